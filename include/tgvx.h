@@ -1,0 +1,169 @@
+/* tgvx: MI355X (gfx950) implementation of the global-TGV hot path.
+ *
+ * C ABI of libtgvx.so. Plain structs, plain pointers and sizes; every call is
+ * synchronous at return; the caller owns all host memory and it is never
+ * retained; a context owns its device buffers and stream and is not
+ * thread-safe (one per thread). All functions return 0 on success and a
+ * non-zero TGVX_E_* code on failure, with a message in tgvx_last_error().
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference root):
+ *   tgvx_histograms    <- compute_histograms        src/histogram.hpp:44-48
+ *   tgvx_mip_build     <- DepthMipmap::build        src/histogram.hpp:27
+ *   tgvx_solve         <- tgv_minimize              src/tgv.hpp:31-36
+ *   tgvx_solve_oc      <- tgv_minimize_oc           src/tgv.hpp:42-48
+ *   tgvx_median_init   <- CubeHistogram::median_indicator, src/histogram.hpp:18
+ *   tgvx_problem_*     <- the level loop of tgv_minimize (src/tgv.cpp:387-416)
+ *                         split into load / solve / read-back so that inputs can
+ *                         stay resident in device memory between solves
+ *   tgvx_level_bench   <- run_level                 src/tgv.cpp:137-232
+ * Struct layouts equal the reference PODs byte for byte:
+ *   tgvx_cube         == OctreeCube   (src/types.hpp:31-35, 16 B)
+ *   tgvx_state        == TGVCube      (src/tgv.hpp:12-17, 52 B)
+ *   tgvx_treetop_leaf == TreetopLeaf  (src/treetop.hpp:8-13, 20 B)
+ */
+#ifndef TGVX_H
+#define TGVX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tgvx_ctx tgvx_ctx;
+
+typedef struct {
+    uint32_t hi, lo; /* 63-bit Morton code of the lower corner on the 2^21 grid */
+    float r_c;       /* sample radius */
+    int32_t depth;
+} tgvx_cube;
+
+typedef struct {
+    float u;
+    float v[3];
+    float p[3];
+    float q[6]; /* Voigt xx yy zz xy xz yz */
+} tgvx_state;
+
+typedef struct {
+    uint32_t hi, lo;
+    int32_t depth;
+    uint32_t first_idx, last_idx; /* inclusive cube index range */
+} tgvx_treetop_leaf;
+
+/* One depth map. Only the base level is passed: the pyramid is built on the
+ * device. world_to_cam is rows 0..2 of the inverse of the reference's
+ * DepthMap::Rt (world <- camera), computed once on the host. */
+typedef struct {
+    const float* depths; /* width*height, row-major, NaN = invalid */
+    int32_t width, height;
+    float fx, fy, cx, cy;
+    float world_to_cam[12];
+    int32_t lidar; /* vote 5 instead of 1 (histogram.cpp:158) */
+} tgvx_view;
+
+enum { TGVX_FP64 = 0, TGVX_FP32 = 1, TGVX_BF16 = 2 };
+enum { TGVX_OP_REF = 0, TGVX_OP_ADJ = 1 };
+enum {
+    TGVX_OK = 0,
+    TGVX_E_ARG = 1,    /* bad argument */
+    TGVX_E_HIP = 2,    /* HIP runtime error */
+    TGVX_E_STATE = 3,  /* call out of order (e.g. solve before load) */
+    TGVX_E_NUMERIC = 4 /* non-finite energy or gap */
+};
+
+/* TGVParams (src/tgv.hpp:24-28) plus the controls the reference hard-codes or
+ * lacks. Defaults (tgvx_default_params) reproduce the reference. */
+typedef struct {
+    float alpha0;      /* weight on |eps(v)|  (reference default 0.02) */
+    float alpha1;      /* weight on |grad u - v| (reference default 0.04) */
+    float lambda;      /* data weight (reference: 1) */
+    int32_t iters;     /* iterations per level when gap_tol <= 0 (reference default 200) */
+    int32_t precision; /* TGVX_FP64 / TGVX_FP32 / TGVX_BF16 (state storage; BF16 computes in FP32) */
+    int32_t op_mode;   /* TGVX_OP_REF: the reference's update rules bit for bit;
+                          TGVX_OP_ADJ: primal step with the exact adjoint of the dual step */
+    int32_t chunk;     /* iterations between gap evaluations (default 50) */
+    int32_t max_iters; /* cap per level when iterating to a gap */
+    double gap_tol;    /* relative duality gap target; <= 0 selects fixed iterations */
+    int32_t warm_fallback; /* with an initial field: per level, keep the cold init when its gap is lower */
+    int32_t use_graph;     /* launch iteration chunks as captured HIP graphs */
+    int32_t block;         /* threads per workgroup (0 = tuned default) */
+    int32_t reserved[5];
+} tgvx_params;
+
+#define TGVX_MAX_LEVELS 22
+
+typedef struct {
+    int32_t n_levels;
+    int32_t level_depth[TGVX_MAX_LEVELS];
+    uint32_t level_leaves[TGVX_MAX_LEVELS];
+    int32_t level_iters[TGVX_MAX_LEVELS];
+    int32_t level_warm[TGVX_MAX_LEVELS]; /* 1: warm init kept, 0: cold */
+    double level_energy[TGVX_MAX_LEVELS]; /* primal energy P at the last evaluation */
+    double level_dual[TGVX_MAX_LEVELS];   /* dual value D at the last evaluation */
+    double leaf_iters;    /* sum over levels of leaves x iterations */
+    double solve_ms;      /* wall time of the level loop, inputs resident on the device */
+    double iterate_ms;    /* time in the iteration kernels alone (events around chunks) */
+    double gap_ms;        /* time in the energy / gap evaluations */
+    double setup_ms;      /* neighbour tables, permutation and per-leaf constants */
+} tgvx_report;
+
+void tgvx_default_params(tgvx_params* p);
+
+int tgvx_create(tgvx_ctx** ctx, int32_t device);
+void tgvx_destroy(tgvx_ctx* ctx);
+const char* tgvx_last_error(const tgvx_ctx* ctx);
+/* Device name and compute units, for reports. */
+int tgvx_device_info(tgvx_ctx* ctx, char* name, int32_t name_len, int32_t* cus);
+
+/* compute_histograms with nb bins (the reference has 8): bins_out holds n*nb
+ * counts, cube-major. Bit-exact with the CPU definition. */
+int tgvx_histograms(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views,
+                    int32_t nviews, const float origin[3], float r_root, int32_t nb, uint32_t* bins_out);
+
+/* DepthMipmap::build of one view: all levels concatenated into out (size
+ * >= width*height*4/3 + 16), (w, h) per level into dims; level count in *nlevels. */
+int tgvx_mip_build(tgvx_ctx* ctx, const tgvx_view* view, float* out, int32_t* dims, int32_t* nlevels);
+
+/* median_indicator of n histograms of nb bins, FP32. */
+int tgvx_median_init(tgvx_ctx* ctx, const uint32_t* bins, uint64_t n, int32_t nb, float* out);
+
+/* tgv_minimize. state_out: n records of 13 values (u, v[3], p[3], q[6]) --
+ * tgvx_state for FP32/BF16, 13 doubles for FP64. init_uv (optional, n x 4
+ * floats: u, vx, vy, vz) replaces the median / parent-copy initialisation of
+ * every level (the learned warm start). report may be NULL. */
+int tgvx_solve(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+               const tgvx_params* params, const float* init_uv, void* state_out, tgvx_report* report);
+
+/* tgv_minimize_oc: per treetop leaf, out-of-leaf u_bar / v_bar read from the
+ * level-start snapshot. All leaves of a level are solved in one launch. */
+int tgvx_solve_oc(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+                  const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* params,
+                  void* state_out, tgvx_report* report);
+
+/* Resident problem: upload cubes + histograms once, build tables on the device. */
+int tgvx_problem_load(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins,
+                      int32_t nb, int32_t precision, int32_t op_mode);
+/* Solve the resident problem from its cold (or given) initialisation. */
+int tgvx_problem_solve(tgvx_ctx* ctx, const tgvx_params* params, const float* init_uv,
+                       tgvx_report* report);
+/* Read back state (layout as tgvx_solve) and, optionally, u_bar/v_bar (n x 4 of the state type). */
+int tgvx_problem_state(tgvx_ctx* ctx, void* state_out, void* ubar_out);
+/* Energy P and dual value D of one level of the resident state (FP64). */
+int tgvx_problem_gap(tgvx_ctx* ctx, int32_t depth, const tgvx_params* params, double* P, double* D);
+/* Per-level sizes of the resident problem: depth and leaf count, returns count. */
+int tgvx_problem_levels(tgvx_ctx* ctx, int32_t* depths, uint32_t* counts, int32_t* nlevels);
+
+/* run_level timing: iters iterations of the level at `depth` of the resident
+ * problem (state continues from where it is). Times measured with HIP events
+ * on the library's stream: total, and the average duration of one launch of
+ * each iteration kernel (dual, primal) from a separately bracketed run. */
+int tgvx_level_bench(tgvx_ctx* ctx, int32_t depth, const tgvx_params* params, int32_t iters,
+                     double* total_ms, double* dual_ms, double* primal_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TGVX_H */

@@ -1,0 +1,229 @@
+"""tgvx: the global-TGV hot path of TGVReconClaude on AMD Instinct MI355X (gfx950).
+
+Python front end of the C ABI in include/tgvx.h. The functions mirror the
+reference's C++ interface (src/tgv.hpp, src/histogram.hpp):
+
+    compute_histograms(cubes, views, origin, r_root, nb)      -> uint32 [n, nb]
+    tgv_minimize(cubes, hists, origin, r_root, params)        -> state records
+    tgv_minimize_oc(cubes, hists, leaves, origin, r_root, params)
+    median_indicator(hists)                                   -> float32 [n]
+
+Every call runs on the GPU through libtgvx.so; there is no CPU fallback. A
+missing or unloadable library raises RuntimeError, as does any failure reported
+by the library (the reference signals errors with std::runtime_error).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import BF16, CUBE_DT, FP32, FP64, LEAF_DT, OP_ADJ, OP_REF, STATE_DT, Params, Report
+
+__all__ = [
+    "Context", "TGVParams", "compute_histograms", "tgv_minimize", "tgv_minimize_oc", "median_indicator",
+    "CUBE_DT", "STATE_DT", "LEAF_DT", "FP64", "FP32", "BF16", "OP_REF", "OP_ADJ", "make_view",
+]
+
+
+def TGVParams(**kw):
+    """TGVParams (src/tgv.hpp:24-28) with the extra controls of tgvx_params.
+
+    Defaults reproduce the reference: alpha0=0.02, alpha1=0.04, iters=200,
+    FP32, reference update rules, fixed iterations.
+    """
+    p = Params()
+    N.lib().tgvx_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "lambda":
+            k = "lambda_"
+        if not hasattr(p, k):
+            raise TypeError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def make_view(depths, fx, fy, cx, cy, world_to_cam, lidar=False):
+    """One depth map (DepthMap + DepthMipmap::build inputs, src/depth_map.hpp:7-12).
+
+    depths: float32 [H, W], NaN = invalid; world_to_cam: 3x4 (rows 0-2 of Rt^-1).
+    Returns (View, keep-alive array).
+    """
+    d = np.ascontiguousarray(depths, dtype=np.float32)
+    v = N.View()
+    v.depths = d.ctypes.data
+    v.height, v.width = d.shape
+    v.fx, v.fy, v.cx, v.cy = fx, fy, cx, cy
+    m = np.asarray(world_to_cam, dtype=np.float32).reshape(12)
+    for i in range(12):
+        v.world_to_cam[i] = float(m[i])
+    v.lidar = 1 if lidar else 0
+    return v, d
+
+
+class Context:
+    """One tgvx context (device buffers + stream) on one GPU."""
+
+    def __init__(self, device=0):
+        self._L = N.lib()
+        h = C.c_void_p()
+        rc = self._L.tgvx_create(C.byref(h), device)
+        if rc != 0:
+            raise RuntimeError(f"tgvx_create failed with code {rc} (no GPU visible?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.tgvx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"tgvx error {rc}: {self._L.tgvx_last_error(self.h).decode()}")
+
+    def device_info(self):
+        name = C.create_string_buffer(64)
+        cus = C.c_int32()
+        self._check(self._L.tgvx_device_info(self.h, name, 64, C.byref(cus)))
+        return name.value.decode(), cus.value
+
+    # ---- histograms
+    def histograms(self, cubes, views, origin, r_root, nb=8):
+        cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+        arr = (N.View * max(1, len(views)))(*[v for v in views])
+        o = np.asarray(origin, dtype=np.float32)
+        out = np.zeros((len(cubes), nb), dtype=np.uint32)
+        self._check(self._L.tgvx_histograms(self.h, N.ptr(cubes), len(cubes), arr, len(views), N.ptr(o),
+                                            float(r_root), nb, N.ptr(out)))
+        return out
+
+    def mip_build(self, view):
+        w, h = view.width, view.height
+        out = np.zeros(w * h * 4 // 3 + 64, dtype=np.float32)
+        dims = np.zeros(2 * 16, dtype=np.int32)
+        nl = C.c_int32()
+        self._check(self._L.tgvx_mip_build(self.h, C.byref(view), N.ptr(out), N.ptr(dims), C.byref(nl)))
+        levels, off = [], 0
+        for l in range(nl.value):
+            lw, lh = int(dims[2 * l]), int(dims[2 * l + 1])
+            levels.append(out[off:off + lw * lh].reshape(lh, lw).copy())
+            off += lw * lh
+        return levels
+
+    def median_indicator(self, hists):
+        h = np.ascontiguousarray(hists, dtype=np.uint32)
+        out = np.zeros(len(h), dtype=np.float32)
+        self._check(self._L.tgvx_median_init(self.h, N.ptr(h), len(h), h.shape[1], N.ptr(out)))
+        return out
+
+    # ---- solves
+    @staticmethod
+    def _state_array(n, precision):
+        if precision == FP64:
+            return np.zeros((n, 13), dtype=np.float64)
+        return np.zeros(n, dtype=STATE_DT)
+
+    def solve(self, cubes, hists, params=None, init_uv=None):
+        """tgv_minimize; returns (state, Report)."""
+        params = params or TGVParams()
+        cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+        h = np.ascontiguousarray(hists, dtype=np.uint32)
+        st = self._state_array(len(cubes), params.precision)
+        rep = Report()
+        init = None if init_uv is None else np.ascontiguousarray(init_uv, dtype=np.float32)
+        self._check(self._L.tgvx_solve(self.h, N.ptr(cubes), len(cubes), N.ptr(h), h.shape[1] if h.ndim == 2 else 8,
+                                       C.byref(params), N.ptr(init), N.ptr(st), C.byref(rep)))
+        return st, rep
+
+    def solve_oc(self, cubes, hists, leaves, params=None):
+        params = params or TGVParams()
+        cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+        h = np.ascontiguousarray(hists, dtype=np.uint32)
+        lv = np.ascontiguousarray(leaves, dtype=LEAF_DT)
+        st = self._state_array(len(cubes), params.precision)
+        rep = Report()
+        self._check(self._L.tgvx_solve_oc(self.h, N.ptr(cubes), len(cubes), N.ptr(h),
+                                          h.shape[1] if h.ndim == 2 else 8, N.ptr(lv), len(lv),
+                                          C.byref(params), N.ptr(st), C.byref(rep)))
+        return st, rep
+
+    # ---- resident problem
+    def load(self, cubes, hists, precision=FP32, op_mode=OP_REF):
+        cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+        h = np.ascontiguousarray(hists, dtype=np.uint32)
+        self._n, self._prec = len(cubes), precision
+        self._check(self._L.tgvx_problem_load(self.h, N.ptr(cubes), len(cubes), N.ptr(h), h.shape[1], precision,
+                                              op_mode))
+
+    def run(self, params, init_uv=None):
+        rep = Report()
+        init = None if init_uv is None else np.ascontiguousarray(init_uv, dtype=np.float32)
+        self._check(self._L.tgvx_problem_solve(self.h, C.byref(params), N.ptr(init), C.byref(rep)))
+        return rep
+
+    def state(self, with_bar=False):
+        st = self._state_array(self._n, self._prec)
+        bar = None
+        if with_bar:
+            bar = np.zeros((self._n, 4), dtype=np.float64 if self._prec == FP64 else np.float32)
+        self._check(self._L.tgvx_problem_state(self.h, N.ptr(st), N.ptr(bar)))
+        return (st, bar) if with_bar else st
+
+    def gap(self, depth, params):
+        P, D = C.c_double(), C.c_double()
+        self._check(self._L.tgvx_problem_gap(self.h, depth, C.byref(params), C.byref(P), C.byref(D)))
+        return P.value, D.value
+
+    def levels(self):
+        d = np.zeros(22, dtype=np.int32)
+        c = np.zeros(22, dtype=np.uint32)
+        n = C.c_int32()
+        self._check(self._L.tgvx_problem_levels(self.h, N.ptr(d), N.ptr(c), C.byref(n)))
+        return [(int(d[i]), int(c[i])) for i in range(n.value)]
+
+    def level_bench(self, depth, params, iters):
+        t, d, p = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._L.tgvx_level_bench(self.h, depth, C.byref(params), iters, C.byref(t), C.byref(d),
+                                             C.byref(p)))
+        return t.value, d.value, p.value
+
+
+_default_ctx = None
+
+
+def _ctx():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def compute_histograms(cubes, views, origin, r_root, nb=8):
+    """compute_histograms (src/histogram.hpp:44-48) with nb bins."""
+    return _ctx().histograms(cubes, views, origin, r_root, nb)
+
+
+def median_indicator(hists):
+    """CubeHistogram::median_indicator (src/histogram.hpp:18) of every row."""
+    return _ctx().median_indicator(hists)
+
+
+def tgv_minimize(cubes, hists, origin=None, r_root=1.0, params=None, init_uv=None):
+    """tgv_minimize (src/tgv.hpp:31-36). origin / r_root are unused, as in the reference."""
+    return _ctx().solve(cubes, hists, params, init_uv)[0]
+
+
+def tgv_minimize_oc(cubes, hists, leaves, origin=None, r_root=1.0, params=None):
+    """tgv_minimize_oc (src/tgv.hpp:42-48)."""
+    return _ctx().solve_oc(cubes, hists, leaves, params)[0]

@@ -1,0 +1,140 @@
+"""ctypes bindings of libtgvx.so (include/tgvx.h) and libtgvx_scene.so (include/tgvx_scene.h).
+
+The structs mirror the C headers byte for byte; the numpy dtypes mirror the
+reference PODs (OctreeCube 16 B, TGVCube 52 B, TreetopLeaf 20 B).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+
+CUBE_DT = np.dtype([("hi", "<u4"), ("lo", "<u4"), ("r_c", "<f4"), ("depth", "<i4")])
+STATE_DT = np.dtype([("u", "<f4"), ("v", "<f4", 3), ("p", "<f4", 3), ("q", "<f4", 6)])
+LEAF_DT = np.dtype([("hi", "<u4"), ("lo", "<u4"), ("depth", "<i4"), ("first_idx", "<u4"), ("last_idx", "<u4")])
+assert CUBE_DT.itemsize == 16 and STATE_DT.itemsize == 52 and LEAF_DT.itemsize == 20
+
+FP64, FP32, BF16 = 0, 1, 2
+OP_REF, OP_ADJ = 0, 1
+MAX_LEVELS = 22
+
+
+class View(C.Structure):
+    _fields_ = [("depths", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32),
+                ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+                ("world_to_cam", C.c_float * 12), ("lidar", C.c_int32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("alpha0", C.c_float), ("alpha1", C.c_float), ("lambda_", C.c_float),
+                ("iters", C.c_int32), ("precision", C.c_int32), ("op_mode", C.c_int32),
+                ("chunk", C.c_int32), ("max_iters", C.c_int32), ("gap_tol", C.c_double),
+                ("warm_fallback", C.c_int32), ("use_graph", C.c_int32), ("block", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+class Report(C.Structure):
+    _fields_ = [("n_levels", C.c_int32),
+                ("level_depth", C.c_int32 * MAX_LEVELS), ("level_leaves", C.c_uint32 * MAX_LEVELS),
+                ("level_iters", C.c_int32 * MAX_LEVELS), ("level_warm", C.c_int32 * MAX_LEVELS),
+                ("level_energy", C.c_double * MAX_LEVELS), ("level_dual", C.c_double * MAX_LEVELS),
+                ("leaf_iters", C.c_double), ("solve_ms", C.c_double), ("iterate_ms", C.c_double),
+                ("gap_ms", C.c_double), ("setup_ms", C.c_double)]
+
+    def levels(self):
+        out = []
+        for i in range(self.n_levels):
+            P, D = self.level_energy[i], self.level_dual[i]
+            out.append(dict(depth=self.level_depth[i], leaves=self.level_leaves[i],
+                            iters=self.level_iters[i], warm=self.level_warm[i], P=P, D=D,
+                            rel_gap=(P - D) / max(1.0, abs(P))))
+        return out
+
+
+class Prim(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("center", C.c_float * 3), ("size", C.c_float * 3), ("rot", C.c_float * 9)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+                ("cam_to_world", C.c_float * 12), ("world_to_cam", C.c_float * 12)]
+
+
+# exported symbols of each library, as declared in include/*.h
+TGVX_SYMBOLS = [
+    "tgvx_default_params", "tgvx_create", "tgvx_destroy", "tgvx_last_error", "tgvx_device_info",
+    "tgvx_histograms", "tgvx_mip_build", "tgvx_median_init", "tgvx_solve", "tgvx_solve_oc",
+    "tgvx_problem_load", "tgvx_problem_solve", "tgvx_problem_state", "tgvx_problem_gap",
+    "tgvx_problem_levels", "tgvx_level_bench",
+]
+SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
+                 "tgvs_balance", "tgvs_treetop", "tgvs_free"]
+
+_lib = None
+_scene = None
+
+
+def _path(name):
+    p = os.path.join(PKG, name)
+    if not os.path.exists(p):
+        raise RuntimeError(f"{name} is not built: run __graft_entry__.build() (or python tgvreconclaude_amd/build.py)")
+    return p
+
+
+def lib():
+    """libtgvx.so (loaded once). Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        L = C.CDLL(_path("libtgvx.so"))
+        vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int32
+        L.tgvx_default_params.argtypes = [C.POINTER(Params)]
+        L.tgvx_default_params.restype = None
+        L.tgvx_create.argtypes = [C.POINTER(vp), i32]
+        L.tgvx_destroy.argtypes = [vp]
+        L.tgvx_destroy.restype = None
+        L.tgvx_last_error.argtypes = [vp]
+        L.tgvx_last_error.restype = C.c_char_p
+        L.tgvx_device_info.argtypes = [vp, C.c_char_p, i32, C.POINTER(i32)]
+        L.tgvx_histograms.argtypes = [vp, vp, u64, C.POINTER(View), i32, vp, C.c_float, i32, vp]
+        L.tgvx_mip_build.argtypes = [vp, C.POINTER(View), vp, vp, C.POINTER(i32)]
+        L.tgvx_median_init.argtypes = [vp, vp, u64, i32, vp]
+        L.tgvx_solve.argtypes = [vp, vp, u64, vp, i32, C.POINTER(Params), vp, vp, C.POINTER(Report)]
+        L.tgvx_solve_oc.argtypes = [vp, vp, u64, vp, i32, vp, u64, C.POINTER(Params), vp, C.POINTER(Report)]
+        L.tgvx_problem_load.argtypes = [vp, vp, u64, vp, i32, i32, i32]
+        L.tgvx_problem_solve.argtypes = [vp, C.POINTER(Params), vp, C.POINTER(Report)]
+        L.tgvx_problem_state.argtypes = [vp, vp, vp]
+        L.tgvx_problem_gap.argtypes = [vp, i32, C.POINTER(Params), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.tgvx_problem_levels.argtypes = [vp, vp, vp, C.POINTER(i32)]
+        L.tgvx_level_bench.argtypes = [vp, i32, C.POINTER(Params), i32, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        for name in TGVX_SYMBOLS:
+            f = getattr(L, name)
+            if f.restype is C.c_int:  # default
+                f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def scene_lib():
+    global _scene
+    if _scene is None:
+        L = C.CDLL(_path("libtgvx_scene.so"))
+        vp, i32, f32 = C.c_void_p, C.c_int32, C.c_float
+        L.tgvs_ring_cameras.argtypes = [i32, f32, i32, i32, f32, vp]
+        L.tgvs_random_cameras.argtypes = [i32, f32, i32, i32, f32, C.c_uint64, vp]
+        L.tgvs_render.argtypes = [vp, i32, vp, i32, i32, i32, f32, f32, f32, f32, C.c_uint64, vp]
+        L.tgvs_build_octree.argtypes = [vp, vp, i32, i32, i32, vp, f32, i32, C.POINTER(vp)]
+        L.tgvs_build_octree.restype = C.c_int64
+        L.tgvs_balance.argtypes = [vp, C.c_uint64, f32, C.POINTER(vp)]
+        L.tgvs_balance.restype = C.c_int64
+        L.tgvs_treetop.argtypes = [vp, C.c_uint64, C.c_uint32, C.POINTER(vp)]
+        L.tgvs_treetop.restype = C.c_int64
+        L.tgvs_free.argtypes = [vp]
+        L.tgvs_free.restype = None
+        _scene = L
+    return _scene
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)

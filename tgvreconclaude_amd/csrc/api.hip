@@ -1,0 +1,182 @@
+// extern "C" entry points of libtgvx.so: argument checks, error capture,
+// context lifetime. Compute lives in hist.hip and solve.hip.
+#include <chrono>
+#include <cstring>
+
+#include "tgvx_internal.h"
+
+namespace tgvx {
+void release_graphs(const tgvx_ctx* c);
+}
+
+namespace {
+template <class F>
+int guarded(tgvx_ctx* c, F&& f) {
+    if (!c) return TGVX_E_ARG;
+    try {
+        TGVX_HIP(hipSetDevice(c->device));
+        f();
+        c->err.clear();
+        return TGVX_OK;
+    } catch (const tgvx::Error& e) {
+        c->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return TGVX_E_HIP;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void tgvx_default_params(tgvx_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->alpha0 = 0.02f;  // tgv.hpp:25
+    p->alpha1 = 0.04f;  // tgv.hpp:26
+    p->lambda = 1.0f;
+    p->iters = 200;     // tgv.hpp:27
+    p->precision = TGVX_FP32;
+    p->op_mode = TGVX_OP_REF;
+    p->chunk = 50;
+    p->max_iters = 20000;
+    p->gap_tol = 0.0;
+    p->warm_fallback = 1;
+    p->use_graph = 1;
+    p->block = 256;
+}
+
+int tgvx_create(tgvx_ctx** out, int32_t device) {
+    if (!out) return TGVX_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return TGVX_E_HIP;
+    if (device < 0 || device >= n) return TGVX_E_ARG;
+    tgvx_ctx* c = new tgvx_ctx;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return TGVX_E_HIP;
+    }
+    for (auto& e : c->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return TGVX_E_HIP;
+        }
+    }
+    *out = c;
+    return TGVX_OK;
+}
+
+void tgvx_destroy(tgvx_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    tgvx::release_graphs(c);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* tgvx_last_error(const tgvx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tgvx_device_info(tgvx_ctx* c, char* name, int32_t name_len, int32_t* cus) {
+    return guarded(c, [&] {
+        hipDeviceProp_t prop;
+        TGVX_HIP(hipGetDeviceProperties(&prop, c->device));
+        if (name && name_len > 0) {
+            std::strncpy(name, prop.gcnArchName, name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (cus) *cus = prop.multiProcessorCount;
+    });
+}
+
+int tgvx_histograms(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views, int32_t nviews,
+                    const float origin[3], float r_root, int32_t nb, uint32_t* bins_out) {
+    return guarded(c, [&] { tgvx::histograms(c, cubes, n, views, nviews, origin, r_root, nb, bins_out); });
+}
+
+int tgvx_mip_build(tgvx_ctx* c, const tgvx_view* view, float* out, int32_t* dims, int32_t* nlevels) {
+    return guarded(c, [&] { tgvx::mip_build(c, view, out, dims, nlevels); });
+}
+
+int tgvx_median_init(tgvx_ctx* c, const uint32_t* bins, uint64_t n, int32_t nb, float* out) {
+    return guarded(c, [&] { tgvx::median_init(c, bins, n, nb, out); });
+}
+
+int tgvx_problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+                      int32_t precision, int32_t op_mode) {
+    return guarded(c, [&] { tgvx::problem_load(c, cubes, n, bins, nb, precision, op_mode); });
+}
+
+int tgvx_problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep) {
+    return guarded(c, [&] { tgvx::problem_solve(c, p, init_uv, rep, nullptr, 0, nullptr); });
+}
+
+int tgvx_problem_state(tgvx_ctx* c, void* state_out, void* ubar_out) {
+    return guarded(c, [&] { tgvx::problem_state(c, state_out, ubar_out); });
+}
+
+int tgvx_problem_gap(tgvx_ctx* c, int32_t depth, const tgvx_params* p, double* P, double* D) {
+    return guarded(c, [&] {
+        if (!p || !P || !D) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        tgvx::problem_gap(c, depth, p, P, D);
+    });
+}
+
+int tgvx_problem_levels(tgvx_ctx* c, int32_t* depths, uint32_t* counts, int32_t* nlevels) {
+    return guarded(c, [&] {
+        const auto& P = c->prob;
+        if (!P.loaded) throw tgvx::Error(TGVX_E_STATE, "no problem loaded");
+        for (size_t i = 0; i < P.levels.size(); ++i) {
+            if (depths) depths[i] = P.levels[i].depth;
+            if (counts) counts[i] = P.levels[i].n;
+        }
+        if (nlevels) *nlevels = (int32_t)P.levels.size();
+    });
+}
+
+int tgvx_level_bench(tgvx_ctx* c, int32_t depth, const tgvx_params* p, int32_t iters, double* total_ms,
+                     double* dual_ms, double* primal_ms) {
+    return guarded(c, [&] {
+        if (!p || !total_ms || !dual_ms || !primal_ms || iters <= 0) throw tgvx::Error(TGVX_E_ARG, "bad argument");
+        tgvx::level_bench(c, depth, p, iters, total_ms, dual_ms, primal_ms);
+    });
+}
+
+static int solve_common(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+                        const tgvx_params* p, const float* init_uv, const tgvx_treetop_leaf* leaves,
+                        uint64_t nleaves, void* state_out, tgvx_report* rep) {
+    return guarded(c, [&] {
+        if (!p) throw tgvx::Error(TGVX_E_ARG, "null params");
+        if (n > 0 && !state_out) throw tgvx::Error(TGVX_E_ARG, "null state_out");
+        tgvx_report r;
+        std::memset(&r, 0, sizeof(r));
+        const auto t0 = std::chrono::steady_clock::now();
+        tgvx::problem_load(c, cubes, n, bins, nb, p->precision, p->op_mode);
+        r.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        tgvx::problem_solve(c, p, init_uv, &r, leaves, nleaves, cubes);
+        tgvx::problem_state(c, state_out, nullptr);
+        if (rep) *rep = r;
+    });
+}
+
+int tgvx_solve(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+               const tgvx_params* params, const float* init_uv, void* state_out, tgvx_report* report) {
+    return solve_common(c, cubes, n, bins, nb, params, init_uv, nullptr, 0, state_out, report);
+}
+
+int tgvx_solve_oc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+                  const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* params, void* state_out,
+                  tgvx_report* report) {
+    if (n > 0 && (!leaves || nleaves == 0)) {
+        if (c) c->err = "tgvx_solve_oc needs at least one treetop leaf";
+        return TGVX_E_ARG;
+    }
+    return solve_common(c, cubes, n, bins, nb, params, nullptr, leaves, nleaves, state_out, report);
+}
+
+}  // extern "C"

@@ -1,0 +1,971 @@
+// Primal-dual TGV iteration over the level-major permuted octree
+// (reference src/tgv.cpp:137-232 run_level, :241-352 run_level_frozen,
+// :356-511 tgv_minimize / tgv_minimize_oc), energy / duality gap in FP64, and
+// the host orchestration of the coarse-to-fine level loop.
+//
+// Layout in device memory (n cubes, positions g = 0..n-1, level-major):
+//   17 state arrays of S (u, v[3], p[3], q[6], u_bar, v_bar[3]), SoA;
+//   6 int32 neighbour arrays (face f of position g -> position or -1);
+//   W (uint32 vote total), med (T median indicator), parent (int32);
+//   incoming-edge CSR of each level (for the exact-adjoint mode and the gap).
+// One reference iteration = five Jacobi passes; they fuse into two kernels
+// with one device-wide dependency between them:
+//   k_dual   = passes 1-2 (p, q ascent + ball projections), reads u_bar, v_bar;
+//   k_primal = passes 3-5 (u prox, v step, extrapolation), reads the new p, q.
+// Every expression keeps the reference's evaluation order; the library is
+// compiled with -ffp-contract=off, and HIP's division and square root are
+// correctly rounded, so FP32 / FP64 results equal the CPU definition bit for bit.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "tgvx_device.h"
+#include "tgvx_internal.h"
+
+namespace tgvx {
+
+enum { F_U = 0, F_VX, F_VY, F_VZ, F_PX, F_PY, F_PZ, F_Q0, F_Q1, F_Q2, F_Q3, F_Q4, F_Q5, F_UB, F_VBX, F_VBY, F_VBZ };
+
+template <class S>
+struct KArgs {
+    using T = typename Io<S>::T;
+    S* s[17];
+    const int32_t* nb[6];
+    const uint32_t* W;
+    const T* med;
+    const int32_t* parent;
+    const int32_t* in_start;
+    const int32_t* in_ent;
+    uint32_t off, n;          // positions iterated by this launch
+    uint32_t lvl_off, lvl_n;  // the level's positions (active set for the gap)
+    T sigma, tau, a0, a1, tl;
+    // frozen-border mode (run_level_frozen): out-of-leaf u_bar / v_bar come from fz[]
+    const S* fz[4];
+    const uint32_t* orig;  // position -> input index
+    uint32_t leaf_first, leaf_last;
+    const float* init;     // warm start field, 4 per position
+    S* save[4];
+};
+
+template <class T>
+__device__ inline void clip3(T& a, T& b, T& c, T r) {
+    const T n2 = a * a + b * b + c * c;
+    if (n2 > r * r) {
+        const T s = r / sqrt(n2);
+        a *= s;
+        b *= s;
+        c *= s;
+    }
+}
+template <class T>
+__device__ inline void clip6(T* q, T r) {
+    T n2 = T(0);
+    for (int i = 0; i < 6; ++i) n2 += q[i] * q[i];
+    if (n2 > r * r) {
+        const T s = r / sqrt(n2);
+        for (int i = 0; i < 6; ++i) q[i] *= s;
+    }
+}
+// prox_data (tgv.cpp:75-94) with the median hoisted out of the iteration
+// (it depends only on the histogram): shrink towards med by tau*lambda*W.
+template <class T>
+__device__ inline T prox(T z, uint32_t W, T med, T tl) {
+    if (W == 0u) return z;
+    const T diff = z - med;
+    const T shrink = tl * (T)W;
+    if (diff > shrink) return z - shrink;
+    if (diff < -shrink) return z + shrink;
+    return med;
+}
+
+// ---- phase D: dual p and q (tgv.cpp:158-189; frozen variant :274-309)
+template <class S, bool FROZEN>
+__global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    const T ubi = IO::ld(a.s[F_UB], g), vxi = IO::ld(a.s[F_VBX], g), vyi = IO::ld(a.s[F_VBY], g),
+            vzi = IO::ld(a.s[F_VBZ], g);
+    const int j0 = a.nb[0][g], j2 = a.nb[2][g], j4 = a.nb[4][g];
+    auto live = [&](int j) -> bool {
+        if (!FROZEN) return true;
+        const uint32_t o = a.orig[j];
+        return o >= a.leaf_first && o <= a.leaf_last;
+    };
+    auto rd = [&](int j, int f, int fzi, T self) -> T {
+        if (j < 0) return self;
+        if (FROZEN && !live(j)) return IO::ld(a.fz[fzi], j);
+        return IO::ld(a.s[f], j);
+    };
+    const T ub0 = rd(j0, F_UB, 0, ubi), ub2 = rd(j2, F_UB, 0, ubi), ub4 = rd(j4, F_UB, 0, ubi);
+    T p0 = IO::ld(a.s[F_PX], g), p1 = IO::ld(a.s[F_PY], g), p2 = IO::ld(a.s[F_PZ], g);
+    p0 += a.sigma * (ub0 - ubi - vxi);
+    p1 += a.sigma * (ub2 - ubi - vyi);
+    p2 += a.sigma * (ub4 - ubi - vzi);
+    clip3(p0, p1, p2, a.a1);
+    IO::st(a.s[F_PX], g, p0);
+    IO::st(a.s[F_PY], g, p1);
+    IO::st(a.s[F_PZ], g, p2);
+
+    const T x0 = rd(j0, F_VBX, 1, vxi), y0 = rd(j0, F_VBY, 2, vyi), z0 = rd(j0, F_VBZ, 3, vzi);
+    const T x2 = rd(j2, F_VBX, 1, vxi), y2 = rd(j2, F_VBY, 2, vyi), z2 = rd(j2, F_VBZ, 3, vzi);
+    const T x4 = rd(j4, F_VBX, 1, vxi), y4 = rd(j4, F_VBY, 2, vyi), z4 = rd(j4, F_VBZ, 3, vzi);
+    const T eps[6] = {x0 - vxi,
+                      y2 - vyi,
+                      z4 - vzi,
+                      T(0.5) * (x2 - vxi + y0 - vyi),
+                      T(0.5) * (x4 - vxi + z0 - vzi),
+                      T(0.5) * (y4 - vyi + z2 - vzi)};
+    T q[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = IO::ld(a.s[F_Q0 + k], g) + a.sigma * eps[k];
+    clip6(q, a.a0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) IO::st(a.s[F_Q0 + k], g, q[k]);
+}
+
+// ---- phase P, reference rules: primal u, primal v, extrapolation
+// (tgv.cpp:191-230).
+template <class S>
+__global__ void __launch_bounds__(256) k_primal_ref(KArgs<S> a) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    const int m0 = a.nb[1][g], m2 = a.nb[3][g], m4 = a.nb[5][g];
+    auto ld = [&](int f, int j) -> T { return j >= 0 ? IO::ld(a.s[f], j) : T(0); };
+    const T px = IO::ld(a.s[F_PX], g), py = IO::ld(a.s[F_PY], g), pz = IO::ld(a.s[F_PZ], g);
+    const T div_p = (px - ld(F_PX, m0)) + (py - ld(F_PY, m2)) + (pz - ld(F_PZ, m4));
+    const T u = IO::ld(a.s[F_U], g);
+    const T un = prox(u + a.tau * div_p, a.W[g], a.med[g], a.tl);
+
+    const T q0 = IO::ld(a.s[F_Q0], g), q1 = IO::ld(a.s[F_Q1], g), q2 = IO::ld(a.s[F_Q2], g),
+            q3 = IO::ld(a.s[F_Q3], g), q4 = IO::ld(a.s[F_Q4], g), q5 = IO::ld(a.s[F_Q5], g);
+    const T dqx = (q0 - ld(F_Q0, m0)) + (q3 - ld(F_Q3, m2)) + (q4 - ld(F_Q4, m4));
+    const T dqy = (q3 - ld(F_Q3, m0)) + (q1 - ld(F_Q1, m2)) + (q5 - ld(F_Q5, m4));
+    const T dqz = (q4 - ld(F_Q4, m0)) + (q5 - ld(F_Q5, m2)) + (q2 - ld(F_Q2, m4));
+    const T vx = IO::ld(a.s[F_VX], g), vy = IO::ld(a.s[F_VY], g), vz = IO::ld(a.s[F_VZ], g);
+    const T vxn = vx + a.tau * (px + dqx);
+    const T vyn = vy + a.tau * (py + dqy);
+    const T vzn = vz + a.tau * (pz + dqz);
+    IO::st(a.s[F_UB], g, T(2) * un - u);
+    IO::st(a.s[F_VBX], g, T(2) * vxn - vx);
+    IO::st(a.s[F_VBY], g, T(2) * vyn - vy);
+    IO::st(a.s[F_VBZ], g, T(2) * vzn - vz);
+    IO::st(a.s[F_U], g, un);
+    IO::st(a.s[F_VX], g, vxn);
+    IO::st(a.s[F_VY], g, vyn);
+    IO::st(a.s[F_VZ], g, vzn);
+}
+
+// q component paired with (v component c, axis a) in eps: the diagonal entry
+// for c == a, else xy / xz / yz for {c, a} = {0,1} / {0,2} / {1,2}.
+__host__ __device__ constexpr int qidx(int c, int ax) {
+    return c == ax ? F_Q0 + c : (c + ax == 1 ? F_Q3 : (c + ax == 2 ? F_Q4 : F_Q5));
+}
+// Field f + ax for a run-time axis, without indexing the argument array.
+template <class S>
+__device__ inline const S* field3(const KArgs<S>& a, int f0, int f1, int f2, int ax) {
+    return ax == 0 ? a.s[f0] : (ax == 1 ? a.s[f1] : a.s[f2]);
+}
+
+// ---- phase P, exact adjoint (no reference counterpart): the primal step
+// uses minus the transpose of the dual step's linear map: self terms only
+// where the +a neighbour exists, incoming terms from every active cube whose
+// +a neighbour this one is, and 1/2 on the off-diagonal eps entries.
+template <class S>
+__global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    T p[3] = {IO::ld(a.s[F_PX], g), IO::ld(a.s[F_PY], g), IO::ld(a.s[F_PZ], g)};
+    T dp = T(0);
+    T dq[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        if (a.nb[2 * ax][g] < 0) continue;
+        dp += p[ax];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T w = (c == ax) ? T(1) : T(0.5);
+            dq[c] += w * IO::ld(a.s[qidx(c, ax)], g);
+        }
+    }
+    const int e0 = a.in_start[g], e1 = a.in_start[g + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int ent = a.in_ent[e];
+        const long k = ent & 0x3fffffff;
+        const int ax = (int)((uint32_t)ent >> 30);
+        dp -= IO::ld(field3(a, F_PX, F_PY, F_PZ, ax), k);
+        // (c, ax) -> q: c = 0: q0 q3 q4; c = 1: q3 q1 q5; c = 2: q4 q5 q2
+        const T w0 = ax == 0 ? T(1) : T(0.5), w1 = ax == 1 ? T(1) : T(0.5), w2 = ax == 2 ? T(1) : T(0.5);
+        dq[0] -= w0 * IO::ld(field3(a, F_Q0, F_Q3, F_Q4, ax), k);
+        dq[1] -= w1 * IO::ld(field3(a, F_Q3, F_Q1, F_Q5, ax), k);
+        dq[2] -= w2 * IO::ld(field3(a, F_Q4, F_Q5, F_Q2, ax), k);
+    }
+    const T u = IO::ld(a.s[F_U], g);
+    const T un = prox(u + a.tau * dp, a.W[g], a.med[g], a.tl);
+    T vn[3], vo[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        vo[c] = IO::ld(a.s[F_VX + c], g);
+        vn[c] = vo[c] + a.tau * (p[c] + dq[c]);
+    }
+    IO::st(a.s[F_UB], g, T(2) * un - u);
+    for (int c = 0; c < 3; ++c) IO::st(a.s[F_VBX + c], g, T(2) * vn[c] - vo[c]);
+    IO::st(a.s[F_U], g, un);
+    for (int c = 0; c < 3; ++c) IO::st(a.s[F_VX + c], g, vn[c]);
+}
+
+// ---- initialisation
+// tgv.cpp:372-385: u = median, everything else 0, u_bar = u.
+template <class S>
+__global__ void k_reset(KArgs<S> a, uint32_t n) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const T m = a.med[g];
+    IO::st(a.s[F_U], g, m);
+    IO::st(a.s[F_UB], g, m);
+#pragma unroll
+    for (int f = F_VX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
+#pragma unroll
+    for (int f = F_VBX; f <= F_VBZ; ++f) IO::st(a.s[f], g, T(0));
+}
+
+// tgv.cpp:391-409: u, v from the cube holding the parent code (if any), p = q
+// = 0, u_bar = u, v_bar = v. A parent found at the same depth holds the code
+// of its own parent and copies itself, so reading it while it is written
+// returns the same value: the gather order does not matter.
+template <class S>
+__global__ void k_parent_init(KArgs<S> a) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    const int pi = a.parent[g];
+    const long src = pi >= 0 ? pi : g;
+    const T u = IO::ld(a.s[F_U], src), vx = IO::ld(a.s[F_VX], src), vy = IO::ld(a.s[F_VY], src),
+            vz = IO::ld(a.s[F_VZ], src);
+    IO::st(a.s[F_U], g, u);
+    IO::st(a.s[F_VX], g, vx);
+    IO::st(a.s[F_VY], g, vy);
+    IO::st(a.s[F_VZ], g, vz);
+#pragma unroll
+    for (int f = F_PX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
+    IO::st(a.s[F_UB], g, u);
+    IO::st(a.s[F_VBX], g, vx);
+    IO::st(a.s[F_VBY], g, vy);
+    IO::st(a.s[F_VBZ], g, vz);
+}
+
+// Warm start: u, v from a given field; mode 1 also saves the current u, v of
+// the level first, mode 2 restores the saved u, v instead.
+template <class S>
+__global__ void k_field_init(KArgs<S> a, int mode) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    T x[4];
+    if (mode == 2) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) x[c] = IO::ld(a.save[c], t);
+    } else {
+        if (mode == 1) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) IO::st(a.save[c], t, IO::ld(a.s[F_U + c], g));
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) x[c] = (T)a.init[g * 4 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::st(a.s[F_U + c], g, x[c]);
+#pragma unroll
+    for (int f = F_PX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::st(a.s[F_UB + c], g, x[c]);
+}
+
+// ---- energy and duality gap (no reference counterpart)
+// Per active cube i (neighbours outside the level frozen at u_bar / v_bar):
+//   P_i = lambda W |u - m| + alpha1 |K1(u, v)| + alpha0 |eps(v)|
+//   D_i = <frozen part, (p, q)> + min_{u in [-1,1]} (lambda W |u - m| + c_u u)
+//         - V (|c_vx| + |c_vy| + |c_vz|)
+// with c_u, c_v the coefficients of u_i, v_i in the Lagrangian. Summed in FP64
+// per block, then over blocks in a fixed order.
+constexpr int kGapBlock = 256;
+
+template <class S>
+__global__ void __launch_bounds__(kGapBlock) k_gap(KArgs<S> a, double lam, double a0, double a1, double vbox,
+                                                    double* part) {
+    using IO = Io<S>;
+    __shared__ double sp[kGapBlock], sd[kGapBlock];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    double P = 0.0, D = 0.0;
+    if (t < a.n) {
+        const long g = (long)a.off + t;
+        auto in_lvl = [&](int j) { return (uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n; };
+        auto L = [&](int f, long j) -> double { return (double)IO::ld(a.s[f], j); };
+        const uint32_t Wc = a.W[g];
+        const double W = (double)Wc, m = (double)a.med[g], ui = L(F_U, g);
+        const double v[3] = {L(F_VX, g), L(F_VY, g), L(F_VZ, g)};
+        const double p[3] = {L(F_PX, g), L(F_PY, g), L(F_PZ, g)};
+        double gr[3], dv[3][3];
+        int jn[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int j = a.nb[2 * ax][g];
+            jn[ax] = j;
+            if (j >= 0) {
+                const bool in = in_lvl(j);
+                gr[ax] = (in ? L(F_U, j) : L(F_UB, j)) - ui;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dv[c][ax] = (in ? L(F_VX + c, j) : L(F_VBX + c, j)) - v[c];
+            } else {
+                gr[ax] = 0.0;
+                for (int c = 0; c < 3; ++c) dv[c][ax] = 0.0;
+            }
+        }
+        const double k1[3] = {gr[0] - v[0], gr[1] - v[1], gr[2] - v[2]};
+        const double e[6] = {dv[0][0], dv[1][1], dv[2][2], 0.5 * (dv[0][1] + dv[1][0]), 0.5 * (dv[0][2] + dv[2][0]),
+                             0.5 * (dv[1][2] + dv[2][1])};
+        const double nk = sqrt(k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]);
+        double ne = 0.0;
+        for (int k = 0; k < 6; ++k) ne += e[k] * e[k];
+        ne = sqrt(ne);
+        P = (Wc ? lam * W * fabs(ui - m) : 0.0) + a1 * nk + a0 * ne;
+
+        double cst = 0.0, cu = 0.0, cv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int j = jn[ax];
+            if (j < 0) continue;
+            cu -= p[ax];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double w = (c == ax) ? 1.0 : 0.5;
+                cv[c] -= w * L(qidx(c, ax), g);
+            }
+            if (!in_lvl(j)) {
+                cst += p[ax] * L(F_UB, j);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double w = (c == ax) ? 1.0 : 0.5;
+                    cst += w * L(qidx(c, ax), g) * L(F_VBX + c, j);
+                }
+            }
+        }
+        for (int e2 = a.in_start[g]; e2 < a.in_start[g + 1]; ++e2) {
+            const int ent = a.in_ent[e2];
+            const long k = ent & 0x3fffffff;
+            const int ax = (int)((uint32_t)ent >> 30);
+            auto LD = [&](const S* f) { return (double)IO::ld(f, k); };
+            cu += LD(field3(a, F_PX, F_PY, F_PZ, ax));
+            cv[0] += (ax == 0 ? 1.0 : 0.5) * LD(field3(a, F_Q0, F_Q3, F_Q4, ax));
+            cv[1] += (ax == 1 ? 1.0 : 0.5) * LD(field3(a, F_Q3, F_Q1, F_Q5, ax));
+            cv[2] += (ax == 2 ? 1.0 : 0.5) * LD(field3(a, F_Q4, F_Q5, F_Q2, ax));
+        }
+        for (int c = 0; c < 3; ++c) cv[c] -= p[c];
+        double mu;
+        if (Wc == 0u) {
+            mu = -fabs(cu);
+        } else {
+            const double g0 = lam * W * fabs(-1.0 - m) + cu * -1.0;
+            const double g1 = lam * W * fabs(m - m) + cu * m;
+            const double g2 = lam * W * fabs(1.0 - m) + cu * 1.0;
+            mu = fmin(g0, fmin(g1, g2));
+        }
+        D = cst + mu - vbox * (fabs(cv[0]) + fabs(cv[1]) + fabs(cv[2]));
+    }
+    sp[threadIdx.x] = P;
+    sd[threadIdx.x] = D;
+    __syncthreads();
+    for (int s = kGapBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sp[threadIdx.x] += sp[threadIdx.x + s];
+            sd[threadIdx.x] += sd[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sp[0];
+        part[2 * blockIdx.x + 1] = sd[0];
+    }
+}
+
+__global__ void __launch_bounds__(kGapBlock) k_sum_pairs(const double* part, uint32_t nparts, double* out) {
+    __shared__ double sp[kGapBlock], sd[kGapBlock];
+    double P = 0.0, D = 0.0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += kGapBlock) {
+        P += part[2 * i];
+        D += part[2 * i + 1];
+    }
+    sp[threadIdx.x] = P;
+    sd[threadIdx.x] = D;
+    __syncthreads();
+    for (int s = kGapBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sp[threadIdx.x] += sp[threadIdx.x + s];
+            sd[threadIdx.x] += sd[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sp[0];
+        out[1] = sd[0];
+    }
+}
+
+// ---- neighbour tables on the device (tgv.cpp:97-133)
+// Lookup in the code map: ucode ascending, uidx = position of the last input
+// cube with that code (the reference's unordered_map "last writer wins").
+__device__ inline int code_lookup(const uint64_t* uc, const int32_t* ui, uint32_t U, uint64_t key) {
+    uint32_t lo = 0, hi = U;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (uc[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < U && uc[lo] == key) ? ui[lo] : -1;
+}
+
+__global__ void k_tables(const uint64_t* __restrict__ codes, const int32_t* __restrict__ depth, uint32_t n,
+                         const uint64_t* __restrict__ uc, const int32_t* __restrict__ ui, uint32_t U,
+                         int32_t* nb0, int32_t* nb1, int32_t* nb2, int32_t* nb3, int32_t* nb4, int32_t* nb5,
+                         int32_t* parent) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint64_t c = codes[g];
+    const int d = depth[g];
+    int32_t* out[6] = {nb0, nb1, nb2, nb3, nb4, nb5};
+    for (int f = 0; f < 6; ++f) {
+        const uint64_t k = face_nb(c, f, d);
+        int j = -1;
+        if (k != kNone) {
+            j = code_lookup(uc, ui, U, k);
+            if (j < 0 && d > 0) j = code_lookup(uc, ui, U, ancestor(k, d - 1));
+        }
+        out[f][g] = j;
+    }
+    parent[g] = d > 0 ? code_lookup(uc, ui, U, ancestor(c, d - 1)) : -1;
+}
+
+// ---------------------------------------------------------------- host side
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class S>
+KArgs<S> make_args(tgvx_ctx* c, const tgvx_params* prm) {
+    using T = typename Io<S>::T;
+    Problem& P = c->prob;
+    KArgs<S> a;
+    std::memset(&a, 0, sizeof(a));
+    for (int i = 0; i < 17; ++i) a.s[i] = P.d_state[i].as<S>();
+    for (int f = 0; f < 6; ++f) a.nb[f] = P.d_nb[f].as<int32_t>();
+    a.W = P.d_W.as<uint32_t>();
+    a.med = P.d_med.as<T>();
+    a.parent = P.d_parent.as<int32_t>();
+    a.in_start = P.d_in_start.as<int32_t>();
+    a.in_ent = P.d_in_ent.as<int32_t>();
+    for (int i = 0; i < 4; ++i) a.save[i] = P.d_save[i].as<S>();
+    a.init = P.d_init.as<float>();
+    if (prm) {
+        a.sigma = T(0.25);
+        a.tau = T(0.25);
+        a.a0 = (T)prm->alpha0;
+        a.a1 = (T)prm->alpha1;
+        a.tl = a.tau * (T)prm->lambda;
+    }
+    return a;
+}
+
+int block_of(const tgvx_params* p) {
+    int b = p && p->block > 0 ? p->block : 256;
+    if (b != 64 && b != 128 && b != 256) throw Error(TGVX_E_ARG, "block must be 64, 128 or 256");
+    return b;
+}
+
+template <class S>
+void launch_iter(tgvx_ctx* c, KArgs<S>& a, int op_mode, int block, bool frozen) {
+    const unsigned grid = (a.n + block - 1) / block;
+    if (frozen) k_dual<S, true><<<grid, block, 0, c->stream>>>(a);
+    else k_dual<S, false><<<grid, block, 0, c->stream>>>(a);
+    if (op_mode == TGVX_OP_ADJ) k_primal_adj<S><<<grid, block, 0, c->stream>>>(a);
+    else k_primal_ref<S><<<grid, block, 0, c->stream>>>(a);
+}
+
+// Cached graph of `iters` iterations of one level.
+struct GraphKey {
+    uint32_t off, n;
+    int iters, op, block, prec;
+    bool operator<(const GraphKey& o) const {
+        return std::tie(off, n, iters, op, block, prec) < std::tie(o.off, o.n, o.iters, o.op, o.block, o.prec);
+    }
+};
+struct GraphCache {
+    std::map<GraphKey, hipGraphExec_t> m;
+    ~GraphCache() { clear(); }
+    void clear() {
+        for (auto& kv : m) (void)hipGraphExecDestroy(kv.second);
+        m.clear();
+    }
+};
+std::map<const tgvx_ctx*, GraphCache> g_graphs;
+
+template <class S>
+void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen) {
+    if (iters <= 0 || a.n == 0) return;
+    const int block = block_of(p);
+    if (p->use_graph && !frozen && iters > 1) {
+        GraphKey key{a.off, a.n, iters, p->op_mode, block, c->prob.precision};
+        GraphCache& gc = g_graphs[c];
+        auto it = gc.m.find(key);
+        if (it == gc.m.end()) {
+            hipGraph_t graph;
+            TGVX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, false);
+            TGVX_HIP(hipStreamEndCapture(c->stream, &graph));
+            hipGraphExec_t exec;
+            TGVX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            TGVX_HIP(hipGraphDestroy(graph));
+            it = gc.m.emplace(key, exec).first;
+        }
+        TGVX_HIP(hipGraphLaunch(it->second, c->stream));
+        return;
+    }
+    for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, frozen);
+    TGVX_HIP(hipGetLastError());
+}
+
+template <class S>
+void eval_gap(tgvx_ctx* c, KArgs<S> a, const tgvx_params* p, double* Pout, double* Dout) {
+    Problem& P = c->prob;
+    a.off = a.lvl_off;
+    a.n = a.lvl_n;
+    const uint32_t blocks = (a.n + kGapBlock - 1) / kGapBlock;
+    P.d_part.ensure(sizeof(double) * 2 * std::max<uint32_t>(blocks, 1));
+    P.d_pd.ensure(sizeof(double) * 2);
+    if (blocks > 0)
+        k_gap<S><<<blocks, kGapBlock, 0, c->stream>>>(a, (double)p->lambda, (double)p->alpha0, (double)p->alpha1,
+                                                      2.0, P.d_part.as<double>());
+    k_sum_pairs<<<1, kGapBlock, 0, c->stream>>>(P.d_part.as<double>(), blocks, P.d_pd.as<double>());
+    TGVX_HIP(hipGetLastError());
+    double pd[2];
+    TGVX_HIP(hipMemcpyAsync(pd, P.d_pd.p, sizeof(pd), hipMemcpyDeviceToHost, c->stream));
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    *Pout = pd[0];
+    *Dout = pd[1];
+}
+
+double rel_gap(double P, double D) { return (P - D) / std::max(1.0, std::fabs(P)); }
+
+template <class S>
+void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
+             const tgvx_treetop_leaf* leaves, uint64_t nleaves) {
+    Problem& P = c->prob;
+    KArgs<S> a = make_args<S>(c, p);
+    const bool to_gap = p->gap_tol > 0.0;
+    const int chunk = p->chunk > 0 ? p->chunk : 50;
+    const int max_iters = p->max_iters > 0 ? p->max_iters : 100000;
+    tgvx_report r;
+    std::memset(&r, 0, sizeof(r));
+    const double t0 = now_ms();
+    if (init_uv) {
+        P.d_init.ensure(sizeof(float) * 4 * P.n);
+        std::vector<float> perm_init(4 * P.n);
+        for (uint64_t g = 0; g < P.n; ++g) std::memcpy(&perm_init[4 * g], init_uv + 4 * (size_t)P.perm[g], 16);
+        TGVX_HIP(hipMemcpyAsync(P.d_init.p, perm_init.data(), sizeof(float) * 4 * P.n, hipMemcpyHostToDevice,
+                                c->stream));
+        TGVX_HIP(hipStreamSynchronize(c->stream));
+        a.init = P.d_init.as<float>();
+    }
+    const double t_solve0 = now_ms();
+    if (P.n > 0) {
+        k_reset<S><<<(unsigned)((P.n + 255) / 256), 256, 0, c->stream>>>(a, (uint32_t)P.n);
+        TGVX_HIP(hipGetLastError());
+    }
+    double it_ms = 0.0, gap_ms = 0.0;
+    std::vector<S*> fz_store;
+    for (size_t li = 0; li < P.levels.size(); ++li) {
+        const Level& L = P.levels[li];
+        a.off = L.off;
+        a.n = L.n;
+        a.lvl_off = L.off;
+        a.lvl_n = L.n;
+        const unsigned grid = (L.n + 255) / 256;
+        int warm_kept = 0;
+        if (init_uv) {
+            if (p->warm_fallback) {
+                if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
+                double Pc, Dc, Pw, Dw;
+                eval_gap<S>(c, a, p, &Pc, &Dc);
+                k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 1);
+                eval_gap<S>(c, a, p, &Pw, &Dw);
+                if (Pw - Dw > Pc - Dc) k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 2);
+                else warm_kept = 1;
+            } else {
+                k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 0);
+                warm_kept = 1;
+            }
+        } else if (li > 0) {
+            k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
+        }
+        TGVX_HIP(hipGetLastError());
+        int done = 0;
+        double Pv = 0.0, Dv = 0.0;
+        if (leaves) {
+            // tgv_minimize_oc: snapshot u_bar / v_bar, then each treetop leaf's
+            // part of the level in turn (tgv.cpp:482-507).
+            for (int i = 0; i < 4; ++i) {
+                TGVX_HIP(hipMemcpyAsync(P.d_save[i].p, P.d_state[F_UB + i].p, P.elem * P.n,
+                                        hipMemcpyDeviceToDevice, c->stream));
+                a.fz[i] = P.d_save[i].as<S>();
+            }
+            a.orig = P.d_orig.as<uint32_t>();
+            const uint32_t* pb = P.perm.data() + L.off;
+            for (uint64_t l = 0; l < nleaves; ++l) {
+                const uint32_t* lo = std::lower_bound(pb, pb + L.n, leaves[l].first_idx);
+                const uint32_t* hi = std::upper_bound(pb, pb + L.n, leaves[l].last_idx);
+                if (lo == hi) continue;
+                a.off = L.off + (uint32_t)(lo - pb);
+                a.n = (uint32_t)(hi - lo);
+                a.leaf_first = leaves[l].first_idx;
+                a.leaf_last = leaves[l].last_idx;
+                run_iters<S>(c, a, p, p->iters, true);
+            }
+            a.off = L.off;
+            a.n = L.n;
+            done = p->iters;
+        } else if (!to_gap) {
+            TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+            run_iters<S>(c, a, p, p->iters, false);
+            TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+            TGVX_HIP(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            it_ms += ms;
+            done = p->iters;
+            const double g0 = now_ms();
+            eval_gap<S>(c, a, p, &Pv, &Dv);
+            gap_ms += now_ms() - g0;
+        } else {
+            double g0 = now_ms();
+            eval_gap<S>(c, a, p, &Pv, &Dv);
+            gap_ms += now_ms() - g0;
+            while (done < max_iters && rel_gap(Pv, Dv) > p->gap_tol) {
+                if (!std::isfinite(Pv) || !std::isfinite(Dv))
+                    throw Error(TGVX_E_NUMERIC, "non-finite energy or gap at depth " + std::to_string(L.depth));
+                const int k = std::min(chunk, max_iters - done);
+                TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+                run_iters<S>(c, a, p, k, false);
+                TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+                done += k;
+                g0 = now_ms();
+                eval_gap<S>(c, a, p, &Pv, &Dv);
+                gap_ms += now_ms() - g0;
+                float ms = 0.f;
+                TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                it_ms += ms;
+            }
+        }
+        if (li < (size_t)TGVX_MAX_LEVELS) {
+            r.level_depth[li] = L.depth;
+            r.level_leaves[li] = L.n;
+            r.level_iters[li] = done;
+            r.level_warm[li] = warm_kept;
+            r.level_energy[li] = Pv;
+            r.level_dual[li] = Dv;
+        }
+        r.leaf_iters += (double)L.n * done;
+    }
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    r.n_levels = (int)std::min<size_t>(P.levels.size(), TGVX_MAX_LEVELS);
+    r.solve_ms = now_ms() - t_solve0;
+    r.iterate_ms = it_ms;
+    r.gap_ms = gap_ms;
+    (void)t0;
+    if (rep) {
+        const double setup = rep->setup_ms;
+        *rep = r;
+        rep->setup_ms = setup;
+    }
+}
+
+template <class S>
+void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
+             double* primal_ms) {
+    Problem& P = c->prob;
+    const Level* L = nullptr;
+    for (const Level& l : P.levels)
+        if (l.depth == depth) L = &l;
+    if (!L) throw Error(TGVX_E_ARG, "no cubes at depth " + std::to_string(depth));
+    KArgs<S> a = make_args<S>(c, p);
+    a.off = a.lvl_off = L->off;
+    a.n = a.lvl_n = L->n;
+    const int block = block_of(p);
+    const unsigned grid = (a.n + block - 1) / block;
+    // whole iterations (graph or plain launches as configured)
+    TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+    run_iters<S>(c, a, p, iters, false);
+    TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+    TGVX_HIP(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    *total_ms = ms;
+    // each kernel bracketed by its own events
+    const int k = std::max(1, std::min(iters, 20));
+    double sd = 0.0, sp = 0.0;
+    for (int i = 0; i < k; ++i) {
+        TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+        k_dual<S, false><<<grid, block, 0, c->stream>>>(a);
+        TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+        if (p->op_mode == TGVX_OP_ADJ) k_primal_adj<S><<<grid, block, 0, c->stream>>>(a);
+        else k_primal_ref<S><<<grid, block, 0, c->stream>>>(a);
+        TGVX_HIP(hipEventRecord(c->ev[2], c->stream));
+        TGVX_HIP(hipEventSynchronize(c->ev[2]));
+        float m1 = 0.f, m2 = 0.f;
+        TGVX_HIP(hipEventElapsedTime(&m1, c->ev[0], c->ev[1]));
+        TGVX_HIP(hipEventElapsedTime(&m2, c->ev[1], c->ev[2]));
+        sd += m1;
+        sp += m2;
+    }
+    *dual_ms = sd / k;
+    *primal_ms = sp / k;
+}
+
+}  // namespace
+
+void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb, int precision,
+                  int op_mode) {
+    if (nb <= 0 || nb > 32) throw Error(TGVX_E_ARG, "nb must be in 1..32");
+    if (n >= (1ull << 30)) throw Error(TGVX_E_ARG, "at most 2^30 - 1 cubes");
+    if (precision < TGVX_FP64 || precision > TGVX_BF16) throw Error(TGVX_E_ARG, "bad precision");
+    if (n > 0 && (!cubes || !bins)) throw Error(TGVX_E_ARG, "null pointer");
+    const double t0 = now_ms();
+    g_graphs[c].clear();
+    Problem& P = c->prob;
+    P.loaded = false;
+    P.n = n;
+    P.nb = nb;
+    P.precision = precision;
+    P.op_mode = op_mode;
+    P.elem = precision == TGVX_FP64 ? 8 : (precision == TGVX_FP32 ? 4 : 2);
+    // level-major permutation (counting sort by depth, stable)
+    std::vector<uint64_t> cnt(22, 0);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (cubes[i].depth < 0 || cubes[i].depth > 21) throw Error(TGVX_E_ARG, "cube depth outside 0..21");
+        cnt[cubes[i].depth]++;
+    }
+    P.levels.clear();
+    std::vector<uint64_t> start(22, 0);
+    uint64_t acc = 0;
+    for (int d = 0; d < 22; ++d) {
+        start[d] = acc;
+        if (cnt[d]) P.levels.push_back({d, (uint32_t)acc, (uint32_t)cnt[d]});
+        acc += cnt[d];
+    }
+    P.perm.assign(n, 0);
+    P.inv.assign(n, 0);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t g = start[cubes[i].depth]++;
+        P.perm[g] = (uint32_t)i;
+        P.inv[i] = (uint32_t)g;
+    }
+    // code map: sorted codes, last writer (largest input index) per code
+    std::vector<std::pair<uint64_t, uint32_t>> kv(n);
+    bool sorted = true;
+    for (uint64_t i = 0; i < n; ++i) {
+        kv[i] = {((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i};
+        if (i && kv[i].first < kv[i - 1].first) sorted = false;
+    }
+    if (!sorted) std::stable_sort(kv.begin(), kv.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    std::vector<uint64_t> ucode;
+    std::vector<int32_t> uidx;
+    ucode.reserve(n);
+    uidx.reserve(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (i + 1 < n && kv[i + 1].first == kv[i].first) continue;
+        ucode.push_back(kv[i].first);
+        uidx.push_back((int32_t)P.inv[kv[i].second]);
+    }
+    std::vector<std::pair<uint64_t, uint32_t>>().swap(kv);
+    // per-position code, depth, vote total, median
+    std::vector<uint64_t> pcode(n);
+    std::vector<int32_t> pdepth(n);
+    std::vector<uint32_t> W(n);
+    std::vector<double> med64(precision == TGVX_FP64 ? n : 0);
+    std::vector<float> med32(precision == TGVX_FP64 ? 0 : n);
+    for (uint64_t g = 0; g < n; ++g) {
+        const uint32_t i = P.perm[g];
+        pcode[g] = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
+        pdepth[g] = cubes[i].depth;
+        const uint32_t* b = bins + (size_t)i * nb;
+        uint32_t tot = 0;
+        for (int k = 0; k < nb; ++k) tot += b[k];
+        W[g] = tot;
+        // median_indicator (histogram.cpp:21-31) in the arithmetic type
+        double m64 = 0.0;
+        float m32 = 0.0f;
+        if (tot) {
+            m64 = 1.0;
+            m32 = 1.0f;
+            uint32_t cum = 0;
+            for (int k = 0; k < nb; ++k) {
+                cum += b[k];
+                if (cum * 2u >= tot) {
+                    m64 = -1.0 + ((double)k + 0.5) / (0.5 * (double)nb);
+                    m32 = -1.0f + ((float)k + 0.5f) / (0.5f * (float)nb);
+                    break;
+                }
+            }
+        }
+        if (precision == TGVX_FP64) med64[g] = m64;
+        else med32[g] = m32;
+    }
+    auto up = [&](DevBuf& b, const void* src, size_t bytes) {
+        b.ensure(bytes);
+        if (bytes) TGVX_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    };
+    DevBuf d_code, d_depth, d_uc, d_ui;
+    up(d_code, pcode.data(), 8 * n);
+    up(d_depth, pdepth.data(), 4 * n);
+    up(d_uc, ucode.data(), 8 * ucode.size());
+    up(d_ui, uidx.data(), 4 * uidx.size());
+    up(P.d_W, W.data(), 4 * n);
+    if (precision == TGVX_FP64) up(P.d_med, med64.data(), 8 * n);
+    else up(P.d_med, med32.data(), 4 * n);
+    for (int f = 0; f < 6; ++f) P.d_nb[f].ensure(4 * n);
+    P.d_parent.ensure(4 * n);
+    if (n > 0) {
+        k_tables<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+            d_code.as<uint64_t>(), d_depth.as<int32_t>(), (uint32_t)n, d_uc.as<uint64_t>(), d_ui.as<int32_t>(),
+            (uint32_t)ucode.size(), P.d_nb[0].as<int32_t>(), P.d_nb[1].as<int32_t>(), P.d_nb[2].as<int32_t>(),
+            P.d_nb[3].as<int32_t>(), P.d_nb[4].as<int32_t>(), P.d_nb[5].as<int32_t>(), P.d_parent.as<int32_t>());
+        TGVX_HIP(hipGetLastError());
+    }
+    // incoming +a edges within each level, ordered by (source position, axis)
+    std::vector<int32_t> nbp[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        nbp[ax].resize(n);
+        if (n)
+            TGVX_HIP(hipMemcpyAsync(nbp[ax].data(), P.d_nb[2 * ax].p, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int32_t> in_start(n + 1, 0);
+    for (uint64_t k = 0; k < n; ++k)
+        for (int ax = 0; ax < 3; ++ax) {
+            const int32_t j = nbp[ax][k];
+            if (j >= 0 && pdepth[j] == pdepth[k]) in_start[j + 1]++;
+        }
+    for (uint64_t i = 0; i < n; ++i) in_start[i + 1] += in_start[i];
+    std::vector<int32_t> ent(in_start[n]);
+    std::vector<int32_t> fill(in_start.begin(), in_start.end() - 1);
+    for (uint64_t k = 0; k < n; ++k)
+        for (int ax = 0; ax < 3; ++ax) {
+            const int32_t j = nbp[ax][k];
+            if (j >= 0 && pdepth[j] == pdepth[k]) ent[fill[j]++] = (int32_t)(k | ((uint64_t)ax << 30));
+        }
+    P.n_in_edges = ent.size();
+    up(P.d_in_start, in_start.data(), 4 * (n + 1));
+    up(P.d_in_ent, ent.data(), 4 * std::max<size_t>(ent.size(), 1));
+    for (int i = 0; i < 17; ++i) P.d_state[i].ensure(P.elem * std::max<uint64_t>(n, 1));
+    for (int i = 0; i < 4; ++i) P.d_save[i].ensure(P.elem * std::max<uint64_t>(n, 1));
+    // position -> input index (the frozen-border mode's in-leaf test reads it)
+    up(P.d_orig, P.perm.data(), 4 * std::max<uint64_t>(n, 1));
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    P.loaded = true;
+    (void)t0;
+}
+
+void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
+                   const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_cube* cubes) {
+    Problem& P = c->prob;
+    if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
+    if (!p) throw Error(TGVX_E_ARG, "null params");
+    if (p->op_mode != P.op_mode && p->op_mode != TGVX_OP_REF && p->op_mode != TGVX_OP_ADJ)
+        throw Error(TGVX_E_ARG, "bad op_mode");
+    if (leaves && init_uv) throw Error(TGVX_E_ARG, "the frozen-border solve takes no initial field");
+    (void)cubes;
+    switch (P.precision) {
+        case TGVX_FP64: solve_t<double>(c, p, init_uv, rep, leaves, nleaves); break;
+        case TGVX_FP32: solve_t<float>(c, p, init_uv, rep, leaves, nleaves); break;
+        default: solve_t<bf16s>(c, p, init_uv, rep, leaves, nleaves); break;
+    }
+}
+
+void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out) {
+    Problem& P = c->prob;
+    if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
+    const size_t n = P.n, e = P.elem;
+    std::vector<uint8_t> col(e * n);
+    const size_t out_e = P.precision == TGVX_FP64 ? 8 : 4;
+    auto widen = [&](const uint8_t* src, uint64_t g, uint8_t* dst) {
+        if (P.precision == TGVX_BF16) {
+            uint16_t h;
+            std::memcpy(&h, src + 2 * g, 2);
+            const uint32_t u = (uint32_t)h << 16;
+            std::memcpy(dst, &u, 4);
+        } else {
+            std::memcpy(dst, src + e * g, e);
+        }
+    };
+    for (int f = 0; f < 17; ++f) {
+        const bool is_bar = f >= F_UB;
+        uint8_t* out = (uint8_t*)(is_bar ? ubar_out : state_out);
+        if (!out) continue;
+        const int rec = is_bar ? 4 : 13, slot = is_bar ? f - F_UB : f;
+        if (n) TGVX_HIP(hipMemcpy(col.data(), P.d_state[f].p, e * n, hipMemcpyDeviceToHost));
+        for (uint64_t g = 0; g < n; ++g) widen(col.data(), g, out + ((size_t)P.perm[g] * rec + slot) * out_e);
+    }
+}
+
+void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* Pv, double* Dv) {
+    Problem& P = c->prob;
+    if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
+    const Level* L = nullptr;
+    for (const Level& l : P.levels)
+        if (l.depth == depth) L = &l;
+    if (!L) throw Error(TGVX_E_ARG, "no cubes at depth " + std::to_string(depth));
+    auto go = [&](auto tag) {
+        using S = decltype(tag);
+        KArgs<S> a = make_args<S>(c, p);
+        a.off = a.lvl_off = L->off;
+        a.n = a.lvl_n = L->n;
+        eval_gap<S>(c, a, p, Pv, Dv);
+    };
+    switch (P.precision) {
+        case TGVX_FP64: go(double{}); break;
+        case TGVX_FP32: go(float{}); break;
+        default: go(bf16s{}); break;
+    }
+}
+
+void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
+                 double* primal_ms) {
+    Problem& P = c->prob;
+    if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
+    switch (P.precision) {
+        case TGVX_FP64: bench_t<double>(c, depth, p, iters, total_ms, dual_ms, primal_ms); break;
+        case TGVX_FP32: bench_t<float>(c, depth, p, iters, total_ms, dual_ms, primal_ms); break;
+        default: bench_t<bf16s>(c, depth, p, iters, total_ms, dual_ms, primal_ms); break;
+    }
+}
+
+void release_graphs(const tgvx_ctx* c) { g_graphs.erase(c); }
+
+}  // namespace tgvx

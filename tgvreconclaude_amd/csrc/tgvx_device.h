@@ -1,0 +1,100 @@
+// Device-side helpers shared by the tgvx kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tgvx {
+
+// ---------------------------------------------------------------- Morton
+// 21 bits per axis, bit b of x/y/z at bit 3b/3b+1/3b+2 of the 63-bit code
+// (reference src/types.hpp:10-11,39-73).
+__host__ __device__ inline uint64_t dilate3(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+__host__ __device__ inline uint32_t compact3(uint64_t x) {
+    x &= 0x1249249249249249ull;
+    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x ^ (x >> 8)) & 0x1f0000ff0000ffull;
+    x = (x ^ (x >> 16)) & 0x1f00000000ffffull;
+    x = (x ^ (x >> 32)) & 0x1fffffull;
+    return (uint32_t)x;
+}
+__host__ __device__ inline uint64_t morton(uint32_t x, uint32_t y, uint32_t z) {
+    return dilate3(x) | (dilate3(y) << 1) | (dilate3(z) << 2);
+}
+// Ancestor at target depth: clear the low 3*(21-d) bits (balance.cpp:22-26).
+__host__ __device__ inline uint64_t ancestor(uint64_t c, int d) {
+    const unsigned s = 3u * (21u - (unsigned)d);
+    return s >= 64u ? 0ull : (c & ~((1ull << s) - 1ull));
+}
+constexpr uint64_t kNone = ~0ull;
+// Same-depth face neighbour, faces 0:+x 1:-x 2:+y 3:-y 4:+z 5:-z; kNone
+// outside [0, 2^21) (balance.cpp:32-49).
+__host__ __device__ inline uint64_t face_nb(uint64_t c, int f, int d) {
+    uint32_t a[3] = {compact3(c), compact3(c >> 1), compact3(c >> 2)};
+    const uint32_t s = 1u << (21 - d);
+    uint32_t& x = a[f >> 1];
+    if (f & 1) {
+        if (x < s) return kNone;
+        x -= s;
+    } else {
+        if (x + s >= (1u << 21)) return kNone;
+        x += s;
+    }
+    return morton(a[0], a[1], a[2]);
+}
+
+// ---------------------------------------------------------------- storage
+// State arrays are stored as S and computed in T. BF16 storage rounds to
+// nearest-even on every store.
+struct bf16s {
+    uint16_t x;
+};
+
+template <class S>
+struct Io;
+template <>
+struct Io<float> {
+    using T = float;
+    __device__ static inline T ld(const float* p, long i) { return p[i]; }
+    __device__ static inline void st(float* p, long i, T v) { p[i] = v; }
+};
+template <>
+struct Io<double> {
+    using T = double;
+    __device__ static inline T ld(const double* p, long i) { return p[i]; }
+    __device__ static inline void st(double* p, long i, T v) { p[i] = v; }
+};
+template <>
+struct Io<bf16s> {
+    using T = float;
+    __device__ static inline T ld(const bf16s* p, long i) {
+        return __uint_as_float((uint32_t)p[i].x << 16);
+    }
+    __device__ static inline void st(bf16s* p, long i, T v) {
+        const uint32_t u = __float_as_uint(v);
+        uint16_t h;
+        if ((u & 0x7fffffffu) > 0x7f800000u) h = (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+        else h = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+        p[i].x = h;
+    }
+};
+
+// ---------------------------------------------------------------- launch
+// Workgroups are dispatched to the 8 XCDs round-robin by id; remap so that
+// each XCD gets one contiguous run of tiles (neighbouring leaves share its L2).
+__device__ inline int xcd_tile() {
+    const int nwg = (int)gridDim.x, b = (int)blockIdx.x;
+    const int per = nwg >> 3, rem = nwg & 7, x = b & 7, k = b >> 3;
+    return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
+}  // namespace tgvx

@@ -1,0 +1,110 @@
+// Host-side internals of libtgvx: context, device buffers, resident problem.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tgvx.h"
+
+namespace tgvx {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define TGVX_HIP(call)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw ::tgvx::Error(TGVX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+// Device allocation owned by the context; freed on reset / destruction.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t b) {
+        if (b <= bytes && p) return;
+        release();
+        if (b == 0) b = 16;
+        TGVX_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <class X>
+    X* as() const { return static_cast<X*>(p); }
+};
+
+struct Level {
+    int depth;
+    uint32_t off, n;  // contiguous range of permuted positions
+};
+
+// A problem resident on the device: cubes permuted level-major (ascending
+// depth; within a depth ascending input index, i.e. the reference's by_depth
+// order), neighbour tables in permuted positions, per-leaf data constants and
+// the state arrays.
+struct Problem {
+    bool loaded = false;
+    uint64_t n = 0;
+    int nb = 0, precision = TGVX_FP32, op_mode = TGVX_OP_REF;
+    size_t elem = 4;  // bytes per stored state value
+    std::vector<Level> levels;
+    std::vector<uint32_t> perm;      // position -> input index
+    std::vector<uint32_t> inv;       // input index -> position
+    DevBuf d_nb[6];                  // int32 per position, faces 0..5
+    DevBuf d_parent;                 // int32: position of the cube holding the parent code, or -1
+    DevBuf d_W;                      // uint32 vote total
+    DevBuf d_med;                    // T median indicator
+    DevBuf d_in_start, d_in_ent;     // incoming +a edges within a level (CSR; ent = k | a << 30)
+    DevBuf d_state[17];              // u vx vy vz px py pz q0..q5 ub vbx vby vbz
+    DevBuf d_save[4];                // scratch copy of u, v of one level
+    DevBuf d_init;                   // float x 4 per position (warm start field), optional
+    DevBuf d_orig;                   // uint32 position -> input index
+    DevBuf d_part;                   // gap partial sums
+    DevBuf d_pd;                     // final P, D
+    uint64_t n_in_edges = 0;
+};
+
+}  // namespace tgvx
+
+struct tgvx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    tgvx::Problem prob;
+    // scratch for histogram calls
+    tgvx::DevBuf h_cubes, h_bins, h_pyr, h_meta;
+};
+
+namespace tgvx {
+// solve.hip
+void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb, int precision,
+                  int op_mode);
+void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
+                   const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_cube* cubes);
+void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out);
+void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* P, double* D);
+void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
+                 double* primal_ms);
+// hist.hip
+void histograms(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views, int nv,
+                const float origin[3], float r_root, int nb, uint32_t* bins_out);
+void mip_build(tgvx_ctx* c, const tgvx_view* v, float* out, int32_t* dims, int32_t* nlevels);
+void median_init(tgvx_ctx* c, const uint32_t* bins, uint64_t n, int nb, float* out);
+}  // namespace tgvx
