@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the global-TGV hot path on one MI355X.
+
+Workload: the depth-10 scene -- torus (R=1, r=0.3) union a box rotated 30
+degrees, 96 seeded views of 1280x960 on the radius-3 sphere, depth noise
+sigma 0.003, 3% outliers, octree clamped at depth 10 (about 6 M leaves), 10
+histogram bins, alpha1 = 1, alpha0 = 2, lambda = 1, FP32 state.
+
+Step: one coarse-to-fine sweep over every level of that octree with 50
+primal-dual iterations per level (the hot path's unit of work: reset, parent
+initialisation, 50 iterations and one FP64 gap evaluation per level), inputs
+resident in HBM. value = leaf-iterations / s over the K timed steps.
+
+Also reported: the roofline of the iteration at the finest level (measured
+live with HIP events on the library's stream), the time to relative duality
+gap <= 1e-4 of a cold solve, and the CPU restatement of the reference's
+run_level timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per leaf-iteration: 17 state values read + 17 written,
+# (W, m) and 6 neighbour indices
+ALG_BYTES = {"fp32": 34 * 4 + 8 + 24, "bf16": 34 * 2 + 8 + 24, "fp64": 34 * 8 + 12 + 24}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp32", "bf16", "fp64"], default="fp32")
+    ap.add_argument("--views", type=int, default=96)
+    ap.add_argument("--width", type=int, default=1280)
+    ap.add_argument("--height", type=int, default=960)
+    ap.add_argument("--max-depth", type=int, default=10)
+    ap.add_argument("--iters", type=int, default=50, help="iterations per level in one step")
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--gap-tol", type=float, default=1e-4)
+    ap.add_argument("--no-gap-run", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        nbytes = args.calibrate_copy << 20
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+        assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+        assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+        for _ in range(3):
+            assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+        hip.hipDeviceSynchronize()
+        hip.hipFree(a)
+        hip.hipFree(b)
+        print(json.dumps({"calibrate_copy_bytes": nbytes}))
+        return
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the solver "
+              "shards across GPUs only by treetop leaves (not run here). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import tgvreconclaude_amd as T
+    from tgvreconclaude_amd import scene
+
+    prec = {"fp32": T.FP32, "bf16": T.BF16, "fp64": T.FP64}[args.precision]
+    t0 = time.time()
+    log(f"[bench] generating scene: {args.views} views {args.width}x{args.height}, depth <= {args.max_depth}")
+    sc = scene.torus_box_scene(views=args.views, W=args.width, H=args.height, max_depth=args.max_depth)
+    cubes = sc.cubes
+    n = len(cubes)
+    t_scene = time.time() - t0
+    log(f"[bench] scene: {n} leaves in {t_scene:.1f} s")
+
+    ctx = T.Context(0)
+    arch, cus = ctx.device_info()
+    t1 = time.time()
+    hist = ctx.histograms(cubes, sc.views(), sc.origin, sc.r_root, 10)
+    t_hist = time.time() - t1
+    log(f"[bench] histograms on the GPU: {t_hist * 1e3:.1f} ms (incl. upload of {sc.depths.nbytes / 1e6:.0f} MB)")
+
+    t1 = time.time()
+    ctx.load(cubes, hist, precision=prec, op_mode=T.OP_ADJ)
+    t_load = time.time() - t1
+    levels = ctx.levels()
+    log(f"[bench] problem resident in {t_load:.2f} s; levels {levels}")
+
+    prm = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, iters=args.iters,
+                      use_graph=0 if args.no_graph else 1, block=args.block)
+    for i in range(args.warmup):
+        ctx.run(prm)
+    log(f"[bench] {args.warmup} warmup steps done")
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
+    ts = time.perf_counter()
+    leaf_iters = 0.0
+    for i in range(args.steps):
+        rep = ctx.run(prm)
+        leaf_iters += rep.leaf_iters
+    hip.hipDeviceSynchronize()
+    dt = time.perf_counter() - ts
+    value = leaf_iters / dt
+    log(f"[bench] {args.steps} steps in {dt:.3f} s: {value:.4g} leaf-iter/s")
+
+    # roofline of the iteration at the finest level: HIP events on the library's stream
+    finest, n_f = levels[-1]
+    total_ms, dual_ms, primal_ms = ctx.level_bench(finest, prm, 100)
+    it_ms = dual_ms + primal_ms
+    ab = ALG_BYTES[args.precision]
+    achieved = ab * n_f / (it_ms * 1e-3) / 1e9
+    log(f"[bench] depth {finest} ({n_f} leaves): dual {dual_ms * 1e3:.1f} us, primal {primal_ms * 1e3:.1f} us, "
+        f"100 its {total_ms:.2f} ms -> {achieved:.0f} GB/s algorithmic")
+
+    # time to relative gap <= tol, cold start (exact-adjoint mode)
+    gap = None
+    if not args.no_gap_run:
+        pg = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, gap_tol=args.gap_tol,
+                         chunk=50, max_iters=20000, use_graph=prm.use_graph, block=args.block)
+        rg = ctx.run(pg)
+        lv = rg.levels()
+        gap = {"tol": args.gap_tol, "wall_ms": round(rg.solve_ms, 2), "iterate_ms": round(rg.iterate_ms, 2),
+               "gap_eval_ms": round(rg.gap_ms, 2), "leaf_iters": rg.leaf_iters,
+               "iters_per_level": [l["iters"] for l in lv], "final_rel_gap": [l["rel_gap"] for l in lv],
+               "reached": all(l["rel_gap"] <= args.gap_tol for l in lv)}
+        log(f"[bench] cold solve to gap {args.gap_tol}: {rg.solve_ms:.1f} ms, iters {gap['iters_per_level']}")
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        op = O.params(alpha0=2.0, alpha1=1.0, op_mode=1, threads=1)
+        t_one, na = O.time_level(cubes, hist, op, finest, 1)
+        k = max(1, int(args.cpu_seconds / max(t_one, 1e-3)))
+        t_k, na = O.time_level(cubes, hist, op, finest, k)
+        cpu = {"value": na * k / t_k, "unit": "leaf-iter/s", "cores": 1, "kind": "port",
+               "sample": f"{k} iterations of the depth-{finest} level ({na} leaves) of this workload, "
+                         f"FP32 restatement of run_level (tgv.cpp:137-232), 1 thread, {t_k:.1f} s"}
+        log(f"[bench] CPU baseline: {cpu['value']:.4g} leaf-iter/s ({k} its, {t_k:.1f} s)")
+
+    out = {
+        "metric": "TGV leaf-iter/s + time-to-gap<=1e-4 (learned warm start vs cold), 1 GPU",
+        "value": value,
+        "unit": "leaf-iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "bf16": "bf16-storage/f32", "fp64": "f64"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": "torus_box_depth10", "leaves": n, "views": args.views,
+                   "image": [args.width, args.height], "max_depth": args.max_depth, "bins": 10,
+                   "iters_per_level_per_step": args.iters, "op_mode": "adjoint", "levels": levels,
+                   "parallelism": "single-gpu", "device": arch, "cus": cus},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_dual + k_primal_adj, depth {finest}, {n_f} leaves",
+                     "alg_bytes_per_leaf_iter": ab, "dual_us": dual_ms * 1e3, "primal_us": primal_ms * 1e3},
+        "time_to_gap": {"cold": gap, "warm": None},
+        "cpu_baseline": cpu,
+        "setup_s": {"scene": round(t_scene, 2), "histograms": round(t_hist, 3), "load": round(t_load, 3)},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

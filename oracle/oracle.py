@@ -64,6 +64,8 @@ def lib():
         L.orc_extract_surface.restype = u64
         L.orc_free.argtypes = [vp]
         L.orc_free.restype = None
+        L.orc_time_level.argtypes = [vp, u64, vp, i32, C.POINTER(OParams), i32, i32, C.POINTER(u64)]
+        L.orc_time_level.restype = C.c_double
         L.orc_log2_level.argtypes = [C.c_float]
         L.orc_log2_rule_mismatches.argtypes = [C.c_float, C.c_float, vp, i32]
         L.orc_log2_rule_mismatches.restype = u64
@@ -229,3 +231,12 @@ def extract_surface(cubes, u, origin, r_root, has_data=None):
 def log2_rule_mismatches(lo, hi, thr):
     t = np.ascontiguousarray(thr, dtype=np.float32)
     return lib().orc_log2_rule_mismatches(lo, hi, _p(t), len(t))
+
+
+def time_level(cubes, hists, prm, depth, iters):
+    """Seconds for `iters` iterations of one level of the CPU restatement, and its cube count."""
+    cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+    h = np.ascontiguousarray(hists, dtype=np.uint32)
+    na = C.c_uint64()
+    t = lib().orc_time_level(_p(cubes), len(cubes), _p(h), h.shape[1], C.byref(prm), depth, iters, C.byref(na))
+    return t, na.value

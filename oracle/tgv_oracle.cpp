@@ -24,6 +24,7 @@
 // Functions with no reference counterpart (energy / duality gap, the exact-adjoint
 // operator mode, the MLP) are marked as such where they are defined.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1035,6 +1036,28 @@ uint64_t orc_log2_rule_mismatches(float lo, float hi, const float* thr, int32_t 
         if (rule != (int)std::log2(x)) ++bad;
     }
     return bad;
+}
+
+// CPU baseline: times `iters` iterations of the level at `depth` of the
+// reference restatement (AoS state, active[] indirection, five passes, the
+// median recomputed inside prox_data) with `threads` OpenMP threads.
+// Level set-up (code map, neighbour table) is excluded. Returns seconds and
+// the number of active cubes.
+double orc_time_level(const orc::Cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb, const orc_params* p,
+                      int32_t depth, int32_t iters, uint64_t* n_active) {
+    orc::Solver<float> s{cubes, n, bins, nb, to_params(p)};
+    s.setup();
+    orc::Level lv;
+    if (depth <= s.max_d) lv.active = s.by_depth[depth];
+    *n_active = lv.active.size();
+    if (lv.active.empty()) return 0.0;
+    orc::build_level(lv, cubes, s.cm, n);
+    const size_t na = lv.active.size();
+    std::vector<float> un(na), vn[3];
+    for (int c = 0; c < 3; ++c) vn[c].resize(na);
+    const auto t0 = std::chrono::steady_clock::now();
+    s.iterate(lv, iters, un, vn, nullptr, nullptr, 0, 0);
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 int32_t orc_sizeof_cube() { return (int32_t)sizeof(orc::Cube); }

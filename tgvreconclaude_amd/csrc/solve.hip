@@ -674,6 +674,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                 TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
                 run_iters<S>(c, a, p, k, false);
                 TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+                TGVX_HIP(hipEventSynchronize(c->ev[1]));
                 done += k;
                 g0 = now_ms();
                 eval_gap<S>(c, a, p, &Pv, &Dv);
