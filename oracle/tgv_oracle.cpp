@@ -323,7 +323,7 @@ static inline int32_t nb_ctf(const CodeMap& cm, uint64_t code, int face, int dep
 
 // Incoming-edge lists for the exact-adjoint operator and the gap (no reference
 // counterpart): for active position ai, every (active position k, axis a) with
-// nb_{+a}(active[k]) == active[ai].
+// nb_{+a}(active[k]) == active[ai], ordered by (a, k).
 struct Incoming {
     std::vector<uint32_t> start;               // size na+1
     std::vector<std::pair<uint32_t, int>> ent;  // (k, a)
@@ -355,8 +355,8 @@ static void build_level(Level& L, const Cube* cubes, const CodeMap& cm, size_t N
     L.in.start = cnt;
     L.in.ent.resize(cnt[na]);
     std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
-    for (size_t k = 0; k < na; ++k)
-        for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < 3; ++a)
+        for (size_t k = 0; k < na; ++k) {
             const int32_t j = L.nb[k * 6 + 2 * a];
             if (j >= 0 && L.pos[j] >= 0) L.in.ent[fill[L.pos[j]]++] = {(uint32_t)k, a};
         }

@@ -34,8 +34,9 @@ struct KArgs {
     using T = typename Io<S>::T;
     S* s[17];
     const int32_t* nb[6];
-    const uint32_t* W;
-    const T* med;
+    const int32_t* nbt[3];    // incoming +a edge of each axis (exact-adjoint mode), or -1
+    const uint32_t* info;     // packed per-cube word, see InfoWord
+    int32_t nbins;
     const int32_t* parent;
     const int32_t* in_start;
     const int32_t* in_ent;
@@ -69,11 +70,33 @@ __device__ inline void clip6(T* q, T r) {
         for (int i = 0; i < 6; ++i) q[i] *= s;
     }
 }
-// prox_data (tgv.cpp:75-94) with the median hoisted out of the iteration
-// (it depends only on the histogram): shrink towards med by tau*lambda*W.
+// Per-cube constant word: the vote total W (bits 0-22), "more than one
+// incoming edge on some axis" (bit 23), the weighted-median bin k (bits
+// 24-28) and "the +x / +y / +z neighbour exists" (bits 29-31).
+struct InfoWord {
+    static constexpr uint32_t kWMax = (1u << 23) - 1;
+    __host__ __device__ static uint32_t W(uint32_t w) { return w & kWMax; }
+    __host__ __device__ static bool multi(uint32_t w) { return (w >> 23) & 1u; }
+    __host__ __device__ static int bin(uint32_t w) { return (int)((w >> 24) & 31u); }
+    __host__ __device__ static bool has(uint32_t w, int ax) { return (w >> (29 + ax)) & 1u; }
+};
+
+// median_indicator (histogram.cpp:21-31) with nb bins, from the median bin:
+// the centre -1 + (k + 0.5) / (nb / 2) of bin k, 0 for an empty histogram.
 template <class T>
-__device__ inline T prox(T z, uint32_t W, T med, T tl) {
+__host__ __device__ inline T median_of(uint32_t w, int nb) {
+    if (InfoWord::W(w) == 0u) return T(0);
+    return T(-1) + (T(InfoWord::bin(w)) + T(0.5)) / (T(0.5) * T(nb));
+}
+
+// prox_data (tgv.cpp:75-94): shrink towards the weighted median by
+// tau*lambda*W; the median depends only on the histogram, so it comes from
+// the packed word instead of a scan of the bins every iteration.
+template <class T>
+__device__ inline T prox(T z, uint32_t w, int nb, T tl) {
+    const uint32_t W = InfoWord::W(w);
     if (W == 0u) return z;
+    const T med = median_of<T>(w, nb);
     const T diff = z - med;
     const T shrink = tl * (T)W;
     if (diff > shrink) return z - shrink;
@@ -143,7 +166,7 @@ __global__ void __launch_bounds__(256) k_primal_ref(KArgs<S> a) {
     const T px = IO::ld(a.s[F_PX], g), py = IO::ld(a.s[F_PY], g), pz = IO::ld(a.s[F_PZ], g);
     const T div_p = (px - ld(F_PX, m0)) + (py - ld(F_PY, m2)) + (pz - ld(F_PZ, m4));
     const T u = IO::ld(a.s[F_U], g);
-    const T un = prox(u + a.tau * div_p, a.W[g], a.med[g], a.tl);
+    const T un = prox(u + a.tau * div_p, a.info[g], a.nbins, a.tl);
 
     const T q0 = IO::ld(a.s[F_Q0], g), q1 = IO::ld(a.s[F_Q1], g), q2 = IO::ld(a.s[F_Q2], g),
             q3 = IO::ld(a.s[F_Q3], g), q4 = IO::ld(a.s[F_Q4], g), q5 = IO::ld(a.s[F_Q5], g);
@@ -186,33 +209,58 @@ __global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a) {
     const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
+    const uint32_t w = a.info[g];
     T p[3] = {IO::ld(a.s[F_PX], g), IO::ld(a.s[F_PY], g), IO::ld(a.s[F_PZ], g)};
     T dp = T(0);
     T dq[3] = {T(0), T(0), T(0)};
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        if (a.nb[2 * ax][g] < 0) continue;
+        if (!InfoWord::has(w, ax)) continue;
         dp += p[ax];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const T w = (c == ax) ? T(1) : T(0.5);
-            dq[c] += w * IO::ld(a.s[qidx(c, ax)], g);
+            const T w2 = (c == ax) ? T(1) : T(0.5);
+            dq[c] += w2 * IO::ld(a.s[qidx(c, ax)], g);
         }
     }
-    const int e0 = a.in_start[g], e1 = a.in_start[g + 1];
-    for (int e = e0; e < e1; ++e) {
-        const int ent = a.in_ent[e];
-        const long k = ent & 0x3fffffff;
-        const int ax = (int)((uint32_t)ent >> 30);
+    // incoming edges in (axis, source) order
+    auto incoming = [&](long k, int ax) {
         dp -= IO::ld(field3(a, F_PX, F_PY, F_PZ, ax), k);
-        // (c, ax) -> q: c = 0: q0 q3 q4; c = 1: q3 q1 q5; c = 2: q4 q5 q2
         const T w0 = ax == 0 ? T(1) : T(0.5), w1 = ax == 1 ? T(1) : T(0.5), w2 = ax == 2 ? T(1) : T(0.5);
         dq[0] -= w0 * IO::ld(field3(a, F_Q0, F_Q3, F_Q4, ax), k);
         dq[1] -= w1 * IO::ld(field3(a, F_Q3, F_Q1, F_Q5, ax), k);
         dq[2] -= w2 * IO::ld(field3(a, F_Q4, F_Q5, F_Q2, ax), k);
+    };
+    if (!InfoWord::multi(w)) {
+        // at most one incoming edge per axis: three direct slots
+        const int k0 = a.nbt[0][g], k1 = a.nbt[1][g], k2 = a.nbt[2][g];
+        if (k0 >= 0) {
+            dp -= IO::ld(a.s[F_PX], k0);
+            dq[0] -= T(1) * IO::ld(a.s[F_Q0], k0);
+            dq[1] -= T(0.5) * IO::ld(a.s[F_Q3], k0);
+            dq[2] -= T(0.5) * IO::ld(a.s[F_Q4], k0);
+        }
+        if (k1 >= 0) {
+            dp -= IO::ld(a.s[F_PY], k1);
+            dq[0] -= T(0.5) * IO::ld(a.s[F_Q3], k1);
+            dq[1] -= T(1) * IO::ld(a.s[F_Q1], k1);
+            dq[2] -= T(0.5) * IO::ld(a.s[F_Q5], k1);
+        }
+        if (k2 >= 0) {
+            dp -= IO::ld(a.s[F_PZ], k2);
+            dq[0] -= T(0.5) * IO::ld(a.s[F_Q4], k2);
+            dq[1] -= T(0.5) * IO::ld(a.s[F_Q5], k2);
+            dq[2] -= T(1) * IO::ld(a.s[F_Q2], k2);
+        }
+    } else {
+        const int e0 = a.in_start[g], e1 = a.in_start[g + 1];
+        for (int e = e0; e < e1; ++e) {
+            const int ent = a.in_ent[e];
+            incoming(ent & 0x3fffffff, (int)((uint32_t)ent >> 30));
+        }
     }
     const T u = IO::ld(a.s[F_U], g);
-    const T un = prox(u + a.tau * dp, a.W[g], a.med[g], a.tl);
+    const T un = prox(u + a.tau * dp, w, a.nbins, a.tl);
     T vn[3], vo[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -220,8 +268,10 @@ __global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a) {
         vn[c] = vo[c] + a.tau * (p[c] + dq[c]);
     }
     IO::st(a.s[F_UB], g, T(2) * un - u);
+#pragma unroll
     for (int c = 0; c < 3; ++c) IO::st(a.s[F_VBX + c], g, T(2) * vn[c] - vo[c]);
     IO::st(a.s[F_U], g, un);
+#pragma unroll
     for (int c = 0; c < 3; ++c) IO::st(a.s[F_VX + c], g, vn[c]);
 }
 
@@ -233,7 +283,7 @@ __global__ void k_reset(KArgs<S> a, uint32_t n) {
     using T = typename IO::T;
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    const T m = a.med[g];
+    const T m = median_of<T>(a.info[g], a.nbins);
     IO::st(a.s[F_U], g, m);
     IO::st(a.s[F_UB], g, m);
 #pragma unroll
@@ -311,6 +361,7 @@ template <class S>
 __global__ void __launch_bounds__(kGapBlock) k_gap(KArgs<S> a, double lam, double a0, double a1, double vbox,
                                                     double* part) {
     using IO = Io<S>;
+    using T = typename IO::T;
     __shared__ double sp[kGapBlock], sd[kGapBlock];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     double P = 0.0, D = 0.0;
@@ -318,8 +369,8 @@ __global__ void __launch_bounds__(kGapBlock) k_gap(KArgs<S> a, double lam, doubl
         const long g = (long)a.off + t;
         auto in_lvl = [&](int j) { return (uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n; };
         auto L = [&](int f, long j) -> double { return (double)IO::ld(a.s[f], j); };
-        const uint32_t Wc = a.W[g];
-        const double W = (double)Wc, m = (double)a.med[g], ui = L(F_U, g);
+        const uint32_t Wc = InfoWord::W(a.info[g]);
+        const double W = (double)Wc, m = (double)median_of<T>(a.info[g], a.nbins), ui = L(F_U, g);
         const double v[3] = {L(F_VX, g), L(F_VY, g), L(F_VZ, g)};
         const double p[3] = {L(F_PX, g), L(F_PY, g), L(F_PZ, g)};
         double gr[3], dv[3][3];
@@ -477,8 +528,9 @@ KArgs<S> make_args(tgvx_ctx* c, const tgvx_params* prm) {
     std::memset(&a, 0, sizeof(a));
     for (int i = 0; i < 17; ++i) a.s[i] = P.d_state[i].as<S>();
     for (int f = 0; f < 6; ++f) a.nb[f] = P.d_nb[f].as<int32_t>();
-    a.W = P.d_W.as<uint32_t>();
-    a.med = P.d_med.as<T>();
+    for (int ax = 0; ax < 3; ++ax) a.nbt[ax] = P.d_nbt[ax].as<int32_t>();
+    a.info = P.d_info.as<uint32_t>();
+    a.nbins = P.nb;
     a.parent = P.d_parent.as<int32_t>();
     a.in_start = P.d_in_start.as<int32_t>();
     a.in_ent = P.d_in_ent.as<int32_t>();
@@ -805,38 +857,28 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
         uidx.push_back((int32_t)P.inv[kv[i].second]);
     }
     std::vector<std::pair<uint64_t, uint32_t>>().swap(kv);
-    // per-position code, depth, vote total, median
+    // per-position code, depth, vote total and median bin
     std::vector<uint64_t> pcode(n);
     std::vector<int32_t> pdepth(n);
-    std::vector<uint32_t> W(n);
-    std::vector<double> med64(precision == TGVX_FP64 ? n : 0);
-    std::vector<float> med32(precision == TGVX_FP64 ? 0 : n);
+    std::vector<uint32_t> info(n);
     for (uint64_t g = 0; g < n; ++g) {
         const uint32_t i = P.perm[g];
         pcode[g] = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
         pdepth[g] = cubes[i].depth;
         const uint32_t* b = bins + (size_t)i * nb;
-        uint32_t tot = 0;
+        uint64_t tot = 0;
         for (int k = 0; k < nb; ++k) tot += b[k];
-        W[g] = tot;
-        // median_indicator (histogram.cpp:21-31) in the arithmetic type
-        double m64 = 0.0;
-        float m32 = 0.0f;
-        if (tot) {
-            m64 = 1.0;
-            m32 = 1.0f;
-            uint32_t cum = 0;
-            for (int k = 0; k < nb; ++k) {
-                cum += b[k];
-                if (cum * 2u >= tot) {
-                    m64 = -1.0 + ((double)k + 0.5) / (0.5 * (double)nb);
-                    m32 = -1.0f + ((float)k + 0.5f) / (0.5f * (float)nb);
-                    break;
-                }
+        if (tot > InfoWord::kWMax) throw Error(TGVX_E_ARG, "vote total of a cube exceeds 2^23 - 1");
+        // median bin (histogram.cpp:21-31): first k with 2 * cumulative >= total
+        uint32_t kmed = 0, cum = 0;
+        for (int k = 0; k < nb; ++k) {
+            cum += b[k];
+            if (cum * 2u >= (uint32_t)tot) {
+                kmed = (uint32_t)k;
+                break;
             }
         }
-        if (precision == TGVX_FP64) med64[g] = m64;
-        else med32[g] = m32;
+        info[g] = (uint32_t)tot | (kmed << 24);
     }
     auto up = [&](DevBuf& b, const void* src, size_t bytes) {
         b.ensure(bytes);
@@ -847,11 +889,8 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     up(d_depth, pdepth.data(), 4 * n);
     up(d_uc, ucode.data(), 8 * ucode.size());
     up(d_ui, uidx.data(), 4 * uidx.size());
-    up(P.d_W, W.data(), 4 * n);
-    if (precision == TGVX_FP64) up(P.d_med, med64.data(), 8 * n);
-    else up(P.d_med, med32.data(), 4 * n);
-    for (int f = 0; f < 6; ++f) P.d_nb[f].ensure(4 * n);
-    P.d_parent.ensure(4 * n);
+    for (int f = 0; f < 6; ++f) P.d_nb[f].ensure(4 * std::max<uint64_t>(n, 1));
+    P.d_parent.ensure(4 * std::max<uint64_t>(n, 1));
     if (n > 0) {
         k_tables<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
             d_code.as<uint64_t>(), d_depth.as<int32_t>(), (uint32_t)n, d_uc.as<uint64_t>(), d_ui.as<int32_t>(),
@@ -859,7 +898,10 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
             P.d_nb[3].as<int32_t>(), P.d_nb[4].as<int32_t>(), P.d_nb[5].as<int32_t>(), P.d_parent.as<int32_t>());
         TGVX_HIP(hipGetLastError());
     }
-    // incoming +a edges within each level, ordered by (source position, axis)
+    // Incoming +a edges within each level: for cube j, every (k, a) with
+    // nb_{+a}(k) == j and k on j's level, ordered by (a, k). Axis a's first
+    // source also goes to slot nbt[a][j]; cubes with two sources on an axis
+    // are flagged in the info word and read the full list.
     std::vector<int32_t> nbp[3];
     for (int ax = 0; ax < 3; ++ax) {
         nbp[ax].resize(n);
@@ -868,20 +910,32 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     }
     TGVX_HIP(hipStreamSynchronize(c->stream));
     std::vector<int32_t> in_start(n + 1, 0);
-    for (uint64_t k = 0; k < n; ++k)
-        for (int ax = 0; ax < 3; ++ax) {
+    std::vector<int32_t> slot[3];
+    std::vector<uint8_t> cnt_ax(3 * n, 0);
+    for (int ax = 0; ax < 3; ++ax) slot[ax].assign(n, -1);
+    for (int ax = 0; ax < 3; ++ax)
+        for (uint64_t k = 0; k < n; ++k) {
             const int32_t j = nbp[ax][k];
-            if (j >= 0 && pdepth[j] == pdepth[k]) in_start[j + 1]++;
+            if (j >= 0) info[k] |= 1u << (29 + ax);
+            if (j >= 0 && pdepth[j] == pdepth[k]) {
+                in_start[j + 1]++;
+                if (slot[ax][j] < 0) slot[ax][j] = (int32_t)k;
+                if (cnt_ax[3 * j + ax] < 2) cnt_ax[3 * j + ax]++;
+            }
         }
+    for (uint64_t j = 0; j < n; ++j)
+        if (cnt_ax[3 * j] > 1 || cnt_ax[3 * j + 1] > 1 || cnt_ax[3 * j + 2] > 1) info[j] |= 1u << 23;
     for (uint64_t i = 0; i < n; ++i) in_start[i + 1] += in_start[i];
     std::vector<int32_t> ent(in_start[n]);
     std::vector<int32_t> fill(in_start.begin(), in_start.end() - 1);
-    for (uint64_t k = 0; k < n; ++k)
-        for (int ax = 0; ax < 3; ++ax) {
+    for (int ax = 0; ax < 3; ++ax)
+        for (uint64_t k = 0; k < n; ++k) {
             const int32_t j = nbp[ax][k];
             if (j >= 0 && pdepth[j] == pdepth[k]) ent[fill[j]++] = (int32_t)(k | ((uint64_t)ax << 30));
         }
     P.n_in_edges = ent.size();
+    up(P.d_info, info.data(), 4 * std::max<uint64_t>(n, 1));
+    for (int ax = 0; ax < 3; ++ax) up(P.d_nbt[ax], slot[ax].data(), 4 * std::max<uint64_t>(n, 1));
     up(P.d_in_start, in_start.data(), 4 * (n + 1));
     up(P.d_in_ent, ent.data(), 4 * std::max<size_t>(ent.size(), 1));
     for (int i = 0; i < 17; ++i) P.d_state[i].ensure(P.elem * std::max<uint64_t>(n, 1));

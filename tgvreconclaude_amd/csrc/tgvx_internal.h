@@ -68,8 +68,8 @@ struct Problem {
     std::vector<uint32_t> inv;       // input index -> position
     DevBuf d_nb[6];                  // int32 per position, faces 0..5
     DevBuf d_parent;                 // int32: position of the cube holding the parent code, or -1
-    DevBuf d_W;                      // uint32 vote total
-    DevBuf d_med;                    // T median indicator
+    DevBuf d_info;                   // uint32 packed vote total / median bin / flags
+    DevBuf d_nbt[3];                 // int32 first incoming +a edge per axis, or -1
     DevBuf d_in_start, d_in_ent;     // incoming +a edges within a level (CSR; ent = k | a << 30)
     DevBuf d_state[17];              // u vx vy vz px py pz q0..q5 ub vbx vby vbz
     DevBuf d_save[4];                // scratch copy of u, v of one level
