@@ -1,0 +1,195 @@
+// The reference's hot-path tests (tests/test_stage5.cpp, tests/test_stage6.cpp
+// of the reference), written against the same C++ interface, running on the
+// GPU through libtgvx. Exit status 0 when every check passes.
+#include <cmath>
+#include <cstdio>
+#include <filesystem>
+#include <string>
+#include <vector>
+
+#include "histogram.hpp"
+#include "tgv.hpp"
+
+static int n_pass = 0, n_fail = 0;
+#define EXPECT(c)                                                      \
+    do {                                                               \
+        if (c) ++n_pass;                                               \
+        else {                                                         \
+            ++n_fail;                                                  \
+            std::printf("FAIL %s:%d  %s\n", __FILE__, __LINE__, #c);  \
+        }                                                              \
+    } while (0)
+#define EXPECT_NEAR(a, b, e) EXPECT(std::fabs((float)(a) - (float)(b)) <= (float)(e))
+
+static DepthMap plane(int w, int h, float depth, float f, float c) {
+    DepthMap dm;
+    dm.width = w;
+    dm.height = h;
+    dm.depths.assign((size_t)w * h, depth);
+    dm.K = Matrix3f::Identity();
+    dm.K(0, 0) = f;
+    dm.K(1, 1) = f;
+    dm.K(0, 2) = c;
+    dm.K(1, 2) = c;
+    dm.Rt = Matrix4f::Identity();
+    return dm;
+}
+
+// test_stage5.cpp:21-41
+static void mip_levels_of_constant_map() {
+    DepthMipmap mip;
+    mip.build(plane(8, 8, 10.0f, 100, 4), false);
+    EXPECT(mip.levels.size() >= 4u);
+    EXPECT(mip.levels[0].width == 8 && mip.levels[1].width == 4 && mip.levels[2].width == 2);
+    EXPECT(mip.depth_at(0, 3, 3) == 10.0f);
+    EXPECT(mip.depth_at(1, 2, 2) == 10.0f);
+    EXPECT(mip.depth_at(2, 1, 1) == 10.0f);
+}
+
+// test_stage5.cpp:45-80
+static void projection_cases() {
+    DepthMipmap mip;
+    mip.build(plane(32, 32, 5.0f, 100, 16), false);
+    int l, u, v;
+    float z;
+    EXPECT(mip.project(Vec3f{0, 0, 5}, 0.04f, l, u, v, z));
+    EXPECT_NEAR(z, 5.0f, 1e-4f);
+    EXPECT(l == 0);
+    EXPECT(!mip.project(Vec3f{0, 0, -1}, 0.1f, l, u, v, z));
+    EXPECT(!mip.project(Vec3f{1000, 0, 5}, 0.1f, l, u, v, z));
+    EXPECT(mip.project(Vec3f{0, 0, 1}, 2.0f, l, u, v, z));
+    EXPECT(l >= 1);
+}
+
+// test_stage5.cpp:84-101
+static void histogram_statistics() {
+    CubeHistogram h;
+    h.bins[4] = 10;
+    EXPECT(h.median_indicator() == 0.125f);
+    EXPECT_NEAR(h.mean_indicator(), 0.125f, 1e-6f);
+    CubeHistogram h2;
+    h2.bins[3] = 5;
+    h2.bins[4] = 5;
+    EXPECT(h2.median_indicator() == -0.125f);
+    CubeHistogram e;
+    EXPECT(e.median_indicator() == 0.0f && e.mean_indicator() == 0.0f);
+}
+
+// test_stage5.cpp:105-156: one cube just behind a plane, one vote in bin 3
+static void one_cube_one_view() {
+    DepthMipmap mip;
+    mip.build(plane(16, 16, 5.0f, 50, 8), false);
+    OctreeCube c;
+    c.depth = 4;
+    c.r_c = 1.0f / 16;
+    c.code = morton_encode(4u << 17, 4u << 17, 8u << 17);
+    auto hists = compute_histograms({c}, {mip}, Vec3f{-0.5f, -0.5f, 4.0f}, 1.0f);
+    EXPECT(hists.size() == 1u);
+    EXPECT(hists[0].total() == 1u);
+    EXPECT(hists[0].bins[3] == 1u);
+    const std::string path = std::filesystem::temp_directory_path().string() + "/tgvx_dropin_hist.h";
+    save_histograms(path, hists);
+    auto back = load_histograms(path);
+    EXPECT(back.size() == 1u && back[0].bins[3] == 1u);
+    std::filesystem::remove(path);
+}
+
+static void chain(std::vector<OctreeCube>& cubes, std::vector<CubeHistogram>& hists, float r_root) {
+    cubes.clear();
+    for (uint32_t ix = 0; ix < 3; ++ix) {
+        OctreeCube c;
+        c.depth = 5;
+        c.r_c = r_root / 32;
+        c.code = morton_encode(ix << 16, 0u, 0u);
+        cubes.push_back(c);
+    }
+    hists.assign(3, CubeHistogram{});
+    hists[0].bins[6] = 50;
+    hists[2].bins[1] = 50;
+}
+
+// test_stage6.cpp:25-66
+static void smoothing_chain() {
+    std::vector<OctreeCube> cubes;
+    std::vector<CubeHistogram> hists;
+    chain(cubes, hists, 32.0f);
+    TGVParams p;
+    p.iters = 50;
+    p.alpha0 = 0.1f;
+    p.alpha1 = 0.2f;
+    auto s = tgv_minimize(cubes, hists, Vec3f::Zero(), 32.0f, p);
+    EXPECT(s.size() == 3u);
+    EXPECT(s[0].u > 0.0f && s[2].u < 0.0f);
+    EXPECT(s[1].u > s[2].u && s[1].u < s[0].u);
+}
+
+// test_stage6.cpp:68-94 (exact: a constant histogram field is a fixed point)
+static void constant_field() {
+    std::vector<OctreeCube> cubes;
+    std::vector<CubeHistogram> hists;
+    for (uint32_t ix = 0; ix < 4; ++ix) {
+        OctreeCube c;
+        c.depth = 4;
+        c.r_c = 1.0f;
+        c.code = morton_encode(ix << 17, 0u, 0u);
+        cubes.push_back(c);
+        CubeHistogram h;
+        h.bins[6] = 10;
+        hists.push_back(h);
+    }
+    TGVParams p;
+    p.iters = 20;
+    for (auto& s : tgv_minimize(cubes, hists, Vec3f::Zero(), 16.0f, p)) EXPECT(s.u == 0.625f);
+}
+
+// test_stage6.cpp:96-109
+static void state_file_roundtrip() {
+    std::vector<TGVCube> st(3);
+    st[0].u = 1.0f;
+    st[0].v[0] = 0.1f;
+    st[2].u = -1.0f;
+    st[2].p[2] = 0.5f;
+    const std::string path = std::filesystem::temp_directory_path().string() + "/tgvx_dropin_state.s";
+    save_tgv(path, st);
+    auto b = load_tgv(path);
+    EXPECT(b.size() == 3u && b[0].u == 1.0f && b[2].p[2] == 0.5f);
+    std::filesystem::remove(path);
+}
+
+// test_stage6.cpp:133-184
+static void out_of_core() {
+    std::vector<OctreeCube> cubes;
+    std::vector<CubeHistogram> hists;
+    chain(cubes, hists, 32.0f);
+    TGVParams p;
+    p.iters = 50;
+    p.alpha0 = 0.1f;
+    p.alpha1 = 0.2f;
+    TreetopLeaf all{cubes[0].code, 5, 0, 2};
+    auto a = tgv_minimize(cubes, hists, Vec3f::Zero(), 32.0f, p);
+    auto b = tgv_minimize_oc(cubes, hists, {all}, Vec3f::Zero(), 32.0f, p);
+    for (int i = 0; i < 3; ++i) EXPECT(a[i].u == b[i].u);
+    TreetopLeaf l0{cubes[0].code, 5, 0, 1}, l1{cubes[2].code, 5, 2, 2};
+    auto s = tgv_minimize_oc(cubes, hists, {l0, l1}, Vec3f::Zero(), 32.0f, p);
+    EXPECT(s[0].u > 0.0f && s[2].u < 0.0f && s[1].u > s[2].u && s[1].u < s[0].u);
+    EXPECT(tgv_minimize_oc({}, {}, {}, Vec3f::Zero(), 1.0f).empty());
+    EXPECT(tgv_minimize({}, {}, Vec3f::Zero(), 1.0f).empty());
+}
+
+int main() {
+    try {
+        mip_levels_of_constant_map();
+        projection_cases();
+        histogram_statistics();
+        one_cube_one_view();
+        smoothing_chain();
+        constant_field();
+        state_file_roundtrip();
+        out_of_core();
+    } catch (const std::exception& e) {
+        std::printf("exception: %s\n", e.what());
+        return 2;
+    }
+    std::printf("Results: %d passed, %d failed\n", n_pass, n_fail);
+    return n_fail > 0 ? 1 : 0;
+}

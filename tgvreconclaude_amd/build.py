@@ -59,6 +59,20 @@ def build_scene():
     return out
 
 
+def build_host():
+    """libtgvx_host.so: the reference's C++ interface over the C ABI, and the
+    C++ drop-in test program linked against it."""
+    host = os.path.join(PKG, "host")
+    out = os.path.join(PKG, "libtgvx_host.so")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + INCLUDE, "-I" + host, "-o", out,
+          os.path.join(host, "tgvx_host.cpp"), "-L" + PKG, "-ltgvx", "-Wl,-rpath,$ORIGIN"])
+    test = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+    _run(["g++", "-std=c++17", "-O2", "-I" + INCLUDE, "-I" + host, "-o", test,
+          os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp"), "-L" + PKG, "-ltgvx_host", "-ltgvx",
+          "-Wl,-rpath," + PKG])
+    return out
+
+
 def build_oracle():
     _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "liboracle.so")
@@ -67,7 +81,9 @@ def build_oracle():
 def build_all():
     with ThreadPoolExecutor(max_workers=3) as ex:
         futs = [ex.submit(build_tgvx), ex.submit(build_scene), ex.submit(build_oracle)]
-        return [f.result() for f in futs]
+        out = [f.result() for f in futs]
+    out.append(build_host())
+    return out
 
 
 if __name__ == "__main__":

@@ -1,0 +1,13 @@
+// TreetopLeaf of the reference interface (src/treetop.hpp:8-13).
+#pragma once
+#include <cstdint>
+
+#include "types.hpp"
+
+struct TreetopLeaf {
+    MortonCode code;
+    int32_t depth;
+    uint32_t first_idx;
+    uint32_t last_idx;
+};
+static constexpr uint32_t N_CUBES_PER_TREETOP_LEAF = 1u << 24;
