@@ -57,7 +57,7 @@ def lib():
         L.orc_tgv_minimize_oc.argtypes = [vp, u64, vp, i32, vp, vp, u64, C.POINTER(OParams), i32, vp]
         L.orc_level_gap.argtypes = [vp, u64, vp, i32, C.POINTER(OParams), i32, vp, vp,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        L.orc_neighbour_table.argtypes = [vp, u64, i32, vp, vp]
+        L.orc_neighbour_table.argtypes = [vp, u64, i32, i32, vp, vp]
         L.orc_neighbour_table.restype = u64
         L.orc_extract_surface.argtypes = [vp, u64, vp, vp, vp, C.c_float, C.POINTER(vp), C.POINTER(vp),
                                           C.POINTER(u64)]
@@ -203,11 +203,12 @@ def level_gap(cubes, hists, prm, level, state, bar):
     return P.value, D.value
 
 
-def neighbour_table(cubes, level):
+def neighbour_table(cubes, level, mode=0):
+    """mode 0: the reference's table; mode 1: the depth-aware table of the exact-adjoint mode."""
     cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
     nb = np.zeros((len(cubes), 6), dtype=np.int32)
     act = np.zeros(len(cubes), dtype=np.uint32)
-    na = lib().orc_neighbour_table(_p(cubes), len(cubes), level, _p(nb), _p(act))
+    na = lib().orc_neighbour_table(_p(cubes), len(cubes), level, mode, _p(nb), _p(act))
     return act[:na].copy(), nb[:na].copy()
 
 

@@ -268,6 +268,16 @@ struct Params {
 
 using CodeMap = std::unordered_map<uint64_t, uint32_t>;
 
+// (code, depth) -> last input index with that pair. Used only by the
+// exact-adjoint mode, whose neighbour table is depth-aware (no reference
+// counterpart; the reference keys its map on the code alone, tgv.cpp:50-54).
+struct PairHash {
+    size_t operator()(const std::pair<uint64_t, int>& k) const {
+        return std::hash<uint64_t>()(k.first * 31u + (uint64_t)k.second);
+    }
+};
+using DepthCodeMap = std::unordered_map<std::pair<uint64_t, int>, uint32_t, PairHash>;
+
 // tgv.cpp:57-72
 template <class T>
 static inline void clip3(T p[3], T r) {
@@ -321,6 +331,23 @@ static inline int32_t nb_ctf(const CodeMap& cm, uint64_t code, int face, int dep
     return -1;
 }
 
+// Geometric neighbour of the exact-adjoint mode: the cube of the same depth
+// at the face-neighbour code, else the cube one level coarser covering it,
+// else none. A cube that is not the last one with its own (code, depth) is a
+// duplicate and gets no neighbours (the lookups find the last one).
+static inline int32_t nb_geo(const DepthCodeMap& dm, uint64_t code, int face, int depth, bool shadow) {
+    if (shadow) return -1;
+    const uint64_t nb = face_nb(code, face, depth);
+    if (nb == NONE) return -1;
+    auto it = dm.find({nb, depth});
+    if (it != dm.end()) return (int32_t)it->second;
+    if (depth > 0) {
+        it = dm.find({parent(nb, depth), depth - 1});
+        if (it != dm.end()) return (int32_t)it->second;
+    }
+    return -1;
+}
+
 // Incoming-edge lists for the exact-adjoint operator and the gap (no reference
 // counterpart): for active position ai, every (active position k, axis a) with
 // nb_{+a}(active[k]) == active[ai], ordered by (a, k).
@@ -331,24 +358,31 @@ struct Incoming {
 
 struct Level {
     std::vector<uint32_t> active;  // global indices, ascending
-    std::vector<int32_t> nb;       // nb[ai*6+f] (global index or -1), tgv.cpp:115-133
+    std::vector<int32_t> nb;       // reference table nb[ai*6+f] (global index or -1), tgv.cpp:115-133
+    std::vector<int32_t> nba;      // exact-adjoint (depth-aware) table, same layout
     std::vector<int32_t> pos;      // global index -> active position or -1 (size N)
-    Incoming in;
+    Incoming in;                   // incoming +a edges of nba within the level
 };
 
-static void build_level(Level& L, const Cube* cubes, const CodeMap& cm, size_t N) {
+static void build_level(Level& L, const Cube* cubes, const CodeMap& cm, const DepthCodeMap& dm, size_t N) {
     const size_t na = L.active.size();
     L.nb.resize(na * 6);
+    L.nba.resize(na * 6);
     for (size_t ai = 0; ai < na; ++ai) {
-        const Cube& c = cubes[L.active[ai]];
-        for (int f = 0; f < 6; ++f) L.nb[ai * 6 + f] = nb_ctf(cm, code_of(c), f, c.depth);
+        const uint32_t i = L.active[ai];
+        const Cube& c = cubes[i];
+        const bool shadow = dm.at({code_of(c), c.depth}) != i;
+        for (int f = 0; f < 6; ++f) {
+            L.nb[ai * 6 + f] = nb_ctf(cm, code_of(c), f, c.depth);
+            L.nba[ai * 6 + f] = nb_geo(dm, code_of(c), f, c.depth, shadow);
+        }
     }
     L.pos.assign(N, -1);
     for (size_t ai = 0; ai < na; ++ai) L.pos[L.active[ai]] = (int32_t)ai;
     std::vector<uint32_t> cnt(na + 1, 0);
     for (size_t k = 0; k < na; ++k)
         for (int a = 0; a < 3; ++a) {
-            const int32_t j = L.nb[k * 6 + 2 * a];
+            const int32_t j = L.nba[k * 6 + 2 * a];
             if (j >= 0 && L.pos[j] >= 0) cnt[L.pos[j] + 1]++;
         }
     for (size_t i = 0; i < na; ++i) cnt[i + 1] += cnt[i];
@@ -357,7 +391,7 @@ static void build_level(Level& L, const Cube* cubes, const CodeMap& cm, size_t N
     std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
     for (int a = 0; a < 3; ++a)
         for (size_t k = 0; k < na; ++k) {
-            const int32_t j = L.nb[k * 6 + 2 * a];
+            const int32_t j = L.nba[k * 6 + 2 * a];
             if (j >= 0 && L.pos[j] >= 0) L.in.ent[fill[L.pos[j]]++] = {(uint32_t)k, a};
         }
 }
@@ -397,7 +431,7 @@ static void level_gap(const Level& L, const std::vector<State<T>>& st, const std
         double g[3], dv[3][3];  // dv[c][a] = D_a v^c
         bool has[3];
         for (int a = 0; a < 3; ++a) {
-            const int32_t j = L.nb[ai * 6 + 2 * a];
+            const int32_t j = L.nba[ai * 6 + 2 * a];
             has[a] = j >= 0;
             g[a] = has[a] ? U(j) - ui : 0.0;
             for (int c = 0; c < 3; ++c) dv[c][a] = has[a] ? Vc(j, c) - (double)s.v[c] : 0.0;
@@ -415,7 +449,7 @@ static void level_gap(const Level& L, const std::vector<State<T>>& st, const std
         double cu = 0;         // coefficient of u_i
         double cv[3] = {0, 0, 0};  // coefficient of v_i
         for (int a = 0; a < 3; ++a) {
-            const int32_t j = L.nb[ai * 6 + 2 * a];
+            const int32_t j = L.nba[ai * 6 + 2 * a];
             if (j < 0) continue;
             cu -= (double)s.p[a];
             for (int c = 0; c < 3; ++c) {
@@ -464,6 +498,7 @@ struct Solver {
     std::vector<State<T>> st;
     std::vector<T> ub, vb[3];
     CodeMap cm;
+    DepthCodeMap dm;
     std::vector<std::vector<uint32_t>> by_depth;
     int min_d = 21, max_d = 0;
     // per-level report
@@ -474,6 +509,8 @@ struct Solver {
         st.assign(N, State<T>{});
         cm.reserve(N);
         for (size_t i = 0; i < N; ++i) cm[code_of(cubes[i])] = (uint32_t)i;  // last writer wins
+        dm.reserve(N);
+        for (size_t i = 0; i < N; ++i) dm[{code_of(cubes[i]), (int)cubes[i].depth}] = (uint32_t)i;
         for (size_t i = 0; i < N; ++i) st[i].u = median_indicator<T>(bins + i * nb, nb);
         for (size_t i = 0; i < N; ++i) {
             min_d = std::min(min_d, (int)cubes[i].depth);
@@ -488,11 +525,19 @@ struct Solver {
 
     // tgv.cpp:391-409: copy u, v from the cube holding the parent code (if any),
     // zero p, q, reset the extrapolated values.
+    // The exact-adjoint mode copies from the cube exactly one level coarser.
     void parent_init(const std::vector<uint32_t>& active, int L) {
         for (uint32_t i : active) {
-            auto it = cm.find(parent(code_of(cubes[i]), L));
-            if (it != cm.end()) {
-                const uint32_t pi = it->second;
+            const uint64_t pc = parent(code_of(cubes[i]), L);
+            int64_t pi = -1;
+            if (prm.op_mode == 0) {
+                auto it = cm.find(pc);
+                if (it != cm.end()) pi = it->second;
+            } else {
+                auto it = dm.find({pc, L - 1});
+                if (it != dm.end()) pi = it->second;
+            }
+            if (pi >= 0) {
                 st[i].u = st[pi].u;
                 for (int c = 0; c < 3; ++c) st[i].v[c] = st[pi].v[c];
             }
@@ -524,6 +569,8 @@ struct Solver {
         const int mode = prm.op_mode;
         auto inleaf = [&](int32_t j) { return (uint32_t)j >= leaf_first && (uint32_t)j <= leaf_last; };
         static const int QI[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+        // dual passes: the reference table, or the depth-aware one (exact adjoint)
+        const std::vector<int32_t>& NB = mode == 0 ? L.nb : L.nba;
 #ifdef _OPENMP
         const int nt = prm.threads > 0 ? prm.threads : 1;
 #endif
@@ -533,7 +580,7 @@ struct Solver {
             for (long ai = 0; ai < na; ++ai) {
                 const uint32_t i = L.active[ai];
                 auto ubf = [&](int f) -> T {
-                    const int32_t j = L.nb[ai * 6 + f];
+                    const int32_t j = NB[ai * 6 + f];
                     if (j < 0) return ub[i];
                     if (fu && !inleaf(j)) return fu[j];
                     return ub[j];
@@ -548,7 +595,7 @@ struct Solver {
             for (long ai = 0; ai < na; ++ai) {
                 const uint32_t i = L.active[ai];
                 auto vbf = [&](int f, int c) -> T {
-                    const int32_t j = L.nb[ai * 6 + f];
+                    const int32_t j = NB[ai * 6 + f];
                     if (j < 0) return vb[c][i];
                     if (fv && !inleaf(j)) return fv[c][j];
                     return vb[c][j];
@@ -602,7 +649,7 @@ struct Solver {
                     T dp = T(0);
                     T dq[3] = {T(0), T(0), T(0)};
                     for (int a = 0; a < 3; ++a) {
-                        if (L.nb[ai * 6 + 2 * a] < 0) continue;
+                        if (L.nba[ai * 6 + 2 * a] < 0) continue;
                         dp += s.p[a];
                         for (int c = 0; c < 3; ++c) {
                             const T w = (c == a) ? T(1) : T(0.5);
@@ -676,7 +723,7 @@ struct Solver {
                 // warm start: compare against the cold init unless disabled
                 if (prm.warm_fallback) {
                     if (L > min_d) parent_init(lv.active, L);
-                    build_level(lv, cubes, cm, N);
+                    build_level(lv, cubes, cm, dm, N);
                     double Pc, Dc;
                     gap(lv, Pc, Dc);
                     std::vector<State<T>> keep(lv.active.size());
@@ -694,11 +741,11 @@ struct Solver {
                     }
                 } else {
                     set_init(lv.active, init_uv);
-                    build_level(lv, cubes, cm, N);
+                    build_level(lv, cubes, cm, dm, N);
                 }
             } else {
                 if (L > min_d) parent_init(lv.active, L);
-                build_level(lv, cubes, cm, N);
+                build_level(lv, cubes, cm, dm, N);
             }
             run(lv, nullptr, nullptr, 0, 0);
         }
@@ -722,7 +769,7 @@ struct Solver {
                 if (lo == hi) continue;
                 Level lv;
                 lv.active.assign(lo, hi);
-                build_level(lv, cubes, cm, N);
+                build_level(lv, cubes, cm, dm, N);
                 run(lv, fu.data(), fv, leaf_first[l], leaf_last[l]);
             }
         }
@@ -966,11 +1013,15 @@ void orc_level_gap(const orc::Cube* cubes, uint64_t n, const uint32_t* bins, int
                    const orc_params* p, int32_t level, const double* state, const double* ubar,
                    double* P, double* D) {
     orc::CodeMap cm;
-    for (size_t i = 0; i < n; ++i) cm[orc::code_of(cubes[i])] = (uint32_t)i;
+    orc::DepthCodeMap dm;
+    for (size_t i = 0; i < n; ++i) {
+        cm[orc::code_of(cubes[i])] = (uint32_t)i;
+        dm[{orc::code_of(cubes[i]), (int)cubes[i].depth}] = (uint32_t)i;
+    }
     orc::Level L;
     for (size_t i = 0; i < n; ++i)
         if (cubes[i].depth == level) L.active.push_back((uint32_t)i);
-    orc::build_level(L, cubes, cm, n);
+    orc::build_level(L, cubes, cm, dm, n);
     std::vector<orc::State<double>> st(n);
     std::vector<double> ub(n), vb[3];
     for (int c = 0; c < 3; ++c) vb[c].resize(n);
@@ -987,20 +1038,27 @@ void orc_level_gap(const orc::Cube* cubes, uint64_t n, const uint32_t* bins, int
 }
 
 // Neighbour table of the level's active set (6 per active cube, global
-// indices or -1), in ascending global index order. Returns the active count.
-uint64_t orc_neighbour_table(const orc::Cube* cubes, uint64_t n, int32_t level, int32_t* nb_out,
+// indices or -1), in ascending global index order: the reference's table
+// (mode 0) or the depth-aware table of the exact-adjoint mode (mode 1).
+// Returns the active count.
+uint64_t orc_neighbour_table(const orc::Cube* cubes, uint64_t n, int32_t level, int32_t mode, int32_t* nb_out,
                              uint32_t* active_out) {
     orc::CodeMap cm;
-    for (size_t i = 0; i < n; ++i) cm[orc::code_of(cubes[i])] = (uint32_t)i;
-    size_t na = 0;
+    orc::DepthCodeMap dm;
     for (size_t i = 0; i < n; ++i) {
-        if (cubes[i].depth != level) continue;
-        if (active_out) active_out[na] = (uint32_t)i;
-        for (int f = 0; f < 6; ++f)
-            nb_out[na * 6 + f] = orc::nb_ctf(cm, orc::code_of(cubes[i]), f, cubes[i].depth);
-        ++na;
+        cm[orc::code_of(cubes[i])] = (uint32_t)i;
+        dm[{orc::code_of(cubes[i]), (int)cubes[i].depth}] = (uint32_t)i;
     }
-    return na;
+    orc::Level L;
+    for (size_t i = 0; i < n; ++i)
+        if (cubes[i].depth == level) L.active.push_back((uint32_t)i);
+    orc::build_level(L, cubes, cm, dm, n);
+    const std::vector<int32_t>& t = mode == 0 ? L.nb : L.nba;
+    for (size_t k = 0; k < L.active.size(); ++k) {
+        if (active_out) active_out[k] = L.active[k];
+        for (int f = 0; f < 6; ++f) nb_out[k * 6 + f] = t[k * 6 + f];
+    }
+    return L.active.size();
 }
 
 // Surface of the u = 0 level set. Returns the vertex count; verts / faces are
@@ -1051,7 +1109,7 @@ double orc_time_level(const orc::Cube* cubes, uint64_t n, const uint32_t* bins, 
     if (depth <= s.max_d) lv.active = s.by_depth[depth];
     *n_active = lv.active.size();
     if (lv.active.empty()) return 0.0;
-    orc::build_level(lv, cubes, s.cm, n);
+    orc::build_level(lv, cubes, s.cm, s.dm, n);
     const size_t na = lv.active.size();
     std::vector<float> un(na), vn[3];
     for (int c = 0; c < 3; ++c) vn[c].resize(na);

@@ -324,7 +324,7 @@ def test_threads_do_not_change_bits(tmp_path):
 def dense_level(cs, level):
     """Independent statement of one level's linear map K(u, v) = (grad u - v, eps(v))
     as a matrix, from the neighbour table alone (all cubes on one level)."""
-    act, nb = O.neighbour_table(cs, level)
+    act, nb = O.neighbour_table(cs, level, mode=1)
     n = len(act)
     pos = {int(a): i for i, a in enumerate(act)}
     # variables x = [u (n), vx (n), vy (n), vz (n)]; rows: p (3n) then q (6n)
@@ -493,3 +493,39 @@ def test_plane_patch_surface():
     assert len({tuple(r) for r in np.round(v * 1e6).astype(np.int64)}) == len(v)
     b, i, irr = _edge_valence(f)
     assert irr == 0 and i > 0
+
+
+# ---------------------------------------------------------------- depth-aware table
+
+def test_depth_aware_table_brute_force():
+    # exact-adjoint mode: same-depth face neighbour, else the cube one level
+    # coarser covering that position, else none; one-to-one within a level
+    rng = np.random.default_rng(11)
+    cells = set()
+    for _ in range(400):
+        d = int(rng.integers(2, 5))
+        x, y, z = rng.integers(0, 1 << d, 3)
+        cells.add((d, int(x), int(y), int(z)))
+    lst = sorted(cells, key=lambda c: (O.morton_encode(c[1] << (21 - c[0]), c[2] << (21 - c[0]),
+                                                       c[3] << (21 - c[0])), c[0]))
+    cs = cubes_of([cube(x, y, z, d) for d, x, y, z in lst])
+    index = {c: i for i, c in enumerate(lst)}
+    for level in (2, 3, 4):
+        act, nb = O.neighbour_table(cs, level, mode=1)
+        seen = set()
+        for row, i in zip(nb, act):
+            d, x, y, z = lst[i]
+            assert d == level
+            for f in range(6):
+                a, sgn = f // 2, (1 if f % 2 == 0 else -1)
+                p = [x, y, z]
+                p[a] += sgn
+                want = -1
+                if 0 <= p[a] < (1 << d):
+                    want = index.get((d, *p), -1)
+                    if want < 0:
+                        want = index.get((d - 1, p[0] >> 1, p[1] >> 1, p[2] >> 1), -1)
+                assert row[f] == want, (lst[i], f, row[f], want)
+                if f % 2 == 0 and want >= 0 and lst[want][0] == level:
+                    assert (want, f) not in seen
+                    seen.add((want, f))

@@ -33,13 +33,13 @@ template <class S>
 struct KArgs {
     using T = typename Io<S>::T;
     S* s[17];
-    const int32_t* nb[6];
-    const int32_t* nbt[3];    // incoming +a edge of each axis (exact-adjoint mode), or -1
+    const int32_t* nbp[3];    // +x, +y, +z neighbours read by the dual pass
+    const int32_t* nbm[3];    // -x, -y, -z neighbours (reference primal pass)
+    const int32_t* nba[3];    // depth-aware +x, +y, +z neighbours (exact-adjoint mode, gap)
+    const int32_t* nbt[3];    // the level's cube whose depth-aware +a neighbour this is, or -1
     const uint32_t* info;     // packed per-cube word, see InfoWord
     int32_t nbins;
     const int32_t* parent;
-    const int32_t* in_start;
-    const int32_t* in_ent;
     uint32_t off, n;          // positions iterated by this launch
     uint32_t lvl_off, lvl_n;  // the level's positions (active set for the gap)
     T sigma, tau, a0, a1, tl;
@@ -70,13 +70,12 @@ __device__ inline void clip6(T* q, T r) {
         for (int i = 0; i < 6; ++i) q[i] *= s;
     }
 }
-// Per-cube constant word: the vote total W (bits 0-22), "more than one
-// incoming edge on some axis" (bit 23), the weighted-median bin k (bits
-// 24-28) and "the +x / +y / +z neighbour exists" (bits 29-31).
+// Per-cube constant word: the vote total W (bits 0-23), the weighted-median
+// bin k (bits 24-28) and "the depth-aware +x / +y / +z neighbour exists"
+// (bits 29-31).
 struct InfoWord {
-    static constexpr uint32_t kWMax = (1u << 23) - 1;
+    static constexpr uint32_t kWMax = (1u << 24) - 1;
     __host__ __device__ static uint32_t W(uint32_t w) { return w & kWMax; }
-    __host__ __device__ static bool multi(uint32_t w) { return (w >> 23) & 1u; }
     __host__ __device__ static int bin(uint32_t w) { return (int)((w >> 24) & 31u); }
     __host__ __device__ static bool has(uint32_t w, int ax) { return (w >> (29 + ax)) & 1u; }
 };
@@ -114,18 +113,23 @@ __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
     const long g = (long)a.off + t;
     const T ubi = IO::ld(a.s[F_UB], g), vxi = IO::ld(a.s[F_VBX], g), vyi = IO::ld(a.s[F_VBY], g),
             vzi = IO::ld(a.s[F_VBZ], g);
-    const int j0 = a.nb[0][g], j2 = a.nb[2][g], j4 = a.nb[4][g];
+    const int j0 = a.nbp[0][g], j2 = a.nbp[1][g], j4 = a.nbp[2][g];
     auto live = [&](int j) -> bool {
         if (!FROZEN) return true;
         const uint32_t o = a.orig[j];
         return o >= a.leaf_first && o <= a.leaf_last;
     };
-    auto rd = [&](int j, int f, int fzi, T self) -> T {
-        if (j < 0) return self;
-        if (FROZEN && !live(j)) return IO::ld(a.fz[fzi], j);
-        return IO::ld(a.s[f], j);
+    // A missing neighbour reads the cube's own value (Neumann, tgv.cpp:163):
+    // loading at g itself gives exactly that, so every load is unconditional.
+    const long s0 = j0 >= 0 ? j0 : g, s2 = j2 >= 0 ? j2 : g, s4 = j4 >= 0 ? j4 : g;
+    auto rd = [&](int j, long sj, int f, int fzi, T self) -> T {
+        if (FROZEN) {
+            if (j < 0) return self;
+            if (!live(j)) return IO::ld(a.fz[fzi], j);
+        }
+        return IO::ld(a.s[f], sj);
     };
-    const T ub0 = rd(j0, F_UB, 0, ubi), ub2 = rd(j2, F_UB, 0, ubi), ub4 = rd(j4, F_UB, 0, ubi);
+    const T ub0 = rd(j0, s0, F_UB, 0, ubi), ub2 = rd(j2, s2, F_UB, 0, ubi), ub4 = rd(j4, s4, F_UB, 0, ubi);
     T p0 = IO::ld(a.s[F_PX], g), p1 = IO::ld(a.s[F_PY], g), p2 = IO::ld(a.s[F_PZ], g);
     p0 += a.sigma * (ub0 - ubi - vxi);
     p1 += a.sigma * (ub2 - ubi - vyi);
@@ -135,9 +139,9 @@ __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
     IO::st(a.s[F_PY], g, p1);
     IO::st(a.s[F_PZ], g, p2);
 
-    const T x0 = rd(j0, F_VBX, 1, vxi), y0 = rd(j0, F_VBY, 2, vyi), z0 = rd(j0, F_VBZ, 3, vzi);
-    const T x2 = rd(j2, F_VBX, 1, vxi), y2 = rd(j2, F_VBY, 2, vyi), z2 = rd(j2, F_VBZ, 3, vzi);
-    const T x4 = rd(j4, F_VBX, 1, vxi), y4 = rd(j4, F_VBY, 2, vyi), z4 = rd(j4, F_VBZ, 3, vzi);
+    const T x0 = rd(j0, s0, F_VBX, 1, vxi), y0 = rd(j0, s0, F_VBY, 2, vyi), z0 = rd(j0, s0, F_VBZ, 3, vzi);
+    const T x2 = rd(j2, s2, F_VBX, 1, vxi), y2 = rd(j2, s2, F_VBY, 2, vyi), z2 = rd(j2, s2, F_VBZ, 3, vzi);
+    const T x4 = rd(j4, s4, F_VBX, 1, vxi), y4 = rd(j4, s4, F_VBY, 2, vyi), z4 = rd(j4, s4, F_VBZ, 3, vzi);
     const T eps[6] = {x0 - vxi,
                       y2 - vyi,
                       z4 - vzi,
@@ -161,18 +165,24 @@ __global__ void __launch_bounds__(256) k_primal_ref(KArgs<S> a) {
     const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
-    const int m0 = a.nb[1][g], m2 = a.nb[3][g], m4 = a.nb[5][g];
-    auto ld = [&](int f, int j) -> T { return j >= 0 ? IO::ld(a.s[f], j) : T(0); };
+    const int m0 = a.nbm[0][g], m2 = a.nbm[1][g], m4 = a.nbm[2][g];
+    // A missing neighbour contributes 0 (tgv.cpp:196,209). Loads go to g
+    // instead and the value is replaced by 0 afterwards, so that every load
+    // is unconditional and they can all be in flight together.
+    const long s0 = m0 >= 0 ? m0 : g, s2 = m2 >= 0 ? m2 : g, s4 = m4 >= 0 ? m4 : g;
+    auto nx = [&](int f) -> T { const T x = IO::ld(a.s[f], s0); return m0 >= 0 ? x : T(0); };
+    auto ny = [&](int f) -> T { const T x = IO::ld(a.s[f], s2); return m2 >= 0 ? x : T(0); };
+    auto nz = [&](int f) -> T { const T x = IO::ld(a.s[f], s4); return m4 >= 0 ? x : T(0); };
     const T px = IO::ld(a.s[F_PX], g), py = IO::ld(a.s[F_PY], g), pz = IO::ld(a.s[F_PZ], g);
-    const T div_p = (px - ld(F_PX, m0)) + (py - ld(F_PY, m2)) + (pz - ld(F_PZ, m4));
+    const T div_p = (px - nx(F_PX)) + (py - ny(F_PY)) + (pz - nz(F_PZ));
     const T u = IO::ld(a.s[F_U], g);
     const T un = prox(u + a.tau * div_p, a.info[g], a.nbins, a.tl);
 
     const T q0 = IO::ld(a.s[F_Q0], g), q1 = IO::ld(a.s[F_Q1], g), q2 = IO::ld(a.s[F_Q2], g),
             q3 = IO::ld(a.s[F_Q3], g), q4 = IO::ld(a.s[F_Q4], g), q5 = IO::ld(a.s[F_Q5], g);
-    const T dqx = (q0 - ld(F_Q0, m0)) + (q3 - ld(F_Q3, m2)) + (q4 - ld(F_Q4, m4));
-    const T dqy = (q3 - ld(F_Q3, m0)) + (q1 - ld(F_Q1, m2)) + (q5 - ld(F_Q5, m4));
-    const T dqz = (q4 - ld(F_Q4, m0)) + (q5 - ld(F_Q5, m2)) + (q2 - ld(F_Q2, m4));
+    const T dqx = (q0 - nx(F_Q0)) + (q3 - ny(F_Q3)) + (q4 - nz(F_Q4));
+    const T dqy = (q3 - nx(F_Q3)) + (q1 - ny(F_Q1)) + (q5 - nz(F_Q5));
+    const T dqz = (q4 - nx(F_Q4)) + (q5 - ny(F_Q5)) + (q2 - nz(F_Q2));
     const T vx = IO::ld(a.s[F_VX], g), vy = IO::ld(a.s[F_VY], g), vz = IO::ld(a.s[F_VZ], g);
     const T vxn = vx + a.tau * (px + dqx);
     const T vyn = vy + a.tau * (py + dqy);
@@ -213,51 +223,46 @@ __global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a) {
     T p[3] = {IO::ld(a.s[F_PX], g), IO::ld(a.s[F_PY], g), IO::ld(a.s[F_PZ], g)};
     T dp = T(0);
     T dq[3] = {T(0), T(0), T(0)};
+    const T qs[6] = {IO::ld(a.s[F_Q0], g), IO::ld(a.s[F_Q1], g), IO::ld(a.s[F_Q2], g),
+                     IO::ld(a.s[F_Q3], g), IO::ld(a.s[F_Q4], g), IO::ld(a.s[F_Q5], g)};
+    // self terms where the +a neighbour exists; a select keeps each sum
+    // exactly as if the term had been skipped
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        if (!InfoWord::has(w, ax)) continue;
-        dp += p[ax];
+        const bool h = InfoWord::has(w, ax);
+        dp = h ? dp + p[ax] : dp;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const T w2 = (c == ax) ? T(1) : T(0.5);
-            dq[c] += w2 * IO::ld(a.s[qidx(c, ax)], g);
+            dq[c] = h ? dq[c] + w2 * qs[qidx(c, ax) - F_Q0] : dq[c];
         }
     }
-    // incoming edges in (axis, source) order
-    auto incoming = [&](long k, int ax) {
-        dp -= IO::ld(field3(a, F_PX, F_PY, F_PZ, ax), k);
-        const T w0 = ax == 0 ? T(1) : T(0.5), w1 = ax == 1 ? T(1) : T(0.5), w2 = ax == 2 ? T(1) : T(0.5);
-        dq[0] -= w0 * IO::ld(field3(a, F_Q0, F_Q3, F_Q4, ax), k);
-        dq[1] -= w1 * IO::ld(field3(a, F_Q3, F_Q1, F_Q5, ax), k);
-        dq[2] -= w2 * IO::ld(field3(a, F_Q4, F_Q5, F_Q2, ax), k);
-    };
-    if (!InfoWord::multi(w)) {
-        // at most one incoming edge per axis: three direct slots
-        const int k0 = a.nbt[0][g], k1 = a.nbt[1][g], k2 = a.nbt[2][g];
-        if (k0 >= 0) {
-            dp -= IO::ld(a.s[F_PX], k0);
-            dq[0] -= T(1) * IO::ld(a.s[F_Q0], k0);
-            dq[1] -= T(0.5) * IO::ld(a.s[F_Q3], k0);
-            dq[2] -= T(0.5) * IO::ld(a.s[F_Q4], k0);
-        }
-        if (k1 >= 0) {
-            dp -= IO::ld(a.s[F_PY], k1);
-            dq[0] -= T(0.5) * IO::ld(a.s[F_Q3], k1);
-            dq[1] -= T(1) * IO::ld(a.s[F_Q1], k1);
-            dq[2] -= T(0.5) * IO::ld(a.s[F_Q5], k1);
-        }
-        if (k2 >= 0) {
-            dp -= IO::ld(a.s[F_PZ], k2);
-            dq[0] -= T(0.5) * IO::ld(a.s[F_Q4], k2);
-            dq[1] -= T(0.5) * IO::ld(a.s[F_Q5], k2);
-            dq[2] -= T(1) * IO::ld(a.s[F_Q2], k2);
-        }
-    } else {
-        const int e0 = a.in_start[g], e1 = a.in_start[g + 1];
-        for (int e = e0; e < e1; ++e) {
-            const int ent = a.in_ent[e];
-            incoming(ent & 0x3fffffff, (int)((uint32_t)ent >> 30));
-        }
+    // incoming edges, one per axis at most (the depth-aware table is one-to-one)
+    const int k0 = a.nbt[0][g], k1 = a.nbt[1][g], k2 = a.nbt[2][g];
+    const long s0 = k0 >= 0 ? k0 : g, s1 = k1 >= 0 ? k1 : g, s2 = k2 >= 0 ? k2 : g;
+    const T p0 = IO::ld(a.s[F_PX], s0), a0 = IO::ld(a.s[F_Q0], s0), a3 = IO::ld(a.s[F_Q3], s0),
+            a4 = IO::ld(a.s[F_Q4], s0);
+    const T p1 = IO::ld(a.s[F_PY], s1), b3 = IO::ld(a.s[F_Q3], s1), b1 = IO::ld(a.s[F_Q1], s1),
+            b5 = IO::ld(a.s[F_Q5], s1);
+    const T p2 = IO::ld(a.s[F_PZ], s2), c4 = IO::ld(a.s[F_Q4], s2), c5 = IO::ld(a.s[F_Q5], s2),
+            c2 = IO::ld(a.s[F_Q2], s2);
+    if (k0 >= 0) {
+        dp -= p0;
+        dq[0] -= T(1) * a0;
+        dq[1] -= T(0.5) * a3;
+        dq[2] -= T(0.5) * a4;
+    }
+    if (k1 >= 0) {
+        dp -= p1;
+        dq[0] -= T(0.5) * b3;
+        dq[1] -= T(1) * b1;
+        dq[2] -= T(0.5) * b5;
+    }
+    if (k2 >= 0) {
+        dp -= p2;
+        dq[0] -= T(0.5) * c4;
+        dq[1] -= T(0.5) * c5;
+        dq[2] -= T(1) * c2;
     }
     const T u = IO::ld(a.s[F_U], g);
     const T un = prox(u + a.tau * dp, w, a.nbins, a.tl);
@@ -377,7 +382,7 @@ __global__ void __launch_bounds__(kGapBlock) k_gap(KArgs<S> a, double lam, doubl
         int jn[3];
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
-            const int j = a.nb[2 * ax][g];
+            const int j = a.nba[ax][g];
             jn[ax] = j;
             if (j >= 0) {
                 const bool in = in_lvl(j);
@@ -418,15 +423,16 @@ __global__ void __launch_bounds__(kGapBlock) k_gap(KArgs<S> a, double lam, doubl
                 }
             }
         }
-        for (int e2 = a.in_start[g]; e2 < a.in_start[g + 1]; ++e2) {
-            const int ent = a.in_ent[e2];
-            const long k = ent & 0x3fffffff;
-            const int ax = (int)((uint32_t)ent >> 30);
-            auto LD = [&](const S* f) { return (double)IO::ld(f, k); };
-            cu += LD(field3(a, F_PX, F_PY, F_PZ, ax));
-            cv[0] += (ax == 0 ? 1.0 : 0.5) * LD(field3(a, F_Q0, F_Q3, F_Q4, ax));
-            cv[1] += (ax == 1 ? 1.0 : 0.5) * LD(field3(a, F_Q3, F_Q1, F_Q5, ax));
-            cv[2] += (ax == 2 ? 1.0 : 0.5) * LD(field3(a, F_Q4, F_Q5, F_Q2, ax));
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int k = a.nbt[ax][g];
+            if (k < 0) continue;
+            cu += L(F_PX + ax, k);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double w = (c == ax) ? 1.0 : 0.5;
+                cv[c] += w * L(qidx(c, ax), k);
+            }
         }
         for (int c = 0; c < 3; ++c) cv[c] -= p[c];
         double mu;
@@ -513,6 +519,66 @@ __global__ void k_tables(const uint64_t* __restrict__ codes, const int32_t* __re
     parent[g] = d > 0 ? code_lookup(uc, ui, U, ancestor(c, d - 1)) : -1;
 }
 
+// Depth-aware table of the exact-adjoint mode (no reference counterpart):
+// dc / dd ascending by (code, depth), dp = position of the last input cube
+// with that pair. Same-depth neighbour, else the cube one level coarser,
+// else -1; duplicates of a (code, depth) pair other than the last get none.
+__device__ inline int pair_lookup(const uint64_t* dc, const int32_t* dd, const int32_t* dp, uint32_t U, uint64_t key,
+                                  int depth) {
+    uint32_t lo = 0, hi = U;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (dc[mid] < key || (dc[mid] == key && dd[mid] < depth)) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < U && dc[lo] == key && dd[lo] == depth) ? dp[lo] : -1;
+}
+
+__global__ void k_tables_geo(const uint64_t* __restrict__ codes, const int32_t* __restrict__ depth, uint32_t n,
+                             const uint64_t* __restrict__ dc, const int32_t* __restrict__ dd,
+                             const int32_t* __restrict__ dp, uint32_t U, int32_t* nba0, int32_t* nba1, int32_t* nba2,
+                             int32_t* parent, uint32_t* info) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint64_t c = codes[g];
+    const int d = depth[g];
+    const bool shadow = pair_lookup(dc, dd, dp, U, c, d) != (int)g;
+    int32_t* out[3] = {nba0, nba1, nba2};
+    uint32_t flags = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        int j = -1;
+        const uint64_t k = shadow ? kNone : face_nb(c, 2 * ax, d);
+        if (k != kNone) {
+            j = pair_lookup(dc, dd, dp, U, k, d);
+            if (j < 0 && d > 0) j = pair_lookup(dc, dd, dp, U, ancestor(k, d - 1), d - 1);
+        }
+        out[ax][g] = j;
+        if (j >= 0) flags |= 1u << (29 + ax);
+    }
+    info[g] |= flags;
+    parent[g] = d > 0 ? pair_lookup(dc, dd, dp, U, ancestor(c, d - 1), d - 1) : -1;
+}
+
+__global__ void k_fill_i32(int32_t* p, uint32_t n, int32_t v) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) p[g] = v;
+}
+
+// The depth-aware table is one-to-one within a level, so each target has at
+// most one writer here.
+__global__ void k_transpose(const int32_t* __restrict__ nba0, const int32_t* __restrict__ nba1,
+                            const int32_t* __restrict__ nba2, const int32_t* __restrict__ depth, uint32_t n,
+                            int32_t* nbt0, int32_t* nbt1, int32_t* nbt2) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t* in[3] = {nba0, nba1, nba2};
+    int32_t* out[3] = {nbt0, nbt1, nbt2};
+    for (int ax = 0; ax < 3; ++ax) {
+        const int j = in[ax][k];
+        if (j >= 0 && depth[j] == depth[k]) out[ax][j] = (int32_t)k;
+    }
+}
+
 // ---------------------------------------------------------------- host side
 namespace {
 
@@ -527,13 +593,17 @@ KArgs<S> make_args(tgvx_ctx* c, const tgvx_params* prm) {
     KArgs<S> a;
     std::memset(&a, 0, sizeof(a));
     for (int i = 0; i < 17; ++i) a.s[i] = P.d_state[i].as<S>();
-    for (int f = 0; f < 6; ++f) a.nb[f] = P.d_nb[f].as<int32_t>();
+    const bool adj = prm && prm->op_mode == TGVX_OP_ADJ;
+    for (int ax = 0; ax < 3; ++ax) {
+        a.nba[ax] = P.d_nba[ax].as<int32_t>();
+        a.nbt[ax] = P.d_nbt[ax].as<int32_t>();
+        a.nbm[ax] = P.d_nb[2 * ax + 1].as<int32_t>();
+        a.nbp[ax] = adj ? a.nba[ax] : P.d_nb[2 * ax].as<int32_t>();
+    }
     for (int ax = 0; ax < 3; ++ax) a.nbt[ax] = P.d_nbt[ax].as<int32_t>();
     a.info = P.d_info.as<uint32_t>();
     a.nbins = P.nb;
-    a.parent = P.d_parent.as<int32_t>();
-    a.in_start = P.d_in_start.as<int32_t>();
-    a.in_ent = P.d_in_ent.as<int32_t>();
+    a.parent = adj ? P.d_parent_a.as<int32_t>() : P.d_parent.as<int32_t>();
     for (int i = 0; i < 4; ++i) a.save[i] = P.d_save[i].as<S>();
     a.init = P.d_init.as<float>();
     if (prm) {
@@ -898,46 +968,48 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
             P.d_nb[3].as<int32_t>(), P.d_nb[4].as<int32_t>(), P.d_nb[5].as<int32_t>(), P.d_parent.as<int32_t>());
         TGVX_HIP(hipGetLastError());
     }
-    // Incoming +a edges within each level: for cube j, every (k, a) with
-    // nb_{+a}(k) == j and k on j's level, ordered by (a, k). Axis a's first
-    // source also goes to slot nbt[a][j]; cubes with two sources on an axis
-    // are flagged in the info word and read the full list.
-    std::vector<int32_t> nbp[3];
-    for (int ax = 0; ax < 3; ++ax) {
-        nbp[ax].resize(n);
-        if (n)
-            TGVX_HIP(hipMemcpyAsync(nbp[ax].data(), P.d_nb[2 * ax].p, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    // depth-aware table (exact-adjoint mode and the gap): (code, depth) pairs
+    // ascending with the position of the last input cube of each pair
+    std::vector<std::pair<uint64_t, uint32_t>> pk(n);
+    for (uint64_t i = 0; i < n; ++i)
+        pk[i] = {((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i};
+    std::stable_sort(pk.begin(), pk.end(), [&](const auto& x, const auto& y) {
+        return x.first != y.first ? x.first < y.first : cubes[x.second].depth < cubes[y.second].depth;
+    });
+    std::vector<uint64_t> dcode;
+    std::vector<int32_t> ddepth, dpos;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (i + 1 < n && pk[i + 1].first == pk[i].first && cubes[pk[i + 1].second].depth == cubes[pk[i].second].depth)
+            continue;
+        dcode.push_back(pk[i].first);
+        ddepth.push_back(cubes[pk[i].second].depth);
+        dpos.push_back((int32_t)P.inv[pk[i].second]);
     }
-    TGVX_HIP(hipStreamSynchronize(c->stream));
-    std::vector<int32_t> in_start(n + 1, 0);
-    std::vector<int32_t> slot[3];
-    std::vector<uint8_t> cnt_ax(3 * n, 0);
-    for (int ax = 0; ax < 3; ++ax) slot[ax].assign(n, -1);
-    for (int ax = 0; ax < 3; ++ax)
-        for (uint64_t k = 0; k < n; ++k) {
-            const int32_t j = nbp[ax][k];
-            if (j >= 0) info[k] |= 1u << (29 + ax);
-            if (j >= 0 && pdepth[j] == pdepth[k]) {
-                in_start[j + 1]++;
-                if (slot[ax][j] < 0) slot[ax][j] = (int32_t)k;
-                if (cnt_ax[3 * j + ax] < 2) cnt_ax[3 * j + ax]++;
-            }
-        }
-    for (uint64_t j = 0; j < n; ++j)
-        if (cnt_ax[3 * j] > 1 || cnt_ax[3 * j + 1] > 1 || cnt_ax[3 * j + 2] > 1) info[j] |= 1u << 23;
-    for (uint64_t i = 0; i < n; ++i) in_start[i + 1] += in_start[i];
-    std::vector<int32_t> ent(in_start[n]);
-    std::vector<int32_t> fill(in_start.begin(), in_start.end() - 1);
-    for (int ax = 0; ax < 3; ++ax)
-        for (uint64_t k = 0; k < n; ++k) {
-            const int32_t j = nbp[ax][k];
-            if (j >= 0 && pdepth[j] == pdepth[k]) ent[fill[j]++] = (int32_t)(k | ((uint64_t)ax << 30));
-        }
-    P.n_in_edges = ent.size();
+    std::vector<std::pair<uint64_t, uint32_t>>().swap(pk);
+    DevBuf d_dc, d_dd, d_dp;
+    up(d_dc, dcode.data(), 8 * std::max<size_t>(dcode.size(), 1));
+    up(d_dd, ddepth.data(), 4 * std::max<size_t>(ddepth.size(), 1));
+    up(d_dp, dpos.data(), 4 * std::max<size_t>(dpos.size(), 1));
     up(P.d_info, info.data(), 4 * std::max<uint64_t>(n, 1));
-    for (int ax = 0; ax < 3; ++ax) up(P.d_nbt[ax], slot[ax].data(), 4 * std::max<uint64_t>(n, 1));
-    up(P.d_in_start, in_start.data(), 4 * (n + 1));
-    up(P.d_in_ent, ent.data(), 4 * std::max<size_t>(ent.size(), 1));
+    for (int ax = 0; ax < 3; ++ax) {
+        P.d_nba[ax].ensure(4 * std::max<uint64_t>(n, 1));
+        P.d_nbt[ax].ensure(4 * std::max<uint64_t>(n, 1));
+    }
+    P.d_parent_a.ensure(4 * std::max<uint64_t>(n, 1));
+    if (n > 0) {
+        const unsigned grid = (unsigned)((n + 255) / 256);
+        k_tables_geo<<<grid, 256, 0, c->stream>>>(d_code.as<uint64_t>(), d_depth.as<int32_t>(), (uint32_t)n,
+                                                  d_dc.as<uint64_t>(), d_dd.as<int32_t>(), d_dp.as<int32_t>(),
+                                                  (uint32_t)dcode.size(), P.d_nba[0].as<int32_t>(),
+                                                  P.d_nba[1].as<int32_t>(), P.d_nba[2].as<int32_t>(),
+                                                  P.d_parent_a.as<int32_t>(), P.d_info.as<uint32_t>());
+        for (int ax = 0; ax < 3; ++ax) k_fill_i32<<<grid, 256, 0, c->stream>>>(P.d_nbt[ax].as<int32_t>(), (uint32_t)n, -1);
+        k_transpose<<<grid, 256, 0, c->stream>>>(P.d_nba[0].as<int32_t>(), P.d_nba[1].as<int32_t>(),
+                                                 P.d_nba[2].as<int32_t>(), d_depth.as<int32_t>(), (uint32_t)n,
+                                                 P.d_nbt[0].as<int32_t>(), P.d_nbt[1].as<int32_t>(),
+                                                 P.d_nbt[2].as<int32_t>());
+        TGVX_HIP(hipGetLastError());
+    }
     for (int i = 0; i < 17; ++i) P.d_state[i].ensure(P.elem * std::max<uint64_t>(n, 1));
     for (int i = 0; i < 4; ++i) P.d_save[i].ensure(P.elem * std::max<uint64_t>(n, 1));
     // position -> input index (the frozen-border mode's in-leaf test reads it)

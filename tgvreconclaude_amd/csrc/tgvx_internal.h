@@ -69,15 +69,15 @@ struct Problem {
     DevBuf d_nb[6];                  // int32 per position, faces 0..5
     DevBuf d_parent;                 // int32: position of the cube holding the parent code, or -1
     DevBuf d_info;                   // uint32 packed vote total / median bin / flags
-    DevBuf d_nbt[3];                 // int32 first incoming +a edge per axis, or -1
-    DevBuf d_in_start, d_in_ent;     // incoming +a edges within a level (CSR; ent = k | a << 30)
+    DevBuf d_nba[3];                 // int32 depth-aware +x, +y, +z neighbours (exact-adjoint mode)
+    DevBuf d_nbt[3];                 // int32 the level's cube whose depth-aware +a neighbour this is, or -1
+    DevBuf d_parent_a;               // int32 cube exactly one level coarser at the parent code, or -1
     DevBuf d_state[17];              // u vx vy vz px py pz q0..q5 ub vbx vby vbz
     DevBuf d_save[4];                // scratch copy of u, v of one level
     DevBuf d_init;                   // float x 4 per position (warm start field), optional
     DevBuf d_orig;                   // uint32 position -> input index
     DevBuf d_part;                   // gap partial sums
     DevBuf d_pd;                     // final P, D
-    uint64_t n_in_edges = 0;
 };
 
 }  // namespace tgvx
