@@ -156,6 +156,13 @@ def main():
                          f"FP32 restatement of run_level (tgv.cpp:137-232), 1 thread, {t_k:.1f} s"}
         log(f"[bench] CPU baseline: {cpu['value']:.4g} leaf-iter/s ({k} its, {t_k:.1f} s)")
 
+    # HBM traffic per iteration from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # summary of this workload (counters cannot be read from inside the run)
+    traffic = None
+    pj = os.path.join(ROOT, "profiles", "tgv_iteration_depth10_torus_box.json")
+    if args.precision == "fp32" and os.path.exists(pj) and n_f == json.load(open(pj))["leaves"]:
+        traffic = json.load(open(pj))["traffic_bytes_per_iteration"]
+
     out = {
         "metric": "TGV leaf-iter/s + time-to-gap<=1e-4 (learned warm start vs cold), 1 GPU",
         "value": value,
@@ -174,7 +181,8 @@ def main():
                    "iters_per_level_per_step": args.iters, "op_mode": "adjoint", "levels": levels,
                    "parallelism": "single-gpu", "device": arch, "cus": cus},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_unit": "bytes per iteration (dual + primal launch), rocprofv3 counters",
                      "kernel": f"k_dual + k_primal_adj, depth {finest}, {n_f} leaves",
                      "alg_bytes_per_leaf_iter": ab, "dual_us": dual_ms * 1e3, "primal_us": primal_ms * 1e3},
         "time_to_gap": {"cold": gap, "warm": None},
