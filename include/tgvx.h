@@ -89,7 +89,10 @@ typedef struct {
     int32_t warm_fallback; /* with an initial field: per level, keep the cold init when its gap is lower */
     int32_t use_graph;     /* launch iteration chunks as captured HIP graphs */
     int32_t block;         /* threads per workgroup (0 = tuned default) */
-    int32_t reserved[5];
+    float dual_eps;        /* > 0: after each level's initialisation set p = alpha1 K1/max(|K1|, dual_eps),
+                              q = alpha0 e/max(|e|, dual_eps) from the initial (u, v) (K1 = grad u - v,
+                              e = eps(v), depth-aware operators); 0: p = q = 0 as the reference */
+    int32_t reserved[4];
 } tgvx_params;
 
 #define TGVX_MAX_LEVELS 22

@@ -264,6 +264,7 @@ struct Params {
     double gap_tol;       // relative gap target; <= 0 means fixed iterations
     int32_t threads;      // OpenMP threads for the passes (1 = reference)
     int32_t warm_fallback;  // with an init field: keep cold init where it has the lower gap
+    float dual_eps;         // with an init field: > 0 recovers p, q from (u, v), see dual_recover
 };
 
 using CodeMap = std::unordered_map<uint64_t, uint32_t>;
@@ -558,6 +559,38 @@ struct Solver {
         }
     }
 
+    // Dual initialisation from a primal guess (no reference counterpart):
+    // the optimality conditions give p = alpha1 K1/|K1| and q = alpha0 e/|e|
+    // wherever K1 = grad u - v resp. e = eps(v) is non-zero; with a guess the
+    // norms are clamped below by eps, so small residuals give small duals.
+    // Operators as in the gap (depth-aware table, frozen values outside A).
+    void dual_recover(const Level& L, T eps) {
+        const T a0 = (T)prm.alpha0, a1 = (T)prm.alpha1;
+        const long na = (long)L.active.size();
+        for (long ai = 0; ai < na; ++ai) {
+            const uint32_t i = L.active[ai];
+            State<T>& s = st[i];
+            T g[3], dv[3][3];
+            for (int a = 0; a < 3; ++a) {
+                const int32_t j = L.nba[ai * 6 + 2 * a];
+                const bool in = j >= 0 && L.pos[j] >= 0;
+                g[a] = j < 0 ? T(0) : (in ? st[j].u : ub[j]) - s.u;
+                for (int c = 0; c < 3; ++c)
+                    dv[c][a] = j < 0 ? T(0) : (in ? st[j].v[c] : vb[c][j]) - s.v[c];
+            }
+            const T k1[3] = {g[0] - s.v[0], g[1] - s.v[1], g[2] - s.v[2]};
+            const T e[6] = {dv[0][0], dv[1][1], dv[2][2], T(0.5) * (dv[0][1] + dv[1][0]),
+                            T(0.5) * (dv[0][2] + dv[2][0]), T(0.5) * (dv[1][2] + dv[2][1])};
+            const T n1 = std::sqrt(k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]);
+            T n0 = T(0);
+            for (int k = 0; k < 6; ++k) n0 += e[k] * e[k];
+            n0 = std::sqrt(n0);
+            const T s1 = a1 / std::max(n1, eps), s0 = a0 / std::max(n0, eps);
+            for (int c = 0; c < 3; ++c) s.p[c] = k1[c] * s1;
+            for (int k = 0; k < 6; ++k) s.q[k] = e[k] * s0;
+        }
+    }
+
     // One iteration = the five passes of tgv.cpp:158-230. frozen_* are the
     // level-start snapshots of run_level_frozen (tgv.cpp:263-298), or null.
     void iterate(const Level& L, int iters, std::vector<T>& un, std::vector<T>* vn, const T* fu,
@@ -719,19 +752,22 @@ struct Solver {
             Level lv;
             lv.active = by_depth[L];
             if (lv.active.empty()) continue;
+            const bool rec = prm.dual_eps > 0;
             if (init_uv) {
-                // warm start: compare against the cold init unless disabled
+                // warm start, kept only where it does not start from a larger gap
                 if (prm.warm_fallback) {
                     if (L > min_d) parent_init(lv.active, L);
                     build_level(lv, cubes, cm, dm, N);
+                    if (rec) dual_recover(lv, (T)prm.dual_eps);
                     double Pc, Dc;
                     gap(lv, Pc, Dc);
                     std::vector<State<T>> keep(lv.active.size());
                     for (size_t k = 0; k < lv.active.size(); ++k) keep[k] = st[lv.active[k]];
                     set_init(lv.active, init_uv);
+                    if (rec) dual_recover(lv, (T)prm.dual_eps);
                     double Pw, Dw;
                     gap(lv, Pw, Dw);
-                    if (Pw - Dw > Pc - Dc) {  // fall back to the cold init
+                    if (Pw - Dw > Pc - Dc) {  // back to the cold init
                         for (size_t k = 0; k < lv.active.size(); ++k) {
                             const uint32_t i = lv.active[k];
                             st[i] = keep[k];
@@ -742,10 +778,12 @@ struct Solver {
                 } else {
                     set_init(lv.active, init_uv);
                     build_level(lv, cubes, cm, dm, N);
+                    if (rec) dual_recover(lv, (T)prm.dual_eps);
                 }
             } else {
                 if (L > min_d) parent_init(lv.active, L);
                 build_level(lv, cubes, cm, dm, N);
+                if (rec) dual_recover(lv, (T)prm.dual_eps);
             }
             run(lv, nullptr, nullptr, 0, 0);
         }
@@ -911,6 +949,7 @@ typedef struct {
     int32_t iters, op_mode, chunk, max_iters;
     double gap_tol;
     int32_t threads, warm_fallback;
+    float dual_eps;
 } orc_params;
 
 void orc_morton_encode(uint32_t x, uint32_t y, uint32_t z, uint32_t* hi, uint32_t* lo) {
@@ -968,7 +1007,7 @@ static orc::Params to_params(const orc_params* p) {
     q.alpha0 = p->alpha0; q.alpha1 = p->alpha1; q.lambda = p->lambda;
     q.iters = p->iters; q.op_mode = p->op_mode; q.chunk = p->chunk > 0 ? p->chunk : 50;
     q.max_iters = p->max_iters; q.gap_tol = p->gap_tol;
-    q.threads = p->threads; q.warm_fallback = p->warm_fallback;
+    q.threads = p->threads; q.warm_fallback = p->warm_fallback; q.dual_eps = p->dual_eps;
     return q;
 }
 

@@ -112,3 +112,32 @@ def test_oc_matches_oracle(ctx, sphere):
     st, rep = ctx.solve_oc(sc.cubes, hist, leaves, prm)
     ref = O.tgv_minimize_oc(sc.cubes, hist, leaves, O.params(alpha0=2.0, alpha1=1.0, iters=60))
     assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("fallback", [0, 1])
+def test_warm_start_and_dual_recovery_bit_exact(ctx, sphere, fallback):
+    # an initial (u, v) field for every level, the dual recovered from it, and
+    # (fallback=1) the per-level choice between warm and cold by initial gap
+    sc, hist = sphere
+    rng = np.random.default_rng(7)
+    ref_cold, _, _, _ = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, iters=200, op_mode=1))
+    init = np.ascontiguousarray(ref_cold[:, :4] + rng.normal(0, 0.02, (len(hist), 4)).astype(np.float32))
+    prm = TGVParams(alpha0=2.0, alpha1=1.0, iters=120, precision=FP32, op_mode=OP_ADJ, dual_eps=1e-4,
+                    warm_fallback=fallback)
+    st, rep = ctx.solve(sc.cubes, hist, prm, init_uv=init)
+    ref, _, it, pd = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, iters=120, op_mode=1,
+                                                             dual_eps=1e-4, warm_fallback=fallback), init_uv=init)
+    assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))
+    if fallback:
+        assert 0 < sum(rep.level_warm[i] for i in range(rep.n_levels)) <= rep.n_levels
+
+
+def test_cold_dual_recovery_to_gap(ctx, sphere):
+    sc, hist = sphere
+    prm = TGVParams(alpha0=2.0, alpha1=1.0, precision=FP32, op_mode=OP_ADJ, gap_tol=1e-3, chunk=50, max_iters=4000,
+                    dual_eps=1e-4)
+    st, rep = ctx.solve(sc.cubes, hist, prm)
+    ref, _, it, pd = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, op_mode=1, gap_tol=1e-3, chunk=50,
+                                                             max_iters=4000, dual_eps=1e-4))
+    assert [l["iters"] for l in rep.levels()] == it.tolist()
+    assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))

@@ -353,6 +353,47 @@ __global__ void k_field_init(KArgs<S> a, int mode) {
     for (int c = 0; c < 4; ++c) IO::st(a.s[F_UB + c], g, x[c]);
 }
 
+// Dual initialisation from the level's initial (u, v) (no reference
+// counterpart): p = alpha1 K1 / max(|K1|, eps), q = alpha0 e / max(|e|, eps)
+// with K1 = grad u - v and e = eps(v) of the depth-aware operators, frozen
+// values outside the level. These are the optimality conditions wherever the
+// residuals are non-zero, so a good primal guess yields a good dual one.
+template <class S>
+__global__ void k_dual_recover(KArgs<S> a, typename Io<S>::T eps) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    auto in_lvl = [&](int j) { return (uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n; };
+    const T u = IO::ld(a.s[F_U], g);
+    const T v[3] = {IO::ld(a.s[F_VX], g), IO::ld(a.s[F_VY], g), IO::ld(a.s[F_VZ], g)};
+    T gr[3], dv[3][3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int j = a.nba[ax][g];
+        const bool in = j >= 0 && in_lvl(j);
+        const long sj = j >= 0 ? j : g;
+        gr[ax] = j < 0 ? T(0) : (in ? IO::ld(a.s[F_U], sj) : IO::ld(a.s[F_UB], sj)) - u;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            dv[c][ax] = j < 0 ? T(0) : (in ? IO::ld(a.s[F_VX + c], sj) : IO::ld(a.s[F_VBX + c], sj)) - v[c];
+    }
+    const T k1[3] = {gr[0] - v[0], gr[1] - v[1], gr[2] - v[2]};
+    const T e[6] = {dv[0][0], dv[1][1], dv[2][2], T(0.5) * (dv[0][1] + dv[1][0]), T(0.5) * (dv[0][2] + dv[2][0]),
+                    T(0.5) * (dv[1][2] + dv[2][1])};
+    const T n1 = sqrt(k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]);
+    T n0 = T(0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) n0 += e[k] * e[k];
+    n0 = sqrt(n0);
+    const T s1 = a.a1 / (n1 < eps ? eps : n1), s0 = a.a0 / (n0 < eps ? eps : n0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) IO::st(a.s[F_PX + c], g, k1[c] * s1);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) IO::st(a.s[F_Q0 + k], g, e[k] * s0);
+}
+
 // ---- energy and duality gap (no reference counterpart)
 // Per active cube i (neighbours outside the level frozen at u_bar / v_bar):
 //   P_i = lambda W |u - m| + alpha1 |K1(u, v)| + alpha0 |eps(v)|
@@ -731,21 +772,35 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         a.lvl_n = L.n;
         const unsigned grid = (L.n + 255) / 256;
         int warm_kept = 0;
+        // the level's initialisation; with dual_eps the dual follows from (u, v)
+        const bool rec = p->dual_eps > 0.0f && !leaves;
+        auto recover = [&] {
+            if (rec) k_dual_recover<S><<<grid, 256, 0, c->stream>>>(a, (typename Io<S>::T)p->dual_eps);
+        };
         if (init_uv) {
             if (p->warm_fallback) {
+                // keep the warm start only where it does not start from a larger gap
                 if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
+                recover();
                 double Pc, Dc, Pw, Dw;
                 eval_gap<S>(c, a, p, &Pc, &Dc);
                 k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 1);
+                recover();
                 eval_gap<S>(c, a, p, &Pw, &Dw);
-                if (Pw - Dw > Pc - Dc) k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 2);
-                else warm_kept = 1;
+                if (Pw - Dw > Pc - Dc) {
+                    k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 2);
+                    recover();
+                } else {
+                    warm_kept = 1;
+                }
             } else {
                 k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 0);
+                recover();
                 warm_kept = 1;
             }
-        } else if (li > 0) {
-            k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
+        } else {
+            if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
+            recover();
         }
         TGVX_HIP(hipGetLastError());
         int done = 0;
