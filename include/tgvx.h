@@ -92,7 +92,9 @@ typedef struct {
     float dual_eps;        /* > 0: after each level's initialisation set p = alpha1 K1/max(|K1|, dual_eps),
                               q = alpha0 e/max(|e|, dual_eps) from the initial (u, v) (K1 = grad u - v,
                               e = eps(v), depth-aware operators); 0: p = q = 0 as the reference */
-    int32_t reserved[4];
+    int32_t warm_start;    /* 1: every level starts from the context's network (tgvx_mlp_*): cold
+                              initialisation + network correction, with warm_fallback as above */
+    int32_t reserved[3];
 } tgvx_params;
 
 #define TGVX_MAX_LEVELS 22
@@ -164,6 +166,27 @@ int tgvx_problem_levels(tgvx_ctx* ctx, int32_t* depths, uint32_t* counts, int32_
  * each iteration kernel (dual, primal) from a separately bracketed run. */
 int tgvx_level_bench(tgvx_ctx* ctx, int32_t depth, const tgvx_params* params, int32_t iters,
                      double* total_ms, double* dual_ms, double* primal_ms);
+
+/* ---- warm-start network (no reference counterpart)
+ * 30 -> 64 -> 64 -> 4 ReLU MLP on FP32 matrix cores; parameters packed as
+ * W1[30][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4]. A context starts with
+ * seeded He-normal weights. Inputs are the 30 per-cube features of
+ * tgvx_problem_features, outputs the correction (du, dvx, dvy, dvz) to the
+ * cold initialisation. */
+#define TGVX_MLP_NPARAMS 6404
+int tgvx_mlp_set(tgvx_ctx* ctx, const float* params);
+int tgvx_mlp_get(tgvx_ctx* ctx, float* params);
+/* y[n][4] = network(x[n][30]) */
+int tgvx_mlp_forward(tgvx_ctx* ctx, const float* x, uint64_t n, float* y);
+/* mean squared error over n rows and 4 outputs, and its gradient (TGVX_MLP_NPARAMS) */
+int tgvx_mlp_loss_grad(tgvx_ctx* ctx, const float* x, const float* t, uint64_t n, float* grad, double* loss);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over shuffled mini-batches; mean loss per epoch */
+int tgvx_mlp_train(tgvx_ctx* ctx, const float* x, const float* t, uint64_t n, int32_t epochs, int32_t batch,
+                   float lr, uint64_t seed, float* epoch_loss);
+/* Features (n_L x 30) of the resident problem's level at `depth` from its
+ * current state, with that level's current (u, v) (n_L x 4) and the input
+ * index of each row: after a solve, the training pairs of the warm start. */
+int tgvx_problem_features(tgvx_ctx* ctx, int32_t depth, float* x_out, float* uv_out, uint32_t* index_out);
 
 #ifdef __cplusplus
 }

@@ -241,3 +241,43 @@ def time_level(cubes, hists, prm, depth, iters):
     na = C.c_uint64()
     t = lib().orc_time_level(_p(cubes), len(cubes), _p(h), h.shape[1], C.byref(prm), depth, iters, C.byref(na))
     return t, na.value
+
+
+# ---------------------------------------------------------------- warm-start network
+MLP_NP = 6404
+
+
+def _mlp_lib():
+    L = lib()
+    if not hasattr(L, "_mlp_ready"):
+        vp = C.c_void_p
+        L.orc_mlp_forward.argtypes = [vp, vp, C.c_int32, vp]
+        L.orc_mlp_loss_grad.argtypes = [vp, vp, vp, C.c_int32, vp]
+        L.orc_mlp_loss_grad.restype = C.c_double
+        L.orc_mlp_adam.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double]
+        L._mlp_ready = True
+    return L
+
+
+def mlp_forward(w, x):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros((len(x), 4))
+    _mlp_lib().orc_mlp_forward(_p(w), _p(x), len(x), _p(y))
+    return y
+
+
+def mlp_loss_grad(w, x, t):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    g = np.zeros(MLP_NP)
+    loss = _mlp_lib().orc_mlp_loss_grad(_p(w), _p(x), _p(t), len(x), _p(g))
+    return loss, g
+
+
+def mlp_adam(w, m, v, g, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    for a in (w, m, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    _mlp_lib().orc_mlp_adam(_p(w), _p(m), _p(v), _p(g), step, lr, b1, b2, eps)

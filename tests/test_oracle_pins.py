@@ -529,3 +529,43 @@ def test_depth_aware_table_brute_force():
                 if f % 2 == 0 and want >= 0 and lst[want][0] == level:
                     assert (want, f) not in seen
                     seen.add((want, f))
+
+
+# ---------------------------------------------------------------- network
+
+def test_mlp_gradient_finite_differences():
+    rng = np.random.default_rng(4)
+    w = rng.normal(0, 0.3, O.MLP_NP)
+    x = rng.normal(0, 1, (7, 30))
+    t = rng.normal(0, 1, (7, 4))
+    loss, g = O.mlp_loss_grad(w, x, t)
+    for i in list(range(0, O.MLP_NP, 97)) + [O.MLP_NP - 1, 1920, 1983]:
+        h = 1e-6
+        wp, wm = w.copy(), w.copy()
+        wp[i] += h
+        wm[i] -= h
+        fd = (O.mlp_loss_grad(wp, x, t)[0] - O.mlp_loss_grad(wm, x, t)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_mlp_forward_matches_numpy():
+    rng = np.random.default_rng(5)
+    w = rng.normal(0, 0.3, O.MLP_NP)
+    x = rng.normal(0, 1, (9, 30))
+    W1, b1 = w[:1920].reshape(30, 64), w[1920:1984]
+    W2, b2 = w[1984:6080].reshape(64, 64), w[6080:6144]
+    W3, b3 = w[6144:6400].reshape(64, 4), w[6400:]
+    h = np.maximum(x @ W1 + b1, 0)
+    h = np.maximum(h @ W2 + b2, 0)
+    np.testing.assert_allclose(O.mlp_forward(w, x), h @ W3 + b3, rtol=1e-12, atol=1e-12)
+
+
+def test_adam_step_closed_form():
+    rng = np.random.default_rng(6)
+    w = rng.normal(size=O.MLP_NP)
+    g = rng.normal(size=O.MLP_NP)
+    m, v = np.zeros(O.MLP_NP), np.zeros(O.MLP_NP)
+    w0 = w.copy()
+    O.mlp_adam(w, m, v, g, 1, 1e-3)
+    # first step with zero moments: w -= lr * sign(g) * |g| / (|g| + eps)
+    np.testing.assert_allclose(w, w0 - 1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-12)

@@ -192,6 +192,47 @@ class Context:
         self._check(self._L.tgvx_problem_levels(self.h, N.ptr(d), N.ptr(c), C.byref(n)))
         return [(int(d[i]), int(c[i])) for i in range(n.value)]
 
+    # ---- warm-start network
+    def mlp_set(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == N.MLP_NPARAMS
+        self._check(self._L.tgvx_mlp_set(self.h, N.ptr(w)))
+
+    def mlp_get(self):
+        w = np.zeros(N.MLP_NPARAMS, dtype=np.float32)
+        self._check(self._L.tgvx_mlp_get(self.h, N.ptr(w)))
+        return w
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((len(x), 4), dtype=np.float32)
+        self._check(self._L.tgvx_mlp_forward(self.h, N.ptr(x), len(x), N.ptr(y)))
+        return y
+
+    def mlp_loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(N.MLP_NPARAMS, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self._L.tgvx_mlp_loss_grad(self.h, N.ptr(x), N.ptr(t), len(x), N.ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def mlp_train(self, x, t, epochs=10, batch=4096, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        out = np.zeros(epochs, dtype=np.float32)
+        self._check(self._L.tgvx_mlp_train(self.h, N.ptr(x), N.ptr(t), len(x), epochs, batch, lr, seed, N.ptr(out)))
+        return out
+
+    def features(self, depth):
+        """(features [n_L, 30], current (u, v) [n_L, 4], input indices [n_L]) of one level."""
+        n = dict(self.levels())[depth]
+        x = np.zeros((n, 30), dtype=np.float32)
+        uv = np.zeros((n, 4), dtype=np.float32)
+        idx = np.zeros(n, dtype=np.uint32)
+        self._check(self._L.tgvx_problem_features(self.h, depth, N.ptr(x), N.ptr(uv), N.ptr(idx)))
+        return x, uv, idx
+
     def level_bench(self, depth, params, iters):
         t, d, p = C.c_double(), C.c_double(), C.c_double()
         self._check(self._L.tgvx_level_bench(self.h, depth, C.byref(params), iters, C.byref(t), C.byref(d),

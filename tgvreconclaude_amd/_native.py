@@ -31,7 +31,7 @@ class Params(C.Structure):
                 ("iters", C.c_int32), ("precision", C.c_int32), ("op_mode", C.c_int32),
                 ("chunk", C.c_int32), ("max_iters", C.c_int32), ("gap_tol", C.c_double),
                 ("warm_fallback", C.c_int32), ("use_graph", C.c_int32), ("block", C.c_int32),
-                ("dual_eps", C.c_float), ("reserved", C.c_int32 * 4)]
+                ("dual_eps", C.c_float), ("warm_start", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Report(C.Structure):
@@ -66,8 +66,10 @@ TGVX_SYMBOLS = [
     "tgvx_default_params", "tgvx_create", "tgvx_destroy", "tgvx_last_error", "tgvx_device_info",
     "tgvx_histograms", "tgvx_mip_build", "tgvx_median_init", "tgvx_solve", "tgvx_solve_oc",
     "tgvx_problem_load", "tgvx_problem_solve", "tgvx_problem_state", "tgvx_problem_gap",
-    "tgvx_problem_levels", "tgvx_level_bench",
+    "tgvx_problem_levels", "tgvx_level_bench", "tgvx_mlp_set", "tgvx_mlp_get", "tgvx_mlp_forward",
+    "tgvx_mlp_loss_grad", "tgvx_mlp_train", "tgvx_problem_features",
 ]
+MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
                  "tgvs_balance", "tgvs_treetop", "tgvs_free"]
 
@@ -108,6 +110,12 @@ def lib():
         L.tgvx_problem_levels.argtypes = [vp, vp, vp, C.POINTER(i32)]
         L.tgvx_level_bench.argtypes = [vp, i32, C.POINTER(Params), i32, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.tgvx_mlp_set.argtypes = [vp, vp]
+        L.tgvx_mlp_get.argtypes = [vp, vp]
+        L.tgvx_mlp_forward.argtypes = [vp, vp, u64, vp]
+        L.tgvx_mlp_loss_grad.argtypes = [vp, vp, vp, u64, vp, C.POINTER(C.c_double)]
+        L.tgvx_mlp_train.argtypes = [vp, vp, vp, u64, i32, i32, C.c_float, C.c_uint64, vp]
+        L.tgvx_problem_features.argtypes = [vp, i32, vp, vp, vp]
         for name in TGVX_SYMBOLS:
             f = getattr(L, name)
             if f.restype is C.c_int:  # default

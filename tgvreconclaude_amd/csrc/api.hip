@@ -147,6 +147,46 @@ int tgvx_level_bench(tgvx_ctx* c, int32_t depth, const tgvx_params* p, int32_t i
     });
 }
 
+int tgvx_mlp_set(tgvx_ctx* c, const float* w) {
+    return guarded(c, [&] {
+        if (!w) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        tgvx::mlp_set(c, w);
+    });
+}
+
+int tgvx_mlp_get(tgvx_ctx* c, float* w) {
+    return guarded(c, [&] {
+        if (!w) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        tgvx::mlp_get(c, w);
+    });
+}
+
+int tgvx_mlp_forward(tgvx_ctx* c, const float* x, uint64_t n, float* y) {
+    return guarded(c, [&] {
+        if (n && (!x || !y)) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        tgvx::mlp_forward(c, x, n, y);
+    });
+}
+
+int tgvx_mlp_loss_grad(tgvx_ctx* c, const float* x, const float* t, uint64_t n, float* grad, double* loss) {
+    return guarded(c, [&] {
+        if (!x || !t) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        tgvx::mlp_loss_grad(c, x, t, n, grad, loss);
+    });
+}
+
+int tgvx_mlp_train(tgvx_ctx* c, const float* x, const float* t, uint64_t n, int32_t epochs, int32_t batch, float lr,
+                   uint64_t seed, float* epoch_loss) {
+    return guarded(c, [&] {
+        if (!x || !t) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        tgvx::mlp_train(c, x, t, n, epochs, batch, lr, seed, epoch_loss);
+    });
+}
+
+int tgvx_problem_features(tgvx_ctx* c, int32_t depth, float* x_out, float* uv_out, uint32_t* index_out) {
+    return guarded(c, [&] { tgvx::problem_features(c, depth, x_out, uv_out, index_out); });
+}
+
 static int solve_common(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
                         const tgvx_params* p, const float* init_uv, const tgvx_treetop_leaf* leaves,
                         uint64_t nleaves, void* state_out, tgvx_report* rep) {

@@ -70,24 +70,6 @@ __device__ inline void clip6(T* q, T r) {
         for (int i = 0; i < 6; ++i) q[i] *= s;
     }
 }
-// Per-cube constant word: the vote total W (bits 0-23), the weighted-median
-// bin k (bits 24-28) and "the depth-aware +x / +y / +z neighbour exists"
-// (bits 29-31).
-struct InfoWord {
-    static constexpr uint32_t kWMax = (1u << 24) - 1;
-    __host__ __device__ static uint32_t W(uint32_t w) { return w & kWMax; }
-    __host__ __device__ static int bin(uint32_t w) { return (int)((w >> 24) & 31u); }
-    __host__ __device__ static bool has(uint32_t w, int ax) { return (w >> (29 + ax)) & 1u; }
-};
-
-// median_indicator (histogram.cpp:21-31) with nb bins, from the median bin:
-// the centre -1 + (k + 0.5) / (nb / 2) of bin k, 0 for an empty histogram.
-template <class T>
-__host__ __device__ inline T median_of(uint32_t w, int nb) {
-    if (InfoWord::W(w) == 0u) return T(0);
-    return T(-1) + (T(InfoWord::bin(w)) + T(0.5)) / (T(0.5) * T(nb));
-}
-
 // prox_data (tgv.cpp:75-94): shrink towards the weighted median by
 // tau*lambda*W; the median depends only on the histogram, so it comes from
 // the packed word instead of a scan of the bins every iteration.
@@ -578,7 +560,7 @@ __device__ inline int pair_lookup(const uint64_t* dc, const int32_t* dd, const i
 __global__ void k_tables_geo(const uint64_t* __restrict__ codes, const int32_t* __restrict__ depth, uint32_t n,
                              const uint64_t* __restrict__ dc, const int32_t* __restrict__ dd,
                              const int32_t* __restrict__ dp, uint32_t U, int32_t* nba0, int32_t* nba1, int32_t* nba2,
-                             int32_t* parent, uint32_t* info) {
+                             int32_t* nbm0, int32_t* nbm1, int32_t* nbm2, int32_t* parent, uint32_t* info) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const uint64_t c = codes[g];
@@ -595,6 +577,17 @@ __global__ void k_tables_geo(const uint64_t* __restrict__ codes, const int32_t* 
         }
         out[ax][g] = j;
         if (j >= 0) flags |= 1u << (29 + ax);
+    }
+    // -x, -y, -z faces (read only by the warm-start features)
+    int32_t* outm[3] = {nbm0, nbm1, nbm2};
+    for (int ax = 0; ax < 3; ++ax) {
+        int j = -1;
+        const uint64_t k = shadow ? kNone : face_nb(c, 2 * ax + 1, d);
+        if (k != kNone) {
+            j = pair_lookup(dc, dd, dp, U, k, d);
+            if (j < 0 && d > 0) j = pair_lookup(dc, dd, dp, U, ancestor(k, d - 1), d - 1);
+        }
+        outm[ax][g] = j;
     }
     info[g] |= flags;
     parent[g] = d > 0 ? pair_lookup(dc, dd, dp, U, ancestor(c, d - 1), d - 1) : -1;
@@ -748,6 +741,8 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
     tgvx_report r;
     std::memset(&r, 0, sizeof(r));
     const double t0 = now_ms();
+    if (p->warm_start && (init_uv || leaves))
+        throw Error(TGVX_E_ARG, "warm_start excludes an explicit initial field and the frozen-border solve");
     if (init_uv) {
         P.d_init.ensure(sizeof(float) * 4 * P.n);
         std::vector<float> perm_init(4 * P.n);
@@ -777,7 +772,13 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         auto recover = [&] {
             if (rec) k_dual_recover<S><<<grid, 256, 0, c->stream>>>(a, (typename Io<S>::T)p->dual_eps);
         };
-        if (init_uv) {
+        const bool field = init_uv || p->warm_start;
+        if (p->warm_start) {
+            // the network needs the parent level's final state: features now
+            warm_init(c, L.off, L.n);
+            a.init = P.d_init.as<float>();
+        }
+        if (field) {
             if (p->warm_fallback) {
                 // keep the warm start only where it does not start from a larger gap
                 if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
@@ -1046,8 +1047,16 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     up(d_dd, ddepth.data(), 4 * std::max<size_t>(ddepth.size(), 1));
     up(d_dp, dpos.data(), 4 * std::max<size_t>(dpos.size(), 1));
     up(P.d_info, info.data(), 4 * std::max<uint64_t>(n, 1));
+    {
+        // histograms in position order (read by the warm-start features)
+        std::vector<uint32_t> pb((size_t)std::max<uint64_t>(n, 1) * nb, 0u);
+        for (uint64_t g = 0; g < n; ++g)
+            std::memcpy(&pb[(size_t)g * nb], bins + (size_t)P.perm[g] * nb, sizeof(uint32_t) * nb);
+        up(P.d_bins, pb.data(), 4 * pb.size());
+    }
     for (int ax = 0; ax < 3; ++ax) {
         P.d_nba[ax].ensure(4 * std::max<uint64_t>(n, 1));
+        P.d_nbam[ax].ensure(4 * std::max<uint64_t>(n, 1));
         P.d_nbt[ax].ensure(4 * std::max<uint64_t>(n, 1));
     }
     P.d_parent_a.ensure(4 * std::max<uint64_t>(n, 1));
@@ -1057,6 +1066,8 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
                                                   d_dc.as<uint64_t>(), d_dd.as<int32_t>(), d_dp.as<int32_t>(),
                                                   (uint32_t)dcode.size(), P.d_nba[0].as<int32_t>(),
                                                   P.d_nba[1].as<int32_t>(), P.d_nba[2].as<int32_t>(),
+                                                  P.d_nbam[0].as<int32_t>(), P.d_nbam[1].as<int32_t>(),
+                                                  P.d_nbam[2].as<int32_t>(),
                                                   P.d_parent_a.as<int32_t>(), P.d_info.as<uint32_t>());
         for (int ax = 0; ax < 3; ++ax) k_fill_i32<<<grid, 256, 0, c->stream>>>(P.d_nbt[ax].as<int32_t>(), (uint32_t)n, -1);
         k_transpose<<<grid, 256, 0, c->stream>>>(P.d_nba[0].as<int32_t>(), P.d_nba[1].as<int32_t>(),

@@ -88,6 +88,24 @@ struct Io<bf16s> {
     }
 };
 
+// Per-cube constant word: the vote total W (bits 0-23), the weighted-median
+// bin k (bits 24-28) and "the depth-aware +x / +y / +z neighbour exists"
+// (bits 29-31).
+struct InfoWord {
+    static constexpr uint32_t kWMax = (1u << 24) - 1;
+    __host__ __device__ static uint32_t W(uint32_t w) { return w & kWMax; }
+    __host__ __device__ static int bin(uint32_t w) { return (int)((w >> 24) & 31u); }
+    __host__ __device__ static bool has(uint32_t w, int ax) { return (w >> (29 + ax)) & 1u; }
+};
+
+// median_indicator (histogram.cpp:21-31) with nb bins, from the median bin:
+// the centre -1 + (k + 0.5) / (nb / 2) of bin k, 0 for an empty histogram.
+template <class T>
+__host__ __device__ inline T median_of(uint32_t w, int nb) {
+    if (InfoWord::W(w) == 0u) return T(0);
+    return T(-1) + (T(InfoWord::bin(w)) + T(0.5)) / (T(0.5) * T(nb));
+}
+
 // ---------------------------------------------------------------- launch
 // Workgroups are dispatched to the 8 XCDs round-robin by id; remap so that
 // each XCD gets one contiguous run of tiles (neighbouring leaves share its L2).

@@ -70,7 +70,9 @@ struct Problem {
     DevBuf d_parent;                 // int32: position of the cube holding the parent code, or -1
     DevBuf d_info;                   // uint32 packed vote total / median bin / flags
     DevBuf d_nba[3];                 // int32 depth-aware +x, +y, +z neighbours (exact-adjoint mode)
+    DevBuf d_nbam[3];                // int32 depth-aware -x, -y, -z neighbours (warm-start features)
     DevBuf d_nbt[3];                 // int32 the level's cube whose depth-aware +a neighbour this is, or -1
+    DevBuf d_bins;                   // uint32 histograms, position order
     DevBuf d_parent_a;               // int32 cube exactly one level coarser at the parent code, or -1
     DevBuf d_state[17];              // u vx vy vz px py pz q0..q5 ub vbx vby vbz
     DevBuf d_save[4];                // scratch copy of u, v of one level
@@ -90,6 +92,9 @@ struct tgvx_ctx {
     tgvx::Problem prob;
     // scratch for histogram calls
     tgvx::DevBuf h_cubes, h_bins, h_pyr, h_meta;
+    // warm-start network: weights, Adam moments, activations and data
+    tgvx::DevBuf mlp_w, mlp_m, mlp_v, mlp_x, mlp_t, mlp_y, mlp_rows, mlp_part;
+    int mlp_step = 0;
 };
 
 namespace tgvx {
@@ -102,6 +107,15 @@ void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out);
 void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* P, double* D);
 void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
                  double* primal_ms);
+// mlp.hip
+void mlp_set(tgvx_ctx* c, const float* w);
+void mlp_get(tgvx_ctx* c, float* w);
+void mlp_forward(tgvx_ctx* c, const float* x, uint64_t n, float* y);
+void mlp_loss_grad(tgvx_ctx* c, const float* x, const float* t, uint64_t n, float* grad, double* loss);
+void mlp_train(tgvx_ctx* c, const float* x, const float* t, uint64_t n, int epochs, int batch, float lr,
+               uint64_t seed, float* epoch_loss);
+void warm_init(tgvx_ctx* c, uint32_t off, uint32_t n);
+void problem_features(tgvx_ctx* c, int depth, float* x_out, float* uv_out, uint32_t* index_out);
 // hist.hip
 void histograms(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views, int nv,
                 const float origin[3], float r_root, int nb, uint32_t* bins_out);
