@@ -94,7 +94,9 @@ typedef struct {
                               e = eps(v), depth-aware operators); 0: p = q = 0 as the reference */
     int32_t warm_start;    /* 1: every level starts from the context's network (tgvx_mlp_*): cold
                               initialisation + network correction, with warm_fallback as above */
-    int32_t reserved[3];
+    int32_t warm_dual;     /* with an initial field and dual_eps > 0: 0 recovers p, q from the field,
+                              1 keeps the p, q recovered from the cold initialisation */
+    int32_t reserved[2];
 } tgvx_params;
 
 #define TGVX_MAX_LEVELS 22

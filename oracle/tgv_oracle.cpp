@@ -265,6 +265,7 @@ struct Params {
     int32_t threads;      // OpenMP threads for the passes (1 = reference)
     int32_t warm_fallback;  // with an init field: keep cold init where it has the lower gap
     float dual_eps;         // with an init field: > 0 recovers p, q from (u, v), see dual_recover
+    int32_t warm_dual;      // with an init field: 1 keeps the dual recovered from the cold init
 };
 
 using CodeMap = std::unordered_map<uint64_t, uint32_t>;
@@ -548,12 +549,14 @@ struct Solver {
             for (int c = 0; c < 3; ++c) vb[c][i] = st[i].v[c];
         }
     }
-    void set_init(const std::vector<uint32_t>& active, const float* init_uv) {
+    void set_init(const std::vector<uint32_t>& active, const float* init_uv, bool keep_dual = false) {
         for (uint32_t i : active) {
             st[i].u = (T)init_uv[(size_t)i * 4];
             for (int c = 0; c < 3; ++c) st[i].v[c] = (T)init_uv[(size_t)i * 4 + 1 + c];
-            for (int c = 0; c < 3; ++c) st[i].p[c] = T(0);
-            for (int k = 0; k < 6; ++k) st[i].q[k] = T(0);
+            if (!keep_dual) {
+                for (int c = 0; c < 3; ++c) st[i].p[c] = T(0);
+                for (int k = 0; k < 6; ++k) st[i].q[k] = T(0);
+            }
             ub[i] = st[i].u;
             for (int c = 0; c < 3; ++c) vb[c][i] = st[i].v[c];
         }
@@ -753,6 +756,7 @@ struct Solver {
             lv.active = by_depth[L];
             if (lv.active.empty()) continue;
             const bool rec = prm.dual_eps > 0;
+            const bool keep = rec && prm.warm_dual;  // dual from the cold init
             if (init_uv) {
                 // warm start, kept only where it does not start from a larger gap
                 if (prm.warm_fallback) {
@@ -761,24 +765,29 @@ struct Solver {
                     if (rec) dual_recover(lv, (T)prm.dual_eps);
                     double Pc, Dc;
                     gap(lv, Pc, Dc);
-                    std::vector<State<T>> keep(lv.active.size());
-                    for (size_t k = 0; k < lv.active.size(); ++k) keep[k] = st[lv.active[k]];
-                    set_init(lv.active, init_uv);
-                    if (rec) dual_recover(lv, (T)prm.dual_eps);
+                    std::vector<State<T>> keepst(lv.active.size());
+                    for (size_t k = 0; k < lv.active.size(); ++k) keepst[k] = st[lv.active[k]];
+                    set_init(lv.active, init_uv, keep);
+                    if (rec && !keep) dual_recover(lv, (T)prm.dual_eps);
                     double Pw, Dw;
                     gap(lv, Pw, Dw);
                     if (Pw - Dw > Pc - Dc) {  // back to the cold init
                         for (size_t k = 0; k < lv.active.size(); ++k) {
                             const uint32_t i = lv.active[k];
-                            st[i] = keep[k];
+                            st[i] = keepst[k];
                             ub[i] = st[i].u;
                             for (int c = 0; c < 3; ++c) vb[c][i] = st[i].v[c];
                         }
                     }
                 } else {
-                    set_init(lv.active, init_uv);
+                    if (keep) {
+                        if (L > min_d) parent_init(lv.active, L);
+                        build_level(lv, cubes, cm, dm, N);
+                        dual_recover(lv, (T)prm.dual_eps);
+                    }
+                    set_init(lv.active, init_uv, keep);
                     build_level(lv, cubes, cm, dm, N);
-                    if (rec) dual_recover(lv, (T)prm.dual_eps);
+                    if (rec && !keep) dual_recover(lv, (T)prm.dual_eps);
                 }
             } else {
                 if (L > min_d) parent_init(lv.active, L);
@@ -950,6 +959,7 @@ typedef struct {
     double gap_tol;
     int32_t threads, warm_fallback;
     float dual_eps;
+    int32_t warm_dual;
 } orc_params;
 
 void orc_morton_encode(uint32_t x, uint32_t y, uint32_t z, uint32_t* hi, uint32_t* lo) {
@@ -1007,7 +1017,7 @@ static orc::Params to_params(const orc_params* p) {
     q.alpha0 = p->alpha0; q.alpha1 = p->alpha1; q.lambda = p->lambda;
     q.iters = p->iters; q.op_mode = p->op_mode; q.chunk = p->chunk > 0 ? p->chunk : 50;
     q.max_iters = p->max_iters; q.gap_tol = p->gap_tol;
-    q.threads = p->threads; q.warm_fallback = p->warm_fallback; q.dual_eps = p->dual_eps;
+    q.threads = p->threads; q.warm_fallback = p->warm_fallback; q.dual_eps = p->dual_eps; q.warm_dual = p->warm_dual;
     return q;
 }
 

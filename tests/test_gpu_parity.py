@@ -114,8 +114,8 @@ def test_oc_matches_oracle(ctx, sphere):
     assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("fallback", [0, 1])
-def test_warm_start_and_dual_recovery_bit_exact(ctx, sphere, fallback):
+@pytest.mark.parametrize("fallback,warm_dual", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_warm_start_and_dual_recovery_bit_exact(ctx, sphere, fallback, warm_dual):
     # an initial (u, v) field for every level, the dual recovered from it, and
     # (fallback=1) the per-level choice between warm and cold by initial gap
     sc, hist = sphere
@@ -123,10 +123,11 @@ def test_warm_start_and_dual_recovery_bit_exact(ctx, sphere, fallback):
     ref_cold, _, _, _ = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, iters=200, op_mode=1))
     init = np.ascontiguousarray(ref_cold[:, :4] + rng.normal(0, 0.02, (len(hist), 4)).astype(np.float32))
     prm = TGVParams(alpha0=2.0, alpha1=1.0, iters=120, precision=FP32, op_mode=OP_ADJ, dual_eps=1e-4,
-                    warm_fallback=fallback)
+                    warm_fallback=fallback, warm_dual=warm_dual)
     st, rep = ctx.solve(sc.cubes, hist, prm, init_uv=init)
     ref, _, it, pd = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, iters=120, op_mode=1,
-                                                             dual_eps=1e-4, warm_fallback=fallback), init_uv=init)
+                                                             dual_eps=1e-4, warm_fallback=fallback,
+                                                             warm_dual=warm_dual), init_uv=init)
     assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))
     if fallback:
         assert 0 < sum(rep.level_warm[i] for i in range(rep.n_levels)) <= rep.n_levels

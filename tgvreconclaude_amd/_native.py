@@ -31,7 +31,8 @@ class Params(C.Structure):
                 ("iters", C.c_int32), ("precision", C.c_int32), ("op_mode", C.c_int32),
                 ("chunk", C.c_int32), ("max_iters", C.c_int32), ("gap_tol", C.c_double),
                 ("warm_fallback", C.c_int32), ("use_graph", C.c_int32), ("block", C.c_int32),
-                ("dual_eps", C.c_float), ("warm_start", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("dual_eps", C.c_float), ("warm_start", C.c_int32), ("warm_dual", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class Report(C.Structure):

@@ -307,9 +307,10 @@ __global__ void k_parent_init(KArgs<S> a) {
 }
 
 // Warm start: u, v from a given field; mode 1 also saves the current u, v of
-// the level first, mode 2 restores the saved u, v instead.
+// the level first, mode 2 restores the saved u, v instead; p, q are zeroed,
+// or left as they are with keep_dual.
 template <class S>
-__global__ void k_field_init(KArgs<S> a, int mode) {
+__global__ void k_field_init(KArgs<S> a, int mode, int keep_dual) {
     using IO = Io<S>;
     using T = typename IO::T;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -329,8 +330,10 @@ __global__ void k_field_init(KArgs<S> a, int mode) {
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) IO::st(a.s[F_U + c], g, x[c]);
+    if (!keep_dual) {
 #pragma unroll
-    for (int f = F_PX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
+        for (int f = F_PX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
+    }
 #pragma unroll
     for (int c = 0; c < 4; ++c) IO::st(a.s[F_UB + c], g, x[c]);
 }
@@ -779,24 +782,30 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             a.init = P.d_init.as<float>();
         }
         if (field) {
+            // warm_dual: keep the dual recovered from the cold initialisation
+            const int keep = rec && p->warm_dual ? 1 : 0;
             if (p->warm_fallback) {
                 // keep the warm start only where it does not start from a larger gap
                 if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
                 recover();
                 double Pc, Dc, Pw, Dw;
                 eval_gap<S>(c, a, p, &Pc, &Dc);
-                k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 1);
-                recover();
+                k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 1, keep);
+                if (!keep) recover();
                 eval_gap<S>(c, a, p, &Pw, &Dw);
                 if (Pw - Dw > Pc - Dc) {
-                    k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 2);
-                    recover();
+                    k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 2, keep);
+                    if (!keep) recover();
                 } else {
                     warm_kept = 1;
                 }
             } else {
-                k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 0);
-                recover();
+                if (keep) {
+                    if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
+                    recover();
+                }
+                k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 0, keep);
+                if (!keep) recover();
                 warm_kept = 1;
             }
         } else {

@@ -1,0 +1,192 @@
+"""Learned warm start: training-data generation, training and evaluation.
+
+Training pairs come from this solver itself: seeded CSG scenes (24 ring views
+of 640x480, octree depth <= 9) are solved to relative gap 1e-4 on the GPU; for
+every level, the features of each cube (computed with the parent level's final
+state, exactly as at inference) are paired with the cube's converged (u, v)
+minus its cold initialisation. The network is trained on the GPU with the
+library's own forward / backward / Adam kernels. Evaluation solves held-out
+depth-10 scenes (24 ring views of 1280x960) to the same gap three ways: cold
+(the reference initialisation, p = q = 0), cold with the dual recovered from
+the initial (u, v), and warm (network + recovered dual, with the per-level
+fallback to cold); each must reach the gap.
+
+    python -m tgvreconclaude_amd.warmstart train --scenes 150 --out weights.npy
+    python -m tgvreconclaude_amd.warmstart eval --scenes 50 --weights weights.npy
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import FP32, OP_ADJ, Context, TGVParams, scene
+
+ALPHA0, ALPHA1 = 2.0, 1.0
+GAP = 1e-4
+DUAL_EPS = 1e-4
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart_mlp.npy")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def params(**kw):
+    base = dict(alpha0=ALPHA0, alpha1=ALPHA1, precision=FP32, op_mode=OP_ADJ, gap_tol=GAP, chunk=50,
+                max_iters=40000)
+    base.update(kw)
+    return TGVParams(**base)
+
+
+def train_scene(seed, max_depth=9, W=640, H=480):
+    return scene.csg_scene(seed, views=24, W=W, H=H, max_depth=max_depth)
+
+
+def eval_scene(seed, max_depth=10, W=1280, H=960):
+    return scene.csg_scene(seed, views=24, W=W, H=H, max_depth=max_depth)
+
+
+def harvest(ctx, sc, cap, rng):
+    """(features, residual targets) of every level of one scene, solved to the gap."""
+    hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
+    ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+    rep = ctx.run(params(dual_eps=DUAL_EPS))
+    xs, ts = [], []
+    for depth, n in ctx.levels():
+        x, uv, _ = ctx.features(depth)
+        if n > cap:
+            keep = rng.choice(n, cap, replace=False)
+            x, uv = x[keep], uv[keep]
+        xs.append(x)
+        ts.append(uv - x[:, 20:24])
+    return np.concatenate(xs), np.concatenate(ts), rep
+
+
+def make_training_set(ctx, n_scenes, seed0=1000, cap=4000):
+    rng = np.random.default_rng(seed0)
+    X, T = [], []
+    t0 = time.time()
+    for k in range(n_scenes):
+        sc = train_scene(seed0 + k)
+        x, t, rep = harvest(ctx, sc, cap, rng)
+        X.append(x)
+        T.append(t)
+        if k % 10 == 0:
+            log(f"[warmstart] scene {k}/{n_scenes}: {len(sc.cubes)} cubes, {len(x)} pairs, "
+                f"solve {rep.solve_ms:.0f} ms ({time.time() - t0:.0f} s)")
+    return np.concatenate(X), np.concatenate(T)
+
+
+def train(ctx, X, T, epochs=12, batch=4096, lr=1e-3, seed=0):
+    ctx.mlp_set(initial_weights(seed))
+    curve = ctx.mlp_train(X, T, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return ctx.mlp_get(), curve
+
+
+def initial_weights(seed):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(6404, dtype=np.float32)
+    w[:1920] = rng.normal(0, np.sqrt(2 / 30), 1920)
+    w[1984:6080] = rng.normal(0, np.sqrt(2 / 64), 4096)
+    w[6144:6400] = rng.normal(0, 0.01, 256)
+    return w
+
+
+def solve_variants(ctx, sc, weights, diagnostics=False):
+    hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
+    ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+    out = {"cubes": int(len(sc.cubes))}
+    runs = {"cold": params(), "cold_dual": params(dual_eps=DUAL_EPS),
+            "warm": params(dual_eps=DUAL_EPS, warm_start=1, warm_fallback=1)}
+    if diagnostics:
+        # the network without the fallback rule, and the converged solution
+        # itself as the initial (u, v): what a perfect predictor could gain
+        runs["warm_no_fallback"] = params(dual_eps=DUAL_EPS, warm_start=1, warm_fallback=0)
+        runs["warm_cold_dual"] = params(dual_eps=DUAL_EPS, warm_start=1, warm_fallback=1, warm_dual=1)
+        runs["warm_cold_dual_no_fallback"] = params(dual_eps=DUAL_EPS, warm_start=1, warm_fallback=0, warm_dual=1)
+        runs["solution_init"] = params(dual_eps=DUAL_EPS, warm_fallback=0)
+    ctx.mlp_set(weights)
+    ctx.run(runs["cold"])  # untimed: builds this problem's iteration graphs
+    st = ctx.state()
+    sol = np.ascontiguousarray(np.concatenate([st["u"][:, None], st["v"]], axis=1))
+    for name, p in runs.items():
+        rep = ctx.run(p, init_uv=sol if name == "solution_init" else None)
+        lv = rep.levels()
+        out[name] = {"ms": rep.solve_ms, "iters": [l["iters"] for l in lv],
+                     "reached": bool(all(l["rel_gap"] <= GAP for l in lv)),
+                     "warm_levels": [l["warm"] for l in lv]}
+    return out
+
+
+def evaluate(ctx, weights, n_scenes, seed0=900000, diagnostics=False):
+    res = []
+    for k in range(n_scenes):
+        sc = eval_scene(seed0 + k)
+        r = solve_variants(ctx, sc, weights, diagnostics)
+        r["seed"] = seed0 + k
+        res.append(r)
+        log(f"[warmstart] eval {k}: {r['cubes']} cubes  cold {r['cold']['ms']:.0f} ms  "
+            f"cold+dual {r['cold_dual']['ms']:.0f} ms  warm {r['warm']['ms']:.0f} ms")
+    cold = np.array([r["cold"]["ms"] for r in res])
+    cd = np.array([r["cold_dual"]["ms"] for r in res])
+    warm = np.array([r["warm"]["ms"] for r in res])
+    summary = {"scenes": n_scenes, "gap": GAP,
+               "all_reached": bool(all(r[v]["reached"] for r in res for v in ("cold", "cold_dual", "warm"))),
+               "speedup_warm_vs_cold_total": float(cold.sum() / warm.sum()),
+               "speedup_warm_vs_cold_median": float(np.median(cold / warm)),
+               "speedup_cold_dual_vs_cold_total": float(cold.sum() / cd.sum()),
+               "speedup_warm_vs_cold_dual_total": float(cd.sum() / warm.sum()),
+               "cold_ms_total": float(cold.sum()), "cold_dual_ms_total": float(cd.sum()),
+               "warm_ms_total": float(warm.sum())}
+    for extra in ("warm_no_fallback", "warm_cold_dual", "warm_cold_dual_no_fallback", "solution_init"):
+        if extra in res[0]:
+            x = np.array([r[extra]["ms"] for r in res])
+            summary[f"speedup_{extra}_vs_cold_total"] = float(cold.sum() / x.sum())
+    return summary, res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train", "eval", "both"])
+    ap.add_argument("--scenes", type=int, default=150)
+    ap.add_argument("--eval-scenes", type=int, default=50)
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--cap", type=int, default=4000, help="training pairs kept per level and scene")
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--out", default=None, help="JSON results file")
+    ap.add_argument("--diagnostics", action="store_true", help="also run without fallback and from the solution")
+    args = ap.parse_args()
+    ctx = Context(0)
+    result = {}
+    if args.cmd in ("train", "both"):
+        t0 = time.time()
+        X, T = make_training_set(ctx, args.scenes, cap=args.cap)
+        t_data = time.time() - t0
+        t0 = time.time()
+        w, curve = train(ctx, X, T, epochs=args.epochs)
+        t_train = time.time() - t0
+        base = float(np.mean(T ** 2))
+        log(f"[warmstart] {len(X)} pairs from {args.scenes} scenes in {t_data:.0f} s; training {t_train:.1f} s; "
+            f"loss {curve.tolist()} (zero-correction loss {base:.4g})")
+        os.makedirs(os.path.dirname(os.path.abspath(args.weights)), exist_ok=True)
+        np.save(args.weights, w)
+        result["train"] = {"scenes": args.scenes, "pairs": int(len(X)), "epochs": args.epochs,
+                           "loss_curve": curve.tolist(), "zero_correction_loss": base,
+                           "data_s": t_data, "train_s": t_train}
+    if args.cmd in ("eval", "both"):
+        w = np.load(args.weights)
+        summary, res = evaluate(ctx, w, args.eval_scenes, diagnostics=args.diagnostics)
+        log(f"[warmstart] {json.dumps(summary)}")
+        result["eval"] = summary
+        result["eval_scenes"] = res
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(result, f, indent=1)
+    print(json.dumps({k: v for k, v in result.items() if k != "eval_scenes"}))
+
+
+if __name__ == "__main__":
+    main()
