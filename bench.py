@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--iters", type=int, default=50, help="iterations per level in one step")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-fused", action="store_true", help="two kernels per iteration instead of the fused one")
     ap.add_argument("--gap-tol", type=float, default=1e-4)
     ap.add_argument("--no-gap-run", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
@@ -104,7 +105,7 @@ def main():
     log(f"[bench] problem resident in {t_load:.2f} s; levels {levels}")
 
     prm = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, iters=args.iters,
-                      use_graph=0 if args.no_graph else 1, block=args.block)
+                      use_graph=0 if args.no_graph else 1, block=args.block, fused=0 if args.no_fused else 1)
     for i in range(args.warmup):
         ctx.run(prm)
     log(f"[bench] {args.warmup} warmup steps done")
@@ -123,18 +124,21 @@ def main():
 
     # roofline of the iteration at the finest level: HIP events on the library's stream
     finest, n_f = levels[-1]
-    total_ms, dual_ms, primal_ms = ctx.level_bench(finest, prm, 100)
-    it_ms = dual_ms + primal_ms
+    total_ms, dual_ms, primal_ms, fused_ms = ctx.level_bench(finest, prm, 100)
+    fused = bool(prm.fused) and not args.no_fused
+    it_ms = fused_ms if fused else dual_ms + primal_ms
+    kernel = (f"k_fused_adj (one launch = one iteration), depth {finest}, {n_f} leaves" if fused else
+              f"k_dual + k_primal_adj, depth {finest}, {n_f} leaves")
     ab = ALG_BYTES[args.precision]
     achieved = ab * n_f / (it_ms * 1e-3) / 1e9
-    log(f"[bench] depth {finest} ({n_f} leaves): dual {dual_ms * 1e3:.1f} us, primal {primal_ms * 1e3:.1f} us, "
-        f"100 its {total_ms:.2f} ms -> {achieved:.0f} GB/s algorithmic")
+    log(f"[bench] depth {finest} ({n_f} leaves): fused {fused_ms * 1e3:.1f} us; two-kernel dual {dual_ms * 1e3:.1f} us "
+        f"+ primal {primal_ms * 1e3:.1f} us; 100 its {total_ms:.2f} ms -> {achieved:.0f} GB/s algorithmic")
 
     # time to relative gap <= tol, cold start (exact-adjoint mode)
     gap = None
     if not args.no_gap_run:
         pg = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, gap_tol=args.gap_tol,
-                         chunk=50, max_iters=20000, use_graph=prm.use_graph, block=args.block)
+                         chunk=50, max_iters=20000, use_graph=prm.use_graph, block=args.block, fused=prm.fused)
         rg = ctx.run(pg)
         lv = rg.levels()
         gap = {"tol": args.gap_tol, "wall_ms": round(rg.solve_ms, 2), "iterate_ms": round(rg.iterate_ms, 2),
@@ -183,7 +187,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_unit": "bytes per iteration (dual + primal launch), rocprofv3 counters",
-                     "kernel": f"k_dual + k_primal_adj, depth {finest}, {n_f} leaves",
+                     "kernel": kernel, "fused_us": fused_ms * 1e3,
                      "alg_bytes_per_leaf_iter": ab, "dual_us": dual_ms * 1e3, "primal_us": primal_ms * 1e3},
         "time_to_gap": {"cold": gap, "warm": None},
         "cpu_baseline": cpu,

@@ -96,7 +96,9 @@ typedef struct {
                               initialisation + network correction, with warm_fallback as above */
     int32_t warm_dual;     /* with an initial field and dual_eps > 0: 0 recovers p, q from the field,
                               1 keeps the p, q recovered from the cold initialisation */
-    int32_t reserved[2];
+    int32_t fused;         /* exact-adjoint mode: 1 (default) runs one fused kernel per iteration,
+                              identical results to the two-kernel iteration (0) */
+    int32_t reserved[1];
 } tgvx_params;
 
 #define TGVX_MAX_LEVELS 22
@@ -164,10 +166,12 @@ int tgvx_problem_levels(tgvx_ctx* ctx, int32_t* depths, uint32_t* counts, int32_
 
 /* run_level timing: iters iterations of the level at `depth` of the resident
  * problem (state continues from where it is). Times measured with HIP events
- * on the library's stream: total, and the average duration of one launch of
- * each iteration kernel (dual, primal) from a separately bracketed run. */
+ * on the library's stream: total (the configured path), and the average
+ * duration of one launch of each iteration kernel (dual, primal and, in the
+ * exact-adjoint mode, the fused pass; fused_ms may be NULL) from separately
+ * bracketed launches. */
 int tgvx_level_bench(tgvx_ctx* ctx, int32_t depth, const tgvx_params* params, int32_t iters,
-                     double* total_ms, double* dual_ms, double* primal_ms);
+                     double* total_ms, double* dual_ms, double* primal_ms, double* fused_ms);
 
 /* ---- warm-start network (no reference counterpart)
  * 30 -> 64 -> 64 -> 4 ReLU MLP on FP32 matrix cores; parameters packed as

@@ -234,10 +234,11 @@ class Context:
         return x, uv, idx
 
     def level_bench(self, depth, params, iters):
-        t, d, p = C.c_double(), C.c_double(), C.c_double()
+        """(total ms of `iters` iterations, avg ms of one dual / primal / fused launch)."""
+        t, d, p, f = C.c_double(), C.c_double(), C.c_double(), C.c_double()
         self._check(self._L.tgvx_level_bench(self.h, depth, C.byref(params), iters, C.byref(t), C.byref(d),
-                                             C.byref(p)))
-        return t.value, d.value, p.value
+                                             C.byref(p), C.byref(f)))
+        return t.value, d.value, p.value, f.value
 
 
 _default_ctx = None

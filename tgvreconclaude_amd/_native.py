@@ -32,7 +32,7 @@ class Params(C.Structure):
                 ("chunk", C.c_int32), ("max_iters", C.c_int32), ("gap_tol", C.c_double),
                 ("warm_fallback", C.c_int32), ("use_graph", C.c_int32), ("block", C.c_int32),
                 ("dual_eps", C.c_float), ("warm_start", C.c_int32), ("warm_dual", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("fused", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class Report(C.Structure):
@@ -110,7 +110,7 @@ def lib():
         L.tgvx_problem_gap.argtypes = [vp, i32, C.POINTER(Params), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tgvx_problem_levels.argtypes = [vp, vp, vp, C.POINTER(i32)]
         L.tgvx_level_bench.argtypes = [vp, i32, C.POINTER(Params), i32, C.POINTER(C.c_double),
-                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tgvx_mlp_set.argtypes = [vp, vp]
         L.tgvx_mlp_get.argtypes = [vp, vp]
         L.tgvx_mlp_forward.argtypes = [vp, vp, u64, vp]

@@ -45,6 +45,7 @@ void tgvx_default_params(tgvx_params* p) {
     p->warm_fallback = 1;
     p->use_graph = 1;
     p->block = 256;
+    p->fused = 1;
 }
 
 int tgvx_create(tgvx_ctx** out, int32_t device) {
@@ -140,10 +141,10 @@ int tgvx_problem_levels(tgvx_ctx* c, int32_t* depths, uint32_t* counts, int32_t*
 }
 
 int tgvx_level_bench(tgvx_ctx* c, int32_t depth, const tgvx_params* p, int32_t iters, double* total_ms,
-                     double* dual_ms, double* primal_ms) {
+                     double* dual_ms, double* primal_ms, double* fused_ms) {
     return guarded(c, [&] {
         if (!p || !total_ms || !dual_ms || !primal_ms || iters <= 0) throw tgvx::Error(TGVX_E_ARG, "bad argument");
-        tgvx::level_bench(c, depth, p, iters, total_ms, dual_ms, primal_ms);
+        tgvx::level_bench(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms);
     });
 }
 

@@ -85,6 +85,30 @@ __device__ inline T prox(T z, uint32_t w, int nb, T tl) {
     return med;
 }
 
+// Passes 1-2 of one cube (tgv.cpp:159-189): p += sigma (grad u_bar - v_bar),
+// q += sigma eps(v_bar), ball projections. bar[0] = this cube's (u_bar,
+// v_bar), bar[1 + a] = its +a neighbour's (or its own when there is none).
+template <class T>
+__device__ inline void dual_step(T p[3], T q[6], const T bar[4][4], T sigma, T a0, T a1) {
+    const T ubi = bar[0][0], vxi = bar[0][1], vyi = bar[0][2], vzi = bar[0][3];
+    p[0] += sigma * (bar[1][0] - ubi - vxi);
+    p[1] += sigma * (bar[2][0] - ubi - vyi);
+    p[2] += sigma * (bar[3][0] - ubi - vzi);
+    clip3(p[0], p[1], p[2], a1);
+    const T x0 = bar[1][1], y0 = bar[1][2], z0 = bar[1][3];
+    const T x2 = bar[2][1], y2 = bar[2][2], z2 = bar[2][3];
+    const T x4 = bar[3][1], y4 = bar[3][2], z4 = bar[3][3];
+    const T eps[6] = {x0 - vxi,
+                      y2 - vyi,
+                      z4 - vzi,
+                      T(0.5) * (x2 - vxi + y0 - vyi),
+                      T(0.5) * (x4 - vxi + z0 - vzi),
+                      T(0.5) * (y4 - vyi + z2 - vzi)};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = q[k] + sigma * eps[k];
+    clip6(q, a0);
+}
+
 // ---- phase D: dual p and q (tgv.cpp:158-189; frozen variant :274-309)
 template <class S, bool FROZEN>
 __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
@@ -111,29 +135,21 @@ __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
         }
         return IO::ld(a.s[f], sj);
     };
-    const T ub0 = rd(j0, s0, F_UB, 0, ubi), ub2 = rd(j2, s2, F_UB, 0, ubi), ub4 = rd(j4, s4, F_UB, 0, ubi);
-    T p0 = IO::ld(a.s[F_PX], g), p1 = IO::ld(a.s[F_PY], g), p2 = IO::ld(a.s[F_PZ], g);
-    p0 += a.sigma * (ub0 - ubi - vxi);
-    p1 += a.sigma * (ub2 - ubi - vyi);
-    p2 += a.sigma * (ub4 - ubi - vzi);
-    clip3(p0, p1, p2, a.a1);
-    IO::st(a.s[F_PX], g, p0);
-    IO::st(a.s[F_PY], g, p1);
-    IO::st(a.s[F_PZ], g, p2);
-
-    const T x0 = rd(j0, s0, F_VBX, 1, vxi), y0 = rd(j0, s0, F_VBY, 2, vyi), z0 = rd(j0, s0, F_VBZ, 3, vzi);
-    const T x2 = rd(j2, s2, F_VBX, 1, vxi), y2 = rd(j2, s2, F_VBY, 2, vyi), z2 = rd(j2, s2, F_VBZ, 3, vzi);
-    const T x4 = rd(j4, s4, F_VBX, 1, vxi), y4 = rd(j4, s4, F_VBY, 2, vyi), z4 = rd(j4, s4, F_VBZ, 3, vzi);
-    const T eps[6] = {x0 - vxi,
-                      y2 - vyi,
-                      z4 - vzi,
-                      T(0.5) * (x2 - vxi + y0 - vyi),
-                      T(0.5) * (x4 - vxi + z0 - vzi),
-                      T(0.5) * (y4 - vyi + z2 - vzi)};
+    // bars: [0] this cube, [1..3] its +x, +y, +z neighbours; each (u, vx, vy, vz)
+    T bar[4][4] = {{ubi, vxi, vyi, vzi}};
+    const int js[3] = {j0, j2, j4};
+    const long ss[3] = {s0, s2, s4};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bar[1 + ax][c] = rd(js[ax], ss[ax], F_UB + c, c, bar[0][c]);
+    T p[3] = {IO::ld(a.s[F_PX], g), IO::ld(a.s[F_PY], g), IO::ld(a.s[F_PZ], g)};
     T q[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) q[k] = IO::ld(a.s[F_Q0 + k], g) + a.sigma * eps[k];
-    clip6(q, a.a0);
+    for (int k = 0; k < 6; ++k) q[k] = IO::ld(a.s[F_Q0 + k], g);
+    dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) IO::st(a.s[F_PX + c], g, p[c]);
 #pragma unroll
     for (int k = 0; k < 6; ++k) IO::st(a.s[F_Q0 + k], g, q[k]);
 }
@@ -194,19 +210,26 @@ __device__ inline const S* field3(const KArgs<S>& a, int f0, int f1, int f2, int
 // uses minus the transpose of the dual step's linear map: self terms only
 // where the +a neighbour exists, incoming terms from every active cube whose
 // +a neighbour this one is, and 1/2 on the off-diagonal eps entries.
+// The 13 iterated values (u, v[3], p[3], q[6]) of every cube, as one array
+// each; the fused iteration ping-pongs between two such sets.
 template <class S>
-__global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a) {
+struct Set13 {
+    S* f[13];
+};
+
+// Passes 3-5 of cube g in the exact-adjoint mode, from the values in src:
+// the new (u, v) in nw, the old ones in od.
+template <class S>
+__device__ inline void primal_adj_at(const KArgs<S>& a, const Set13<S>& src, long g, typename Io<S>::T nw[4],
+                                     typename Io<S>::T od[4]) {
     using IO = Io<S>;
     using T = typename IO::T;
-    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
-    const long g = (long)a.off + t;
     const uint32_t w = a.info[g];
-    T p[3] = {IO::ld(a.s[F_PX], g), IO::ld(a.s[F_PY], g), IO::ld(a.s[F_PZ], g)};
+    T p[3] = {IO::ld(src.f[F_PX], g), IO::ld(src.f[F_PY], g), IO::ld(src.f[F_PZ], g)};
     T dp = T(0);
     T dq[3] = {T(0), T(0), T(0)};
-    const T qs[6] = {IO::ld(a.s[F_Q0], g), IO::ld(a.s[F_Q1], g), IO::ld(a.s[F_Q2], g),
-                     IO::ld(a.s[F_Q3], g), IO::ld(a.s[F_Q4], g), IO::ld(a.s[F_Q5], g)};
+    const T qs[6] = {IO::ld(src.f[F_Q0], g), IO::ld(src.f[F_Q1], g), IO::ld(src.f[F_Q2], g),
+                     IO::ld(src.f[F_Q3], g), IO::ld(src.f[F_Q4], g), IO::ld(src.f[F_Q5], g)};
     // self terms where the +a neighbour exists; a select keeps each sum
     // exactly as if the term had been skipped
 #pragma unroll
@@ -222,12 +245,12 @@ __global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a) {
     // incoming edges, one per axis at most (the depth-aware table is one-to-one)
     const int k0 = a.nbt[0][g], k1 = a.nbt[1][g], k2 = a.nbt[2][g];
     const long s0 = k0 >= 0 ? k0 : g, s1 = k1 >= 0 ? k1 : g, s2 = k2 >= 0 ? k2 : g;
-    const T p0 = IO::ld(a.s[F_PX], s0), a0 = IO::ld(a.s[F_Q0], s0), a3 = IO::ld(a.s[F_Q3], s0),
-            a4 = IO::ld(a.s[F_Q4], s0);
-    const T p1 = IO::ld(a.s[F_PY], s1), b3 = IO::ld(a.s[F_Q3], s1), b1 = IO::ld(a.s[F_Q1], s1),
-            b5 = IO::ld(a.s[F_Q5], s1);
-    const T p2 = IO::ld(a.s[F_PZ], s2), c4 = IO::ld(a.s[F_Q4], s2), c5 = IO::ld(a.s[F_Q5], s2),
-            c2 = IO::ld(a.s[F_Q2], s2);
+    const T p0 = IO::ld(src.f[F_PX], s0), a0 = IO::ld(src.f[F_Q0], s0), a3 = IO::ld(src.f[F_Q3], s0),
+            a4 = IO::ld(src.f[F_Q4], s0);
+    const T p1 = IO::ld(src.f[F_PY], s1), b3 = IO::ld(src.f[F_Q3], s1), b1 = IO::ld(src.f[F_Q1], s1),
+            b5 = IO::ld(src.f[F_Q5], s1);
+    const T p2 = IO::ld(src.f[F_PZ], s2), c4 = IO::ld(src.f[F_Q4], s2), c5 = IO::ld(src.f[F_Q5], s2),
+            c2 = IO::ld(src.f[F_Q2], s2);
     if (k0 >= 0) {
         dp -= p0;
         dq[0] -= T(1) * a0;
@@ -246,20 +269,88 @@ __global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a) {
         dq[1] -= T(0.5) * c5;
         dq[2] -= T(1) * c2;
     }
-    const T u = IO::ld(a.s[F_U], g);
-    const T un = prox(u + a.tau * dp, w, a.nbins, a.tl);
-    T vn[3], vo[3];
+    const T u = IO::ld(src.f[F_U], g);
+    od[0] = u;
+    nw[0] = prox(u + a.tau * dp, w, a.nbins, a.tl);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        vo[c] = IO::ld(a.s[F_VX + c], g);
-        vn[c] = vo[c] + a.tau * (p[c] + dq[c]);
+        od[1 + c] = IO::ld(src.f[F_VX + c], g);
+        nw[1 + c] = od[1 + c] + a.tau * (p[c] + dq[c]);
     }
-    IO::st(a.s[F_UB], g, T(2) * un - u);
+}
+
+// ---- phase P, exact adjoint (no reference counterpart): the primal step
+// uses minus the transpose of the dual step's linear map: self terms only
+// where the +a neighbour exists, incoming terms from the cube whose +a
+// neighbour this one is, and 1/2 on the off-diagonal eps entries. Reads src,
+// writes (u, v) to dst and u_bar, v_bar to the main set; copy_dual also
+// moves p, q from src to dst.
+template <class S>
+__global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a, Set13<S> src, Set13<S> dst, int copy_dual) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    T nw[4], od[4];
+    primal_adj_at(a, src, g, nw, od);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) IO::st(a.s[F_VBX + c], g, T(2) * vn[c] - vo[c]);
-    IO::st(a.s[F_U], g, un);
+    for (int c = 0; c < 4; ++c) IO::st(a.s[F_UB + c], g, T(2) * nw[c] - od[c]);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) IO::st(a.s[F_VX + c], g, vn[c]);
+    for (int c = 0; c < 4; ++c) IO::st(dst.f[F_U + c], g, nw[c]);
+    if (copy_dual) {
+#pragma unroll
+        for (int f = F_PX; f <= F_Q5; ++f) IO::st(dst.f[f], g, IO::ld(src.f[f], g));
+    }
+}
+
+// ---- one fused iteration, exact-adjoint mode (no reference counterpart):
+// the primal passes of iteration n for this cube and, redundantly, for its
+// +x, +y, +z neighbours, then the dual passes of iteration n + 1 for this
+// cube. u_bar, v_bar never leave registers; src holds (u^n, v^n, p^n+1,
+// q^n+1), dst receives (u^n+1, v^n+1, p^n+2, q^n+2). Every value is computed
+// by the same expressions as k_primal_adj followed by k_dual, so the results
+// are identical to the two-kernel iteration.
+template <class S>
+__global__ void __launch_bounds__(256) k_fused_adj(KArgs<S> a, Set13<S> src, Set13<S> dst) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    T nw[4], od[4];
+    primal_adj_at(a, src, g, nw, od);
+    T bar[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bar[0][c] = IO::rt(T(2) * nw[c] - od[c]);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int j = a.nbp[ax][g];
+        if (j < 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bar[1 + ax][c] = bar[0][c];
+        } else if ((uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n) {
+            T n2[4], o2[4];
+            primal_adj_at(a, src, j, n2, o2);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bar[1 + ax][c] = IO::rt(T(2) * n2[c] - o2[c]);
+        } else {
+            // a cube of another level: its frozen u_bar, v_bar
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bar[1 + ax][c] = IO::ld(a.s[F_UB + c], j);
+        }
+    }
+    T p[3] = {IO::ld(src.f[F_PX], g), IO::ld(src.f[F_PY], g), IO::ld(src.f[F_PZ], g)};
+    T q[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = IO::ld(src.f[F_Q0 + k], g);
+    dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::st(dst.f[F_U + c], g, nw[c]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) IO::st(dst.f[F_PX + c], g, p[c]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) IO::st(dst.f[F_Q0 + k], g, q[k]);
 }
 
 // ---- initialisation
@@ -660,20 +751,64 @@ int block_of(const tgvx_params* p) {
 }
 
 template <class S>
+Set13<S> main_set(const KArgs<S>& a) {
+    Set13<S> m;
+    for (int f = 0; f < 13; ++f) m.f[f] = a.s[f];
+    return m;
+}
+
+template <class S>
+Set13<S> alt_set(tgvx_ctx* c) {
+    Problem& P = c->prob;
+    Set13<S> m;
+    for (int f = 0; f < 13; ++f) {
+        P.d_alt[f].ensure(P.elem * std::max<uint64_t>(P.n, 1));
+        m.f[f] = P.d_alt[f].as<S>();
+    }
+    return m;
+}
+
+template <class S>
 void launch_iter(tgvx_ctx* c, KArgs<S>& a, int op_mode, int block, bool frozen) {
     const unsigned grid = (a.n + block - 1) / block;
     if (frozen) k_dual<S, true><<<grid, block, 0, c->stream>>>(a);
     else k_dual<S, false><<<grid, block, 0, c->stream>>>(a);
-    if (op_mode == TGVX_OP_ADJ) k_primal_adj<S><<<grid, block, 0, c->stream>>>(a);
-    else k_primal_ref<S><<<grid, block, 0, c->stream>>>(a);
+    if (op_mode == TGVX_OP_ADJ) {
+        const Set13<S> m = main_set(a);
+        k_primal_adj<S><<<grid, block, 0, c->stream>>>(a, m, m, 0);
+    } else {
+        k_primal_ref<S><<<grid, block, 0, c->stream>>>(a);
+    }
+}
+
+// `iters` >= 2 exact-adjoint iterations as: dual of the first, iters - 1
+// fused (primal n, dual n+1) passes alternating between the main and the
+// alternate set, primal of the last back into the main set.
+template <class S>
+void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, int block) {
+    const unsigned grid = (a.n + block - 1) / block;
+    Set13<S> cur = main_set(a), oth = alt_set<S>(c);
+    const Set13<S> mainset = cur;
+    k_dual<S, false><<<grid, block, 0, c->stream>>>(a);
+    for (int i = 1; i < iters; ++i) {
+        k_fused_adj<S><<<grid, block, 0, c->stream>>>(a, cur, oth);
+        std::swap(cur, oth);
+    }
+    const int copy = cur.f[0] != mainset.f[0];
+    k_primal_adj<S><<<grid, block, 0, c->stream>>>(a, cur, mainset, copy);
+}
+
+bool use_fused(const tgvx_params* p, bool frozen, int iters) {
+    return p->op_mode == TGVX_OP_ADJ && p->fused && !frozen && iters >= 2;
 }
 
 // Cached graph of `iters` iterations of one level.
 struct GraphKey {
     uint32_t off, n;
-    int iters, op, block, prec;
+    int iters, op, block, prec, fused;
     bool operator<(const GraphKey& o) const {
-        return std::tie(off, n, iters, op, block, prec) < std::tie(o.off, o.n, o.iters, o.op, o.block, o.prec);
+        return std::tie(off, n, iters, op, block, prec, fused) <
+               std::tie(o.off, o.n, o.iters, o.op, o.block, o.prec, o.fused);
     }
 };
 struct GraphCache {
@@ -690,14 +825,18 @@ template <class S>
 void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen) {
     if (iters <= 0 || a.n == 0) return;
     const int block = block_of(p);
+    const bool fused = use_fused(p, frozen, iters);
+    if (fused) alt_set<S>(c);  // allocate outside any capture
     if (p->use_graph && !frozen && iters > 1) {
-        GraphKey key{a.off, a.n, iters, p->op_mode, block, c->prob.precision};
+        GraphKey key{a.off, a.n, iters, p->op_mode, block, c->prob.precision, fused ? 1 : 0};
         GraphCache& gc = g_graphs[c];
         auto it = gc.m.find(key);
         if (it == gc.m.end()) {
             hipGraph_t graph;
             TGVX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, false);
+            if (fused) launch_fused<S>(c, a, iters, block);
+            else
+                for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, false);
             TGVX_HIP(hipStreamEndCapture(c->stream, &graph));
             hipGraphExec_t exec;
             TGVX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -707,7 +846,9 @@ void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool f
         TGVX_HIP(hipGraphLaunch(it->second, c->stream));
         return;
     }
-    for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, frozen);
+    if (fused) launch_fused<S>(c, a, iters, block);
+    else
+        for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, frozen);
     TGVX_HIP(hipGetLastError());
 }
 
@@ -896,7 +1037,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
 
 template <class S>
 void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
-             double* primal_ms) {
+             double* primal_ms, double* fused_ms) {
     Problem& P = c->prob;
     const Level* L = nullptr;
     for (const Level& l : P.levels)
@@ -922,8 +1063,12 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
         TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
         k_dual<S, false><<<grid, block, 0, c->stream>>>(a);
         TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
-        if (p->op_mode == TGVX_OP_ADJ) k_primal_adj<S><<<grid, block, 0, c->stream>>>(a);
-        else k_primal_ref<S><<<grid, block, 0, c->stream>>>(a);
+        if (p->op_mode == TGVX_OP_ADJ) {
+            const Set13<S> m = main_set(a);
+            k_primal_adj<S><<<grid, block, 0, c->stream>>>(a, m, m, 0);
+        } else {
+            k_primal_ref<S><<<grid, block, 0, c->stream>>>(a);
+        }
         TGVX_HIP(hipEventRecord(c->ev[2], c->stream));
         TGVX_HIP(hipEventSynchronize(c->ev[2]));
         float m1 = 0.f, m2 = 0.f;
@@ -934,6 +1079,24 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     }
     *dual_ms = sd / k;
     *primal_ms = sp / k;
+    // the fused pass alone (an even count, so the main set ends current)
+    double sf = 0.0;
+    if (p->op_mode == TGVX_OP_ADJ) {
+        Set13<S> cur = main_set(a), oth = alt_set<S>(c);
+        const int kf = 2 * k;
+        for (int i = 0; i < kf; ++i) {
+            TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+            k_fused_adj<S><<<grid, block, 0, c->stream>>>(a, cur, oth);
+            TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+            TGVX_HIP(hipEventSynchronize(c->ev[1]));
+            float m1 = 0.f;
+            TGVX_HIP(hipEventElapsedTime(&m1, c->ev[0], c->ev[1]));
+            sf += m1;
+            std::swap(cur, oth);
+        }
+        sf /= kf;
+    }
+    if (fused_ms) *fused_ms = sf;
 }
 
 }  // namespace
@@ -1158,13 +1321,13 @@ void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* Pv, doubl
 }
 
 void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
-                 double* primal_ms) {
+                 double* primal_ms, double* fused_ms) {
     Problem& P = c->prob;
     if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
     switch (P.precision) {
-        case TGVX_FP64: bench_t<double>(c, depth, p, iters, total_ms, dual_ms, primal_ms); break;
-        case TGVX_FP32: bench_t<float>(c, depth, p, iters, total_ms, dual_ms, primal_ms); break;
-        default: bench_t<bf16s>(c, depth, p, iters, total_ms, dual_ms, primal_ms); break;
+        case TGVX_FP64: bench_t<double>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
+        case TGVX_FP32: bench_t<float>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
+        default: bench_t<bf16s>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
     }
 }
 

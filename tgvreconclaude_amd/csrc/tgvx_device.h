@@ -66,12 +66,14 @@ struct Io<float> {
     using T = float;
     __device__ static inline T ld(const float* p, long i) { return p[i]; }
     __device__ static inline void st(float* p, long i, T v) { p[i] = v; }
+    __device__ static inline T rt(T v) { return v; }  // value after a store / load round trip
 };
 template <>
 struct Io<double> {
     using T = double;
     __device__ static inline T ld(const double* p, long i) { return p[i]; }
     __device__ static inline void st(double* p, long i, T v) { p[i] = v; }
+    __device__ static inline T rt(T v) { return v; }
 };
 template <>
 struct Io<bf16s> {
@@ -85,6 +87,11 @@ struct Io<bf16s> {
         if ((u & 0x7fffffffu) > 0x7f800000u) h = (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
         else h = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
         p[i].x = h;
+    }
+    __device__ static inline T rt(T v) {
+        bf16s b;
+        st(&b, 0, v);
+        return ld(&b, 0);
     }
 };
 
