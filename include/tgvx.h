@@ -173,6 +173,28 @@ int tgvx_problem_levels(tgvx_ctx* ctx, int32_t* depths, uint32_t* counts, int32_
 int tgvx_level_bench(tgvx_ctx* ctx, int32_t depth, const tgvx_params* params, int32_t iters,
                      double* total_ms, double* dual_ms, double* primal_ms, double* fused_ms);
 
+/* ---- several independent octrees in one problem (no reference counterpart)
+ * Blocks are concatenated level-major: every iteration of a depth is one
+ * launch over that depth's cubes of all blocks; neighbour lookups stay inside
+ * each block. With gap_tol > 0 a depth is iterated until every block has
+ * reached it (blocks that reached it keep iterating with the rest). */
+typedef struct {
+    const tgvx_cube* cubes;
+    uint64_t n;
+    const uint32_t* bins; /* n x nb */
+} tgvx_block;
+int tgvx_problem_load_blocks(tgvx_ctx* ctx, const tgvx_block* blocks, uint32_t nblocks, int32_t nb,
+                             int32_t precision, int32_t op_mode);
+/* per level (as tgvx_problem_levels) x block, row-major: the iterations
+ * after which the block's relative gap first was <= gap_tol and that gap
+ * (-1 and 0: the block has no cube at that depth; -1 and the final gap: not
+ * reached within max_iters); with a fixed iteration count, that count and
+ * the final gap. Either pointer may be NULL. */
+int tgvx_problem_block_report(tgvx_ctx* ctx, int32_t* iters, double* rel_gap);
+/* load + solve + per-block read-back; state_out[b] as in tgvx_solve */
+int tgvx_solve_batched(tgvx_ctx* ctx, const tgvx_block* blocks, uint32_t nblocks, int32_t nb,
+                       const tgvx_params* params, void* const* state_out, tgvx_report* report);
+
 /* ---- warm-start network (no reference counterpart)
  * 30 -> 64 -> 64 -> 4 ReLU MLP on FP32 matrix cores; parameters packed as
  * W1[30][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4]. A context starts with

@@ -166,6 +166,49 @@ class Context:
         self._check(self._L.tgvx_problem_load(self.h, N.ptr(cubes), len(cubes), N.ptr(h), h.shape[1], precision,
                                               op_mode))
 
+    def load_blocks(self, blocks, precision=FP32, op_mode=OP_ADJ):
+        """Several independent octrees [(cubes, hists), ...] as one resident problem."""
+        keep = [(np.ascontiguousarray(c, dtype=CUBE_DT), np.ascontiguousarray(h, dtype=np.uint32)) for c, h in blocks]
+        arr = (N.Block * len(keep))()
+        for i, (c, h) in enumerate(keep):
+            arr[i].cubes, arr[i].n, arr[i].bins = c.ctypes.data, len(c), h.ctypes.data
+        nb = keep[0][1].shape[1]
+        self._n, self._prec = sum(len(c) for c, _ in keep), precision
+        self._block_sizes = [len(c) for c, _ in keep]
+        self._check(self._L.tgvx_problem_load_blocks(self.h, arr, len(keep), nb, precision, op_mode))
+
+    def solve_batched(self, blocks, params=None):
+        """tgv_minimize of several independent octrees in one resident problem; (states per block, Report)."""
+        params = params or TGVParams()
+        keep = [(np.ascontiguousarray(c, dtype=CUBE_DT), np.ascontiguousarray(h, dtype=np.uint32)) for c, h in blocks]
+        arr = (N.Block * len(keep))()
+        states = [self._state_array(len(c), params.precision) for c, _ in keep]
+        outs = (C.c_void_p * len(keep))()
+        for i, (c, h) in enumerate(keep):
+            arr[i].cubes, arr[i].n, arr[i].bins = c.ctypes.data, len(c), h.ctypes.data
+            outs[i] = states[i].ctypes.data
+        rep = Report()
+        self._check(self._L.tgvx_solve_batched(self.h, arr, len(keep), keep[0][1].shape[1], C.byref(params), outs,
+                                               C.byref(rep)))
+        self._n, self._prec = sum(len(c) for c, _ in keep), params.precision
+        self._block_sizes = [len(c) for c, _ in keep]
+        return states, rep
+
+    def block_report(self):
+        """(iterations [levels, blocks], relative gaps [levels, blocks]) of the last solve."""
+        nl, nb = len(self.levels()), len(self._block_sizes)
+        it = np.zeros((nl, nb), dtype=np.int32)
+        g = np.zeros((nl, nb), dtype=np.float64)
+        self._check(self._L.tgvx_problem_block_report(self.h, N.ptr(it), N.ptr(g)))
+        return it, g
+
+    def split_blocks(self, state):
+        out, o = [], 0
+        for n in self._block_sizes:
+            out.append(state[o:o + n])
+            o += n
+        return out
+
     def run(self, params, init_uv=None):
         rep = Report()
         init = None if init_uv is None else np.ascontiguousarray(init_uv, dtype=np.float32)

@@ -53,6 +53,10 @@ class Report(C.Structure):
         return out
 
 
+class Block(C.Structure):
+    _fields_ = [("cubes", C.c_void_p), ("n", C.c_uint64), ("bins", C.c_void_p)]
+
+
 class Prim(C.Structure):
     _fields_ = [("kind", C.c_int32), ("center", C.c_float * 3), ("size", C.c_float * 3), ("rot", C.c_float * 9)]
 
@@ -68,7 +72,8 @@ TGVX_SYMBOLS = [
     "tgvx_histograms", "tgvx_mip_build", "tgvx_median_init", "tgvx_solve", "tgvx_solve_oc",
     "tgvx_problem_load", "tgvx_problem_solve", "tgvx_problem_state", "tgvx_problem_gap",
     "tgvx_problem_levels", "tgvx_level_bench", "tgvx_mlp_set", "tgvx_mlp_get", "tgvx_mlp_forward",
-    "tgvx_mlp_loss_grad", "tgvx_mlp_train", "tgvx_problem_features",
+    "tgvx_mlp_loss_grad", "tgvx_mlp_train", "tgvx_problem_features", "tgvx_problem_load_blocks",
+    "tgvx_problem_block_report", "tgvx_solve_batched",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
@@ -117,6 +122,9 @@ def lib():
         L.tgvx_mlp_loss_grad.argtypes = [vp, vp, vp, u64, vp, C.POINTER(C.c_double)]
         L.tgvx_mlp_train.argtypes = [vp, vp, vp, u64, i32, i32, C.c_float, C.c_uint64, vp]
         L.tgvx_problem_features.argtypes = [vp, i32, vp, vp, vp]
+        L.tgvx_problem_load_blocks.argtypes = [vp, vp, C.c_uint32, i32, i32, i32]
+        L.tgvx_problem_block_report.argtypes = [vp, vp, vp]
+        L.tgvx_solve_batched.argtypes = [vp, vp, C.c_uint32, i32, C.POINTER(Params), vp, C.POINTER(Report)]
         for name in TGVX_SYMBOLS:
             f = getattr(L, name)
             if f.restype is C.c_int:  # default

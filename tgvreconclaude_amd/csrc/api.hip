@@ -1,7 +1,9 @@
 // extern "C" entry points of libtgvx.so: argument checks, error capture,
 // context lifetime. Compute lives in hist.hip and solve.hip.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <vector>
 
 #include "tgvx_internal.h"
 
@@ -186,6 +188,72 @@ int tgvx_mlp_train(tgvx_ctx* c, const float* x, const float* t, uint64_t n, int3
 
 int tgvx_problem_features(tgvx_ctx* c, int32_t depth, float* x_out, float* uv_out, uint32_t* index_out) {
     return guarded(c, [&] { tgvx::problem_features(c, depth, x_out, uv_out, index_out); });
+}
+
+namespace {
+// concatenated cubes / bins of several blocks and their input ranges
+struct Concat {
+    std::vector<tgvx_cube> cubes;
+    std::vector<uint32_t> bins;
+    std::vector<uint64_t> start;
+};
+Concat concat(const tgvx_block* blocks, uint32_t nblocks, int32_t nb) {
+    if (!blocks || nblocks == 0) throw tgvx::Error(TGVX_E_ARG, "no blocks");
+    if (nb <= 0 || nb > 32) throw tgvx::Error(TGVX_E_ARG, "nb must be in 1..32");
+    Concat c;
+    c.start.push_back(0);
+    uint64_t tot = 0;
+    for (uint32_t b = 0; b < nblocks; ++b) tot += blocks[b].n;
+    c.cubes.reserve(tot);
+    c.bins.reserve(tot * nb);
+    for (uint32_t b = 0; b < nblocks; ++b) {
+        const tgvx_block& k = blocks[b];
+        if (k.n && (!k.cubes || !k.bins)) throw tgvx::Error(TGVX_E_ARG, "null block arrays");
+        c.cubes.insert(c.cubes.end(), k.cubes, k.cubes + k.n);
+        c.bins.insert(c.bins.end(), k.bins, k.bins + k.n * nb);
+        c.start.push_back(c.start.back() + k.n);
+    }
+    return c;
+}
+}  // namespace
+
+int tgvx_problem_load_blocks(tgvx_ctx* c, const tgvx_block* blocks, uint32_t nblocks, int32_t nb, int32_t precision,
+                             int32_t op_mode) {
+    return guarded(c, [&] {
+        Concat k = concat(blocks, nblocks, nb);
+        tgvx::problem_load(c, k.cubes.data(), k.cubes.size(), k.bins.data(), nb, precision, op_mode, k.start.data(),
+                           nblocks);
+    });
+}
+
+int tgvx_problem_block_report(tgvx_ctx* c, int32_t* iters, double* rel_gap) {
+    return guarded(c, [&] {
+        const auto& P = c->prob;
+        if (!P.loaded) throw tgvx::Error(TGVX_E_STATE, "no problem loaded");
+        if (iters) std::copy(P.block_iters.begin(), P.block_iters.end(), iters);
+        if (rel_gap) std::copy(P.block_gap.begin(), P.block_gap.end(), rel_gap);
+    });
+}
+
+int tgvx_solve_batched(tgvx_ctx* c, const tgvx_block* blocks, uint32_t nblocks, int32_t nb, const tgvx_params* p,
+                       void* const* state_out, tgvx_report* rep) {
+    return guarded(c, [&] {
+        if (!p || !state_out) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        Concat k = concat(blocks, nblocks, nb);
+        tgvx_report r;
+        std::memset(&r, 0, sizeof(r));
+        const auto t0 = std::chrono::steady_clock::now();
+        tgvx::problem_load(c, k.cubes.data(), k.cubes.size(), k.bins.data(), nb, p->precision, p->op_mode,
+                           k.start.data(), nblocks);
+        r.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        tgvx::problem_solve(c, p, nullptr, &r, nullptr, 0, nullptr);
+        const size_t rec = p->precision == TGVX_FP64 ? 13 * sizeof(double) : sizeof(tgvx_state);
+        std::vector<uint8_t> all(rec * std::max<size_t>(k.cubes.size(), 1));
+        tgvx::problem_state(c, all.data(), nullptr);
+        for (uint32_t b = 0; b < nblocks; ++b)
+            if (blocks[b].n) std::memcpy(state_out[b], all.data() + rec * k.start[b], rec * blocks[b].n);
+        if (rep) *rep = r;
+    });
 }
 
 static int solve_common(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,

@@ -49,7 +49,16 @@ struct KArgs {
     uint32_t leaf_first, leaf_last;
     const float* init;     // warm start field, 4 per position
     S* save[4];
+    // several blocks iterated to a gap: a cube is skipped while its block's
+    // byte in bact is 0 (the block has converged); null: every cube runs
+    const int32_t* blk;
+    const uint8_t* bact;
 };
+
+template <class S>
+__device__ inline bool masked(const KArgs<S>& a, long g) {
+    return a.bact && !a.bact[a.blk[g]];
+}
 
 template <class T>
 __device__ inline void clip3(T& a, T& b, T& c, T r) {
@@ -117,6 +126,7 @@ __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
     const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
+    if (masked(a, g)) return;
     const T ubi = IO::ld(a.s[F_UB], g), vxi = IO::ld(a.s[F_VBX], g), vyi = IO::ld(a.s[F_VBY], g),
             vzi = IO::ld(a.s[F_VBZ], g);
     const int j0 = a.nbp[0][g], j2 = a.nbp[1][g], j4 = a.nbp[2][g];
@@ -163,6 +173,7 @@ __global__ void __launch_bounds__(256) k_primal_ref(KArgs<S> a) {
     const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
+    if (masked(a, g)) return;
     const int m0 = a.nbm[0][g], m2 = a.nbm[1][g], m4 = a.nbm[2][g];
     // A missing neighbour contributes 0 (tgv.cpp:196,209). Loads go to g
     // instead and the value is replaced by 0 afterwards, so that every load
@@ -292,6 +303,7 @@ __global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a, Set13<S> src, Se
     const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
+    if (masked(a, g)) return;
     T nw[4], od[4];
     primal_adj_at(a, src, g, nw, od);
 #pragma unroll
@@ -318,6 +330,7 @@ __global__ void __launch_bounds__(256) k_fused_adj(KArgs<S> a, Set13<S> src, Set
     const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
+    if (masked(a, g)) return;
     T nw[4], od[4];
     primal_adj_at(a, src, g, nw, od);
     T bar[4][4];
@@ -479,90 +492,94 @@ __global__ void k_dual_recover(KArgs<S> a, typename Io<S>::T eps) {
 // per block, then over blocks in a fixed order.
 constexpr int kGapBlock = 256;
 
+// Contribution of cube g to the level's energy P and dual value D.
+template <class S>
+__device__ inline void gap_cube(const KArgs<S>& a, long g, double lam, double a0, double a1, double vbox, double& P,
+                                double& D) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    auto in_lvl = [&](int j) { return (uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n; };
+    auto L = [&](int f, long j) -> double { return (double)IO::ld(a.s[f], j); };
+    const uint32_t Wc = InfoWord::W(a.info[g]);
+    const double W = (double)Wc, m = (double)median_of<T>(a.info[g], a.nbins), ui = L(F_U, g);
+    const double v[3] = {L(F_VX, g), L(F_VY, g), L(F_VZ, g)};
+    const double p[3] = {L(F_PX, g), L(F_PY, g), L(F_PZ, g)};
+    double gr[3], dv[3][3];
+    int jn[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int j = a.nba[ax][g];
+        jn[ax] = j;
+        if (j >= 0) {
+            const bool in = in_lvl(j);
+            gr[ax] = (in ? L(F_U, j) : L(F_UB, j)) - ui;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dv[c][ax] = (in ? L(F_VX + c, j) : L(F_VBX + c, j)) - v[c];
+        } else {
+            gr[ax] = 0.0;
+            for (int c = 0; c < 3; ++c) dv[c][ax] = 0.0;
+        }
+    }
+    const double k1[3] = {gr[0] - v[0], gr[1] - v[1], gr[2] - v[2]};
+    const double e[6] = {dv[0][0], dv[1][1], dv[2][2], 0.5 * (dv[0][1] + dv[1][0]), 0.5 * (dv[0][2] + dv[2][0]),
+                         0.5 * (dv[1][2] + dv[2][1])};
+    const double nk = sqrt(k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]);
+    double ne = 0.0;
+    for (int k = 0; k < 6; ++k) ne += e[k] * e[k];
+    ne = sqrt(ne);
+    P = (Wc ? lam * W * fabs(ui - m) : 0.0) + a1 * nk + a0 * ne;
+
+    double cst = 0.0, cu = 0.0, cv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int j = jn[ax];
+        if (j < 0) continue;
+        cu -= p[ax];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double w = (c == ax) ? 1.0 : 0.5;
+            cv[c] -= w * L(qidx(c, ax), g);
+        }
+        if (!in_lvl(j)) {
+            cst += p[ax] * L(F_UB, j);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double w = (c == ax) ? 1.0 : 0.5;
+                cst += w * L(qidx(c, ax), g) * L(F_VBX + c, j);
+            }
+        }
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int k = a.nbt[ax][g];
+        if (k < 0) continue;
+        cu += L(F_PX + ax, k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double w = (c == ax) ? 1.0 : 0.5;
+            cv[c] += w * L(qidx(c, ax), k);
+        }
+    }
+    for (int c = 0; c < 3; ++c) cv[c] -= p[c];
+    double mu;
+    if (Wc == 0u) {
+        mu = -fabs(cu);
+    } else {
+        const double g0 = lam * W * fabs(-1.0 - m) + cu * -1.0;
+        const double g1 = lam * W * fabs(m - m) + cu * m;
+        const double g2 = lam * W * fabs(1.0 - m) + cu * 1.0;
+        mu = fmin(g0, fmin(g1, g2));
+    }
+    D = cst + mu - vbox * (fabs(cv[0]) + fabs(cv[1]) + fabs(cv[2]));
+}
+
 template <class S>
 __global__ void __launch_bounds__(kGapBlock) k_gap(KArgs<S> a, double lam, double a0, double a1, double vbox,
                                                     double* part) {
-    using IO = Io<S>;
-    using T = typename IO::T;
     __shared__ double sp[kGapBlock], sd[kGapBlock];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     double P = 0.0, D = 0.0;
-    if (t < a.n) {
-        const long g = (long)a.off + t;
-        auto in_lvl = [&](int j) { return (uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n; };
-        auto L = [&](int f, long j) -> double { return (double)IO::ld(a.s[f], j); };
-        const uint32_t Wc = InfoWord::W(a.info[g]);
-        const double W = (double)Wc, m = (double)median_of<T>(a.info[g], a.nbins), ui = L(F_U, g);
-        const double v[3] = {L(F_VX, g), L(F_VY, g), L(F_VZ, g)};
-        const double p[3] = {L(F_PX, g), L(F_PY, g), L(F_PZ, g)};
-        double gr[3], dv[3][3];
-        int jn[3];
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const int j = a.nba[ax][g];
-            jn[ax] = j;
-            if (j >= 0) {
-                const bool in = in_lvl(j);
-                gr[ax] = (in ? L(F_U, j) : L(F_UB, j)) - ui;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) dv[c][ax] = (in ? L(F_VX + c, j) : L(F_VBX + c, j)) - v[c];
-            } else {
-                gr[ax] = 0.0;
-                for (int c = 0; c < 3; ++c) dv[c][ax] = 0.0;
-            }
-        }
-        const double k1[3] = {gr[0] - v[0], gr[1] - v[1], gr[2] - v[2]};
-        const double e[6] = {dv[0][0], dv[1][1], dv[2][2], 0.5 * (dv[0][1] + dv[1][0]), 0.5 * (dv[0][2] + dv[2][0]),
-                             0.5 * (dv[1][2] + dv[2][1])};
-        const double nk = sqrt(k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]);
-        double ne = 0.0;
-        for (int k = 0; k < 6; ++k) ne += e[k] * e[k];
-        ne = sqrt(ne);
-        P = (Wc ? lam * W * fabs(ui - m) : 0.0) + a1 * nk + a0 * ne;
-
-        double cst = 0.0, cu = 0.0, cv[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const int j = jn[ax];
-            if (j < 0) continue;
-            cu -= p[ax];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double w = (c == ax) ? 1.0 : 0.5;
-                cv[c] -= w * L(qidx(c, ax), g);
-            }
-            if (!in_lvl(j)) {
-                cst += p[ax] * L(F_UB, j);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double w = (c == ax) ? 1.0 : 0.5;
-                    cst += w * L(qidx(c, ax), g) * L(F_VBX + c, j);
-                }
-            }
-        }
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const int k = a.nbt[ax][g];
-            if (k < 0) continue;
-            cu += L(F_PX + ax, k);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double w = (c == ax) ? 1.0 : 0.5;
-                cv[c] += w * L(qidx(c, ax), k);
-            }
-        }
-        for (int c = 0; c < 3; ++c) cv[c] -= p[c];
-        double mu;
-        if (Wc == 0u) {
-            mu = -fabs(cu);
-        } else {
-            const double g0 = lam * W * fabs(-1.0 - m) + cu * -1.0;
-            const double g1 = lam * W * fabs(m - m) + cu * m;
-            const double g2 = lam * W * fabs(1.0 - m) + cu * 1.0;
-            mu = fmin(g0, fmin(g1, g2));
-        }
-        D = cst + mu - vbox * (fabs(cv[0]) + fabs(cv[1]) + fabs(cv[2]));
-    }
+    if (t < a.n) gap_cube(a, (long)a.off + t, lam, a0, a1, vbox, P, D);
     sp[threadIdx.x] = P;
     sd[threadIdx.x] = D;
     __syncthreads();
@@ -576,6 +593,36 @@ __global__ void __launch_bounds__(kGapBlock) k_gap(KArgs<S> a, double lam, doubl
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = sp[0];
         part[2 * blockIdx.x + 1] = sd[0];
+    }
+}
+
+// Per-block energy and dual value of a level shared by several blocks:
+// workgroup b sums its block's segment [seg[b], seg[b+1]) of the level.
+template <class S>
+__global__ void __launch_bounds__(kGapBlock) k_gap_seg(KArgs<S> a, const uint32_t* __restrict__ seg, double lam,
+                                                        double a0, double a1, double vbox, double* out) {
+    __shared__ double sp[kGapBlock], sd[kGapBlock];
+    const uint32_t b = blockIdx.x;
+    double P = 0.0, D = 0.0;
+    for (uint32_t t = seg[b] + threadIdx.x; t < seg[b + 1]; t += kGapBlock) {
+        double p1, d1;
+        gap_cube(a, (long)a.off + t, lam, a0, a1, vbox, p1, d1);
+        P += p1;
+        D += d1;
+    }
+    sp[threadIdx.x] = P;
+    sd[threadIdx.x] = D;
+    __syncthreads();
+    for (int s = kGapBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sp[threadIdx.x] += sp[threadIdx.x + s];
+            sd[threadIdx.x] += sd[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * b] = sp[0];
+        out[2 * b + 1] = sd[0];
     }
 }
 
@@ -615,14 +662,21 @@ __device__ inline int code_lookup(const uint64_t* uc, const int32_t* ui, uint32_
     return (lo < U && uc[lo] == key) ? ui[lo] : -1;
 }
 
+// Several independent octrees (blocks) may share one problem: each position's
+// lookups are confined to its block's segment [uoff[b], uoff[b+1]) of the
+// sorted code array (blk == nullptr: one block, the whole array).
 __global__ void k_tables(const uint64_t* __restrict__ codes, const int32_t* __restrict__ depth, uint32_t n,
-                         const uint64_t* __restrict__ uc, const int32_t* __restrict__ ui, uint32_t U,
+                         const uint64_t* __restrict__ uc0, const int32_t* __restrict__ ui0, uint32_t U0,
+                         const int32_t* __restrict__ blk, const uint32_t* __restrict__ uoff,
                          int32_t* nb0, int32_t* nb1, int32_t* nb2, int32_t* nb3, int32_t* nb4, int32_t* nb5,
                          int32_t* parent) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const uint64_t c = codes[g];
     const int d = depth[g];
+    const uint32_t lo = blk ? uoff[blk[g]] : 0u, U = blk ? uoff[blk[g] + 1] - lo : U0;
+    const uint64_t* uc = uc0 + lo;
+    const int32_t* ui = ui0 + lo;
     int32_t* out[6] = {nb0, nb1, nb2, nb3, nb4, nb5};
     for (int f = 0; f < 6; ++f) {
         const uint64_t k = face_nb(c, f, d);
@@ -652,13 +706,18 @@ __device__ inline int pair_lookup(const uint64_t* dc, const int32_t* dd, const i
 }
 
 __global__ void k_tables_geo(const uint64_t* __restrict__ codes, const int32_t* __restrict__ depth, uint32_t n,
-                             const uint64_t* __restrict__ dc, const int32_t* __restrict__ dd,
-                             const int32_t* __restrict__ dp, uint32_t U, int32_t* nba0, int32_t* nba1, int32_t* nba2,
+                             const uint64_t* __restrict__ dc0, const int32_t* __restrict__ dd0,
+                             const int32_t* __restrict__ dp0, uint32_t U0, const int32_t* __restrict__ blk,
+                             const uint32_t* __restrict__ doff, int32_t* nba0, int32_t* nba1, int32_t* nba2,
                              int32_t* nbm0, int32_t* nbm1, int32_t* nbm2, int32_t* parent, uint32_t* info) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const uint64_t c = codes[g];
     const int d = depth[g];
+    const uint32_t lo = blk ? doff[blk[g]] : 0u, U = blk ? doff[blk[g] + 1] - lo : U0;
+    const uint64_t* dc = dc0 + lo;
+    const int32_t* dd = dd0 + lo;
+    const int32_t* dp = dp0 + lo;
     const bool shadow = pair_lookup(dc, dd, dp, U, c, d) != (int)g;
     int32_t* out[3] = {nba0, nba1, nba2};
     uint32_t flags = 0;
@@ -805,10 +864,10 @@ bool use_fused(const tgvx_params* p, bool frozen, int iters) {
 // Cached graph of `iters` iterations of one level.
 struct GraphKey {
     uint32_t off, n;
-    int iters, op, block, prec, fused;
+    int iters, op, block, prec, fused, mask;
     bool operator<(const GraphKey& o) const {
-        return std::tie(off, n, iters, op, block, prec, fused) <
-               std::tie(o.off, o.n, o.iters, o.op, o.block, o.prec, o.fused);
+        return std::tie(off, n, iters, op, block, prec, fused, mask) <
+               std::tie(o.off, o.n, o.iters, o.op, o.block, o.prec, o.fused, o.mask);
     }
 };
 struct GraphCache {
@@ -828,7 +887,7 @@ void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool f
     const bool fused = use_fused(p, frozen, iters);
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (p->use_graph && !frozen && iters > 1) {
-        GraphKey key{a.off, a.n, iters, p->op_mode, block, c->prob.precision, fused ? 1 : 0};
+        GraphKey key{a.off, a.n, iters, p->op_mode, block, c->prob.precision, fused ? 1 : 0, a.bact ? 1 : 0};
         GraphCache& gc = g_graphs[c];
         auto it = gc.m.find(key);
         if (it == gc.m.end()) {
@@ -874,6 +933,27 @@ void eval_gap(tgvx_ctx* c, KArgs<S> a, const tgvx_params* p, double* Pout, doubl
 
 double rel_gap(double P, double D) { return (P - D) / std::max(1.0, std::fabs(P)); }
 
+// Per-block relative gaps of level li of a problem holding several blocks.
+template <class S>
+void eval_gap_blocks(tgvx_ctx* c, KArgs<S> a, const tgvx_params* p, size_t li, std::vector<double>& rg) {
+    Problem& P = c->prob;
+    const uint32_t nb = P.nblocks;
+    P.d_seg.ensure(sizeof(uint32_t) * (nb + 1));
+    P.d_pd.ensure(sizeof(double) * 2 * std::max<uint32_t>(nb, 1));
+    TGVX_HIP(hipMemcpyAsync(P.d_seg.p, P.seg[li].data(), sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice,
+                            c->stream));
+    a.off = a.lvl_off;
+    a.n = a.lvl_n;
+    k_gap_seg<S><<<nb, kGapBlock, 0, c->stream>>>(a, P.d_seg.as<uint32_t>(), (double)p->lambda, (double)p->alpha0,
+                                                   (double)p->alpha1, 2.0, P.d_pd.as<double>());
+    TGVX_HIP(hipGetLastError());
+    std::vector<double> pd(2 * nb);
+    TGVX_HIP(hipMemcpyAsync(pd.data(), P.d_pd.p, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream));
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    rg.resize(nb);
+    for (uint32_t b = 0; b < nb; ++b) rg[b] = rel_gap(pd[2 * b], pd[2 * b + 1]);
+}
+
 template <class S>
 void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
              const tgvx_treetop_leaf* leaves, uint64_t nleaves) {
@@ -887,6 +967,10 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
     const double t0 = now_ms();
     if (p->warm_start && (init_uv || leaves))
         throw Error(TGVX_E_ARG, "warm_start excludes an explicit initial field and the frozen-border solve");
+    if (P.nblocks > 1 && (p->warm_start || init_uv || leaves))
+        throw Error(TGVX_E_ARG, "a problem of several blocks takes neither warm starts nor treetop leaves");
+    P.block_iters.assign(P.levels.size() * P.nblocks, -1);
+    P.block_gap.assign(P.levels.size() * P.nblocks, 0.0);
     if (init_uv) {
         P.d_init.ensure(sizeof(float) * 4 * P.n);
         std::vector<float> perm_init(4 * P.n);
@@ -955,6 +1039,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         }
         TGVX_HIP(hipGetLastError());
         int done = 0;
+        double work = -1.0;  // cube-iterations actually run, when not n * done
         double Pv = 0.0, Dv = 0.0;
         if (leaves) {
             // tgv_minimize_oc: snapshot u_bar / v_bar, then each treetop leaf's
@@ -990,7 +1075,71 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             done = p->iters;
             const double g0 = now_ms();
             eval_gap<S>(c, a, p, &Pv, &Dv);
+            if (P.nblocks > 1) {
+                std::vector<double> rg;
+                eval_gap_blocks<S>(c, a, p, li, rg);
+                for (uint32_t b = 0; b < P.nblocks; ++b) {
+                    const bool has = P.seg[li][b + 1] > P.seg[li][b];
+                    P.block_iters[li * P.nblocks + b] = has ? done : -1;
+                    P.block_gap[li * P.nblocks + b] = has ? rg[b] : 0.0;
+                }
+            }
             gap_ms += now_ms() - g0;
+        } else if (P.nblocks > 1) {
+            // several blocks: iterate the whole level until every block has
+            // reached the gap; each block's count is where it first did
+            std::vector<double> rg;
+            int* bi = &P.block_iters[li * P.nblocks];
+            std::vector<uint8_t> act(P.nblocks);
+            auto upload = [&] {
+                for (uint32_t b = 0; b < P.nblocks; ++b) act[b] = bi[b] < 0 && P.seg[li][b + 1] > P.seg[li][b];
+                TGVX_HIP(hipMemcpyAsync(P.d_bact.p, act.data(), P.nblocks, hipMemcpyHostToDevice, c->stream));
+            };
+            a.blk = P.d_blk.as<int32_t>();
+            a.bact = P.d_bact.as<uint8_t>();
+            double g0 = now_ms();
+            eval_gap_blocks<S>(c, a, p, li, rg);
+            gap_ms += now_ms() - g0;
+            auto mark = [&] {
+                bool all = true;
+                for (uint32_t b = 0; b < P.nblocks; ++b) {
+                    if (!std::isfinite(rg[b]))
+                        throw Error(TGVX_E_NUMERIC, "non-finite gap at depth " + std::to_string(L.depth));
+                    if (P.seg[li][b + 1] == P.seg[li][b]) continue;
+                    if (rg[b] <= p->gap_tol) {
+                        if (bi[b] < 0) {
+                            bi[b] = done;
+                            P.block_gap[li * P.nblocks + b] = rg[b];
+                        }
+                    } else {
+                        all = false;
+                    }
+                }
+                return all;
+            };
+            while (!mark() && done < max_iters) {
+                const int k = std::min(chunk, max_iters - done);
+                upload();  // blocks that reached the gap keep their state
+                TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+                run_iters<S>(c, a, p, k, false);
+                TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+                TGVX_HIP(hipEventSynchronize(c->ev[1]));
+                done += k;
+                g0 = now_ms();
+                eval_gap_blocks<S>(c, a, p, li, rg);
+                gap_ms += now_ms() - g0;
+                float ms = 0.f;
+                TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                it_ms += ms;
+            }
+            work = 0.0;
+            for (uint32_t b = 0; b < P.nblocks; ++b) {
+                if (bi[b] < 0) P.block_gap[li * P.nblocks + b] = rg[b];  // not reached (or no cube there)
+                work += (double)(P.seg[li][b + 1] - P.seg[li][b]) * (bi[b] < 0 ? done : bi[b]);
+            }
+            a.blk = nullptr;
+            a.bact = nullptr;
+            eval_gap<S>(c, a, p, &Pv, &Dv);
         } else {
             double g0 = now_ms();
             eval_gap<S>(c, a, p, &Pv, &Dv);
@@ -1020,7 +1169,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             r.level_energy[li] = Pv;
             r.level_dual[li] = Dv;
         }
-        r.leaf_iters += (double)L.n * done;
+        r.leaf_iters += work >= 0.0 ? work : (double)L.n * done;
     }
     TGVX_HIP(hipStreamSynchronize(c->stream));
     r.n_levels = (int)std::min<size_t>(P.levels.size(), TGVX_MAX_LEVELS);
@@ -1102,7 +1251,7 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
 }  // namespace
 
 void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb, int precision,
-                  int op_mode) {
+                  int op_mode, const uint64_t* blk_start, uint32_t nblocks) {
     if (nb <= 0 || nb > 32) throw Error(TGVX_E_ARG, "nb must be in 1..32");
     if (n >= (1ull << 30)) throw Error(TGVX_E_ARG, "at most 2^30 - 1 cubes");
     if (precision < TGVX_FP64 || precision > TGVX_BF16) throw Error(TGVX_E_ARG, "bad precision");
@@ -1116,6 +1265,19 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     P.precision = precision;
     P.op_mode = op_mode;
     P.elem = precision == TGVX_FP64 ? 8 : (precision == TGVX_FP32 ? 4 : 2);
+    // blocks: contiguous input ranges [bs[b], bs[b+1]) (one block by default)
+    std::vector<uint64_t> bs;
+    if (blk_start && nblocks > 0) {
+        bs.assign(blk_start, blk_start + nblocks + 1);
+        if (bs[0] != 0 || bs[nblocks] != n) throw Error(TGVX_E_ARG, "block ranges must cover the cubes");
+        for (uint32_t b = 0; b < nblocks; ++b)
+            if (bs[b + 1] < bs[b]) throw Error(TGVX_E_ARG, "block ranges must be ascending");
+    } else {
+        bs = {0, n};
+        nblocks = 1;
+    }
+    P.nblocks = nblocks;
+    P.block_start = bs;
     // level-major permutation (counting sort by depth, stable)
     std::vector<uint64_t> cnt(22, 0);
     for (uint64_t i = 0; i < n; ++i) {
@@ -1137,24 +1299,45 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
         P.perm[g] = (uint32_t)i;
         P.inv[i] = (uint32_t)g;
     }
-    // code map: sorted codes, last writer (largest input index) per code
-    std::vector<std::pair<uint64_t, uint32_t>> kv(n);
-    bool sorted = true;
-    for (uint64_t i = 0; i < n; ++i) {
-        kv[i] = {((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i};
-        if (i && kv[i].first < kv[i - 1].first) sorted = false;
+    // per-level, per-block segments of the positions (a level's positions are
+    // in input order, hence grouped by block)
+    P.seg.assign(P.levels.size(), std::vector<uint32_t>(nblocks + 1, 0));
+    std::vector<int32_t> pblk(n);
+    for (size_t li = 0; li < P.levels.size(); ++li) {
+        const Level& L = P.levels[li];
+        std::vector<uint32_t>& sg = P.seg[li];
+        uint32_t b = 0;
+        for (uint32_t t = 0; t < L.n; ++t) {
+            const uint64_t i = P.perm[L.off + t];
+            while (i >= bs[b + 1]) ++b;
+            pblk[L.off + t] = (int32_t)b;
+            sg[b + 1]++;
+        }
+        for (uint32_t k = 0; k < nblocks; ++k) sg[k + 1] += sg[k];
     }
-    if (!sorted) std::stable_sort(kv.begin(), kv.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    // code map of each block: sorted codes, last writer (largest input index)
     std::vector<uint64_t> ucode;
     std::vector<int32_t> uidx;
+    std::vector<uint32_t> uoff(nblocks + 1, 0);
     ucode.reserve(n);
     uidx.reserve(n);
-    for (uint64_t i = 0; i < n; ++i) {
-        if (i + 1 < n && kv[i + 1].first == kv[i].first) continue;
-        ucode.push_back(kv[i].first);
-        uidx.push_back((int32_t)P.inv[kv[i].second]);
+    for (uint32_t b = 0; b < nblocks; ++b) {
+        std::vector<std::pair<uint64_t, uint32_t>> kv;
+        kv.reserve(bs[b + 1] - bs[b]);
+        bool sorted = true;
+        for (uint64_t i = bs[b]; i < bs[b + 1]; ++i) {
+            kv.push_back({((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i});
+            if (kv.size() > 1 && kv.back().first < kv[kv.size() - 2].first) sorted = false;
+        }
+        if (!sorted)
+            std::stable_sort(kv.begin(), kv.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (size_t i = 0; i < kv.size(); ++i) {
+            if (i + 1 < kv.size() && kv[i + 1].first == kv[i].first) continue;
+            ucode.push_back(kv[i].first);
+            uidx.push_back((int32_t)P.inv[kv[i].second]);
+        }
+        uoff[b + 1] = (uint32_t)ucode.size();
     }
-    std::vector<std::pair<uint64_t, uint32_t>>().swap(kv);
     // per-position code, depth, vote total and median bin
     std::vector<uint64_t> pcode(n);
     std::vector<int32_t> pdepth(n);
@@ -1182,38 +1365,51 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
         b.ensure(bytes);
         if (bytes) TGVX_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
     };
-    DevBuf d_code, d_depth, d_uc, d_ui;
+    DevBuf d_code, d_depth, d_uc, d_ui, d_uoff, d_doff;
     up(d_code, pcode.data(), 8 * n);
     up(d_depth, pdepth.data(), 4 * n);
-    up(d_uc, ucode.data(), 8 * ucode.size());
-    up(d_ui, uidx.data(), 4 * uidx.size());
+    up(d_uc, ucode.data(), 8 * std::max<size_t>(ucode.size(), 1));
+    up(d_ui, uidx.data(), 4 * std::max<size_t>(uidx.size(), 1));
+    const bool multi = nblocks > 1;
+    if (multi) {
+        up(P.d_blk, pblk.data(), 4 * n);
+        up(d_uoff, uoff.data(), 4 * uoff.size());
+        P.d_bact.ensure(nblocks);
+    }
+    const int32_t* kblk = multi ? P.d_blk.as<int32_t>() : nullptr;
     for (int f = 0; f < 6; ++f) P.d_nb[f].ensure(4 * std::max<uint64_t>(n, 1));
     P.d_parent.ensure(4 * std::max<uint64_t>(n, 1));
     if (n > 0) {
         k_tables<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
             d_code.as<uint64_t>(), d_depth.as<int32_t>(), (uint32_t)n, d_uc.as<uint64_t>(), d_ui.as<int32_t>(),
-            (uint32_t)ucode.size(), P.d_nb[0].as<int32_t>(), P.d_nb[1].as<int32_t>(), P.d_nb[2].as<int32_t>(),
+            (uint32_t)ucode.size(), kblk, d_uoff.as<uint32_t>(), P.d_nb[0].as<int32_t>(), P.d_nb[1].as<int32_t>(), P.d_nb[2].as<int32_t>(),
             P.d_nb[3].as<int32_t>(), P.d_nb[4].as<int32_t>(), P.d_nb[5].as<int32_t>(), P.d_parent.as<int32_t>());
         TGVX_HIP(hipGetLastError());
     }
-    // depth-aware table (exact-adjoint mode and the gap): (code, depth) pairs
-    // ascending with the position of the last input cube of each pair
-    std::vector<std::pair<uint64_t, uint32_t>> pk(n);
-    for (uint64_t i = 0; i < n; ++i)
-        pk[i] = {((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i};
-    std::stable_sort(pk.begin(), pk.end(), [&](const auto& x, const auto& y) {
-        return x.first != y.first ? x.first < y.first : cubes[x.second].depth < cubes[y.second].depth;
-    });
+    // depth-aware table (exact-adjoint mode and the gap): per block, (code,
+    // depth) pairs ascending with the position of the last input cube of each
     std::vector<uint64_t> dcode;
     std::vector<int32_t> ddepth, dpos;
-    for (uint64_t i = 0; i < n; ++i) {
-        if (i + 1 < n && pk[i + 1].first == pk[i].first && cubes[pk[i + 1].second].depth == cubes[pk[i].second].depth)
-            continue;
-        dcode.push_back(pk[i].first);
-        ddepth.push_back(cubes[pk[i].second].depth);
-        dpos.push_back((int32_t)P.inv[pk[i].second]);
+    std::vector<uint32_t> doff(nblocks + 1, 0);
+    for (uint32_t b = 0; b < nblocks; ++b) {
+        std::vector<std::pair<uint64_t, uint32_t>> pk;
+        pk.reserve(bs[b + 1] - bs[b]);
+        for (uint64_t i = bs[b]; i < bs[b + 1]; ++i)
+            pk.push_back({((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i});
+        std::stable_sort(pk.begin(), pk.end(), [&](const auto& x, const auto& y) {
+            return x.first != y.first ? x.first < y.first : cubes[x.second].depth < cubes[y.second].depth;
+        });
+        for (size_t i = 0; i < pk.size(); ++i) {
+            if (i + 1 < pk.size() && pk[i + 1].first == pk[i].first &&
+                cubes[pk[i + 1].second].depth == cubes[pk[i].second].depth)
+                continue;
+            dcode.push_back(pk[i].first);
+            ddepth.push_back(cubes[pk[i].second].depth);
+            dpos.push_back((int32_t)P.inv[pk[i].second]);
+        }
+        doff[b + 1] = (uint32_t)dcode.size();
     }
-    std::vector<std::pair<uint64_t, uint32_t>>().swap(pk);
+    if (multi) up(d_doff, doff.data(), 4 * doff.size());
     DevBuf d_dc, d_dd, d_dp;
     up(d_dc, dcode.data(), 8 * std::max<size_t>(dcode.size(), 1));
     up(d_dd, ddepth.data(), 4 * std::max<size_t>(ddepth.size(), 1));
@@ -1236,7 +1432,8 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
         const unsigned grid = (unsigned)((n + 255) / 256);
         k_tables_geo<<<grid, 256, 0, c->stream>>>(d_code.as<uint64_t>(), d_depth.as<int32_t>(), (uint32_t)n,
                                                   d_dc.as<uint64_t>(), d_dd.as<int32_t>(), d_dp.as<int32_t>(),
-                                                  (uint32_t)dcode.size(), P.d_nba[0].as<int32_t>(),
+                                                  (uint32_t)dcode.size(), kblk, d_doff.as<uint32_t>(),
+                                                  P.d_nba[0].as<int32_t>(),
                                                   P.d_nba[1].as<int32_t>(), P.d_nba[2].as<int32_t>(),
                                                   P.d_nbam[0].as<int32_t>(), P.d_nbam[1].as<int32_t>(),
                                                   P.d_nbam[2].as<int32_t>(),

@@ -64,6 +64,11 @@ struct Problem {
     int nb = 0, precision = TGVX_FP32, op_mode = TGVX_OP_REF;
     size_t elem = 4;  // bytes per stored state value
     std::vector<Level> levels;
+    uint32_t nblocks = 1;                    // independent octrees sharing the problem
+    std::vector<uint64_t> block_start;       // input index range of each block
+    std::vector<std::vector<uint32_t>> seg;  // per level: block b owns positions off + [seg[b], seg[b+1])
+    std::vector<int32_t> block_iters;        // per level x block: iterations to reach the gap
+    std::vector<double> block_gap;           // per level x block: final relative gap
     std::vector<uint32_t> perm;      // position -> input index
     std::vector<uint32_t> inv;       // input index -> position
     DevBuf d_nb[6];                  // int32 per position, faces 0..5
@@ -73,6 +78,9 @@ struct Problem {
     DevBuf d_nbam[3];                // int32 depth-aware -x, -y, -z neighbours (warm-start features)
     DevBuf d_nbt[3];                 // int32 the level's cube whose depth-aware +a neighbour this is, or -1
     DevBuf d_bins;                   // uint32 histograms, position order
+    DevBuf d_blk;                    // int32 block of each position (several blocks only)
+    DevBuf d_seg;                    // uint32 one level's block segments
+    DevBuf d_bact;                   // uint8 per block: still iterating (to-gap solves of several blocks)
     DevBuf d_parent_a;               // int32 cube exactly one level coarser at the parent code, or -1
     DevBuf d_state[17];              // u vx vy vz px py pz q0..q5 ub vbx vby vbz
     DevBuf d_save[4];                // scratch copy of u, v of one level
@@ -101,7 +109,7 @@ struct tgvx_ctx {
 namespace tgvx {
 // solve.hip
 void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb, int precision,
-                  int op_mode);
+                  int op_mode, const uint64_t* blk_start = nullptr, uint32_t nblocks = 1);
 void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
                    const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_cube* cubes);
 void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out);
