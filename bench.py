@@ -26,9 +26,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per leaf-iteration: 17 state values read + 17 written,
-# (W, m) and 6 neighbour indices
-ALG_BYTES = {"fp32": 34 * 4 + 8 + 24, "bf16": 34 * 2 + 8 + 24, "fp64": 34 * 8 + 12 + 24}
+ELEM = {"fp32": 4, "bf16": 2, "fp64": 8}
+# algorithmic bytes per leaf-iteration, each value counted once per iteration:
+# two kernels: the 17 state values (u, v, p, q, u_bar, v_bar) read and
+# written, the packed info word (4 B, counted as 8 for (W, m)) and 6
+# neighbour indices; fused: u, v, p, q read and written (u_bar, v_bar never
+# leave the chip), the info word and 6 neighbour indices
+ALG_BYTES_SPLIT = {k: 34 * e + 8 + 24 for k, e in ELEM.items()}
+ALG_BYTES_FUSED = {k: 26 * e + 4 + 24 for k, e in ELEM.items()}
+KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj"}
 
 
 def log(msg):
@@ -48,7 +54,10 @@ def main():
     ap.add_argument("--iters", type=int, default=50, help="iterations per level in one step")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--no-fused", action="store_true", help="two kernels per iteration instead of the fused one")
+    ap.add_argument("--fused", type=int, choices=[0, 1, 2], default=1,
+                    help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused kernel")
+    ap.add_argument("--no-fused", action="store_true", help="same as --fused 0")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start time-to-gap runs")
     ap.add_argument("--gap-tol", type=float, default=1e-4)
     ap.add_argument("--no-gap-run", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
@@ -73,8 +82,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the solver "
-              "shards across GPUs only by treetop leaves (not run here). Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1): the multi-GPU path, sharding by treetop leaves, "
+              "is not built.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -104,8 +113,9 @@ def main():
     levels = ctx.levels()
     log(f"[bench] problem resident in {t_load:.2f} s; levels {levels}")
 
+    variant = 0 if args.no_fused else args.fused
     prm = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, iters=args.iters,
-                      use_graph=0 if args.no_graph else 1, block=args.block, fused=0 if args.no_fused else 1)
+                      use_graph=0 if args.no_graph else 1, block=args.block, fused=variant)
     for i in range(args.warmup):
         ctx.run(prm)
     log(f"[bench] {args.warmup} warmup steps done")
@@ -125,11 +135,10 @@ def main():
     # roofline of the iteration at the finest level: HIP events on the library's stream
     finest, n_f = levels[-1]
     total_ms, dual_ms, primal_ms, fused_ms = ctx.level_bench(finest, prm, 100)
-    fused = bool(prm.fused) and not args.no_fused
+    fused = variant != 0
     it_ms = fused_ms if fused else dual_ms + primal_ms
-    kernel = (f"k_fused_adj (one launch = one iteration), depth {finest}, {n_f} leaves" if fused else
-              f"k_dual + k_primal_adj, depth {finest}, {n_f} leaves")
-    ab = ALG_BYTES[args.precision]
+    kernel = f"{KERNEL_NAMES[variant]}, depth {finest}, {n_f} leaves" + (" (one launch = one iteration)" if fused else "")
+    ab = (ALG_BYTES_FUSED if fused else ALG_BYTES_SPLIT)[args.precision]
     achieved = ab * n_f / (it_ms * 1e-3) / 1e9
     log(f"[bench] depth {finest} ({n_f} leaves): fused {fused_ms * 1e3:.1f} us; two-kernel dual {dual_ms * 1e3:.1f} us "
         f"+ primal {primal_ms * 1e3:.1f} us; 100 its {total_ms:.2f} ms -> {achieved:.0f} GB/s algorithmic")
@@ -146,6 +155,26 @@ def main():
                "iters_per_level": [l["iters"] for l in lv], "final_rel_gap": [l["rel_gap"] for l in lv],
                "reached": all(l["rel_gap"] <= args.gap_tol for l in lv)}
         log(f"[bench] cold solve to gap {args.gap_tol}: {rg.solve_ms:.1f} ms, iters {gap['iters_per_level']}")
+
+    # the same solve warm-started: dual recovered from the initial field, and
+    # the trained network (committed weights) with the per-level fallback
+    warm = None
+    wpath = os.path.join(ROOT, "tgvreconclaude_amd", "weights", "warmstart_mlp.npy")
+    if gap is not None and not args.no_warm and os.path.exists(wpath):
+        ctx.mlp_set(np.load(wpath))
+        warm = {}
+        for name, kw in (("cold_dual", dict(dual_eps=1e-4)),
+                         ("warm", dict(dual_eps=1e-4, warm_start=1, warm_fallback=1))):
+            pw = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, gap_tol=args.gap_tol,
+                             chunk=50, max_iters=20000, use_graph=prm.use_graph, block=args.block, fused=prm.fused,
+                             **kw)
+            rw = ctx.run(pw)
+            lv = rw.levels()
+            warm[name] = {"wall_ms": round(rw.solve_ms, 2), "leaf_iters": rw.leaf_iters,
+                          "iters_per_level": [l["iters"] for l in lv], "warm_levels": [l["warm"] for l in lv],
+                          "reached": all(l["rel_gap"] <= args.gap_tol for l in lv),
+                          "speedup_vs_cold": round(gap["wall_ms"] / rw.solve_ms, 3)}
+            log(f"[bench] {name} solve to gap: {rw.solve_ms:.1f} ms ({warm[name]['speedup_vs_cold']}x cold)")
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -164,8 +193,10 @@ def main():
     # summary of this workload (counters cannot be read from inside the run)
     traffic = None
     pj = os.path.join(ROOT, "profiles", "tgv_iteration_depth10_torus_box.json")
-    if args.precision == "fp32" and os.path.exists(pj) and n_f == json.load(open(pj))["leaves"]:
-        traffic = json.load(open(pj))["traffic_bytes_per_iteration"]
+    if args.precision == "fp32" and os.path.exists(pj):
+        rec = json.load(open(pj)).get(KERNEL_NAMES[variant])
+        if rec and rec["leaves"] == n_f:
+            traffic = rec["traffic_bytes_per_iteration"]
 
     out = {
         "metric": "TGV leaf-iter/s + time-to-gap<=1e-4 (learned warm start vs cold), 1 GPU",
@@ -186,10 +217,10 @@ def main():
                    "parallelism": "single-gpu", "device": arch, "cus": cus},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_unit": "bytes per iteration (dual + primal launch), rocprofv3 counters",
+                     "traffic_unit": "HBM bytes per iteration at the finest level, rocprofv3 FETCH_SIZE/WRITE_SIZE",
                      "kernel": kernel, "fused_us": fused_ms * 1e3,
                      "alg_bytes_per_leaf_iter": ab, "dual_us": dual_ms * 1e3, "primal_us": primal_ms * 1e3},
-        "time_to_gap": {"cold": gap, "warm": None},
+        "time_to_gap": {"cold": gap, "warm": warm},
         "cpu_baseline": cpu,
         "setup_s": {"scene": round(t_scene, 2), "histograms": round(t_hist, 3), "load": round(t_load, 3)},
     }

@@ -96,8 +96,10 @@ typedef struct {
                               initialisation + network correction, with warm_fallback as above */
     int32_t warm_dual;     /* with an initial field and dual_eps > 0: 0 recovers p, q from the field,
                               1 keeps the p, q recovered from the cold initialisation */
-    int32_t fused;         /* exact-adjoint mode: 1 (default) runs one fused kernel per iteration,
-                              identical results to the two-kernel iteration (0) */
+    int32_t fused;         /* exact-adjoint mode, results identical in all cases: 0 a dual and a
+                              primal kernel per iteration; 1 (default) one fused kernel per
+                              iteration sharing a tile's primal values through LDS; 2 one fused
+                              kernel recomputing every neighbour's primal values */
     int32_t reserved[1];
 } tgvx_params;
 

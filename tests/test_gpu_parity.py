@@ -144,12 +144,14 @@ def test_cold_dual_recovery_to_gap(ctx, sphere):
     assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("variant", [1, 2])
 @pytest.mark.parametrize("prec", [FP32, FP64])
-def test_fused_iteration_identical_to_two_kernels(ctx, sphere, prec):
-    # exact-adjoint mode: one fused kernel per iteration == dual + primal kernels, bit for bit
+def test_fused_iteration_identical_to_two_kernels(ctx, sphere, prec, variant):
+    # exact-adjoint mode: one fused kernel per iteration (LDS-tiled or not) ==
+    # dual + primal kernels, bit for bit
     sc, hist = sphere
     a, _ = ctx.solve(sc.cubes, hist, TGVParams(alpha0=2.0, alpha1=1.0, iters=75, precision=prec, op_mode=OP_ADJ,
-                                               fused=1))
+                                               fused=variant))
     b, _ = ctx.solve(sc.cubes, hist, TGVParams(alpha0=2.0, alpha1=1.0, iters=75, precision=prec, op_mode=OP_ADJ,
                                                fused=0))
     a, b = _state13(a), _state13(b)
@@ -157,10 +159,11 @@ def test_fused_iteration_identical_to_two_kernels(ctx, sphere, prec):
                           b.view(np.uint32 if b.dtype == np.float32 else np.uint64))
 
 
-def test_fused_bf16_matches_two_kernels(ctx, sphere):
+@pytest.mark.parametrize("variant", [1, 2])
+def test_fused_bf16_matches_two_kernels(ctx, sphere, variant):
     sc, hist = sphere
     a, _ = ctx.solve(sc.cubes, hist, TGVParams(alpha0=2.0, alpha1=1.0, iters=40, precision=BF16, op_mode=OP_ADJ,
-                                               fused=1))
+                                               fused=variant))
     b, _ = ctx.solve(sc.cubes, hist, TGVParams(alpha0=2.0, alpha1=1.0, iters=40, precision=BF16, op_mode=OP_ADJ,
                                                fused=0))
     assert np.array_equal(_state13(a).view(np.uint32), _state13(b).view(np.uint32))

@@ -230,9 +230,11 @@ struct Set13 {
 
 // Passes 3-5 of cube g in the exact-adjoint mode, from the values in src:
 // the new (u, v) in nw, the old ones in od.
+// p_own / q_own receive the cube's own p, q from src.
 template <class S>
 __device__ inline void primal_adj_at(const KArgs<S>& a, const Set13<S>& src, long g, typename Io<S>::T nw[4],
-                                     typename Io<S>::T od[4]) {
+                                     typename Io<S>::T od[4], typename Io<S>::T p_own[3],
+                                     typename Io<S>::T q_own[6]) {
     using IO = Io<S>;
     using T = typename IO::T;
     const uint32_t w = a.info[g];
@@ -288,6 +290,16 @@ __device__ inline void primal_adj_at(const KArgs<S>& a, const Set13<S>& src, lon
         od[1 + c] = IO::ld(src.f[F_VX + c], g);
         nw[1 + c] = od[1 + c] + a.tau * (p[c] + dq[c]);
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p_own[c] = p[c];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q_own[k] = qs[k];
+}
+template <class S>
+__device__ inline void primal_adj_at(const KArgs<S>& a, const Set13<S>& src, long g, typename Io<S>::T nw[4],
+                                     typename Io<S>::T od[4]) {
+    typename Io<S>::T p[3], q[6];
+    primal_adj_at(a, src, g, nw, od, p, q);
 }
 
 // ---- phase P, exact adjoint (no reference counterpart): the primal step
@@ -357,6 +369,94 @@ __global__ void __launch_bounds__(256) k_fused_adj(KArgs<S> a, Set13<S> src, Set
     T q[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) q[k] = IO::ld(src.f[F_Q0 + k], g);
+    dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::st(dst.f[F_U + c], g, nw[c]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) IO::st(dst.f[F_PX + c], g, p[c]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) IO::st(dst.f[F_Q0 + k], g, q[k]);
+}
+
+// ---- one fused iteration over a tile of kFuseTile consecutive positions
+// (no reference counterpart). Same values as k_fused_adj, but the primal
+// results of the tile's own cubes are shared through LDS: only +x, +y, +z
+// neighbours outside the tile (a few tenths per cube in Morton order) are
+// recomputed, once per reference, by the whole workgroup.
+constexpr int kFuseTile = 256;
+
+template <class S>
+__global__ void __launch_bounds__(kFuseTile) k_fused_tile(KArgs<S> a, Set13<S> src, Set13<S> dst) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    __shared__ T own_bar[4][kFuseTile];
+    __shared__ T halo_bar[4][3 * kFuseTile];
+    __shared__ int32_t halo_j[3 * kFuseTile];
+    __shared__ int32_t n_halo;
+    const uint32_t t0 = (uint32_t)xcd_tile() * kFuseTile;
+    const uint32_t t = t0 + threadIdx.x;
+    const long g = (long)a.off + t;
+    const bool run = t < a.n && !masked(a, g);
+    if (threadIdx.x == 0) n_halo = 0;
+    T nw[4], p[3], q[6];
+    if (run) {
+        T od[4];
+        primal_adj_at(a, src, g, nw, od, p, q);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) own_bar[c][threadIdx.x] = IO::rt(T(2) * nw[c] - od[c]);
+    }
+    __syncthreads();
+    // where each +a neighbour's bar comes from: >= 0 an LDS slot (own tile
+    // below kFuseTile, halo above), -1 this cube (no neighbour), -2 a cube of
+    // another level (its frozen u_bar, v_bar)
+    int32_t src_slot[3] = {-1, -1, -1};
+    int32_t js[3] = {-1, -1, -1};
+    const long tile_lo = (long)a.off + t0;
+    const long tile_hi = (long)a.off + min(t0 + (uint32_t)kFuseTile, a.n);
+    if (run) {
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int j = a.nbp[ax][g];
+            js[ax] = j;
+            if (j < 0) {
+                src_slot[ax] = -1;
+            } else if (j >= tile_lo && j < tile_hi) {
+                src_slot[ax] = (int32_t)(j - tile_lo);
+            } else if ((uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n) {
+                const int h = atomicAdd(&n_halo, 1);
+                halo_j[h] = j;
+                src_slot[ax] = kFuseTile + h;
+            } else {
+                src_slot[ax] = -2;
+            }
+        }
+    }
+    __syncthreads();
+    const int nh = n_halo;
+    for (int h = threadIdx.x; h < nh; h += kFuseTile) {
+        T n2[4], o2[4];
+        primal_adj_at(a, src, (long)halo_j[h], n2, o2);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) halo_bar[c][h] = IO::rt(T(2) * n2[c] - o2[c]);
+    }
+    __syncthreads();
+    if (!run) return;
+    T bar[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bar[0][c] = own_bar[c][threadIdx.x];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int sl = src_slot[ax];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            T v;
+            if (sl == -1) v = bar[0][c];
+            else if (sl == -2) v = IO::ld(a.s[F_UB + c], js[ax]);
+            else if (sl < kFuseTile) v = own_bar[c][sl];
+            else v = halo_bar[c][sl - kFuseTile];
+            bar[1 + ax][c] = v;
+        }
+    }
     dual_step(p, q, bar, a.sigma, a.a0, a.a1);
 #pragma unroll
     for (int c = 0; c < 4; ++c) IO::st(dst.f[F_U + c], g, nw[c]);
@@ -843,14 +943,17 @@ void launch_iter(tgvx_ctx* c, KArgs<S>& a, int op_mode, int block, bool frozen) 
 // `iters` >= 2 exact-adjoint iterations as: dual of the first, iters - 1
 // fused (primal n, dual n+1) passes alternating between the main and the
 // alternate set, primal of the last back into the main set.
+// variant 1: k_fused_tile, 2: k_fused_adj
 template <class S>
-void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, int block) {
+void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, int block, int variant) {
     const unsigned grid = (a.n + block - 1) / block;
     Set13<S> cur = main_set(a), oth = alt_set<S>(c);
     const Set13<S> mainset = cur;
     k_dual<S, false><<<grid, block, 0, c->stream>>>(a);
+    const unsigned tiles = (a.n + kFuseTile - 1) / kFuseTile;
     for (int i = 1; i < iters; ++i) {
-        k_fused_adj<S><<<grid, block, 0, c->stream>>>(a, cur, oth);
+        if (variant == 1) k_fused_tile<S><<<tiles, kFuseTile, 0, c->stream>>>(a, cur, oth);
+        else k_fused_adj<S><<<grid, block, 0, c->stream>>>(a, cur, oth);
         std::swap(cur, oth);
     }
     const int copy = cur.f[0] != mainset.f[0];
@@ -887,13 +990,13 @@ void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool f
     const bool fused = use_fused(p, frozen, iters);
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (p->use_graph && !frozen && iters > 1) {
-        GraphKey key{a.off, a.n, iters, p->op_mode, block, c->prob.precision, fused ? 1 : 0, a.bact ? 1 : 0};
+        GraphKey key{a.off, a.n, iters, p->op_mode, block, c->prob.precision, fused ? p->fused : 0, a.bact ? 1 : 0};
         GraphCache& gc = g_graphs[c];
         auto it = gc.m.find(key);
         if (it == gc.m.end()) {
             hipGraph_t graph;
             TGVX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            if (fused) launch_fused<S>(c, a, iters, block);
+            if (fused) launch_fused<S>(c, a, iters, block, p->fused);
             else
                 for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, false);
             TGVX_HIP(hipStreamEndCapture(c->stream, &graph));
@@ -905,7 +1008,7 @@ void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool f
         TGVX_HIP(hipGraphLaunch(it->second, c->stream));
         return;
     }
-    if (fused) launch_fused<S>(c, a, iters, block);
+    if (fused) launch_fused<S>(c, a, iters, block, p->fused);
     else
         for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, frozen);
     TGVX_HIP(hipGetLastError());
@@ -1228,14 +1331,16 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     }
     *dual_ms = sd / k;
     *primal_ms = sp / k;
-    // the fused pass alone (an even count, so the main set ends current)
+    // the fused pass alone (variant as in launch_fused; an even count, so the
+    // main set ends current)
     double sf = 0.0;
     if (p->op_mode == TGVX_OP_ADJ) {
         Set13<S> cur = main_set(a), oth = alt_set<S>(c);
         const int kf = 2 * k;
         for (int i = 0; i < kf; ++i) {
             TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-            k_fused_adj<S><<<grid, block, 0, c->stream>>>(a, cur, oth);
+            if (p->fused == 2) k_fused_adj<S><<<grid, block, 0, c->stream>>>(a, cur, oth);
+            else k_fused_tile<S><<<(a.n + kFuseTile - 1) / kFuseTile, kFuseTile, 0, c->stream>>>(a, cur, oth);
             TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
             TGVX_HIP(hipEventSynchronize(c->ev[1]));
             float m1 = 0.f;
