@@ -61,6 +61,12 @@ def main():
     ap.add_argument("--gap-tol", type=float, default=1e-4)
     ap.add_argument("--no-gap-run", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--workload", choices=["depth10", "batched"], default="depth10",
+                    help="batched: 256 independent blocks of log-uniform size in one problem vs one problem per block")
+    ap.add_argument("--blocks", type=int, default=256)
+    ap.add_argument("--tune", type=int, default=1, help="1: autotune launch configurations before timing")
+    ap.add_argument("--batched-gap", type=float, default=0.0,
+                    help="batched workload: also time solves to this relative gap (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -92,6 +98,8 @@ def main():
     from tgvreconclaude_amd import scene
 
     prec = {"fp32": T.FP32, "bf16": T.BF16, "fp64": T.FP64}[args.precision]
+    if args.workload == "batched":
+        return batched(args, T, scene, prec)
     t0 = time.time()
     log(f"[bench] generating scene: {args.views} views {args.width}x{args.height}, depth <= {args.max_depth}")
     sc = scene.torus_box_scene(views=args.views, W=args.width, H=args.height, max_depth=args.max_depth)
@@ -116,6 +124,12 @@ def main():
     variant = 0 if args.no_fused else args.fused
     prm = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, iters=args.iters,
                       use_graph=0 if args.no_graph else 1, block=args.block, fused=variant)
+    tuning = None
+    if args.tune:
+        tuning = ctx.autotune(prm, iters=20)
+        prm.tune = 1
+        log(f"[bench] autotuned (depth, variant, block, xcd, us): "
+            f"{[(l['depth'], l['variant'], l['block'], l['xcd'], round(l['best_us'], 1)) for l in tuning]}")
     for i in range(args.warmup):
         ctx.run(prm)
     log(f"[bench] {args.warmup} warmup steps done")
@@ -135,9 +149,13 @@ def main():
     # roofline of the iteration at the finest level: HIP events on the library's stream
     finest, n_f = levels[-1]
     total_ms, dual_ms, primal_ms, fused_ms = ctx.level_bench(finest, prm, 100)
+    if tuning:
+        variant = tuning[-1]["variant"]
     fused = variant != 0
     it_ms = fused_ms if fused else dual_ms + primal_ms
     kernel = f"{KERNEL_NAMES[variant]}, depth {finest}, {n_f} leaves" + (" (one launch = one iteration)" if fused else "")
+    if tuning:
+        kernel += f", workgroup {tuning[-1]['block']}, {'XCD-contiguous' if tuning[-1]['xcd'] else 'plain'} tile order"
     ab = (ALG_BYTES_FUSED if fused else ALG_BYTES_SPLIT)[args.precision]
     achieved = ab * n_f / (it_ms * 1e-3) / 1e9
     log(f"[bench] depth {finest} ({n_f} leaves): fused {fused_ms * 1e3:.1f} us; two-kernel dual {dual_ms * 1e3:.1f} us "
@@ -147,7 +165,8 @@ def main():
     gap = None
     if not args.no_gap_run:
         pg = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, gap_tol=args.gap_tol,
-                         chunk=50, max_iters=20000, use_graph=prm.use_graph, block=args.block, fused=prm.fused)
+                         chunk=50, max_iters=20000, use_graph=prm.use_graph, block=args.block, fused=prm.fused,
+                         tune=prm.tune)
         rg = ctx.run(pg)
         lv = rg.levels()
         gap = {"tol": args.gap_tol, "wall_ms": round(rg.solve_ms, 2), "iterate_ms": round(rg.iterate_ms, 2),
@@ -167,7 +186,7 @@ def main():
                          ("warm", dict(dual_eps=1e-4, warm_start=1, warm_fallback=1))):
             pw = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, gap_tol=args.gap_tol,
                              chunk=50, max_iters=20000, use_graph=prm.use_graph, block=args.block, fused=prm.fused,
-                             **kw)
+                             tune=prm.tune, **kw)
             rw = ctx.run(pw)
             lv = rw.levels()
             warm[name] = {"wall_ms": round(rw.solve_ms, 2), "leaf_iters": rw.leaf_iters,
@@ -222,7 +241,99 @@ def main():
                      "alg_bytes_per_leaf_iter": ab, "dual_us": dual_ms * 1e3, "primal_us": primal_ms * 1e3},
         "time_to_gap": {"cold": gap, "warm": warm},
         "cpu_baseline": cpu,
+        "autotune": tuning,
         "setup_s": {"scene": round(t_scene, 2), "histograms": round(t_hist, 3), "load": round(t_load, 3)},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def batched(args, T, scene, prec):
+    """256 independent octree blocks (seeded unions of 1-6 spheres and boxes,
+    about 4k-512k cubes each, log-uniform, 10 bins), solved as one resident
+    problem -- every iteration of a depth is one launch over that depth's
+    cubes of all blocks, 50-iteration chunks as graphs -- against one problem
+    per block. A step is one coarse-to-fine sweep of 50 iterations per level."""
+    import numpy as np
+
+    ctx = T.Context(0)
+    arch, cus = ctx.device_info()
+    targets = scene.batch_targets(args.blocks)
+    blocks, t0 = [], time.time()
+    for b, tgt in enumerate(targets):
+        sc = scene.batch_block(4000 + b, tgt)
+        blocks.append((sc.cubes, ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)))
+        if b % 16 == 15:
+            log(f"[bench] {b + 1}/{len(targets)} blocks generated ({time.time() - t0:.0f} s)")
+    counts = np.array([len(c) for c, _ in blocks])
+    n = int(counts.sum())
+    log(f"[bench] {n} cubes in {len(blocks)} blocks (min {counts.min()}, median {int(np.median(counts))}, "
+        f"max {counts.max()}) in {time.time() - t0:.0f} s")
+
+    prm = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, iters=args.iters,
+                      use_graph=0 if args.no_graph else 1, fused=0 if args.no_fused else args.fused)
+    ctx.load_blocks(blocks, precision=prec, op_mode=T.OP_ADJ)
+    tuning = None
+    if args.tune:
+        tuning = ctx.autotune(prm, iters=20)
+        prm.tune = 1
+        log(f"[bench] autotuned: {[(l['depth'], l['variant'], l['block'], l['xcd']) for l in tuning]}")
+    for _ in range(args.warmup):
+        ctx.run(prm)
+    ts = time.perf_counter()
+    li = 0.0
+    for _ in range(args.steps):
+        li += ctx.run(prm).leaf_iters
+    dt = time.perf_counter() - ts
+    value = li / dt
+    log(f"[bench] batched: {args.steps} sweeps in {dt:.3f} s: {value:.4g} leaf-iter/s")
+
+    # one problem per block: the same sweep, each block loaded and run alone
+    per_t, per_li = 0.0, 0.0
+    for cubes, hist in blocks:
+        ctx.load(cubes, hist, precision=prec, op_mode=T.OP_ADJ)
+        ctx.run(prm)  # builds this block's graphs
+        t1 = time.perf_counter()
+        per_li += ctx.run(prm).leaf_iters
+        per_t += time.perf_counter() - t1
+    per_value = per_li / per_t
+    log(f"[bench] one problem per block: {per_t * 1e3:.1f} ms per sweep of all blocks: {per_value:.4g} leaf-iter/s")
+
+    gap = None
+    if args.batched_gap > 0:
+        pg = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, gap_tol=args.batched_gap,
+                         chunk=50, max_iters=20000, use_graph=prm.use_graph, fused=prm.fused, tune=prm.tune)
+        ctx.load_blocks(blocks, precision=prec, op_mode=T.OP_ADJ)
+        ctx.run(pg)
+        t1 = time.perf_counter()
+        rb = ctx.run(pg)
+        tb = time.perf_counter() - t1
+        it, _ = ctx.block_report()
+        tp = 0.0
+        for cubes, hist in blocks:
+            ctx.load(cubes, hist, precision=prec, op_mode=T.OP_ADJ)
+            ctx.run(pg)
+            t1 = time.perf_counter()
+            ctx.run(pg)
+            tp += time.perf_counter() - t1
+        gap = {"tol": args.batched_gap, "batched_s": round(tb, 3), "per_block_s": round(tp, 3),
+               "speedup": round(tp / tb, 3), "batched_leaf_iters": rb.leaf_iters,
+               "max_block_iters_per_level": it.max(axis=1).tolist()}
+        log(f"[bench] to gap {args.batched_gap}: batched {tb:.2f} s, one problem per block {tp:.2f} s")
+
+    out = {
+        "metric": "TGV leaf-iter/s, 256 independent blocks batched, 1 GPU",
+        "value": value, "unit": "leaf-iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp32": "f32", "bf16": "bf16-storage/f32", "fp64": "f64"}[args.precision], "data": "synthetic",
+        "config": {"workload": f"batched_{len(blocks)}_blocks", "cubes": n, "blocks": len(blocks),
+                   "cubes_min_median_max": [int(counts.min()), int(np.median(counts)), int(counts.max())],
+                   "iters_per_level_per_step": args.iters, "op_mode": "adjoint", "bins": 10,
+                   "parallelism": "single-gpu", "device": arch, "cus": cus},
+        "one_problem_per_block": {"value": per_value, "ms_per_sweep": per_t * 1e3,
+                                  "speedup_batched": round(value / per_value, 3)},
+        "to_gap": gap,
+        "autotune": tuning,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -100,7 +100,8 @@ typedef struct {
                               primal kernel per iteration; 1 (default) one fused kernel per
                               iteration sharing a tile's primal values through LDS; 2 one fused
                               kernel recomputing every neighbour's primal values */
-    int32_t reserved[1];
+    int32_t tune;          /* 1: take each level's launch configuration from tgvx_autotune's table
+                              (levels without an entry use fused / block as given) */
 } tgvx_params;
 
 #define TGVX_MAX_LEVELS 22
@@ -174,6 +175,27 @@ int tgvx_problem_levels(tgvx_ctx* ctx, int32_t* depths, uint32_t* counts, int32_
  * bracketed launches. */
 int tgvx_level_bench(tgvx_ctx* ctx, int32_t depth, const tgvx_params* params, int32_t iters,
                      double* total_ms, double* dual_ms, double* primal_ms, double* fused_ms);
+
+/* ---- launch autotuner (no reference counterpart)
+ * For every level of the loaded problem, times `iters` iterations of each
+ * launch configuration -- kernel variant (dual + primal kernels, LDS-tiled
+ * fused, untiled fused; exact-adjoint mode only for the fused ones),
+ * workgroup / tile size (64, 128, 256) and tile order (XCD-contiguous or
+ * plain) -- and stores the fastest in the context, per (precision, operator
+ * mode, floor(log2 leaves)); solves with params.tune = 1 use it. Every
+ * configuration gives identical results. The problem's state is overwritten
+ * (a later solve re-initialises it). */
+typedef struct {
+    int32_t n_levels;
+    int32_t level_depth[TGVX_MAX_LEVELS];
+    uint32_t level_leaves[TGVX_MAX_LEVELS];
+    int32_t variant[TGVX_MAX_LEVELS]; /* 0 two kernels, 1 tiled fused, 2 untiled fused */
+    int32_t block[TGVX_MAX_LEVELS];
+    int32_t xcd[TGVX_MAX_LEVELS];
+    double best_us[TGVX_MAX_LEVELS];    /* per iteration, chosen configuration */
+    double default_us[TGVX_MAX_LEVELS]; /* per iteration, params' own configuration */
+} tgvx_tune_report;
+int tgvx_autotune(tgvx_ctx* ctx, const tgvx_params* params, int32_t iters, tgvx_tune_report* report);
 
 /* ---- several independent octrees in one problem (no reference counterpart)
  * Blocks are concatenated level-major: every iteration of a depth is one

@@ -1,0 +1,48 @@
+"""Launch configurations and the autotuner.
+
+Every launch configuration (kernel variant, workgroup / tile size, tile
+order) runs the same arithmetic per cube, so results must be bit-identical;
+the autotuner must pick, per level, a configuration at least as fast as the
+parameters' own (which is among its candidates) and solves that use its
+table must match untuned solves bit for bit.
+"""
+import numpy as np
+import pytest
+
+from tgvreconclaude_amd import BF16, FP32, FP64, OP_ADJ, OP_REF, TGVParams
+
+pytestmark = pytest.mark.gpu
+
+
+def _raw(st):
+    return np.ascontiguousarray(st).view(np.uint8)
+
+
+@pytest.mark.parametrize("fused,block", [(0, 64), (0, 128), (1, 128), (2, 64), (2, 128)])
+def test_launch_shapes_identical(ctx, sphere, fused, block):
+    sc, hist = sphere
+    base = dict(alpha0=2.0, alpha1=1.0, iters=60, precision=FP32, op_mode=OP_ADJ)
+    ref, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=0, block=256, **base))
+    got, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=fused, block=block, **base))
+    assert np.array_equal(_raw(got), _raw(ref))
+
+
+@pytest.mark.parametrize("prec,mode", [(FP32, OP_ADJ), (BF16, OP_ADJ), (FP32, OP_REF), (FP64, OP_ADJ)])
+def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
+    sc, hist = sphere
+    p = TGVParams(alpha0=2.0, alpha1=1.0, iters=60, precision=prec, op_mode=mode)
+    ctx.load(sc.cubes, hist, precision=prec, op_mode=mode)
+    levels = ctx.autotune(p, iters=10)
+    assert [(l["depth"], l["leaves"]) for l in levels] == [tuple(x) for x in ctx.levels()]
+    for l in levels:
+        assert l["variant"] in ((0, 1, 2) if mode == OP_ADJ else (0,))
+        assert l["block"] in (64, 128, 256) and l["xcd"] in (0, 1)
+        if l["variant"] == 1:
+            assert l["block"] in (128, 256)
+        assert 0 < l["best_us"] <= l["default_us"]
+    ctx.run(p)
+    plain = ctx.state()
+    p.tune = 1
+    ctx.run(p)
+    tuned = ctx.state()
+    assert np.array_equal(_raw(tuned), _raw(plain))

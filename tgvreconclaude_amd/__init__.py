@@ -177,6 +177,12 @@ class Context:
         self._block_sizes = [len(c) for c, _ in keep]
         self._check(self._L.tgvx_problem_load_blocks(self.h, arr, len(keep), nb, precision, op_mode))
 
+    def autotune(self, params, iters=20):
+        """Choose each level's launch configuration for the loaded problem (see tgvx_autotune)."""
+        rep = N.TuneReport()
+        self._check(self._L.tgvx_autotune(self.h, C.byref(params), iters, C.byref(rep)))
+        return rep.levels()
+
     def solve_batched(self, blocks, params=None):
         """tgv_minimize of several independent octrees in one resident problem; (states per block, Report)."""
         params = params or TGVParams()

@@ -32,7 +32,7 @@ class Params(C.Structure):
                 ("chunk", C.c_int32), ("max_iters", C.c_int32), ("gap_tol", C.c_double),
                 ("warm_fallback", C.c_int32), ("use_graph", C.c_int32), ("block", C.c_int32),
                 ("dual_eps", C.c_float), ("warm_start", C.c_int32), ("warm_dual", C.c_int32),
-                ("fused", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("fused", C.c_int32), ("tune", C.c_int32)]
 
 
 class Report(C.Structure):
@@ -51,6 +51,20 @@ class Report(C.Structure):
                             iters=self.level_iters[i], warm=self.level_warm[i], P=P, D=D,
                             rel_gap=(P - D) / max(1.0, abs(P))))
         return out
+
+
+ML = MAX_LEVELS
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("n_levels", C.c_int32), ("level_depth", C.c_int32 * ML), ("level_leaves", C.c_uint32 * ML),
+                ("variant", C.c_int32 * ML), ("block", C.c_int32 * ML), ("xcd", C.c_int32 * ML),
+                ("best_us", C.c_double * ML), ("default_us", C.c_double * ML)]
+
+    def levels(self):
+        return [{"depth": self.level_depth[i], "leaves": self.level_leaves[i], "variant": self.variant[i],
+                 "block": self.block[i], "xcd": self.xcd[i], "best_us": self.best_us[i],
+                 "default_us": self.default_us[i]} for i in range(self.n_levels)]
 
 
 class Block(C.Structure):
@@ -73,7 +87,7 @@ TGVX_SYMBOLS = [
     "tgvx_problem_load", "tgvx_problem_solve", "tgvx_problem_state", "tgvx_problem_gap",
     "tgvx_problem_levels", "tgvx_level_bench", "tgvx_mlp_set", "tgvx_mlp_get", "tgvx_mlp_forward",
     "tgvx_mlp_loss_grad", "tgvx_mlp_train", "tgvx_problem_features", "tgvx_problem_load_blocks",
-    "tgvx_problem_block_report", "tgvx_solve_batched",
+    "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
@@ -123,6 +137,7 @@ def lib():
         L.tgvx_mlp_train.argtypes = [vp, vp, vp, u64, i32, i32, C.c_float, C.c_uint64, vp]
         L.tgvx_problem_features.argtypes = [vp, i32, vp, vp, vp]
         L.tgvx_problem_load_blocks.argtypes = [vp, vp, C.c_uint32, i32, i32, i32]
+        L.tgvx_autotune.argtypes = [vp, C.POINTER(Params), i32, C.POINTER(TuneReport)]
         L.tgvx_problem_block_report.argtypes = [vp, vp, vp]
         L.tgvx_solve_batched.argtypes = [vp, vp, C.c_uint32, i32, C.POINTER(Params), vp, C.POINTER(Report)]
         for name in TGVX_SYMBOLS:

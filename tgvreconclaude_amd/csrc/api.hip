@@ -217,6 +217,10 @@ Concat concat(const tgvx_block* blocks, uint32_t nblocks, int32_t nb) {
 }
 }  // namespace
 
+int tgvx_autotune(tgvx_ctx* c, const tgvx_params* p, int32_t iters, tgvx_tune_report* rep) {
+    return guarded(c, [&] { tgvx::autotune(c, p, iters, rep); });
+}
+
 int tgvx_problem_load_blocks(tgvx_ctx* c, const tgvx_block* blocks, uint32_t nblocks, int32_t nb, int32_t precision,
                              int32_t op_mode) {
     return guarded(c, [&] {

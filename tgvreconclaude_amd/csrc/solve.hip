@@ -60,6 +60,12 @@ __device__ inline bool masked(const KArgs<S>& a, long g) {
     return a.bact && !a.bact[a.blk[g]];
 }
 
+// Workgroup -> tile: XCD-contiguous runs of tiles (xcd_tile) or plain order.
+template <bool XCD>
+__device__ inline int tile_id() {
+    return XCD ? xcd_tile() : (int)blockIdx.x;
+}
+
 template <class T>
 __device__ inline void clip3(T& a, T& b, T& c, T r) {
     const T n2 = a * a + b * b + c * c;
@@ -119,11 +125,11 @@ __device__ inline void dual_step(T p[3], T q[6], const T bar[4][4], T sigma, T a
 }
 
 // ---- phase D: dual p and q (tgv.cpp:158-189; frozen variant :274-309)
-template <class S, bool FROZEN>
+template <class S, bool FROZEN, bool XCD = true>
 __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
     using IO = Io<S>;
     using T = typename IO::T;
-    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
+    const uint32_t t = (uint32_t)tile_id<XCD>() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
     if (masked(a, g)) return;
@@ -166,11 +172,11 @@ __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
 
 // ---- phase P, reference rules: primal u, primal v, extrapolation
 // (tgv.cpp:191-230).
-template <class S>
+template <class S, bool XCD = true>
 __global__ void __launch_bounds__(256) k_primal_ref(KArgs<S> a) {
     using IO = Io<S>;
     using T = typename IO::T;
-    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
+    const uint32_t t = (uint32_t)tile_id<XCD>() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
     if (masked(a, g)) return;
@@ -308,11 +314,11 @@ __device__ inline void primal_adj_at(const KArgs<S>& a, const Set13<S>& src, lon
 // neighbour this one is, and 1/2 on the off-diagonal eps entries. Reads src,
 // writes (u, v) to dst and u_bar, v_bar to the main set; copy_dual also
 // moves p, q from src to dst.
-template <class S>
+template <class S, bool XCD = true>
 __global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a, Set13<S> src, Set13<S> dst, int copy_dual) {
     using IO = Io<S>;
     using T = typename IO::T;
-    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
+    const uint32_t t = (uint32_t)tile_id<XCD>() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
     if (masked(a, g)) return;
@@ -335,11 +341,11 @@ __global__ void __launch_bounds__(256) k_primal_adj(KArgs<S> a, Set13<S> src, Se
 // q^n+1), dst receives (u^n+1, v^n+1, p^n+2, q^n+2). Every value is computed
 // by the same expressions as k_primal_adj followed by k_dual, so the results
 // are identical to the two-kernel iteration.
-template <class S>
+template <class S, bool XCD = true>
 __global__ void __launch_bounds__(256) k_fused_adj(KArgs<S> a, Set13<S> src, Set13<S> dst) {
     using IO = Io<S>;
     using T = typename IO::T;
-    const uint32_t t = (uint32_t)xcd_tile() * blockDim.x + threadIdx.x;
+    const uint32_t t = (uint32_t)tile_id<XCD>() * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
     if (masked(a, g)) return;
@@ -378,22 +384,22 @@ __global__ void __launch_bounds__(256) k_fused_adj(KArgs<S> a, Set13<S> src, Set
     for (int k = 0; k < 6; ++k) IO::st(dst.f[F_Q0 + k], g, q[k]);
 }
 
-// ---- one fused iteration over a tile of kFuseTile consecutive positions
+// ---- one fused iteration over a tile of TILE consecutive positions
 // (no reference counterpart). Same values as k_fused_adj, but the primal
 // results of the tile's own cubes are shared through LDS: only +x, +y, +z
 // neighbours outside the tile (a few tenths per cube in Morton order) are
 // recomputed, once per reference, by the whole workgroup.
 constexpr int kFuseTile = 256;
 
-template <class S>
-__global__ void __launch_bounds__(kFuseTile) k_fused_tile(KArgs<S> a, Set13<S> src, Set13<S> dst) {
+template <class S, int TILE = kFuseTile, bool XCD = true>
+__global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, Set13<S> dst) {
     using IO = Io<S>;
     using T = typename IO::T;
-    __shared__ T own_bar[4][kFuseTile];
-    __shared__ T halo_bar[4][3 * kFuseTile];
-    __shared__ int32_t halo_j[3 * kFuseTile];
+    __shared__ T own_bar[4][TILE];
+    __shared__ T halo_bar[4][3 * TILE];
+    __shared__ int32_t halo_j[3 * TILE];
     __shared__ int32_t n_halo;
-    const uint32_t t0 = (uint32_t)xcd_tile() * kFuseTile;
+    const uint32_t t0 = (uint32_t)tile_id<XCD>() * TILE;
     const uint32_t t = t0 + threadIdx.x;
     const long g = (long)a.off + t;
     const bool run = t < a.n && !masked(a, g);
@@ -407,12 +413,12 @@ __global__ void __launch_bounds__(kFuseTile) k_fused_tile(KArgs<S> a, Set13<S> s
     }
     __syncthreads();
     // where each +a neighbour's bar comes from: >= 0 an LDS slot (own tile
-    // below kFuseTile, halo above), -1 this cube (no neighbour), -2 a cube of
+    // below TILE, halo above), -1 this cube (no neighbour), -2 a cube of
     // another level (its frozen u_bar, v_bar)
     int32_t src_slot[3] = {-1, -1, -1};
     int32_t js[3] = {-1, -1, -1};
     const long tile_lo = (long)a.off + t0;
-    const long tile_hi = (long)a.off + min(t0 + (uint32_t)kFuseTile, a.n);
+    const long tile_hi = (long)a.off + min(t0 + (uint32_t)TILE, a.n);
     if (run) {
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
@@ -425,7 +431,7 @@ __global__ void __launch_bounds__(kFuseTile) k_fused_tile(KArgs<S> a, Set13<S> s
             } else if ((uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n) {
                 const int h = atomicAdd(&n_halo, 1);
                 halo_j[h] = j;
-                src_slot[ax] = kFuseTile + h;
+                src_slot[ax] = TILE + h;
             } else {
                 src_slot[ax] = -2;
             }
@@ -433,7 +439,7 @@ __global__ void __launch_bounds__(kFuseTile) k_fused_tile(KArgs<S> a, Set13<S> s
     }
     __syncthreads();
     const int nh = n_halo;
-    for (int h = threadIdx.x; h < nh; h += kFuseTile) {
+    for (int h = threadIdx.x; h < nh; h += TILE) {
         T n2[4], o2[4];
         primal_adj_at(a, src, (long)halo_j[h], n2, o2);
 #pragma unroll
@@ -452,8 +458,8 @@ __global__ void __launch_bounds__(kFuseTile) k_fused_tile(KArgs<S> a, Set13<S> s
             T v;
             if (sl == -1) v = bar[0][c];
             else if (sl == -2) v = IO::ld(a.s[F_UB + c], js[ax]);
-            else if (sl < kFuseTile) v = own_bar[c][sl];
-            else v = halo_bar[c][sl - kFuseTile];
+            else if (sl < TILE) v = own_bar[c][sl];
+            else v = halo_bar[c][sl - TILE];
             bar[1 + ax][c] = v;
         }
     }
@@ -909,6 +915,35 @@ int block_of(const tgvx_params* p) {
     return b;
 }
 
+// How one level's iterations are launched: variant 0 = dual + primal
+// kernels, 1 = LDS-tiled fused kernel, 2 = untiled fused kernel; block = the
+// workgroup size (the tile size for variant 1); xcd = XCD-aware tile order.
+struct Launch {
+    int variant, block, xcd;
+};
+
+int size_bucket(uint32_t n) {
+    int b = 0;
+    while (n > 1u) {
+        n >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool frozen) {
+    const bool adj = p->op_mode == TGVX_OP_ADJ;
+    if (p->fused < 0 || p->fused > 2) throw Error(TGVX_E_ARG, "fused must be 0, 1 or 2");
+    Launch L{adj && !frozen ? p->fused : 0, block_of(p), 1};
+    if (L.variant == 1 && p->block <= 0) L.block = kFuseTile;
+    if (L.variant == 1 && L.block == 64) throw Error(TGVX_E_ARG, "the tiled fused kernel takes block 128 or 256");
+    if (p->tune && !frozen) {
+        auto it = c->tuned.find(std::make_tuple(c->prob.precision, p->op_mode, size_bucket(n)));
+        if (it != c->tuned.end()) L = Launch{it->second.variant, it->second.block, it->second.xcd};
+    }
+    return L;
+}
+
 template <class S>
 Set13<S> main_set(const KArgs<S>& a) {
     Set13<S> m;
@@ -927,50 +962,69 @@ Set13<S> alt_set(tgvx_ctx* c) {
     return m;
 }
 
-template <class S>
-void launch_iter(tgvx_ctx* c, KArgs<S>& a, int op_mode, int block, bool frozen) {
+template <class S, bool X>
+void launch_iter_x(tgvx_ctx* c, KArgs<S>& a, int op_mode, int block, bool frozen) {
     const unsigned grid = (a.n + block - 1) / block;
-    if (frozen) k_dual<S, true><<<grid, block, 0, c->stream>>>(a);
-    else k_dual<S, false><<<grid, block, 0, c->stream>>>(a);
+    if (frozen) k_dual<S, true, X><<<grid, block, 0, c->stream>>>(a);
+    else k_dual<S, false, X><<<grid, block, 0, c->stream>>>(a);
     if (op_mode == TGVX_OP_ADJ) {
         const Set13<S> m = main_set(a);
-        k_primal_adj<S><<<grid, block, 0, c->stream>>>(a, m, m, 0);
+        k_primal_adj<S, X><<<grid, block, 0, c->stream>>>(a, m, m, 0);
     } else {
-        k_primal_ref<S><<<grid, block, 0, c->stream>>>(a);
+        k_primal_ref<S, X><<<grid, block, 0, c->stream>>>(a);
     }
+}
+template <class S>
+void launch_iter(tgvx_ctx* c, KArgs<S>& a, int op_mode, const Launch& L, bool frozen) {
+    if (L.xcd) launch_iter_x<S, true>(c, a, op_mode, L.block, frozen);
+    else launch_iter_x<S, false>(c, a, op_mode, L.block, frozen);
+}
+
+// one fused pass (primal n, dual n+1) from src into dst
+template <class S, bool X>
+void launch_fused_pass_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, const Launch& L) {
+    if (L.variant == 1) {
+        if (L.block == 128) k_fused_tile<S, 128, X><<<(a.n + 127) / 128, 128, 0, c->stream>>>(a, src, dst);
+        else k_fused_tile<S, 256, X><<<(a.n + 255) / 256, 256, 0, c->stream>>>(a, src, dst);
+    } else {
+        k_fused_adj<S, X><<<(a.n + L.block - 1) / L.block, L.block, 0, c->stream>>>(a, src, dst);
+    }
+}
+template <class S>
+void launch_fused_pass(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, const Launch& L) {
+    if (L.xcd) launch_fused_pass_x<S, true>(c, a, src, dst, L);
+    else launch_fused_pass_x<S, false>(c, a, src, dst, L);
 }
 
 // `iters` >= 2 exact-adjoint iterations as: dual of the first, iters - 1
 // fused (primal n, dual n+1) passes alternating between the main and the
 // alternate set, primal of the last back into the main set.
-// variant 1: k_fused_tile, 2: k_fused_adj
-template <class S>
-void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, int block, int variant) {
-    const unsigned grid = (a.n + block - 1) / block;
+template <class S, bool X>
+void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
+    const unsigned grid = (a.n + L.block - 1) / L.block;
     Set13<S> cur = main_set(a), oth = alt_set<S>(c);
     const Set13<S> mainset = cur;
-    k_dual<S, false><<<grid, block, 0, c->stream>>>(a);
-    const unsigned tiles = (a.n + kFuseTile - 1) / kFuseTile;
+    k_dual<S, false, X><<<grid, L.block, 0, c->stream>>>(a);
     for (int i = 1; i < iters; ++i) {
-        if (variant == 1) k_fused_tile<S><<<tiles, kFuseTile, 0, c->stream>>>(a, cur, oth);
-        else k_fused_adj<S><<<grid, block, 0, c->stream>>>(a, cur, oth);
+        launch_fused_pass_x<S, X>(c, a, cur, oth, L);
         std::swap(cur, oth);
     }
     const int copy = cur.f[0] != mainset.f[0];
-    k_primal_adj<S><<<grid, block, 0, c->stream>>>(a, cur, mainset, copy);
+    k_primal_adj<S, X><<<grid, L.block, 0, c->stream>>>(a, cur, mainset, copy);
 }
-
-bool use_fused(const tgvx_params* p, bool frozen, int iters) {
-    return p->op_mode == TGVX_OP_ADJ && p->fused && !frozen && iters >= 2;
+template <class S>
+void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
+    if (L.xcd) launch_fused_x<S, true>(c, a, iters, L);
+    else launch_fused_x<S, false>(c, a, iters, L);
 }
 
 // Cached graph of `iters` iterations of one level.
 struct GraphKey {
     uint32_t off, n;
-    int iters, op, block, prec, fused, mask;
+    int iters, op, block, prec, variant, xcd, mask;
     bool operator<(const GraphKey& o) const {
-        return std::tie(off, n, iters, op, block, prec, fused, mask) <
-               std::tie(o.off, o.n, o.iters, o.op, o.block, o.prec, o.fused, o.mask);
+        return std::tie(off, n, iters, op, block, prec, variant, xcd, mask) <
+               std::tie(o.off, o.n, o.iters, o.op, o.block, o.prec, o.variant, o.xcd, o.mask);
     }
 };
 struct GraphCache {
@@ -984,21 +1038,21 @@ struct GraphCache {
 std::map<const tgvx_ctx*, GraphCache> g_graphs;
 
 template <class S>
-void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen) {
+void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, const Launch& L) {
     if (iters <= 0 || a.n == 0) return;
-    const int block = block_of(p);
-    const bool fused = use_fused(p, frozen, iters);
+    const bool fused = L.variant != 0 && iters >= 2;
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (p->use_graph && !frozen && iters > 1) {
-        GraphKey key{a.off, a.n, iters, p->op_mode, block, c->prob.precision, fused ? p->fused : 0, a.bact ? 1 : 0};
+        GraphKey key{a.off, a.n, iters, p->op_mode, L.block, c->prob.precision, fused ? L.variant : 0, L.xcd,
+                     a.bact ? 1 : 0};
         GraphCache& gc = g_graphs[c];
         auto it = gc.m.find(key);
         if (it == gc.m.end()) {
             hipGraph_t graph;
             TGVX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            if (fused) launch_fused<S>(c, a, iters, block, p->fused);
+            if (fused) launch_fused<S>(c, a, iters, L);
             else
-                for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, false);
+                for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, false);
             TGVX_HIP(hipStreamEndCapture(c->stream, &graph));
             hipGraphExec_t exec;
             TGVX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1008,10 +1062,15 @@ void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool f
         TGVX_HIP(hipGraphLaunch(it->second, c->stream));
         return;
     }
-    if (fused) launch_fused<S>(c, a, iters, block, p->fused);
+    if (fused) launch_fused<S>(c, a, iters, L);
     else
-        for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, block, frozen);
+        for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, frozen);
     TGVX_HIP(hipGetLastError());
+}
+
+template <class S>
+void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen) {
+    run_launch<S>(c, a, p, iters, frozen, launch_for(c, p, a.n, frozen));
 }
 
 template <class S>
@@ -1288,6 +1347,14 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
 }
 
 template <class S>
+KArgs<S> level_args(tgvx_ctx* c, const tgvx_params* p, const Level& L) {
+    KArgs<S> a = make_args<S>(c, p);
+    a.off = a.lvl_off = L.off;
+    a.n = a.lvl_n = L.n;
+    return a;
+}
+
+template <class S>
 void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
              double* primal_ms, double* fused_ms) {
     Problem& P = c->prob;
@@ -1295,31 +1362,36 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     for (const Level& l : P.levels)
         if (l.depth == depth) L = &l;
     if (!L) throw Error(TGVX_E_ARG, "no cubes at depth " + std::to_string(depth));
-    KArgs<S> a = make_args<S>(c, p);
-    a.off = a.lvl_off = L->off;
-    a.n = a.lvl_n = L->n;
-    const int block = block_of(p);
-    const unsigned grid = (a.n + block - 1) / block;
+    KArgs<S> a = level_args<S>(c, p, *L);
+    const Launch cfg = launch_for(c, p, a.n, false);
     // whole iterations (graph or plain launches as configured)
     TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-    run_iters<S>(c, a, p, iters, false);
+    run_launch<S>(c, a, p, iters, false, cfg);
     TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
     TGVX_HIP(hipEventSynchronize(c->ev[1]));
     float ms = 0.f;
     TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     *total_ms = ms;
-    // each kernel bracketed by its own events
+    // dual and primal kernels, each bracketed by its own events (the
+    // configuration's workgroup size and tile order)
     const int k = std::max(1, std::min(iters, 20));
+    Launch split = cfg;
+    split.variant = 0;
+    if (split.block > 256) split.block = 256;
+    const unsigned grid = (a.n + split.block - 1) / split.block;
     double sd = 0.0, sp = 0.0;
     for (int i = 0; i < k; ++i) {
         TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-        k_dual<S, false><<<grid, block, 0, c->stream>>>(a);
+        if (split.xcd) k_dual<S, false, true><<<grid, split.block, 0, c->stream>>>(a);
+        else k_dual<S, false, false><<<grid, split.block, 0, c->stream>>>(a);
         TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
         if (p->op_mode == TGVX_OP_ADJ) {
             const Set13<S> m = main_set(a);
-            k_primal_adj<S><<<grid, block, 0, c->stream>>>(a, m, m, 0);
+            if (split.xcd) k_primal_adj<S, true><<<grid, split.block, 0, c->stream>>>(a, m, m, 0);
+            else k_primal_adj<S, false><<<grid, split.block, 0, c->stream>>>(a, m, m, 0);
         } else {
-            k_primal_ref<S><<<grid, block, 0, c->stream>>>(a);
+            if (split.xcd) k_primal_ref<S, true><<<grid, split.block, 0, c->stream>>>(a);
+            else k_primal_ref<S, false><<<grid, split.block, 0, c->stream>>>(a);
         }
         TGVX_HIP(hipEventRecord(c->ev[2], c->stream));
         TGVX_HIP(hipEventSynchronize(c->ev[2]));
@@ -1331,16 +1403,17 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     }
     *dual_ms = sd / k;
     *primal_ms = sp / k;
-    // the fused pass alone (variant as in launch_fused; an even count, so the
-    // main set ends current)
+    // the fused pass alone (the configuration's variant, the tiled kernel if
+    // it has none; an even count, so the main set ends current)
     double sf = 0.0;
     if (p->op_mode == TGVX_OP_ADJ) {
+        Launch fz = cfg;
+        if (fz.variant == 0) fz = Launch{1, kFuseTile, cfg.xcd};
         Set13<S> cur = main_set(a), oth = alt_set<S>(c);
         const int kf = 2 * k;
         for (int i = 0; i < kf; ++i) {
             TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-            if (p->fused == 2) k_fused_adj<S><<<grid, block, 0, c->stream>>>(a, cur, oth);
-            else k_fused_tile<S><<<(a.n + kFuseTile - 1) / kFuseTile, kFuseTile, 0, c->stream>>>(a, cur, oth);
+            launch_fused_pass<S>(c, a, cur, oth, fz);
             TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
             TGVX_HIP(hipEventSynchronize(c->ev[1]));
             float m1 = 0.f;
@@ -1351,6 +1424,62 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
         sf /= kf;
     }
     if (fused_ms) *fused_ms = sf;
+}
+
+// Autotuner: for every level of the loaded problem, time `iters` iterations
+// of each launch configuration (as a captured graph when use_graph is set)
+// and keep the fastest per (precision, operator mode, log2 size bucket).
+template <class S>
+void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* rep) {
+    Problem& P = c->prob;
+    std::vector<Launch> cand;
+    for (int x = 1; x >= 0; --x) {
+        for (int b : {256, 128, 64}) cand.push_back(Launch{0, b, x});
+        if (p->op_mode == TGVX_OP_ADJ) {
+            for (int b : {256, 128}) cand.push_back(Launch{1, b, x});
+            for (int b : {256, 128}) cand.push_back(Launch{2, b, x});
+        }
+    }
+    tgvx_params q = *p;
+    q.tune = 0;
+    const Launch base = launch_for(c, &q, 1u << 20, false);
+    if (rep) std::memset(rep, 0, sizeof(*rep));
+    int li = 0;
+    for (const Level& L : P.levels) {
+        KArgs<S> a = level_args<S>(c, p, L);
+        double best = 1e300, def = 0.0;
+        Launch win = base;
+        for (const Launch& cf : cand) {
+            run_launch<S>(c, a, &q, iters, false, cf);  // warm-up (and graph build)
+            TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+            run_launch<S>(c, a, &q, iters, false, cf);
+            TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+            TGVX_HIP(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            const double us = 1e3 * ms / iters;
+            const bool is_base = cf.variant == base.variant && cf.block == base.block && cf.xcd == base.xcd;
+            if (is_base) def = us;
+            if (us < best) {
+                best = us;
+                win = cf;
+            }
+        }
+        c->tuned[std::make_tuple(P.precision, p->op_mode, size_bucket(L.n))] = TuneEntry{win.variant, win.block,
+                                                                                         win.xcd, best};
+        if (rep && li < TGVX_MAX_LEVELS) {
+            rep->level_depth[li] = L.depth;
+            rep->level_leaves[li] = L.n;
+            rep->variant[li] = win.variant;
+            rep->block[li] = win.block;
+            rep->xcd[li] = win.xcd;
+            rep->best_us[li] = best;
+            rep->default_us[li] = def;
+        }
+        ++li;
+    }
+    if (rep) rep->n_levels = std::min(li, TGVX_MAX_LEVELS);
+    release_graphs(c);  // the candidates' graphs
 }
 
 }  // namespace
@@ -1619,6 +1748,18 @@ void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* Pv, doubl
         case TGVX_FP64: go(double{}); break;
         case TGVX_FP32: go(float{}); break;
         default: go(bf16s{}); break;
+    }
+}
+
+void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* rep) {
+    Problem& P = c->prob;
+    if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
+    if (!p) throw Error(TGVX_E_ARG, "null params");
+    if (iters < 2 || iters > 10000) throw Error(TGVX_E_ARG, "iters must be in 2..10000");
+    switch (P.precision) {
+        case TGVX_FP64: autotune_t<double>(c, p, iters, rep); break;
+        case TGVX_FP32: autotune_t<float>(c, p, iters, rep); break;
+        default: autotune_t<bf16s>(c, p, iters, rep); break;
     }
 }
 

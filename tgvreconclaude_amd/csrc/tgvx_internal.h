@@ -7,6 +7,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "tgvx.h"
@@ -93,6 +94,14 @@ struct Problem {
 
 }  // namespace tgvx
 
+namespace tgvx {
+// the autotuner's choice for one (precision, operator mode, log2 size) bucket
+struct TuneEntry {
+    int variant, block, xcd;
+    double us;
+};
+}  // namespace tgvx
+
 struct tgvx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -104,6 +113,8 @@ struct tgvx_ctx {
     // warm-start network: weights, Adam moments, activations and data
     tgvx::DevBuf mlp_w, mlp_m, mlp_v, mlp_x, mlp_t, mlp_y, mlp_rows, mlp_part;
     int mlp_step = 0;
+    // launch configurations chosen by tgvx_autotune
+    std::map<std::tuple<int, int, int>, tgvx::TuneEntry> tuned;
 };
 
 namespace tgvx {
@@ -116,6 +127,8 @@ void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out);
 void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* P, double* D);
 void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
                  double* primal_ms, double* fused_ms);
+void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* rep);
+void release_graphs(const tgvx_ctx* c);
 // mlp.hip
 void mlp_set(tgvx_ctx* c, const float* w);
 void mlp_get(tgvx_ctx* c, float* w);

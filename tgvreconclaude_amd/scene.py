@@ -153,21 +153,65 @@ def torus_box_scene(views=96, W=1280, H=960, sigma=0.003, outliers=0.03, seed=1,
     return make_scene(torus_box_prims(), cams, W, H, sigma, outliers, seed, max_depth, name="torus_box")
 
 
-def csg_prims(seed):
-    """Seeded union of 1-6 spheres and boxes inside the root cube."""
+def csg_prims(seed, scale=1.0):
+    """Seeded union of 1-6 spheres and boxes inside the root cube (scaled about the origin)."""
     rng = np.random.default_rng(seed)
     k = int(rng.integers(1, 7))
     out = []
     for _ in range(k):
-        c = rng.uniform(-0.6, 0.6, 3)
+        c = rng.uniform(-0.6, 0.6, 3) * scale
         if rng.random() < 0.5:
-            out.append(prim(SPHERE, c, (float(rng.uniform(0.2, 0.55)),)))
+            out.append(prim(SPHERE, c, (float(rng.uniform(0.2, 0.55)) * scale,)))
         else:
             ax = rng.normal(size=3)
-            out.append(prim(BOX, c, tuple(rng.uniform(0.15, 0.45, 3)), rot_axis_angle(ax, float(rng.uniform(0, 90)))))
+            out.append(prim(BOX, c, tuple(rng.uniform(0.15, 0.45, 3) * scale),
+                            rot_axis_angle(ax, float(rng.uniform(0, 90)))))
     return out
 
 
 def csg_scene(seed, views=24, W=320, H=240, sigma=0.005, outliers=0.02, max_depth=9):
     cams = cameras_ring(views, 3.0, W, H)
     return make_scene(csg_prims(seed), cams, W, H, sigma, outliers, seed, max_depth, name=f"csg{seed}")
+
+
+def batch_block(seed, target, views=12, sigma=0.005, outliers=0.02):
+    """One block of the batched workload: a seeded union of 1-6 spheres and
+    boxes whose octree has about `target` cubes. The depth clamp is the
+    smallest whose estimated count (the count at clamp 7 times 4 per extra
+    depth) reaches the target; the primitives are then scaled by
+    sqrt(target / estimate), so the surface, and with it the count, shrinks
+    to the target, corrected from the measured count where leaves reach the
+    pixel footprint. Images are 640x480, or 1280x960 for clamps above 8."""
+    def make(scale, depth):
+        W, H = (1280, 960) if depth > 8 else (640, 480)
+        cams = cameras_ring(views, 3.0, W, H)
+        return make_scene(csg_prims(seed, scale), cams, W, H, sigma, outliers, seed, depth, name=f"block{seed}",
+                          target=int(target), scale=scale)
+
+    probe = make(1.0, 7)
+    c7 = max(len(probe.cubes), 1)
+    depth, est = 10, c7 * 4.0 ** 3
+    for d in range(4, 11):
+        if c7 * 4.0 ** (d - 7) >= target:
+            depth, est = d, c7 * 4.0 ** (d - 7)
+            break
+    scale = float(np.clip(np.sqrt(target / est), 0.3, 1.0))
+    sc = probe if (depth == 7 and scale == 1.0) else make(scale, depth)
+    # counts stop growing fourfold once leaves approach the pixel footprint:
+    # correct the scale (then the clamp) from the measured count, twice at most
+    for _ in range(2):
+        n = len(sc.cubes)
+        if n >= 0.8 * target or (scale >= 1.0 and depth >= 10):
+            break
+        if scale < 1.0:
+            scale = float(min(1.0, scale * np.sqrt(target / max(n, 1))))
+        else:
+            depth += 1
+        sc = make(scale, depth)
+    return sc
+
+
+def batch_targets(n_blocks=256, seed=4, lo=4096, hi=524288):
+    """Log-uniform target cube counts of the batched workload's blocks."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), n_blocks))
