@@ -160,6 +160,11 @@ int tgvx_problem_load(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const u
 /* Solve the resident problem from its cold (or given) initialisation. */
 int tgvx_problem_solve(tgvx_ctx* ctx, const tgvx_params* params, const float* init_uv,
                        tgvx_report* report);
+/* tgvx_problem_solve with a fixed iteration count per level (n_levels = the
+ * number of depths present, as tgvx_problem_levels; gap_tol must be 0): runs
+ * another precision for exactly the iterations a to-gap solve took */
+int tgvx_problem_solve_levels(tgvx_ctx* ctx, const tgvx_params* params, const int32_t* level_iters, int32_t n_levels,
+                              tgvx_report* report);
 /* Read back state (layout as tgvx_solve) and, optionally, u_bar/v_bar (n x 4 of the state type). */
 int tgvx_problem_state(tgvx_ctx* ctx, void* state_out, void* ubar_out);
 /* Energy P and dual value D of one level of the resident state (FP64). */
@@ -175,6 +180,18 @@ int tgvx_problem_levels(tgvx_ctx* ctx, int32_t* depths, uint32_t* counts, int32_
  * bracketed launches. */
 int tgvx_level_bench(tgvx_ctx* ctx, int32_t depth, const tgvx_params* params, int32_t iters,
                      double* total_ms, double* dual_ms, double* primal_ms, double* fused_ms);
+
+/* ---- iso-surface extraction (marching_cubes.cpp:63-181 extract_surface), on the device
+ * Over the resident problem's cubes with u = the given field (n floats in input
+ * order) or, when u is NULL, the problem's current state; has_data NULL or n
+ * bytes (a pair emits only if both cubes have data). One quad (two triangles)
+ * per face-adjacent pair with a sign change, at the interpolated zero crossing;
+ * vertices merged on the quantised position key and numbered by first
+ * occurrence, as the reference. The mesh stays in the context until the next
+ * extraction; tgvx_surface_copy reads it (verts n_verts x 3, tris n_tris x 3). */
+int tgvx_surface_extract(tgvx_ctx* ctx, const float* u, const uint8_t* has_data, const float origin[3],
+                         float r_root, uint64_t* n_verts, uint64_t* n_tris);
+int tgvx_surface_copy(tgvx_ctx* ctx, float* verts, uint32_t* tris);
 
 /* ---- launch autotuner (no reference counterpart)
  * For every level of the loaded problem, times `iters` iterations of each

@@ -1,0 +1,137 @@
+"""Storage-precision accuracy of the TGV solve against FP64 (test support).
+
+For a storage precision X (FP32, or BF16 storage with FP32 arithmetic) the
+scene is solved cold to relative duality gap `tol` in the exact-adjoint mode,
+or run for given per-level iteration counts. The same problem is then run in
+FP64 for exactly the per-level iteration counts the X run took (the FP64 path
+is bit-identical to the FP64 CPU oracle, pinned by the parity tests), and the
+two are compared: L_inf of u over all cubes and over the cubes with votes,
+and the Hausdorff distance between the surfaces extracted from both
+(marching_cubes.cpp:63-181 over all cubes, faces only between cubes with
+votes, by the library's device extraction, which the surface tests pin
+bit for bit to the oracle's), in units of the finest leaf edge. The distance
+is taken between the mesh vertex sets, both ways. The FP64 solve to the same
+gap is also run, for its iteration counts.
+
+    python tests/accuracy.py --out gpurun_out/accuracy_depth10.json   # the depth-10 torus+box scene
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, ROOT)
+
+from tgvreconclaude_amd import BF16, FP32, FP64, OP_ADJ, TGVParams  # noqa: E402
+
+NAMES = {FP32: "fp32", BF16: "bf16", FP64: "fp64"}
+
+
+def _params(prec, **kw):
+    base = dict(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=OP_ADJ, chunk=50, max_iters=20000)
+    base.update(kw)
+    return TGVParams(**base)
+
+
+def hausdorff(a, b):
+    """Symmetric Hausdorff distance between two point sets (inf if exactly one is empty)."""
+    from scipy.spatial import cKDTree
+    if len(a) == 0 and len(b) == 0:
+        return 0.0
+    if len(a) == 0 or len(b) == 0:
+        return float("inf")
+    da, _ = cKDTree(b).query(a, k=1, workers=-1)
+    db, _ = cKDTree(a).query(b, k=1, workers=-1)
+    return float(max(da.max(), db.max()))
+
+
+def surface(ctx, sc, u, has_data):
+    """The library's extraction over the loaded problem (bit-identical to the
+    oracle's, pinned by the surface tests)."""
+    return ctx.surface(sc.origin, sc.r_root, u=np.ascontiguousarray(u, dtype=np.float32), has_data=has_data)
+
+
+def log(msg):
+    print(f"[accuracy] {msg}", flush=True)
+
+
+def u_of(st):
+    return st["u"].astype(np.float64) if st.dtype.names else st[:, 0].copy()
+
+
+def compare(ctx, sc, hist, prec, tol=1e-4, level_iters=None, max_iters=20000):
+    """Report for one storage precision against FP64 run for the same
+    iterations: solved to relative gap tol, or for the given per-level
+    iteration counts."""
+    has = (hist.sum(axis=1) > 0).astype(np.uint8)
+    leaf = 2.0 * sc.r_root / 2 ** int(sc.cubes["depth"].max())
+    ctx.load(sc.cubes, hist, precision=prec, op_mode=OP_ADJ)
+    t0 = time.time()
+    if level_iters is None:
+        rep = ctx.run(_params(prec, gap_tol=tol, max_iters=max_iters))
+    else:
+        rep = ctx.run_levels(_params(prec), level_iters)
+    its = [l["iters"] for l in rep.levels()]
+    gaps = [l["rel_gap"] for l in rep.levels()]
+    reached = all(g <= tol for g in gaps)
+    u = u_of(ctx.state())
+    log(f"{NAMES[prec]}: {rep.solve_ms:.0f} ms, iterations {its}")
+    ctx.load(sc.cubes, hist, precision=FP64, op_mode=OP_ADJ)
+    r64 = ctx.run_levels(_params(FP64), its)
+    u64 = u_of(ctx.state())
+    log(f"fp64 for the same iterations: {r64.solve_ms:.0f} ms")
+    d = np.abs(u - u64)
+    va, _ = surface(ctx, sc, u, has)
+    vb, _ = surface(ctx, sc, u64, has)
+    h = hausdorff(va, vb)
+    log(f"surfaces of {len(va)} and {len(vb)} vertices, Hausdorff {h:.3g}")
+    return {"precision": NAMES[prec], "gap_tol": tol if level_iters is None else None, "reached": reached,
+            "iters_per_level": its, "final_rel_gap": gaps,
+            "solve_ms": rep.solve_ms, "linf_u": float(d.max()), "linf_u_voted": float(d[has > 0].max()),
+            "mean_abs_u": float(d.mean()), "hausdorff": h, "leaf_edge": leaf, "hausdorff_leaf": h / leaf,
+            "surface_vertices": [int(len(va)), int(len(vb))], "wall_s": round(time.time() - t0, 2)}
+
+
+def fp64_to_gap(ctx, sc, hist, tol=1e-4):
+    ctx.load(sc.cubes, hist, precision=FP64, op_mode=OP_ADJ)
+    rep = ctx.run(_params(FP64, gap_tol=tol))
+    return {"iters_per_level": [l["iters"] for l in rep.levels()],
+            "reached": all(l["rel_gap"] <= tol for l in rep.levels()), "solve_ms": rep.solve_ms}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--bf16-max-iters", type=int, default=6000, help="per-level cap of the BF16 to-gap attempt")
+    args = ap.parse_args()
+    from tgvreconclaude_amd import Context, scene
+    t0 = time.time()
+    sc = scene.torus_box_scene()
+    print(f"[accuracy] scene: {len(sc.cubes)} cubes ({time.time() - t0:.0f} s)", flush=True)
+    ctx = Context(0)
+    hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
+    out = {"workload": "torus_box_depth10", "cubes": int(len(sc.cubes)), "results": []}
+    # FP32 to the gap; BF16 for the FP32 solve's iterations; BF16 to the gap
+    r32 = compare(ctx, sc, hist, FP32, args.tol)
+    runs = [r32, compare(ctx, sc, hist, BF16, args.tol, level_iters=r32["iters_per_level"]),
+            compare(ctx, sc, hist, BF16, args.tol, max_iters=args.bf16_max_iters)]
+    for r in runs:
+        out["results"].append(r)
+        print(f"[accuracy] {r['precision']} (gap_tol {r['gap_tol']}): iters {r['iters_per_level']} "
+              f"reached {r['reached']} max rel gap {max(r['final_rel_gap']):.3g} linf_u {r['linf_u']:.3g} "
+              f"hausdorff {r['hausdorff_leaf']:.3f} leaf ({r['wall_s']} s)", flush=True)
+    out["fp64_to_gap"] = fp64_to_gap(ctx, sc, hist, args.tol)
+    print(f"[accuracy] fp64 to gap: iters {out['fp64_to_gap']['iters_per_level']}", flush=True)
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -221,6 +221,26 @@ class Context:
         self._check(self._L.tgvx_problem_solve(self.h, C.byref(params), N.ptr(init), C.byref(rep)))
         return rep
 
+    def surface(self, origin, r_root, u=None, has_data=None):
+        """extract_surface over the resident problem: (verts [nv, 3] float32, tris [nt, 3] uint32)."""
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float32)
+        hd = None if has_data is None else np.ascontiguousarray(has_data, dtype=np.uint8)
+        o = np.ascontiguousarray(origin, dtype=np.float32)
+        nv, nt = C.c_uint64(), C.c_uint64()
+        self._check(self._L.tgvx_surface_extract(self.h, N.ptr(uu), N.ptr(hd), N.ptr(o), float(r_root),
+                                                 C.byref(nv), C.byref(nt)))
+        verts = np.zeros((nv.value, 3), dtype=np.float32)
+        tris = np.zeros((nt.value, 3), dtype=np.uint32)
+        self._check(self._L.tgvx_surface_copy(self.h, N.ptr(verts), N.ptr(tris)))
+        return verts, tris
+
+    def run_levels(self, params, level_iters):
+        """Solve the resident problem with a fixed iteration count per level."""
+        it = np.ascontiguousarray(level_iters, dtype=np.int32)
+        rep = Report()
+        self._check(self._L.tgvx_problem_solve_levels(self.h, C.byref(params), N.ptr(it), len(it), C.byref(rep)))
+        return rep
+
     def state(self, with_bar=False):
         st = self._state_array(self._n, self._prec)
         bar = None

@@ -87,7 +87,8 @@ TGVX_SYMBOLS = [
     "tgvx_problem_load", "tgvx_problem_solve", "tgvx_problem_state", "tgvx_problem_gap",
     "tgvx_problem_levels", "tgvx_level_bench", "tgvx_mlp_set", "tgvx_mlp_get", "tgvx_mlp_forward",
     "tgvx_mlp_loss_grad", "tgvx_mlp_train", "tgvx_problem_features", "tgvx_problem_load_blocks",
-    "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune",
+    "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune", "tgvx_problem_solve_levels",
+    "tgvx_surface_extract", "tgvx_surface_copy",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
@@ -138,6 +139,9 @@ def lib():
         L.tgvx_problem_features.argtypes = [vp, i32, vp, vp, vp]
         L.tgvx_problem_load_blocks.argtypes = [vp, vp, C.c_uint32, i32, i32, i32]
         L.tgvx_autotune.argtypes = [vp, C.POINTER(Params), i32, C.POINTER(TuneReport)]
+        L.tgvx_problem_solve_levels.argtypes = [vp, C.POINTER(Params), vp, i32, C.POINTER(Report)]
+        L.tgvx_surface_extract.argtypes = [vp, vp, vp, vp, C.c_float, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.tgvx_surface_copy.argtypes = [vp, vp, vp]
         L.tgvx_problem_block_report.argtypes = [vp, vp, vp]
         L.tgvx_solve_batched.argtypes = [vp, vp, C.c_uint32, i32, C.POINTER(Params), vp, C.POINTER(Report)]
         for name in TGVX_SYMBOLS:

@@ -119,6 +119,11 @@ int tgvx_problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, 
     return guarded(c, [&] { tgvx::problem_solve(c, p, init_uv, rep, nullptr, 0, nullptr); });
 }
 
+int tgvx_problem_solve_levels(tgvx_ctx* c, const tgvx_params* p, const int32_t* level_iters, int32_t n_levels,
+                              tgvx_report* rep) {
+    return guarded(c, [&] { tgvx::problem_solve_levels(c, p, level_iters, n_levels, rep); });
+}
+
 int tgvx_problem_state(tgvx_ctx* c, void* state_out, void* ubar_out) {
     return guarded(c, [&] { tgvx::problem_state(c, state_out, ubar_out); });
 }
@@ -216,6 +221,15 @@ Concat concat(const tgvx_block* blocks, uint32_t nblocks, int32_t nb) {
     return c;
 }
 }  // namespace
+
+int tgvx_surface_extract(tgvx_ctx* c, const float* u, const uint8_t* has_data, const float origin[3], float r_root,
+                         uint64_t* nv, uint64_t* nt) {
+    return guarded(c, [&] { tgvx::surface_extract(c, u, has_data, origin, r_root, nv, nt); });
+}
+
+int tgvx_surface_copy(tgvx_ctx* c, float* verts, uint32_t* tris) {
+    return guarded(c, [&] { tgvx::surface_copy(c, verts, tris); });
+}
 
 int tgvx_autotune(tgvx_ctx* c, const tgvx_params* p, int32_t iters, tgvx_tune_report* rep) {
     return guarded(c, [&] { tgvx::autotune(c, p, iters, rep); });

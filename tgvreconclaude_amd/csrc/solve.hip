@@ -1227,14 +1227,15 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             a.n = L.n;
             done = p->iters;
         } else if (!to_gap) {
+            const int fixed = P.level_iters.empty() ? p->iters : P.level_iters[li];
             TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-            run_iters<S>(c, a, p, p->iters, false);
+            run_iters<S>(c, a, p, fixed, false);
             TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
             TGVX_HIP(hipEventSynchronize(c->ev[1]));
             float ms = 0.f;
             TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
             it_ms += ms;
-            done = p->iters;
+            done = fixed;
             const double g0 = now_ms();
             eval_gap<S>(c, a, p, &Pv, &Dv);
             if (P.nblocks > 1) {
@@ -1599,7 +1600,9 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
         b.ensure(bytes);
         if (bytes) TGVX_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
     };
-    DevBuf d_code, d_depth, d_uc, d_ui, d_uoff, d_doff;
+    DevBuf& d_code = P.d_code;
+    DevBuf& d_depth = P.d_depth;
+    DevBuf d_uc, d_ui, d_uoff, d_doff;
     up(d_code, pcode.data(), 8 * n);
     up(d_depth, pdepth.data(), 4 * n);
     up(d_uc, ucode.data(), 8 * std::max<size_t>(ucode.size(), 1));
@@ -1702,6 +1705,24 @@ void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx
         case TGVX_FP32: solve_t<float>(c, p, init_uv, rep, leaves, nleaves); break;
         default: solve_t<bf16s>(c, p, init_uv, rep, leaves, nleaves); break;
     }
+}
+
+void problem_solve_levels(tgvx_ctx* c, const tgvx_params* p, const int32_t* level_iters, int32_t n_levels,
+                          tgvx_report* rep) {
+    Problem& P = c->prob;
+    if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
+    if (!p || !level_iters) throw Error(TGVX_E_ARG, "null pointer");
+    if (n_levels != (int32_t)P.levels.size())
+        throw Error(TGVX_E_ARG, "level_iters needs one count per level (" + std::to_string(P.levels.size()) + ")");
+    if (p->gap_tol > 0.0) throw Error(TGVX_E_ARG, "per-level iteration counts exclude gap_tol");
+    for (int32_t i = 0; i < n_levels; ++i)
+        if (level_iters[i] < 0) throw Error(TGVX_E_ARG, "negative iteration count");
+    P.level_iters.assign(level_iters, level_iters + n_levels);
+    struct Clear {
+        Problem& P;
+        ~Clear() { P.level_iters.clear(); }
+    } clear{P};
+    problem_solve(c, p, nullptr, rep, nullptr, 0, nullptr);
 }
 
 void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out) {

@@ -70,8 +70,11 @@ struct Problem {
     std::vector<std::vector<uint32_t>> seg;  // per level: block b owns positions off + [seg[b], seg[b+1])
     std::vector<int32_t> block_iters;        // per level x block: iterations to reach the gap
     std::vector<double> block_gap;           // per level x block: final relative gap
+    std::vector<int32_t> level_iters;        // fixed iterations per level for the next solve (empty: params)
     std::vector<uint32_t> perm;      // position -> input index
     std::vector<uint32_t> inv;       // input index -> position
+    DevBuf d_code;                   // uint64 Morton code per position
+    DevBuf d_depth;                  // int32 depth per position
     DevBuf d_nb[6];                  // int32 per position, faces 0..5
     DevBuf d_parent;                 // int32: position of the cube holding the parent code, or -1
     DevBuf d_info;                   // uint32 packed vote total / median bin / flags
@@ -113,6 +116,9 @@ struct tgvx_ctx {
     // warm-start network: weights, Adam moments, activations and data
     tgvx::DevBuf mlp_w, mlp_m, mlp_v, mlp_x, mlp_t, mlp_y, mlp_rows, mlp_part;
     int mlp_step = 0;
+    // last extracted surface: vertices (x, y, z) and triangles
+    tgvx::DevBuf srf_verts, srf_tris, srf_tmp[8];
+    uint64_t srf_nv = 0, srf_nt = 0;
     // launch configurations chosen by tgvx_autotune
     std::map<std::tuple<int, int, int>, tgvx::TuneEntry> tuned;
 };
@@ -123,11 +129,17 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
                   int op_mode, const uint64_t* blk_start = nullptr, uint32_t nblocks = 1);
 void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
                    const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_cube* cubes);
+void problem_solve_levels(tgvx_ctx* c, const tgvx_params* p, const int32_t* level_iters, int32_t n_levels,
+                          tgvx_report* rep);
 void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out);
 void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* P, double* D);
 void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
                  double* primal_ms, double* fused_ms);
 void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* rep);
+// surface.hip
+void surface_extract(tgvx_ctx* c, const float* u, const uint8_t* has_data, const float origin[3], float r_root,
+                     uint64_t* nv, uint64_t* nt);
+void surface_copy(tgvx_ctx* c, float* verts, uint32_t* tris);
 void release_graphs(const tgvx_ctx* c);
 // mlp.hip
 void mlp_set(tgvx_ctx* c, const float* w);
