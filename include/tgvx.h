@@ -102,6 +102,8 @@ typedef struct {
                               kernel recomputing every neighbour's primal values */
     int32_t tune;          /* 1: take each level's launch configuration from tgvx_autotune's table
                               (levels without an entry use fused / block as given) */
+    float sigma, tau;      /* primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150).
+                              Exact-adjoint mode converges for sigma * tau <= 1/16 */
 } tgvx_params;
 
 #define TGVX_MAX_LEVELS 22

@@ -24,7 +24,8 @@ class OParams(C.Structure):
     _fields_ = [("alpha0", C.c_float), ("alpha1", C.c_float), ("lambda_", C.c_float),
                 ("iters", C.c_int32), ("op_mode", C.c_int32), ("chunk", C.c_int32),
                 ("max_iters", C.c_int32), ("gap_tol", C.c_double), ("threads", C.c_int32),
-                ("warm_fallback", C.c_int32), ("dual_eps", C.c_float), ("warm_dual", C.c_int32)]
+                ("warm_fallback", C.c_int32), ("dual_eps", C.c_float), ("warm_dual", C.c_int32),
+                ("sigma", C.c_float), ("tau", C.c_float)]
 
 
 _L = None
@@ -78,9 +79,9 @@ def _p(a):
 
 
 def params(alpha0=0.02, alpha1=0.04, lam=1.0, iters=200, op_mode=0, chunk=50, max_iters=20000, gap_tol=0.0,
-           threads=1, warm_fallback=1, dual_eps=0.0, warm_dual=0):
+           threads=1, warm_fallback=1, dual_eps=0.0, warm_dual=0, sigma=0.0, tau=0.0):
     return OParams(alpha0, alpha1, lam, iters, op_mode, chunk, max_iters, gap_tol, threads, warm_fallback, dual_eps,
-                   warm_dual)
+                   warm_dual, sigma, tau)
 
 
 def view(depths, fx, fy, cx, cy, m, lidar=False):

@@ -266,6 +266,7 @@ struct Params {
     int32_t warm_fallback;  // with an init field: keep cold init where it has the lower gap
     float dual_eps;         // with an init field: > 0 recovers p, q from (u, v), see dual_recover
     int32_t warm_dual;      // with an init field: 1 keeps the dual recovered from the cold init
+    float sigma, tau;       // primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150)
 };
 
 using CodeMap = std::unordered_map<uint64_t, uint32_t>;
@@ -598,7 +599,7 @@ struct Solver {
     // level-start snapshots of run_level_frozen (tgv.cpp:263-298), or null.
     void iterate(const Level& L, int iters, std::vector<T>& un, std::vector<T>* vn, const T* fu,
                  const T* const* fv, uint32_t leaf_first, uint32_t leaf_last) {
-        const T sigma = T(0.25), tau = T(0.25);
+        const T sigma = prm.sigma > 0.f ? (T)prm.sigma : T(0.25), tau = prm.tau > 0.f ? (T)prm.tau : T(0.25);
         const T a0 = (T)prm.alpha0, a1 = (T)prm.alpha1;
         const T tl = tau * (T)prm.lambda;
         const long na = (long)L.active.size();
@@ -960,6 +961,7 @@ typedef struct {
     int32_t threads, warm_fallback;
     float dual_eps;
     int32_t warm_dual;
+    float sigma, tau;
 } orc_params;
 
 void orc_morton_encode(uint32_t x, uint32_t y, uint32_t z, uint32_t* hi, uint32_t* lo) {
@@ -1018,6 +1020,7 @@ static orc::Params to_params(const orc_params* p) {
     q.iters = p->iters; q.op_mode = p->op_mode; q.chunk = p->chunk > 0 ? p->chunk : 50;
     q.max_iters = p->max_iters; q.gap_tol = p->gap_tol;
     q.threads = p->threads; q.warm_fallback = p->warm_fallback; q.dual_eps = p->dual_eps; q.warm_dual = p->warm_dual;
+    q.sigma = p->sigma; q.tau = p->tau;
     return q;
 }
 

@@ -167,3 +167,38 @@ def test_fused_bf16_matches_two_kernels(ctx, sphere, variant):
     b, _ = ctx.solve(sc.cubes, hist, TGVParams(alpha0=2.0, alpha1=1.0, iters=40, precision=BF16, op_mode=OP_ADJ,
                                                fused=0))
     assert np.array_equal(_state13(a).view(np.uint32), _state13(b).view(np.uint32))
+
+
+def test_parameter_change_on_resident_problem(ctx, sphere):
+    # graphs captured for one set of weights / steps must not be replayed for
+    # another: a second solve with other parameters equals a fresh solve
+    sc, hist = sphere
+    a = TGVParams(alpha0=2.0, alpha1=1.0, iters=60, op_mode=OP_ADJ)
+    b = TGVParams(alpha0=0.5, alpha1=0.3, iters=60, op_mode=OP_ADJ, sigma=0.5, tau=0.125)
+    ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+    ctx.run(a)
+    ctx.run(b)
+    got = ctx.state()
+    ref, _ = ctx.solve(sc.cubes, hist, b)
+    assert np.array_equal(_state13(got).view(np.uint32), _state13(ref).view(np.uint32))
+    ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+    ctx.run(a)
+    first = ctx.state()
+    ctx.run(b)
+    ctx.run(a)
+    assert np.array_equal(_state13(ctx.state()).view(np.uint32), _state13(first).view(np.uint32))
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_to_gap_adj_large_step_ratio(ctx, sphere, fused):
+    # sigma / tau = 1024 (sigma = 8, tau = 1/128, sigma * tau = 1/16): the
+    # same iteration with other steps, bit-exact with the oracle, counts equal
+    sc, hist = sphere
+    prm = TGVParams(alpha0=2.0, alpha1=1.0, op_mode=OP_ADJ, gap_tol=1e-4, chunk=50, max_iters=8000,
+                    sigma=8.0, tau=1.0 / 128, fused=fused)
+    st, rep = ctx.solve(sc.cubes, hist, prm)
+    ref, _, it, pd = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, op_mode=1, gap_tol=1e-4,
+                                                             chunk=50, max_iters=8000, sigma=8.0, tau=1.0 / 128))
+    assert [l["iters"] for l in rep.levels()] == it.tolist()
+    assert all(l["rel_gap"] <= 1e-4 for l in rep.levels())
+    assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))

@@ -32,7 +32,8 @@ class Params(C.Structure):
                 ("chunk", C.c_int32), ("max_iters", C.c_int32), ("gap_tol", C.c_double),
                 ("warm_fallback", C.c_int32), ("use_graph", C.c_int32), ("block", C.c_int32),
                 ("dual_eps", C.c_float), ("warm_start", C.c_int32), ("warm_dual", C.c_int32),
-                ("fused", C.c_int32), ("tune", C.c_int32)]
+                ("fused", C.c_int32), ("tune", C.c_int32),
+                ("sigma", C.c_float), ("tau", C.c_float)]
 
 
 class Report(C.Structure):

@@ -900,8 +900,8 @@ KArgs<S> make_args(tgvx_ctx* c, const tgvx_params* prm) {
     for (int i = 0; i < 4; ++i) a.save[i] = P.d_save[i].as<S>();
     a.init = P.d_init.as<float>();
     if (prm) {
-        a.sigma = T(0.25);
-        a.tau = T(0.25);
+        a.sigma = prm->sigma > 0.f ? (T)prm->sigma : T(0.25);
+        a.tau = prm->tau > 0.f ? (T)prm->tau : T(0.25);
         a.a0 = (T)prm->alpha0;
         a.a1 = (T)prm->alpha1;
         a.tl = a.tau * (T)prm->lambda;
@@ -1019,12 +1019,17 @@ void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
 }
 
 // Cached graph of `iters` iterations of one level.
+// Everything a captured graph bakes in: the launch shape and the kernel
+// arguments' scalars (steps, weights), so a solve with other parameters on
+// the same problem never replays a stale graph.
 struct GraphKey {
-    uint32_t off, n;
+    uint32_t off, n, lvl_off, lvl_n;
     int iters, op, block, prec, variant, xcd, mask;
+    double sigma, tau, a0, a1, tl;
     bool operator<(const GraphKey& o) const {
-        return std::tie(off, n, iters, op, block, prec, variant, xcd, mask) <
-               std::tie(o.off, o.n, o.iters, o.op, o.block, o.prec, o.variant, o.xcd, o.mask);
+        return std::tie(off, n, lvl_off, lvl_n, iters, op, block, prec, variant, xcd, mask, sigma, tau, a0, a1, tl) <
+               std::tie(o.off, o.n, o.lvl_off, o.lvl_n, o.iters, o.op, o.block, o.prec, o.variant, o.xcd, o.mask,
+                        o.sigma, o.tau, o.a0, o.a1, o.tl);
     }
 };
 struct GraphCache {
@@ -1043,8 +1048,22 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
     const bool fused = L.variant != 0 && iters >= 2;
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (p->use_graph && !frozen && iters > 1) {
-        GraphKey key{a.off, a.n, iters, p->op_mode, L.block, c->prob.precision, fused ? L.variant : 0, L.xcd,
-                     a.bact ? 1 : 0};
+        GraphKey key{a.off,
+                     a.n,
+                     a.lvl_off,
+                     a.lvl_n,
+                     iters,
+                     p->op_mode,
+                     L.block,
+                     c->prob.precision,
+                     fused ? L.variant : 0,
+                     L.xcd,
+                     a.bact ? 1 : 0,
+                     (double)a.sigma,
+                     (double)a.tau,
+                     (double)a.a0,
+                     (double)a.a1,
+                     (double)a.tl};
         GraphCache& gc = g_graphs[c];
         auto it = gc.m.find(key);
         if (it == gc.m.end()) {
