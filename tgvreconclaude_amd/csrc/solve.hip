@@ -385,45 +385,129 @@ __global__ void __launch_bounds__(256) k_fused_adj(KArgs<S> a, Set13<S> src, Set
 }
 
 // ---- one fused iteration over a tile of TILE consecutive positions
-// (no reference counterpart). Same values as k_fused_adj, but the primal
-// results of the tile's own cubes are shared through LDS: only +x, +y, +z
-// neighbours outside the tile (a few tenths per cube in Morton order) are
-// recomputed, once per reference, by the whole workgroup.
+// (no reference counterpart). Same values as k_fused_adj, but the tile's
+// p, q and its primal results are shared through LDS: the primal step reads
+// the -a neighbours' p, q from LDS when they lie in the tile, the dual step
+// reads the +a neighbours' u_bar, v_bar from LDS, and only the +a neighbours
+// outside the tile (a few tenths per cube in Morton order) have their primal
+// values recomputed, once per reference, by the whole workgroup.
 constexpr int kFuseTile = 256;
+
+// Passes 3-5 of cube g (as primal_adj_at) with its own p, q given and the
+// -a neighbours' p, q taken from the tile's copy in LDS where they are in
+// [tile_lo, tile_hi). Same operations in the same order as primal_adj_at.
+template <class S, int TILE>
+__device__ inline void primal_tile(const KArgs<S>& a, const Set13<S>& src, long g, const typename Io<S>::T p[3],
+                                   const typename Io<S>::T qs[6], const typename Io<S>::T (*lpq)[TILE], long tile_lo,
+                                   long tile_hi, typename Io<S>::T nw[4], typename Io<S>::T od[4]) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t w = a.info[g];
+    T dp = T(0);
+    T dq[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const bool h = InfoWord::has(w, ax);
+        dp = h ? dp + p[ax] : dp;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T w2 = (c == ax) ? T(1) : T(0.5);
+            dq[c] = h ? dq[c] + w2 * qs[qidx(c, ax) - F_Q0] : dq[c];
+        }
+    }
+    // incoming edge of axis ax: p_ax and the q entries of row ax of eps
+    constexpr int kF[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
+    const int ks[3] = {a.nbt[0][g], a.nbt[1][g], a.nbt[2][g]};
+    T in[3][4];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const long k = ks[ax];
+        if (k >= tile_lo && k < tile_hi) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) in[ax][e] = lpq[kF[ax][e] - F_PX][k - tile_lo];
+        } else {
+            const long sk = k >= 0 ? k : g;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) in[ax][e] = IO::ld(src.f[kF[ax][e]], sk);
+        }
+    }
+    if (ks[0] >= 0) {
+        dp -= in[0][0];
+        dq[0] -= T(1) * in[0][1];
+        dq[1] -= T(0.5) * in[0][2];
+        dq[2] -= T(0.5) * in[0][3];
+    }
+    if (ks[1] >= 0) {
+        dp -= in[1][0];
+        dq[0] -= T(0.5) * in[1][1];
+        dq[1] -= T(1) * in[1][2];
+        dq[2] -= T(0.5) * in[1][3];
+    }
+    if (ks[2] >= 0) {
+        dp -= in[2][0];
+        dq[0] -= T(0.5) * in[2][1];
+        dq[1] -= T(0.5) * in[2][2];
+        dq[2] -= T(1) * in[2][3];
+    }
+    const T u = IO::ld(src.f[F_U], g);
+    od[0] = u;
+    nw[0] = prox(u + a.tau * dp, w, a.nbins, a.tl);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        od[1 + c] = IO::ld(src.f[F_VX + c], g);
+        nw[1 + c] = od[1 + c] + a.tau * (p[c] + dq[c]);
+    }
+}
 
 template <class S, int TILE = kFuseTile, bool XCD = true>
 __global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, Set13<S> dst) {
     using IO = Io<S>;
     using T = typename IO::T;
+    // LDS: the tile's p, q (phases 0-1), reused for the halo's bars (phases 2-3)
+    constexpr int kPQ = 9 * TILE, kHalo = 4 * 3 * TILE;
+    __shared__ T shared_pq_halo[kPQ > kHalo ? kPQ : kHalo];
     __shared__ T own_bar[4][TILE];
-    __shared__ T halo_bar[4][3 * TILE];
     __shared__ int32_t halo_j[3 * TILE];
     __shared__ int32_t n_halo;
+    T(*own_pq)[TILE] = reinterpret_cast<T(*)[TILE]>(shared_pq_halo);
+    T(*halo_bar)[3 * TILE] = reinterpret_cast<T(*)[3 * TILE]>(shared_pq_halo);
     const uint32_t t0 = (uint32_t)tile_id<XCD>() * TILE;
     const uint32_t t = t0 + threadIdx.x;
     const long g = (long)a.off + t;
     const bool run = t < a.n && !masked(a, g);
-    if (threadIdx.x == 0) n_halo = 0;
-    T nw[4], p[3], q[6];
-    if (run) {
-        T od[4];
-        primal_adj_at(a, src, g, nw, od, p, q);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) own_bar[c][threadIdx.x] = IO::rt(T(2) * nw[c] - od[c]);
-    }
-    __syncthreads();
-    // where each +a neighbour's bar comes from: >= 0 an LDS slot (own tile
-    // below TILE, halo above), -1 this cube (no neighbour), -2 a cube of
-    // another level (its frozen u_bar, v_bar)
-    int32_t src_slot[3] = {-1, -1, -1};
-    int32_t js[3] = {-1, -1, -1};
     const long tile_lo = (long)a.off + t0;
     const long tile_hi = (long)a.off + min(t0 + (uint32_t)TILE, a.n);
+    if (threadIdx.x == 0) n_halo = 0;
+    // phase 0: the tile's p, q into LDS
+    T p[3], q[6];
     if (run) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            p[c] = IO::ld(src.f[F_PX + c], g);
+            own_pq[c][threadIdx.x] = p[c];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            q[k] = IO::ld(src.f[F_Q0 + k], g);
+            own_pq[3 + k][threadIdx.x] = q[k];
+        }
+    }
+    __syncthreads();
+    // phase 1: own primal values; where each +a neighbour's bar comes from:
+    // >= 0 an LDS slot (own tile below TILE, halo above), -1 this cube (no
+    // neighbour), -2 a cube of another level (its frozen u_bar, v_bar, read
+    // here already)
+    T nw[4];
+    int32_t src_slot[3] = {-1, -1, -1};
+    T frozen[3][4];
+    if (run) {
+        T od[4];
+        primal_tile<S, TILE>(a, src, g, p, q, own_pq, tile_lo, tile_hi, nw, od);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) own_bar[c][threadIdx.x] = IO::rt(T(2) * nw[c] - od[c]);
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             const int j = a.nbp[ax][g];
-            js[ax] = j;
             if (j < 0) {
                 src_slot[ax] = -1;
             } else if (j >= tile_lo && j < tile_hi) {
@@ -434,10 +518,13 @@ __global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, S
                 src_slot[ax] = TILE + h;
             } else {
                 src_slot[ax] = -2;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) frozen[ax][c] = IO::ld(a.s[F_UB + c], j);
             }
         }
     }
     __syncthreads();
+    // phase 2: primal values of the out-of-tile neighbours (LDS p, q are dead)
     const int nh = n_halo;
     for (int h = threadIdx.x; h < nh; h += TILE) {
         T n2[4], o2[4];
@@ -446,6 +533,7 @@ __global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, S
         for (int c = 0; c < 4; ++c) halo_bar[c][h] = IO::rt(T(2) * n2[c] - o2[c]);
     }
     __syncthreads();
+    // phase 3: dual step
     if (!run) return;
     T bar[4][4];
 #pragma unroll
@@ -457,7 +545,7 @@ __global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, S
         for (int c = 0; c < 4; ++c) {
             T v;
             if (sl == -1) v = bar[0][c];
-            else if (sl == -2) v = IO::ld(a.s[F_UB + c], js[ax]);
+            else if (sl == -2) v = frozen[ax][c];
             else if (sl < TILE) v = own_bar[c][sl];
             else v = halo_bar[c][sl - TILE];
             bar[1 + ax][c] = v;
@@ -911,7 +999,10 @@ KArgs<S> make_args(tgvx_ctx* c, const tgvx_params* prm) {
 
 int block_of(const tgvx_params* p) {
     int b = p && p->block > 0 ? p->block : 256;
-    if (b != 64 && b != 128 && b != 256) throw Error(TGVX_E_ARG, "block must be 64, 128 or 256");
+    if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
+        throw Error(TGVX_E_ARG, "block must be 64, 128, 256, 512 or 1024");
+    const bool tiled = p && p->op_mode == TGVX_OP_ADJ && p->fused == 1;
+    if (b > 256 && !tiled) throw Error(TGVX_E_ARG, "blocks above 256 exist only for the tiled fused kernel");
     return b;
 }
 
@@ -936,7 +1027,7 @@ Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool froz
     if (p->fused < 0 || p->fused > 2) throw Error(TGVX_E_ARG, "fused must be 0, 1 or 2");
     Launch L{adj && !frozen ? p->fused : 0, block_of(p), 1};
     if (L.variant == 1 && p->block <= 0) L.block = kFuseTile;
-    if (L.variant == 1 && L.block == 64) throw Error(TGVX_E_ARG, "the tiled fused kernel takes block 128 or 256");
+    if (L.variant == 1 && L.block == 64) throw Error(TGVX_E_ARG, "the tiled fused kernel takes block 128 or more");
     if (p->tune && !frozen) {
         auto it = c->tuned.find(std::make_tuple(c->prob.precision, p->op_mode, size_bucket(n)));
         if (it != c->tuned.end()) L = Launch{it->second.variant, it->second.block, it->second.xcd};
@@ -964,6 +1055,7 @@ Set13<S> alt_set(tgvx_ctx* c) {
 
 template <class S, bool X>
 void launch_iter_x(tgvx_ctx* c, KArgs<S>& a, int op_mode, int block, bool frozen) {
+    if (block > 256) block = 256;  // the dual / primal kernels take at most 256
     const unsigned grid = (a.n + block - 1) / block;
     if (frozen) k_dual<S, true, X><<<grid, block, 0, c->stream>>>(a);
     else k_dual<S, false, X><<<grid, block, 0, c->stream>>>(a);
@@ -984,8 +1076,12 @@ void launch_iter(tgvx_ctx* c, KArgs<S>& a, int op_mode, const Launch& L, bool fr
 template <class S, bool X>
 void launch_fused_pass_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, const Launch& L) {
     if (L.variant == 1) {
-        if (L.block == 128) k_fused_tile<S, 128, X><<<(a.n + 127) / 128, 128, 0, c->stream>>>(a, src, dst);
-        else k_fused_tile<S, 256, X><<<(a.n + 255) / 256, 256, 0, c->stream>>>(a, src, dst);
+        switch (L.block) {
+            case 128: k_fused_tile<S, 128, X><<<(a.n + 127) / 128, 128, 0, c->stream>>>(a, src, dst); break;
+            case 512: k_fused_tile<S, 512, X><<<(a.n + 511) / 512, 512, 0, c->stream>>>(a, src, dst); break;
+            case 1024: k_fused_tile<S, 1024, X><<<(a.n + 1023) / 1024, 1024, 0, c->stream>>>(a, src, dst); break;
+            default: k_fused_tile<S, 256, X><<<(a.n + 255) / 256, 256, 0, c->stream>>>(a, src, dst); break;
+        }
     } else {
         k_fused_adj<S, X><<<(a.n + L.block - 1) / L.block, L.block, 0, c->stream>>>(a, src, dst);
     }
@@ -1001,16 +1097,17 @@ void launch_fused_pass(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set1
 // alternate set, primal of the last back into the main set.
 template <class S, bool X>
 void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
-    const unsigned grid = (a.n + L.block - 1) / L.block;
+    const int wg = L.block > 256 ? 256 : L.block;  // the dual / primal kernels take at most 256
+    const unsigned grid = (a.n + wg - 1) / wg;
     Set13<S> cur = main_set(a), oth = alt_set<S>(c);
     const Set13<S> mainset = cur;
-    k_dual<S, false, X><<<grid, L.block, 0, c->stream>>>(a);
+    k_dual<S, false, X><<<grid, wg, 0, c->stream>>>(a);
     for (int i = 1; i < iters; ++i) {
         launch_fused_pass_x<S, X>(c, a, cur, oth, L);
         std::swap(cur, oth);
     }
     const int copy = cur.f[0] != mainset.f[0];
-    k_primal_adj<S, X><<<grid, L.block, 0, c->stream>>>(a, cur, mainset, copy);
+    k_primal_adj<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, copy);
 }
 template <class S>
 void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
@@ -1456,7 +1553,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
     for (int x = 1; x >= 0; --x) {
         for (int b : {256, 128, 64}) cand.push_back(Launch{0, b, x});
         if (p->op_mode == TGVX_OP_ADJ) {
-            for (int b : {256, 128}) cand.push_back(Launch{1, b, x});
+            for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{1, b, x});
             for (int b : {256, 128}) cand.push_back(Launch{2, b, x});
         }
     }
