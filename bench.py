@@ -36,7 +36,8 @@ ELEM = {"fp32": 4, "bf16": 2, "fp64": 8}
 # bytes than this (26 state values, a 4 B info word), so for them the
 # figure is the work done, not the traffic.
 ALG_BYTES = {k: 34 * e + 8 + 24 for k, e in ELEM.items()}
-KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj"}
+KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj",
+                3: "k_fused_tile (2 cubes per thread)"}
 
 
 def log(msg):
@@ -56,8 +57,9 @@ def main():
     ap.add_argument("--iters", type=int, default=50, help="iterations per level in one step")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--fused", type=int, choices=[0, 1, 2], default=1,
-                    help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused kernel")
+    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3], default=1,
+                    help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "
+                         "kernel; 3: LDS-tiled, two cubes per thread")
     ap.add_argument("--no-fused", action="store_true", help="same as --fused 0")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start time-to-gap runs")
     ap.add_argument("--step-ratio", type=float, default=1024.0,

@@ -144,7 +144,7 @@ def test_cold_dual_recovery_to_gap(ctx, sphere):
     assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("prec", [FP32, FP64])
 def test_fused_iteration_identical_to_two_kernels(ctx, sphere, prec, variant):
     # exact-adjoint mode: one fused kernel per iteration (LDS-tiled or not) ==
@@ -159,7 +159,7 @@ def test_fused_iteration_identical_to_two_kernels(ctx, sphere, prec, variant):
                           b.view(np.uint32 if b.dtype == np.float32 else np.uint64))
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 def test_fused_bf16_matches_two_kernels(ctx, sphere, variant):
     sc, hist = sphere
     a, _ = ctx.solve(sc.cubes, hist, TGVParams(alpha0=2.0, alpha1=1.0, iters=40, precision=BF16, op_mode=OP_ADJ,
