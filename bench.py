@@ -55,7 +55,7 @@ def main():
     ap.add_argument("--height", type=int, default=960)
     ap.add_argument("--max-depth", type=int, default=10)
     ap.add_argument("--iters", type=int, default=50, help="iterations per level in one step")
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--block", type=int, default=0, help="workgroup / tile size (0: the library default)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3], default=1,
                     help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "

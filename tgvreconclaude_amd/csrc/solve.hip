@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(256) k_fused_adj(KArgs<S> a, Set13<S> src, Set
 // reads the +a neighbours' u_bar, v_bar from LDS, and only the +a neighbours
 // outside the tile (a few tenths per cube in Morton order) have their primal
 // values recomputed, once per reference, by the whole workgroup.
-constexpr int kFuseTile = 256;
+constexpr int kFuseTile = 512;  // default tile of the tiled fused kernel
 
 // Passes 3-5 of cube g (as primal_adj_at) with its own p, q given and the
 // -a neighbours' p, q taken from the tile's copy in LDS where they are in

@@ -6,10 +6,11 @@ every level, the features of each cube (computed with the parent level's final
 state, exactly as at inference) are paired with the cube's converged (u, v)
 minus its cold initialisation. The network is trained on the GPU with the
 library's own forward / backward / Adam kernels. Evaluation solves held-out
-depth-10 scenes (24 ring views of 1280x960) to the same gap three ways: cold
-(the reference initialisation, p = q = 0), cold with the dual recovered from
-the initial (u, v), and warm (network + recovered dual, with the per-level
-fallback to cold); each must reach the gap.
+depth-10 scenes (24 ring views of 1280x960) to the same gap: the plain cold
+loop with the reference's steps (sigma = tau = 0.25, p = q = 0), and with steps
+sigma / tau = 1024: cold, cold with the dual recovered from the initial
+(u, v), and warm (network + recovered dual, with the per-level fallback to
+cold); each must reach the gap.
 
     python -m tgvreconclaude_amd.warmstart train --scenes 150 --out weights.npy
     python -m tgvreconclaude_amd.warmstart eval --scenes 50 --weights weights.npy
@@ -27,6 +28,11 @@ from . import FP32, OP_ADJ, Context, TGVParams, scene
 ALPHA0, ALPHA1 = 2.0, 1.0
 GAP = 1e-4
 DUAL_EPS = 1e-4
+# primal-dual steps: sigma / tau = 1024 with sigma * tau = 1/16 (the reference's
+# 0.25 * 0.25); at the reference's sigma = tau = 0.25 the same solves take
+# 5-8x longer
+SIGMA, TAU = 8.0, 1.0 / 128
+REFERENCE_STEPS = dict(sigma=0.25, tau=0.25)
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart_mlp.npy")
 
 
@@ -36,7 +42,7 @@ def log(msg):
 
 def params(**kw):
     base = dict(alpha0=ALPHA0, alpha1=ALPHA1, precision=FP32, op_mode=OP_ADJ, gap_tol=GAP, chunk=50,
-                max_iters=40000)
+                max_iters=40000, sigma=SIGMA, tau=TAU)
     base.update(kw)
     return TGVParams(**base)
 
@@ -99,7 +105,8 @@ def solve_variants(ctx, sc, weights, diagnostics=False):
     hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
     ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
     out = {"cubes": int(len(sc.cubes))}
-    runs = {"cold": params(), "cold_dual": params(dual_eps=DUAL_EPS),
+    runs = {"cold_reference_steps": params(**REFERENCE_STEPS), "cold": params(),
+            "cold_dual": params(dual_eps=DUAL_EPS),
             "warm": params(dual_eps=DUAL_EPS, warm_start=1, warm_fallback=1)}
     if diagnostics:
         # the network without the fallback rule, and the converged solution
@@ -109,7 +116,8 @@ def solve_variants(ctx, sc, weights, diagnostics=False):
         runs["warm_cold_dual_no_fallback"] = params(dual_eps=DUAL_EPS, warm_start=1, warm_fallback=0, warm_dual=1)
         runs["solution_init"] = params(dual_eps=DUAL_EPS, warm_fallback=0)
     ctx.mlp_set(weights)
-    ctx.run(runs["cold"])  # untimed: builds this problem's iteration graphs
+    ctx.run(params(max_iters=50, **REFERENCE_STEPS))  # untimed: graphs of the reference steps
+    ctx.run(runs["cold"])  # untimed: graphs of the tuned steps, and the solution
     st = ctx.state()
     sol = np.ascontiguousarray(np.concatenate([st["u"][:, None], st["v"]], axis=1))
     for name, p in runs.items():
@@ -128,13 +136,22 @@ def evaluate(ctx, weights, n_scenes, seed0=900000, diagnostics=False):
         r = solve_variants(ctx, sc, weights, diagnostics)
         r["seed"] = seed0 + k
         res.append(r)
-        log(f"[warmstart] eval {k}: {r['cubes']} cubes  cold {r['cold']['ms']:.0f} ms  "
+        log(f"[warmstart] eval {k}: {r['cubes']} cubes  cold (reference steps) "
+            f"{r['cold_reference_steps']['ms']:.0f} ms  cold {r['cold']['ms']:.0f} ms  "
             f"cold+dual {r['cold_dual']['ms']:.0f} ms  warm {r['warm']['ms']:.0f} ms")
+    ref = np.array([r["cold_reference_steps"]["ms"] for r in res])
     cold = np.array([r["cold"]["ms"] for r in res])
     cd = np.array([r["cold_dual"]["ms"] for r in res])
     warm = np.array([r["warm"]["ms"] for r in res])
-    summary = {"scenes": n_scenes, "gap": GAP,
-               "all_reached": bool(all(r[v]["reached"] for r in res for v in ("cold", "cold_dual", "warm"))),
+    summary = {"scenes": n_scenes, "gap": GAP, "sigma": SIGMA, "tau": TAU,
+               "all_reached": bool(all(r[v]["reached"] for r in res
+                                       for v in ("cold_reference_steps", "cold", "cold_dual", "warm"))),
+               "reached_per_variant": {v: int(sum(r[v]["reached"] for r in res))
+                                       for v in ("cold_reference_steps", "cold", "cold_dual", "warm")},
+               "speedup_warm_vs_cold_reference_steps_total": float(ref.sum() / warm.sum()),
+               "speedup_warm_vs_cold_reference_steps_median": float(np.median(ref / warm)),
+               "speedup_cold_vs_cold_reference_steps_total": float(ref.sum() / cold.sum()),
+               "cold_reference_steps_ms_total": float(ref.sum()),
                "speedup_warm_vs_cold_total": float(cold.sum() / warm.sum()),
                "speedup_warm_vs_cold_median": float(np.median(cold / warm)),
                "speedup_cold_dual_vs_cold_total": float(cold.sum() / cd.sum()),
