@@ -37,7 +37,7 @@ ELEM = {"fp32": 4, "bf16": 2, "fp64": 8}
 # figure is the work done, not the traffic.
 ALG_BYTES = {k: 34 * e + 8 + 24 for k, e in ELEM.items()}
 KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj",
-                3: "k_fused_tile (2 cubes per thread)"}
+                3: "k_fused_tile (p, q staged in LDS)"}
 
 
 def log(msg):
@@ -59,7 +59,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3], default=1,
                     help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "
-                         "kernel; 3: LDS-tiled, two cubes per thread")
+                         "kernel; 3: LDS-tiled with the tile's p, q staged in LDS")
     ap.add_argument("--no-fused", action="store_true", help="same as --fused 0")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start time-to-gap runs")
     ap.add_argument("--step-ratio", type=float, default=1024.0,
@@ -232,7 +232,10 @@ def main():
     traffic = None
     pj = os.path.join(ROOT, "profiles", "tgv_iteration_depth10_torus_box.json")
     if args.precision == "fp32" and os.path.exists(pj):
-        rec = json.load(open(pj)).get(KERNEL_NAMES[variant])
+        name = KERNEL_NAMES[variant]
+        if variant in (1, 3):
+            name += f", tile {tuning[-1]['block'] if tuning else (args.block or 256)}"
+        rec = json.load(open(pj)).get(name)
         if rec and rec["leaves"] == n_f:
             traffic = rec["traffic_bytes_per_iteration"]
 

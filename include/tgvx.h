@@ -99,8 +99,9 @@ typedef struct {
     int32_t fused;         /* exact-adjoint mode, results identical in all cases: 0 a dual and a
                               primal kernel per iteration; 1 (default) one fused kernel per
                               iteration sharing a tile's primal values through LDS; 2 one fused
-                              kernel recomputing every neighbour's primal values; 3 as 1 with two
-                              cubes per thread (block = the tile: 256, 512 or 1024) */
+                              kernel recomputing every neighbour's primal values; 3 as 1 with the
+                              tile's p, q staged in LDS first. For 1 and 3, block is the tile
+                              (128, 256 = default, 512, 1024) */
     int32_t tune;          /* 1: take each level's launch configuration from tgvx_autotune's table
                               (levels without an entry use fused / block as given) */
     float sigma, tau;      /* primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150).
@@ -209,7 +210,7 @@ typedef struct {
     int32_t n_levels;
     int32_t level_depth[TGVX_MAX_LEVELS];
     uint32_t level_leaves[TGVX_MAX_LEVELS];
-    int32_t variant[TGVX_MAX_LEVELS]; /* 0 two kernels, 1 tiled fused, 2 untiled fused, 3 tiled, 2 per thread */
+    int32_t variant[TGVX_MAX_LEVELS]; /* 0 two kernels, 1 tiled fused, 2 untiled fused, 3 tiled + LDS p, q */
     int32_t block[TGVX_MAX_LEVELS];
     int32_t xcd[TGVX_MAX_LEVELS];
     double best_us[TGVX_MAX_LEVELS];    /* per iteration, chosen configuration */

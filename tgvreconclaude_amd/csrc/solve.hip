@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(256) k_fused_adj(KArgs<S> a, Set13<S> src, Set
 // reads the +a neighbours' u_bar, v_bar from LDS, and only the +a neighbours
 // outside the tile (a few tenths per cube in Morton order) have their primal
 // values recomputed, once per reference, by the whole workgroup.
-constexpr int kFuseTile = 512;  // default tile of the tiled fused kernel
+constexpr int kFuseTile = 256;  // default tile of the tiled fused kernel
 
 // Passes 3-5 of cube g (as primal_adj_at) with its own p, q given and the
 // -a neighbours' p, q taken from the tile's copy in LDS where they are in
@@ -459,14 +459,16 @@ __device__ inline void primal_tile(const KArgs<S>& a, const Set13<S>& src, long 
     }
 }
 
-// CPT cubes per thread (TILE / CPT threads): more independent loads in
-// flight per wavefront.
-template <class S, int TILE = kFuseTile, bool XCD = true, int CPT = 1>
-__global__ void __launch_bounds__(TILE / CPT) k_fused_tile(KArgs<S> a, Set13<S> src, Set13<S> dst) {
+// STAGE: the tile's p, q go through LDS first, so the primal step's incoming
+// edges inside the tile are read from LDS (one more barrier); otherwise every
+// cube's primal step reads them from memory. Which is faster depends on the
+// level and the device: the autotuner decides.
+template <class S, int TILE = kFuseTile, bool XCD = true, bool STAGE = false>
+__global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, Set13<S> dst) {
     using IO = Io<S>;
     using T = typename IO::T;
-    constexpr int NT = TILE / CPT;
-    // LDS: the tile's p, q (phases 0-1), reused for the halo's bars (phases 2-3)
+    // LDS: the tile's p, q (STAGE, phases 0-1), reused for the halo's bars
+    // (phases 2-3)
     constexpr int kPQ = 9 * TILE, kHalo = 4 * 3 * TILE;
     __shared__ T shared_pq_halo[kPQ > kHalo ? kPQ : kHalo];
     __shared__ T own_bar[4][TILE];
@@ -475,72 +477,63 @@ __global__ void __launch_bounds__(TILE / CPT) k_fused_tile(KArgs<S> a, Set13<S> 
     T(*own_pq)[TILE] = reinterpret_cast<T(*)[TILE]>(shared_pq_halo);
     T(*halo_bar)[3 * TILE] = reinterpret_cast<T(*)[3 * TILE]>(shared_pq_halo);
     const uint32_t t0 = (uint32_t)tile_id<XCD>() * TILE;
+    const uint32_t t = t0 + threadIdx.x;
+    const long g = (long)a.off + t;
+    const bool run = t < a.n && !masked(a, g);
     const long tile_lo = (long)a.off + t0;
     const long tile_hi = (long)a.off + min(t0 + (uint32_t)TILE, a.n);
-    long g[CPT];
-    bool run[CPT];
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const uint32_t t = t0 + threadIdx.x + k * NT;
-        g[k] = (long)a.off + t;
-        run[k] = t < a.n && !masked(a, g[k]);
-    }
     if (threadIdx.x == 0) n_halo = 0;
-    // phase 0: the tile's p, q into LDS
-    T p[CPT][3], q[CPT][6];
+    T p[3], q[6], nw[4];
+    if (STAGE) {
+        // phase 0: the tile's p, q into LDS
+        if (run) {
 #pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        if (!run[k]) continue;
-        const int l = threadIdx.x + k * NT;
+            for (int c = 0; c < 3; ++c) {
+                p[c] = IO::ld(src.f[F_PX + c], g);
+                own_pq[c][threadIdx.x] = p[c];
+            }
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            p[k][c] = IO::ld(src.f[F_PX + c], g[k]);
-            own_pq[c][l] = p[k][c];
+            for (int k = 0; k < 6; ++k) {
+                q[k] = IO::ld(src.f[F_Q0 + k], g);
+                own_pq[3 + k][threadIdx.x] = q[k];
+            }
         }
-#pragma unroll
-        for (int e = 0; e < 6; ++e) {
-            q[k][e] = IO::ld(src.f[F_Q0 + e], g[k]);
-            own_pq[3 + e][l] = q[k][e];
-        }
+        __syncthreads();
     }
-    __syncthreads();
     // phase 1: own primal values; where each +a neighbour's bar comes from:
     // >= 0 an LDS slot (own tile below TILE, halo above), -1 this cube (no
     // neighbour), -2 a cube of another level (its frozen u_bar, v_bar, read
     // here already)
-    T nw[CPT][4];
-    int32_t src_slot[CPT][3];
-    T frozen[CPT][3][4];
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        if (!run[k]) continue;
-        const int l = threadIdx.x + k * NT;
+    int32_t src_slot[3] = {-1, -1, -1};
+    T frozen[3][4];
+    if (run) {
         T od[4];
-        primal_tile<S, TILE>(a, src, g[k], p[k], q[k], own_pq, tile_lo, tile_hi, nw[k], od);
+        if (STAGE) primal_tile<S, TILE>(a, src, g, p, q, own_pq, tile_lo, tile_hi, nw, od);
+        else primal_adj_at(a, src, g, nw, od, p, q);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) own_bar[c][l] = IO::rt(T(2) * nw[k][c] - od[c]);
+        for (int c = 0; c < 4; ++c) own_bar[c][threadIdx.x] = IO::rt(T(2) * nw[c] - od[c]);
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
-            const int j = a.nbp[ax][g[k]];
+            const int j = a.nbp[ax][g];
             if (j < 0) {
-                src_slot[k][ax] = -1;
+                src_slot[ax] = -1;
             } else if (j >= tile_lo && j < tile_hi) {
-                src_slot[k][ax] = (int32_t)(j - tile_lo);
+                src_slot[ax] = (int32_t)(j - tile_lo);
             } else if ((uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n) {
                 const int h = atomicAdd(&n_halo, 1);
                 halo_j[h] = j;
-                src_slot[k][ax] = TILE + h;
+                src_slot[ax] = TILE + h;
             } else {
-                src_slot[k][ax] = -2;
+                src_slot[ax] = -2;
 #pragma unroll
-                for (int c = 0; c < 4; ++c) frozen[k][ax][c] = IO::ld(a.s[F_UB + c], j);
+                for (int c = 0; c < 4; ++c) frozen[ax][c] = IO::ld(a.s[F_UB + c], j);
             }
         }
     }
     __syncthreads();
     // phase 2: primal values of the out-of-tile neighbours (LDS p, q are dead)
     const int nh = n_halo;
-    for (int h = threadIdx.x; h < nh; h += NT) {
+    for (int h = threadIdx.x; h < nh; h += TILE) {
         T n2[4], o2[4];
         primal_adj_at(a, src, (long)halo_j[h], n2, o2);
 #pragma unroll
@@ -548,34 +541,30 @@ __global__ void __launch_bounds__(TILE / CPT) k_fused_tile(KArgs<S> a, Set13<S> 
     }
     __syncthreads();
     // phase 3: dual step
+    if (!run) return;
+    T bar[4][4];
 #pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        if (!run[k]) continue;
-        const int l = threadIdx.x + k * NT;
-        T bar[4][4];
+    for (int c = 0; c < 4; ++c) bar[0][c] = own_bar[c][threadIdx.x];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) bar[0][c] = own_bar[c][l];
+    for (int ax = 0; ax < 3; ++ax) {
+        const int sl = src_slot[ax];
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const int sl = src_slot[k][ax];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                T v;
-                if (sl == -1) v = bar[0][c];
-                else if (sl == -2) v = frozen[k][ax][c];
-                else if (sl < TILE) v = own_bar[c][sl];
-                else v = halo_bar[c][sl - TILE];
-                bar[1 + ax][c] = v;
-            }
+        for (int c = 0; c < 4; ++c) {
+            T v;
+            if (sl == -1) v = bar[0][c];
+            else if (sl == -2) v = frozen[ax][c];
+            else if (sl < TILE) v = own_bar[c][sl];
+            else v = halo_bar[c][sl - TILE];
+            bar[1 + ax][c] = v;
         }
-        dual_step(p[k], q[k], bar, a.sigma, a.a0, a.a1);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) IO::st(dst.f[F_U + c], g[k], nw[k][c]);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) IO::st(dst.f[F_PX + c], g[k], p[k][c]);
-#pragma unroll
-        for (int e = 0; e < 6; ++e) IO::st(dst.f[F_Q0 + e], g[k], q[k][e]);
     }
+    dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::st(dst.f[F_U + c], g, nw[c]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) IO::st(dst.f[F_PX + c], g, p[c]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) IO::st(dst.f[F_Q0 + k], g, q[k]);
 }
 
 // ---- initialisation
@@ -1045,9 +1034,9 @@ Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool froz
     if (p->fused < 0 || p->fused > 3) throw Error(TGVX_E_ARG, "fused must be 0, 1, 2 or 3");
     Launch L{adj && !frozen ? p->fused : 0, block_of(p), 1};
     if (L.variant == 1 && p->block <= 0) L.block = kFuseTile;
-    if (L.variant == 3 && p->block <= 0) L.block = 512;
-    if (L.variant == 1 && L.block == 64) throw Error(TGVX_E_ARG, "the tiled fused kernel takes block 128 or more");
-    if (L.variant == 3 && L.block < 256) throw Error(TGVX_E_ARG, "two cubes per thread take tiles of 256 or more");
+    if (L.variant == 3 && p->block <= 0) L.block = kFuseTile;
+    if ((L.variant == 1 || L.variant == 3) && L.block == 64)
+        throw Error(TGVX_E_ARG, "the tiled fused kernel takes block 128 or more");
     if (p->tune && !frozen) {
         auto it = c->tuned.find(std::make_tuple(c->prob.precision, p->op_mode, size_bucket(n)));
         if (it != c->tuned.end()) L = Launch{it->second.variant, it->second.block, it->second.xcd};
@@ -1095,18 +1084,25 @@ void launch_iter(tgvx_ctx* c, KArgs<S>& a, int op_mode, const Launch& L, bool fr
 // one fused pass (primal n, dual n+1) from src into dst
 template <class S, bool X>
 void launch_fused_pass_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, const Launch& L) {
-    if (L.variant == 1) {
+    if (L.variant == 1 || L.variant == 3) {
+        const bool stage = L.variant == 3;
         switch (L.block) {
-            case 128: k_fused_tile<S, 128, X><<<(a.n + 127) / 128, 128, 0, c->stream>>>(a, src, dst); break;
-            case 512: k_fused_tile<S, 512, X><<<(a.n + 511) / 512, 512, 0, c->stream>>>(a, src, dst); break;
-            case 1024: k_fused_tile<S, 1024, X><<<(a.n + 1023) / 1024, 1024, 0, c->stream>>>(a, src, dst); break;
-            default: k_fused_tile<S, 256, X><<<(a.n + 255) / 256, 256, 0, c->stream>>>(a, src, dst); break;
-        }
-    } else if (L.variant == 3) {  // two cubes per thread: block = tile / 2 threads
-        switch (L.block) {
-            case 256: k_fused_tile<S, 256, X, 2><<<(a.n + 255) / 256, 128, 0, c->stream>>>(a, src, dst); break;
-            case 1024: k_fused_tile<S, 1024, X, 2><<<(a.n + 1023) / 1024, 512, 0, c->stream>>>(a, src, dst); break;
-            default: k_fused_tile<S, 512, X, 2><<<(a.n + 511) / 512, 256, 0, c->stream>>>(a, src, dst); break;
+            case 128:
+                if (stage) k_fused_tile<S, 128, X, true><<<(a.n + 127) / 128, 128, 0, c->stream>>>(a, src, dst);
+                else k_fused_tile<S, 128, X, false><<<(a.n + 127) / 128, 128, 0, c->stream>>>(a, src, dst);
+                break;
+            case 256:
+                if (stage) k_fused_tile<S, 256, X, true><<<(a.n + 255) / 256, 256, 0, c->stream>>>(a, src, dst);
+                else k_fused_tile<S, 256, X, false><<<(a.n + 255) / 256, 256, 0, c->stream>>>(a, src, dst);
+                break;
+            case 1024:
+                if (stage) k_fused_tile<S, 1024, X, true><<<(a.n + 1023) / 1024, 1024, 0, c->stream>>>(a, src, dst);
+                else k_fused_tile<S, 1024, X, false><<<(a.n + 1023) / 1024, 1024, 0, c->stream>>>(a, src, dst);
+                break;
+            default:
+                if (stage) k_fused_tile<S, 512, X, true><<<(a.n + 511) / 512, 512, 0, c->stream>>>(a, src, dst);
+                else k_fused_tile<S, 512, X, false><<<(a.n + 511) / 512, 512, 0, c->stream>>>(a, src, dst);
+                break;
         }
     } else {
         k_fused_adj<S, X><<<(a.n + L.block - 1) / L.block, L.block, 0, c->stream>>>(a, src, dst);
@@ -1579,8 +1575,8 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
     for (int x = 1; x >= 0; --x) {
         for (int b : {256, 128, 64}) cand.push_back(Launch{0, b, x});
         if (p->op_mode == TGVX_OP_ADJ) {
-            for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{1, b, x});
-            for (int b : {256, 512, 1024}) cand.push_back(Launch{3, b, x});
+            for (int v : {1, 3})
+                for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{v, b, x});
             for (int b : {256, 128}) cand.push_back(Launch{2, b, x});
         }
     }
