@@ -158,6 +158,17 @@ int tgvx_solve_oc(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint3
                   const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* params,
                   void* state_out, tgvx_report* report);
 
+/* tgv_minimize_streaming (tgv.hpp:57-64, tgv.cpp:802-1070) on arrays instead of
+ * files: cubes sorted by code (as the balanced cube file), leaves as from
+ * load_treetop. Per treetop leaf, the leaf's cubes are looked up by code
+ * (last one wins), all others by the cube file's bisection; neighbours not
+ * iterated in the leaf give their level-start (u, v), p, q outside the leaf
+ * count as zero; parents come from the bisection. The reference rules
+ * (op_mode 0) with params.iters per level. state_out as tgvx_solve. */
+int tgvx_solve_streaming(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+                         const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* params,
+                         void* state_out, tgvx_report* report);
+
 /* Resident problem: upload cubes + histograms once, build tables on the device. */
 int tgvx_problem_load(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins,
                       int32_t nb, int32_t precision, int32_t op_mode);

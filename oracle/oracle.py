@@ -56,6 +56,7 @@ def lib():
         L.orc_histograms.argtypes = [vp, u64, vp, i32, vp, C.c_float, i32, vp]
         L.orc_tgv_minimize.argtypes = [vp, u64, vp, i32, C.POINTER(OParams), i32, vp, vp, vp, vp, vp, i32]
         L.orc_tgv_minimize_oc.argtypes = [vp, u64, vp, i32, vp, vp, u64, C.POINTER(OParams), i32, vp]
+        L.orc_tgv_minimize_streaming.argtypes = [vp, u64, vp, i32, vp, vp, u64, C.POINTER(OParams), i32, vp]
         L.orc_level_gap.argtypes = [vp, u64, vp, i32, C.POINTER(OParams), i32, vp, vp,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_neighbour_table.argtypes = [vp, u64, i32, i32, vp, vp]
@@ -212,6 +213,19 @@ def neighbour_table(cubes, level, mode=0):
     act = np.zeros(len(cubes), dtype=np.uint32)
     na = lib().orc_neighbour_table(_p(cubes), len(cubes), level, mode, _p(nb), _p(act))
     return act[:na].copy(), nb[:na].copy()
+
+
+def tgv_minimize_streaming(cubes, hists, leaves, prm=None, f64=False):
+    """tgv_minimize_streaming on arrays: state [n, 13]."""
+    prm = prm or params()
+    cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+    h = np.ascontiguousarray(hists, dtype=np.uint32)
+    first = np.ascontiguousarray(leaves["first_idx"], dtype=np.uint32)
+    last = np.ascontiguousarray(leaves["last_idx"], dtype=np.uint32)
+    st = np.zeros((len(cubes), 13), dtype=np.float64 if f64 else np.float32)
+    lib().orc_tgv_minimize_streaming(_p(cubes), len(cubes), _p(h), h.shape[1], _p(first), _p(last), len(first),
+                                     C.byref(prm), int(f64), _p(st))
+    return st
 
 
 def extract_surface(cubes, u, origin, r_root, has_data=None):

@@ -824,6 +824,180 @@ struct Solver {
     }
 };
 
+// ---------------------------------------------------------------- streaming
+// tgv_minimize_streaming (tgv.cpp:515-1070) held in memory instead of files.
+// Per treetop leaf: the leaf's own cubes are found by code with the last
+// writer winning (the leaf map, tgv.cpp:916-920), every other cube by the
+// cube file's bisection (CubeFileReader::lookup, tgv.cpp:674-683), which
+// returns the first probe that hits, not necessarily the last cube with the
+// code. Neighbour values: a cube iterated in the same leaf reads the live
+// u_bar, v_bar; every other cube the (u, v) snapshot taken at level start;
+// p, q of cubes outside the leaf read as zero (tgv.cpp:1023-1039). Parents
+// are copied leaf by leaf from the bisection's parent (tgv.cpp:865-912).
+template <class T>
+struct Streaming {
+    const Cube* cubes;
+    size_t N;
+    const uint32_t* bins;
+    int nb;
+    Params prm;
+    std::vector<State<T>> st;
+
+    int64_t bisect(uint64_t key) const {
+        uint64_t lo = 0, hi = N;
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            const uint64_t k = code_of(cubes[mid]);
+            if (k == key) return (int64_t)mid;
+            if (k < key) lo = mid + 1;
+            else hi = mid;
+        }
+        return -1;
+    }
+
+    void run(const uint32_t* leaf_first, const uint32_t* leaf_last, size_t nleaves) {
+        st.assign(N, State<T>{});
+        for (size_t i = 0; i < N; ++i) st[i].u = median_indicator<T>(bins + i * nb, nb);
+        if (N == 0) return;
+        int min_d = 21, max_d = 0;
+        for (size_t i = 0; i < N; ++i) {
+            min_d = std::min(min_d, cubes[i].depth);
+            max_d = std::max(max_d, cubes[i].depth);
+        }
+        // per cube: its leaf, and its six neighbours as the leaf's table has them
+        std::vector<int64_t> leaf_of(N, -1);
+        std::vector<int64_t> nbr(N * 6, -1);
+        for (size_t l = 0; l < nleaves; ++l) {
+            const uint32_t f0 = leaf_first[l], f1 = std::min<uint64_t>(leaf_last[l], N - 1);
+            if (f0 > f1) continue;
+            CodeMap leaf_map;
+            for (uint32_t i = f0; i <= f1; ++i) leaf_map[code_of(cubes[i])] = i;
+            for (uint32_t i = f0; i <= f1; ++i) {
+                leaf_of[i] = (int64_t)l;
+                const uint64_t c = code_of(cubes[i]);
+                const int d = cubes[i].depth;
+                for (int f = 0; f < 6; ++f) {
+                    const uint64_t k = face_nb(c, f, d);
+                    if (k == NONE) continue;
+                    auto it = leaf_map.find(k);
+                    int64_t j = it != leaf_map.end() ? (int64_t)it->second : bisect(k);
+                    if (j < 0 && d > 0) {
+                        const uint64_t pk = parent(k, d);
+                        auto pt = leaf_map.find(pk);
+                        j = pt != leaf_map.end() ? (int64_t)pt->second : bisect(pk);
+                    }
+                    nbr[i * 6 + f] = j;
+                }
+            }
+        }
+        const T sigma = prm.sigma > 0.f ? (T)prm.sigma : T(0.25), tau = prm.tau > 0.f ? (T)prm.tau : T(0.25);
+        const T a0 = (T)prm.alpha0, a1 = (T)prm.alpha1, tl = tau * (T)prm.lambda;
+        std::vector<T> ub(N), vb[3] = {std::vector<T>(N), std::vector<T>(N), std::vector<T>(N)};
+        std::vector<T> su(N), sv[3] = {std::vector<T>(N), std::vector<T>(N), std::vector<T>(N)};
+        for (int L = min_d; L <= max_d; ++L) {
+            // parent copy, leaf after leaf: each leaf reads its parents first
+            if (L > min_d) {
+                for (size_t l = 0; l < nleaves; ++l) {
+                    const uint32_t f0 = leaf_first[l], f1 = std::min<uint64_t>(leaf_last[l], N - 1);
+                    std::vector<uint32_t> lvl;
+                    std::vector<int64_t> par;
+                    for (uint32_t i = f0; i <= f1 && f0 <= f1; ++i)
+                        if (cubes[i].depth == L) {
+                            lvl.push_back(i);
+                            par.push_back(bisect(parent(code_of(cubes[i]), L)));
+                        }
+                    std::vector<State<T>> ps(lvl.size());
+                    for (size_t k = 0; k < lvl.size(); ++k)
+                        if (par[k] >= 0) ps[k] = st[(size_t)par[k]];
+                    for (size_t k = 0; k < lvl.size(); ++k) {
+                        State<T>& s = st[lvl[k]];
+                        if (par[k] >= 0) {
+                            s.u = ps[k].u;
+                            for (int c = 0; c < 3; ++c) s.v[c] = ps[k].v[c];
+                        }
+                        for (int c = 0; c < 3; ++c) s.p[c] = T(0);
+                        for (int c = 0; c < 6; ++c) s.q[c] = T(0);
+                    }
+                }
+            }
+            // level start: snapshot and bars of every cube
+            std::vector<uint32_t> act;
+            for (size_t i = 0; i < N; ++i) {
+                su[i] = ub[i] = st[i].u;
+                for (int c = 0; c < 3; ++c) sv[c][i] = vb[c][i] = st[i].v[c];
+                if (cubes[i].depth == L && leaf_of[i] >= 0) act.push_back((uint32_t)i);
+            }
+            if (act.empty()) continue;
+            auto live = [&](uint32_t i, int64_t j) {
+                return leaf_of[(size_t)j] == leaf_of[i] && cubes[(size_t)j].depth == L;
+            };
+            auto inside = [&](uint32_t i, int64_t j) { return leaf_of[(size_t)j] == leaf_of[i]; };
+            const size_t na = act.size();
+            std::vector<T> un(na), vn[3] = {std::vector<T>(na), std::vector<T>(na), std::vector<T>(na)};
+            for (int it = 0; it < prm.iters; ++it) {
+                for (size_t ai = 0; ai < na; ++ai) {  // dual p (tgv.cpp:738-748)
+                    const uint32_t i = act[ai];
+                    auto ubf = [&](int f) -> T {
+                        const int64_t j = nbr[i * 6 + f];
+                        if (j < 0) return ub[i];
+                        return live(i, j) ? ub[(size_t)j] : su[(size_t)j];
+                    };
+                    st[i].p[0] += sigma * (ubf(0) - ub[i] - vb[0][i]);
+                    st[i].p[1] += sigma * (ubf(2) - ub[i] - vb[1][i]);
+                    st[i].p[2] += sigma * (ubf(4) - ub[i] - vb[2][i]);
+                    clip3(st[i].p, a1);
+                }
+                for (size_t ai = 0; ai < na; ++ai) {  // dual q (tgv.cpp:751-767)
+                    const uint32_t i = act[ai];
+                    auto vbf = [&](int f, int c) -> T {
+                        const int64_t j = nbr[i * 6 + f];
+                        if (j < 0) return vb[c][i];
+                        return live(i, j) ? vb[c][(size_t)j] : sv[c][(size_t)j];
+                    };
+                    T eps[6] = {vbf(0, 0) - vb[0][i],
+                                vbf(2, 1) - vb[1][i],
+                                vbf(4, 2) - vb[2][i],
+                                T(0.5) * (vbf(2, 0) - vb[0][i] + vbf(0, 1) - vb[1][i]),
+                                T(0.5) * (vbf(4, 0) - vb[0][i] + vbf(0, 2) - vb[2][i]),
+                                T(0.5) * (vbf(4, 1) - vb[1][i] + vbf(2, 2) - vb[2][i])};
+                    for (int k = 0; k < 6; ++k) st[i].q[k] += sigma * eps[k];
+                    clip6(st[i].q, a0);
+                }
+                for (size_t ai = 0; ai < na; ++ai) {  // primal u (tgv.cpp:770-779)
+                    const uint32_t i = act[ai];
+                    auto gp = [&](int f, int c) -> T {
+                        const int64_t j = nbr[i * 6 + f];
+                        return (j >= 0 && inside(i, j)) ? st[(size_t)j].p[c] : T(0);
+                    };
+                    T div_p = (st[i].p[0] - gp(1, 0)) + (st[i].p[1] - gp(3, 1)) + (st[i].p[2] - gp(5, 2));
+                    un[ai] = prox_data(st[i].u + tau * div_p, tl, bins + (size_t)i * nb, nb);
+                }
+                for (size_t ai = 0; ai < na; ++ai) {  // primal v (tgv.cpp:782-795)
+                    const uint32_t i = act[ai];
+                    auto gq = [&](int f, int c) -> T {
+                        const int64_t j = nbr[i * 6 + f];
+                        return (j >= 0 && inside(i, j)) ? st[(size_t)j].q[c] : T(0);
+                    };
+                    const State<T>& s = st[i];
+                    T dqx = (s.q[0] - gq(1, 0)) + (s.q[3] - gq(3, 3)) + (s.q[4] - gq(5, 4));
+                    T dqy = (s.q[3] - gq(1, 3)) + (s.q[1] - gq(3, 1)) + (s.q[5] - gq(5, 5));
+                    T dqz = (s.q[4] - gq(1, 4)) + (s.q[5] - gq(3, 5)) + (s.q[2] - gq(5, 2));
+                    vn[0][ai] = s.v[0] + tau * (s.p[0] + dqx);
+                    vn[1][ai] = s.v[1] + tau * (s.p[1] + dqy);
+                    vn[2][ai] = s.v[2] + tau * (s.p[2] + dqz);
+                }
+                for (size_t ai = 0; ai < na; ++ai) {  // extrapolation (tgv.cpp:798-810)
+                    const uint32_t i = act[ai];
+                    ub[i] = T(2) * un[ai] - st[i].u;
+                    for (int c = 0; c < 3; ++c) vb[c][i] = T(2) * vn[c][ai] - st[i].v[c];
+                    st[i].u = un[ai];
+                    for (int c = 0; c < 3; ++c) st[i].v[c] = vn[c][ai];
+                }
+            }
+        }
+    }
+};
+
 // ---------------------------------------------------------------- surface
 // marching_cubes.cpp:16-181: one quad per face-adjacent pair with a sign
 // change in u, at the interpolated zero crossing between the cube centres,
@@ -1056,6 +1230,34 @@ int32_t orc_tgv_minimize_oc(const orc::Cube* cubes, uint64_t n, const uint32_t* 
     s.minimize_oc(leaf_first, leaf_last, nleaves);
     copy_out(s, state_out, nullptr, nullptr, nullptr, 0);
     return (int32_t)s.level_iters.size();
+}
+
+// tgv_minimize_streaming on arrays (cubes sorted by code, as the cube file);
+// leaves as inclusive [first, last] index ranges. state_out: 13 values per
+// cube of the chosen type.
+int32_t orc_tgv_minimize_streaming(const orc::Cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+                                   const uint32_t* leaf_first, const uint32_t* leaf_last, uint64_t nleaves,
+                                   const orc_params* p, int32_t is_f64, void* state_out) {
+    auto out = [&](auto& s, auto* dst) {
+        for (size_t i = 0; i < s.N; ++i) {
+            const auto& x = s.st[i];
+            auto* r = dst + i * 13;
+            r[0] = x.u;
+            for (int c = 0; c < 3; ++c) r[1 + c] = x.v[c];
+            for (int c = 0; c < 3; ++c) r[4 + c] = x.p[c];
+            for (int c = 0; c < 6; ++c) r[7 + c] = x.q[c];
+        }
+    };
+    if (is_f64) {
+        orc::Streaming<double> s{cubes, n, bins, nb, to_params(p), {}};
+        s.run(leaf_first, leaf_last, nleaves);
+        out(s, (double*)state_out);
+    } else {
+        orc::Streaming<float> s{cubes, n, bins, nb, to_params(p), {}};
+        s.run(leaf_first, leaf_last, nleaves);
+        out(s, (float*)state_out);
+    }
+    return 0;
 }
 
 // Energy / gap of one level for a given full state (FP64 in, FP64 out): the

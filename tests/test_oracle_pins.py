@@ -307,6 +307,50 @@ def test_oc_two_leaf_polarity():
     assert u[0] > 0 and u[2] < 0 and u[2] < u[1] < u[0]
 
 
+def _leaves(*ranges):
+    return np.array(list(ranges), dtype=[("first_idx", "<u4"), ("last_idx", "<u4")])
+
+
+def test_streaming_single_leaf_equals_in_memory_bitwise():
+    # tests/test_stage6.cpp:241-275 allows 1e-4; one leaf, one depth: no
+    # snapshot and no border is ever read, so the results are identical
+    cs, h = chain3()
+    prm = O.params(alpha0=0.1, alpha1=0.2, iters=50)
+    a = O.tgv_minimize(cs, h, prm)[0]
+    b = O.tgv_minimize_streaming(cs, h, _leaves((0, 2)), prm)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_streaming_two_leaf_polarity():
+    # tests/test_stage6.cpp:277-310
+    cs, h = chain3()
+    st = O.tgv_minimize_streaming(cs, h, _leaves((0, 1), (2, 2)), O.params(alpha0=0.1, alpha1=0.2, iters=50))
+    u = st[:, 0]
+    assert u[0] > 0 and u[2] < 0 and u[2] < u[1] < u[0]
+    assert np.all(np.isfinite(st))
+
+
+def test_streaming_parent_copy_follows_the_bisection():
+    # A chain of octant-0 descendants shares code 0 at depths 1-4, then a
+    # cube B at depth 3 with a larger code. The cube file's bisection for
+    # code 0 over five cubes probes index 2 first and stops there (the depth-3
+    # cube), so with no iterations every parent copy reads index 2's median:
+    # u = [m0, m2, m2, m2, m2]. The in-memory solve takes the last cube with
+    # the code (index 3) instead.
+    cs = cubes_of([cube(0, 0, 0, 1), cube(0, 0, 0, 2), cube(0, 0, 0, 3), cube(0, 0, 0, 4), cube(1, 0, 0, 3)])
+    codes = [(int(c["hi"]) << 32) | int(c["lo"]) for c in cs]
+    assert codes[:4] == [0, 0, 0, 0] and codes[4] > 0
+    h = np.zeros((5, 8), np.uint32)
+    for i, k in enumerate((0, 2, 4, 6, 7)):
+        h[i, k] = 5
+    m = [O.median_indicator(x) for x in h]
+    st = O.tgv_minimize_streaming(cs, h, _leaves((0, 4)), O.params(iters=0))
+    assert st[:, 0].tolist() == [m[0], m[2], m[2], m[2], m[2]]
+    assert np.all(st[:, 1:] == 0)
+    mem = O.tgv_minimize(cs, h, O.params(iters=0))[0]
+    assert mem[1, 0] == m[3]
+
+
 def test_threads_do_not_change_bits(tmp_path):
     # Every pass is a Jacobi sweep: splitting it over threads changes nothing.
     rng = np.random.default_rng(3)

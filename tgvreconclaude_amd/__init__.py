@@ -146,6 +146,18 @@ class Context:
                                        C.byref(params), N.ptr(init), N.ptr(st), C.byref(rep)))
         return st, rep
 
+    def solve_streaming(self, cubes, hists, leaves, params=None):
+        """tgv_minimize_streaming on arrays; returns (state, Report)."""
+        params = params or TGVParams()
+        cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+        h = np.ascontiguousarray(hists, dtype=np.uint32)
+        lv = np.ascontiguousarray(leaves, dtype=LEAF_DT)
+        st = self._state_array(len(cubes), params.precision)
+        rep = Report()
+        self._check(self._L.tgvx_solve_streaming(self.h, N.ptr(cubes), len(cubes), N.ptr(h), h.shape[1], N.ptr(lv),
+                                                 len(lv), C.byref(params), N.ptr(st), C.byref(rep)))
+        return st, rep
+
     def solve_oc(self, cubes, hists, leaves, params=None):
         params = params or TGVParams()
         cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)

@@ -89,7 +89,7 @@ TGVX_SYMBOLS = [
     "tgvx_problem_levels", "tgvx_level_bench", "tgvx_mlp_set", "tgvx_mlp_get", "tgvx_mlp_forward",
     "tgvx_mlp_loss_grad", "tgvx_mlp_train", "tgvx_problem_features", "tgvx_problem_load_blocks",
     "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune", "tgvx_problem_solve_levels",
-    "tgvx_surface_extract", "tgvx_surface_copy",
+    "tgvx_surface_extract", "tgvx_surface_copy", "tgvx_solve_streaming",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
@@ -143,6 +143,8 @@ def lib():
         L.tgvx_problem_solve_levels.argtypes = [vp, C.POINTER(Params), vp, i32, C.POINTER(Report)]
         L.tgvx_surface_extract.argtypes = [vp, vp, vp, vp, C.c_float, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.tgvx_surface_copy.argtypes = [vp, vp, vp]
+        L.tgvx_solve_streaming.argtypes = [vp, vp, C.c_uint64, vp, i32, vp, C.c_uint64, C.POINTER(Params), vp,
+                                           C.POINTER(Report)]
         L.tgvx_problem_block_report.argtypes = [vp, vp, vp]
         L.tgvx_solve_batched.argtypes = [vp, vp, C.c_uint32, i32, C.POINTER(Params), vp, C.POINTER(Report)]
         for name in TGVX_SYMBOLS:

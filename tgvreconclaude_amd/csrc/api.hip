@@ -231,6 +231,28 @@ int tgvx_surface_copy(tgvx_ctx* c, float* verts, uint32_t* tris) {
     return guarded(c, [&] { tgvx::surface_copy(c, verts, tris); });
 }
 
+int tgvx_solve_streaming(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+                         const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* p, void* state_out,
+                         tgvx_report* rep) {
+    return guarded(c, [&] {
+        if (!p || !state_out || (n && (!cubes || !bins))) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        // the cube file is sorted by code; its bisection relies on it
+        for (uint64_t i = 1; i < n; ++i) {
+            const uint64_t a = ((uint64_t)cubes[i - 1].hi << 32) | cubes[i - 1].lo;
+            const uint64_t b = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
+            if (b < a) throw tgvx::Error(TGVX_E_ARG, "cubes must be sorted by Morton code");
+        }
+        tgvx_report r;
+        std::memset(&r, 0, sizeof(r));
+        const auto t0 = std::chrono::steady_clock::now();
+        tgvx::problem_load(c, cubes, n, bins, nb, p->precision, TGVX_OP_REF);
+        r.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        tgvx::problem_solve_streaming(c, p, leaves, nleaves, &r);
+        tgvx::problem_state(c, state_out, nullptr);
+        if (rep) *rep = r;
+    });
+}
+
 int tgvx_autotune(tgvx_ctx* c, const tgvx_params* p, int32_t iters, tgvx_tune_report* rep) {
     return guarded(c, [&] { tgvx::autotune(c, p, iters, rep); });
 }

@@ -53,6 +53,12 @@ struct KArgs {
     // byte in bact is 0 (the block has converged); null: every cube runs
     const int32_t* blk;
     const uint8_t* bact;
+    // streaming mode (tgv_minimize_streaming): per cube, bit f of face f set
+    // for the +a faces (0, 2, 4) when that neighbour is iterated in the same
+    // treetop leaf (else its level-start snapshot in fz is read), and for the
+    // -a faces (1, 3, 5) when that neighbour lies outside the leaf (its p, q
+    // read as 0); bit 6: the cube belongs to a leaf. Null otherwise.
+    const uint8_t* sflag;
 };
 
 template <class S>
@@ -124,8 +130,12 @@ __device__ inline void dual_step(T p[3], T q[6], const T bar[4][4], T sigma, T a
     clip6(q, a0);
 }
 
-// ---- phase D: dual p and q (tgv.cpp:158-189; frozen variant :274-309)
-template <class S, bool FROZEN, bool XCD = true>
+// ---- phase D: dual p and q (tgv.cpp:158-189; frozen variant :274-309;
+// streaming variant :738-767). MODE 0: every neighbour live; 1 (frozen
+// borders): live inside the treetop leaf [leaf_first, leaf_last]; 2
+// (streaming): live as sflag says. A neighbour that is not live gives its
+// level-start snapshot fz.
+template <class S, int MODE, bool XCD = true>
 __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
     using IO = Io<S>;
     using T = typename IO::T;
@@ -133,21 +143,24 @@ __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
     if (t >= a.n) return;
     const long g = (long)a.off + t;
     if (masked(a, g)) return;
+    const uint32_t sf = MODE == 2 ? a.sflag[g] : 0u;
+    if (MODE == 2 && !(sf & 64u)) return;  // in no leaf: never iterated
     const T ubi = IO::ld(a.s[F_UB], g), vxi = IO::ld(a.s[F_VBX], g), vyi = IO::ld(a.s[F_VBY], g),
             vzi = IO::ld(a.s[F_VBZ], g);
     const int j0 = a.nbp[0][g], j2 = a.nbp[1][g], j4 = a.nbp[2][g];
-    auto live = [&](int j) -> bool {
-        if (!FROZEN) return true;
+    auto live = [&](int j, int ax) -> bool {
+        if (MODE == 0) return true;
+        if (MODE == 2) return (sf >> (2 * ax)) & 1u;
         const uint32_t o = a.orig[j];
         return o >= a.leaf_first && o <= a.leaf_last;
     };
     // A missing neighbour reads the cube's own value (Neumann, tgv.cpp:163):
     // loading at g itself gives exactly that, so every load is unconditional.
     const long s0 = j0 >= 0 ? j0 : g, s2 = j2 >= 0 ? j2 : g, s4 = j4 >= 0 ? j4 : g;
-    auto rd = [&](int j, long sj, int f, int fzi, T self) -> T {
-        if (FROZEN) {
+    auto rd = [&](int j, long sj, int f, int fzi, T self, int ax) -> T {
+        if (MODE != 0) {
             if (j < 0) return self;
-            if (!live(j)) return IO::ld(a.fz[fzi], j);
+            if (!live(j, ax)) return IO::ld(a.fz[fzi], j);
         }
         return IO::ld(a.s[f], sj);
     };
@@ -158,7 +171,7 @@ __global__ void __launch_bounds__(256) k_dual(KArgs<S> a) {
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) bar[1 + ax][c] = rd(js[ax], ss[ax], F_UB + c, c, bar[0][c]);
+        for (int c = 0; c < 4; ++c) bar[1 + ax][c] = rd(js[ax], ss[ax], F_UB + c, c, bar[0][c], ax);
     T p[3] = {IO::ld(a.s[F_PX], g), IO::ld(a.s[F_PY], g), IO::ld(a.s[F_PZ], g)};
     T q[6];
 #pragma unroll
@@ -180,7 +193,15 @@ __global__ void __launch_bounds__(256) k_primal_ref(KArgs<S> a) {
     if (t >= a.n) return;
     const long g = (long)a.off + t;
     if (masked(a, g)) return;
-    const int m0 = a.nbm[0][g], m2 = a.nbm[1][g], m4 = a.nbm[2][g];
+    int m0 = a.nbm[0][g], m2 = a.nbm[1][g], m4 = a.nbm[2][g];
+    if (a.sflag) {
+        // streaming: p, q of a cube outside the leaf count as 0, as a missing one
+        const uint32_t sf = a.sflag[g];
+        if (!(sf & 64u)) return;
+        m0 = (sf & 2u) ? -1 : m0;
+        m2 = (sf & 8u) ? -1 : m2;
+        m4 = (sf & 32u) ? -1 : m4;
+    }
     // A missing neighbour contributes 0 (tgv.cpp:196,209). Loads go to g
     // instead and the value is replaced by 0 afterwards, so that every load
     // is unconditional and they can all be in flight together.
@@ -514,7 +535,7 @@ __global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, S
         for (int c = 0; c < 4; ++c) own_bar[c][threadIdx.x] = IO::rt(T(2) * nw[c] - od[c]);
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
-            const int j = a.nbp[ax][g];
+            const int j = __builtin_nontemporal_load(&a.nbp[ax][g]);  // read once
             if (j < 0) {
                 src_slot[ax] = -1;
             } else if (j >= tile_lo && j < tile_hi) {
@@ -560,11 +581,134 @@ __global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, S
     }
     dual_step(p, q, bar, a.sigma, a.a0, a.a1);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) IO::st(dst.f[F_U + c], g, nw[c]);
+    // streaming stores: nothing reads dst in this launch, so keep L2 for the
+    // gathered neighbour values
+    for (int c = 0; c < 4; ++c) IO::stn(dst.f[F_U + c], g, nw[c]);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) IO::st(dst.f[F_PX + c], g, p[c]);
+    for (int c = 0; c < 3; ++c) IO::stn(dst.f[F_PX + c], g, p[c]);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) IO::st(dst.f[F_Q0 + k], g, q[k]);
+    for (int k = 0; k < 6; ++k) IO::stn(dst.f[F_Q0 + k], g, q[k]);
+}
+
+// ---- streaming mode (tgv_minimize_streaming, tgv.cpp:515-1070)
+// The cube file's bisection (CubeFileReader::lookup, tgv.cpp:674-683): the
+// first probe whose code matches, not necessarily the last cube with it.
+__device__ inline int64_t bisect_code(const uint64_t* code, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        const uint64_t k = code[mid];
+        if (k == key) return mid;
+        if (k < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return -1;
+}
+// The leaf map (tgv.cpp:916-920): the last cube of [f0, f1] with this code;
+// the cube array is sorted by code.
+__device__ inline int64_t leaf_code(const uint64_t* code, uint32_t f0, uint32_t f1, uint64_t key) {
+    uint32_t lo = f0, hi = f1 + 1;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (code[mid] <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo > f0 && code[lo - 1] == key) ? (int64_t)(lo - 1) : -1;
+}
+
+// Per input cube i (codes and depths in input order): its six neighbours as
+// its leaf's table has them (tgv.cpp:925-975), the flags of KArgs::sflag and
+// the bisection's parent (tgv.cpp:877-887), all stored at i's position.
+__global__ void k_stream_tables(const uint64_t* __restrict__ code, const int32_t* __restrict__ depth, uint32_t n,
+                                const int32_t* __restrict__ leaf_of, const uint32_t* __restrict__ lfirst,
+                                const uint32_t* __restrict__ llast, const uint32_t* __restrict__ inv, int32_t* nb0,
+                                int32_t* nb1, int32_t* nb2, int32_t* nb3, int32_t* nb4, int32_t* nb5,
+                                uint8_t* __restrict__ flag, int32_t* __restrict__ par) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = inv[i];
+    int32_t* out[6] = {nb0, nb1, nb2, nb3, nb4, nb5};
+    const int l = leaf_of[i];
+    if (l < 0) {
+        for (int f = 0; f < 6; ++f) out[f][g] = -1;
+        flag[g] = 0;
+        par[g] = -1;
+        return;
+    }
+    const uint32_t f0 = lfirst[l], f1 = llast[l];
+    const uint64_t c = code[i];
+    const int d = depth[i];
+    uint32_t fl = 64u;
+    for (int f = 0; f < 6; ++f) {
+        const uint64_t k = face_nb(c, f, d);
+        int64_t j = -1;
+        if (k != kNone) {
+            j = leaf_code(code, f0, f1, k);
+            if (j < 0) j = bisect_code(code, n, k);
+            if (j < 0 && d > 0) {
+                const uint64_t pk = ancestor(k, d - 1);
+                j = leaf_code(code, f0, f1, pk);
+                if (j < 0) j = bisect_code(code, n, pk);
+            }
+        }
+        out[f][g] = j >= 0 ? (int32_t)inv[j] : -1;
+        if (j >= 0) {
+            const bool in = (uint64_t)j >= f0 && (uint64_t)j <= f1;
+            if ((f & 1) == 0 && in && depth[j] == d) fl |= 1u << f;
+            if ((f & 1) == 1 && !in) fl |= 1u << f;
+        }
+    }
+    flag[g] = (uint8_t)fl;
+    const int64_t pj = d > 0 ? bisect_code(code, n, ancestor(c, d - 1)) : -1;
+    par[g] = pj >= 0 ? (int32_t)inv[pj] : -1;
+}
+
+__global__ void k_stream_inputs(const uint64_t* __restrict__ pcode, const int32_t* __restrict__ pdepth,
+                                const uint32_t* __restrict__ inv, uint32_t n, uint64_t* __restrict__ code,
+                                int32_t* __restrict__ depth) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    code[i] = pcode[inv[i]];
+    depth[i] = pdepth[inv[i]];
+}
+
+// Parent copy of one leaf's part of a level (tgv.cpp:895-911): read every
+// parent first, then write.
+template <class S>
+__global__ void k_stream_parent_read(KArgs<S> a, const int32_t* __restrict__ par, typename Io<S>::T* __restrict__ tmp) {
+    using IO = Io<S>;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    const int j = par[g];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) tmp[4 * t + c] = j >= 0 ? IO::ld(a.s[F_U + c], j) : IO::ld(a.s[F_U + c], g);
+}
+template <class S>
+__global__ void k_stream_parent_write(KArgs<S> a, const typename Io<S>::T* __restrict__ tmp) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::st(a.s[F_U + c], g, tmp[4 * t + c]);
+#pragma unroll
+    for (int f = F_PX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
+}
+// Level start (tgv.cpp:914, 1041-1056): every cube's bars and snapshot are
+// its (u, v).
+template <class S>
+__global__ void k_stream_snapshot(KArgs<S> a, uint32_t n) {
+    using IO = Io<S>;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const typename IO::T x = IO::ld(a.s[F_U + c], g);
+        IO::st(a.s[F_UB + c], g, x);
+        IO::st(const_cast<S*>(a.fz[c]), g, x);
+    }
 }
 
 // ---- initialisation
@@ -1066,8 +1210,8 @@ template <class S, bool X>
 void launch_iter_x(tgvx_ctx* c, KArgs<S>& a, int op_mode, int block, bool frozen) {
     if (block > 256) block = 256;  // the dual / primal kernels take at most 256
     const unsigned grid = (a.n + block - 1) / block;
-    if (frozen) k_dual<S, true, X><<<grid, block, 0, c->stream>>>(a);
-    else k_dual<S, false, X><<<grid, block, 0, c->stream>>>(a);
+    if (frozen) k_dual<S, 1, X><<<grid, block, 0, c->stream>>>(a);
+    else k_dual<S, 0, X><<<grid, block, 0, c->stream>>>(a);
     if (op_mode == TGVX_OP_ADJ) {
         const Set13<S> m = main_set(a);
         k_primal_adj<S, X><<<grid, block, 0, c->stream>>>(a, m, m, 0);
@@ -1123,7 +1267,7 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
     const unsigned grid = (a.n + wg - 1) / wg;
     Set13<S> cur = main_set(a), oth = alt_set<S>(c);
     const Set13<S> mainset = cur;
-    k_dual<S, false, X><<<grid, wg, 0, c->stream>>>(a);
+    k_dual<S, 0, X><<<grid, wg, 0, c->stream>>>(a);
     for (int i = 1; i < iters; ++i) {
         launch_fused_pass_x<S, X>(c, a, cur, oth, L);
         std::swap(cur, oth);
@@ -1521,8 +1665,8 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     double sd = 0.0, sp = 0.0;
     for (int i = 0; i < k; ++i) {
         TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-        if (split.xcd) k_dual<S, false, true><<<grid, split.block, 0, c->stream>>>(a);
-        else k_dual<S, false, false><<<grid, split.block, 0, c->stream>>>(a);
+        if (split.xcd) k_dual<S, 0, true><<<grid, split.block, 0, c->stream>>>(a);
+        else k_dual<S, 0, false><<<grid, split.block, 0, c->stream>>>(a);
         TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
         if (p->op_mode == TGVX_OP_ADJ) {
             const Set13<S> m = main_set(a);
@@ -1620,6 +1764,119 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
     }
     if (rep) rep->n_levels = std::min(li, TGVX_MAX_LEVELS);
     release_graphs(c);  // the candidates' graphs
+}
+
+// tgv_minimize_streaming (tgv.cpp:802-1070) with every treetop leaf of a
+// level iterated in the same launch: within a level the leaves are
+// independent (border values come from the level-start snapshot, border
+// p, q are zero), so this equals the reference's leaf-by-leaf order. The
+// parent copy still runs leaf after leaf, as there.
+template <class S>
+void solve_stream_t(tgvx_ctx* c, const tgvx_params* p, const tgvx_treetop_leaf* leaves, uint64_t nleaves,
+                    tgvx_report* rep) {
+    using T = typename Io<S>::T;
+    Problem& P = c->prob;
+    const uint32_t n = (uint32_t)P.n;
+    tgvx_report r;
+    std::memset(&r, 0, sizeof(r));
+    const double t0 = now_ms();
+    // the leaf of every cube (first leaf wins) and the leaf ranges
+    std::vector<int32_t> leaf_of(n, -1);
+    std::vector<uint32_t> lf(std::max<uint64_t>(nleaves, 1), 1), ll(std::max<uint64_t>(nleaves, 1), 0);
+    for (uint64_t l = 0; l < nleaves; ++l) {
+        if (leaves[l].first_idx > leaves[l].last_idx || leaves[l].first_idx >= n) continue;
+        lf[l] = leaves[l].first_idx;
+        ll[l] = std::min<uint32_t>(leaves[l].last_idx, n - 1);
+        for (uint32_t i = lf[l]; i <= ll[l]; ++i)
+            if (leaf_of[i] < 0) leaf_of[i] = (int32_t)l;
+    }
+    auto up = [&](DevBuf& b, const void* src, size_t bytes) {
+        b.ensure(bytes);
+        TGVX_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    };
+    DevBuf d_inv, d_leaf, d_lf, d_ll, d_code, d_depth;
+    up(d_inv, P.inv.data(), 4ull * std::max<uint32_t>(n, 1));
+    up(d_leaf, leaf_of.data(), 4ull * std::max<uint32_t>(n, 1));
+    up(d_lf, lf.data(), 4 * lf.size());
+    up(d_ll, ll.data(), 4 * ll.size());
+    d_code.ensure(8ull * std::max<uint32_t>(n, 1));
+    d_depth.ensure(4ull * std::max<uint32_t>(n, 1));
+    for (int f = 0; f < 6; ++f) P.d_snb[f].ensure(4ull * std::max<uint32_t>(n, 1));
+    P.d_sflag.ensure(std::max<uint32_t>(n, 1));
+    P.d_spar.ensure(4ull * std::max<uint32_t>(n, 1));
+    const unsigned grid_all = (n + 255) / 256;
+    if (n) {
+        k_stream_inputs<<<grid_all, 256, 0, c->stream>>>(P.d_code.as<uint64_t>(), P.d_depth.as<int32_t>(),
+                                                         d_inv.as<uint32_t>(), n, d_code.as<uint64_t>(),
+                                                         d_depth.as<int32_t>());
+        k_stream_tables<<<grid_all, 256, 0, c->stream>>>(
+            d_code.as<uint64_t>(), d_depth.as<int32_t>(), n, d_leaf.as<int32_t>(), d_lf.as<uint32_t>(),
+            d_ll.as<uint32_t>(), d_inv.as<uint32_t>(), P.d_snb[0].as<int32_t>(), P.d_snb[1].as<int32_t>(),
+            P.d_snb[2].as<int32_t>(), P.d_snb[3].as<int32_t>(), P.d_snb[4].as<int32_t>(), P.d_snb[5].as<int32_t>(),
+            P.d_sflag.as<uint8_t>(), P.d_spar.as<int32_t>());
+    }
+    KArgs<S> a = make_args<S>(c, p);
+    for (int ax = 0; ax < 3; ++ax) {
+        a.nbp[ax] = P.d_snb[2 * ax].as<int32_t>();
+        a.nbm[ax] = P.d_snb[2 * ax + 1].as<int32_t>();
+    }
+    a.sflag = P.d_sflag.as<uint8_t>();
+    for (int i = 0; i < 4; ++i) a.fz[i] = P.d_save[i].as<S>();
+    // tgv.cpp:821-845: u = median, everything else 0
+    if (n) k_reset<S><<<grid_all, 256, 0, c->stream>>>(a, n);
+    uint32_t max_level = 1;
+    for (const Level& L : P.levels) max_level = std::max(max_level, L.n);
+    DevBuf tmp;
+    tmp.ensure(4 * sizeof(T) * (size_t)max_level);
+    double it_ms = 0.0;
+    for (size_t li = 0; li < P.levels.size(); ++li) {
+        const Level& L = P.levels[li];
+        if (li > 0) {
+            const uint32_t* pb = P.perm.data() + L.off;
+            for (uint64_t l = 0; l < nleaves; ++l) {
+                if (lf[l] > ll[l]) continue;
+                const uint32_t* lo = std::lower_bound(pb, pb + L.n, lf[l]);
+                const uint32_t* hi = std::upper_bound(pb, pb + L.n, ll[l]);
+                if (lo == hi) continue;
+                KArgs<S> b = a;
+                b.off = L.off + (uint32_t)(lo - pb);
+                b.n = (uint32_t)(hi - lo);
+                const unsigned gl = (b.n + 255) / 256;
+                k_stream_parent_read<S><<<gl, 256, 0, c->stream>>>(b, P.d_spar.as<int32_t>(), tmp.as<T>());
+                k_stream_parent_write<S><<<gl, 256, 0, c->stream>>>(b, tmp.as<T>());
+            }
+        }
+        k_stream_snapshot<S><<<grid_all, 256, 0, c->stream>>>(a, n);
+        a.off = a.lvl_off = L.off;
+        a.n = a.lvl_n = L.n;
+        const unsigned grid = (L.n + 255) / 256;
+        TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+        for (int it = 0; it < p->iters; ++it) {
+            k_dual<S, 2, true><<<grid, 256, 0, c->stream>>>(a);
+            k_primal_ref<S, true><<<grid, 256, 0, c->stream>>>(a);
+        }
+        TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+        TGVX_HIP(hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        it_ms += ms;
+        if (li < (size_t)TGVX_MAX_LEVELS) {
+            r.level_depth[li] = L.depth;
+            r.level_leaves[li] = L.n;
+            r.level_iters[li] = p->iters;
+        }
+        r.leaf_iters += (double)L.n * p->iters;
+    }
+    TGVX_HIP(hipGetLastError());
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    r.n_levels = (int)std::min<size_t>(P.levels.size(), TGVX_MAX_LEVELS);
+    r.solve_ms = now_ms() - t0;
+    r.iterate_ms = it_ms;
+    if (rep) {
+        const double setup = rep->setup_ms;
+        *rep = r;
+        rep->setup_ms = setup;
+    }
 }
 
 }  // namespace
@@ -1862,6 +2119,21 @@ void problem_solve_levels(tgvx_ctx* c, const tgvx_params* p, const int32_t* leve
         ~Clear() { P.level_iters.clear(); }
     } clear{P};
     problem_solve(c, p, nullptr, rep, nullptr, 0, nullptr);
+}
+
+void problem_solve_streaming(tgvx_ctx* c, const tgvx_params* p, const tgvx_treetop_leaf* leaves, uint64_t nleaves,
+                             tgvx_report* rep) {
+    Problem& P = c->prob;
+    if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
+    if (!p || (nleaves && !leaves)) throw Error(TGVX_E_ARG, "null pointer");
+    if (p->op_mode != TGVX_OP_REF) throw Error(TGVX_E_ARG, "the streaming solve runs the reference rules (op_mode 0)");
+    if (p->iters < 0) throw Error(TGVX_E_ARG, "negative iteration count");
+    if (P.nblocks > 1) throw Error(TGVX_E_ARG, "a problem of several blocks has no streaming solve");
+    switch (P.precision) {
+        case TGVX_FP64: solve_stream_t<double>(c, p, leaves, nleaves, rep); break;
+        case TGVX_FP32: solve_stream_t<float>(c, p, leaves, nleaves, rep); break;
+        default: solve_stream_t<bf16s>(c, p, leaves, nleaves, rep); break;
+    }
 }
 
 void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out) {

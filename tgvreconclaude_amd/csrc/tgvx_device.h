@@ -66,6 +66,7 @@ struct Io<float> {
     using T = float;
     __device__ static inline T ld(const float* p, long i) { return p[i]; }
     __device__ static inline void st(float* p, long i, T v) { p[i] = v; }
+    __device__ static inline void stn(float* p, long i, T v) { __builtin_nontemporal_store(v, p + i); }
     __device__ static inline T rt(T v) { return v; }  // value after a store / load round trip
 };
 template <>
@@ -73,6 +74,7 @@ struct Io<double> {
     using T = double;
     __device__ static inline T ld(const double* p, long i) { return p[i]; }
     __device__ static inline void st(double* p, long i, T v) { p[i] = v; }
+    __device__ static inline void stn(double* p, long i, T v) { __builtin_nontemporal_store(v, p + i); }
     __device__ static inline T rt(T v) { return v; }
 };
 template <>
@@ -88,6 +90,7 @@ struct Io<bf16s> {
         else h = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
         p[i].x = h;
     }
+    __device__ static inline void stn(bf16s* p, long i, T v) { st(p, i, v); }
     __device__ static inline T rt(T v) {
         bf16s b;
         st(&b, 0, v);

@@ -89,6 +89,7 @@ struct Problem {
     DevBuf d_state[17];              // u vx vy vz px py pz q0..q5 ub vbx vby vbz
     DevBuf d_save[4];                // scratch copy of u, v of one level
     DevBuf d_alt[13];                // second (u, v, p, q) set of the fused iteration
+    DevBuf d_snb[6], d_sflag, d_spar;  // streaming mode: per-leaf faces, flags, bisection parent
     DevBuf d_init;                   // float x 4 per position (warm start field), optional
     DevBuf d_orig;                   // uint32 position -> input index
     DevBuf d_part;                   // gap partial sums
@@ -132,6 +133,8 @@ void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx
 void problem_solve_levels(tgvx_ctx* c, const tgvx_params* p, const int32_t* level_iters, int32_t n_levels,
                           tgvx_report* rep);
 void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out);
+void problem_solve_streaming(tgvx_ctx* c, const tgvx_params* p, const tgvx_treetop_leaf* leaves, uint64_t nleaves,
+                             tgvx_report* rep);
 void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* P, double* D);
 void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
                  double* primal_ms, double* fused_ms);
