@@ -1,13 +1,18 @@
-// The reference's hot-path tests (tests/test_stage5.cpp, tests/test_stage6.cpp
-// of the reference), written against the same C++ interface, running on the
+// The reference's hot-path tests (tests/test_stage5.cpp, tests/test_stage6.cpp,
+// tests/test_stage7.cpp of the reference), written against the same C++ interface, running on the
 // GPU through libtgvx. Exit status 0 when every check passes.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <filesystem>
 #include <string>
 #include <vector>
 
+#include <fstream>
+
 #include "histogram.hpp"
+#include "marching_cubes.hpp"
+#include "octree_builder.hpp"
 #include "tgv.hpp"
 
 static int n_pass = 0, n_fail = 0;
@@ -176,6 +181,126 @@ static void out_of_core() {
     EXPECT(tgv_minimize({}, {}, Vec3f::Zero(), 1.0f).empty());
 }
 
+// the reference tests' own file writers (test_stage6.cpp:190-238)
+static std::string tmp_path(const char* name) {
+    return std::filesystem::temp_directory_path().string() + "/" + name;
+}
+static void write_cubes(const std::string& path, float r_root, const std::vector<OctreeCube>& cubes) {
+    std::ofstream out(path, std::ios::binary);
+    CubeFileHeader h;
+    h.r_root = r_root;
+    h.cube_count = (uint32_t)cubes.size();
+    out.write(reinterpret_cast<const char*>(&h), sizeof(h));
+    out.write(reinterpret_cast<const char*>(cubes.data()), cubes.size() * sizeof(OctreeCube));
+}
+static void write_treetop(const std::string& path, float r_root, const std::vector<TreetopLeaf>& leaves) {
+    struct {
+        char magic[4] = {'T', 'G', 'V', 'T'};
+        uint32_t version = 1;
+        float r_root = 0.0f;
+        uint32_t leaf_count = 0;
+    } h;
+    h.r_root = r_root;
+    h.leaf_count = (uint32_t)leaves.size();
+    std::ofstream out(path, std::ios::binary);
+    out.write(reinterpret_cast<const char*>(&h), sizeof(h));
+    out.write(reinterpret_cast<const char*>(leaves.data()), leaves.size() * sizeof(TreetopLeaf));
+}
+
+// test_stage6.cpp:240-344
+static void streaming() {
+    std::vector<OctreeCube> cubes;
+    std::vector<CubeHistogram> hists;
+    chain(cubes, hists, 32.0f);
+    const std::string cp = tmp_path("tgvx_s_cubes.bin"), hp = tmp_path("tgvx_s_hists.bin"),
+                      tp = tmp_path("tgvx_s_tt.bin"), sp = tmp_path("tgvx_s_state.bin");
+    write_cubes(cp, 32.0f, cubes);
+    save_histograms(hp, hists);
+    TGVParams p;
+    p.iters = 50;
+    p.alpha0 = 0.1f;
+    p.alpha1 = 0.2f;
+    // one leaf: as in memory
+    write_treetop(tp, 32.0f, {TreetopLeaf{cubes[0].code, 5, 0, 2}});
+    auto ref = tgv_minimize(cubes, hists, Vec3f::Zero(), 32.0f, p);
+    tgv_minimize_streaming(cp, hp, tp, Vec3f::Zero(), 32.0f, sp, p);
+    auto s1 = load_tgv(sp);
+    EXPECT(s1.size() == 3u);
+    for (size_t i = 0; i < s1.size() && i < 3; ++i) EXPECT_NEAR(ref[i].u, s1[i].u, 1e-4f);
+    // two leaves: polarity across the leaf border
+    write_treetop(tp, 32.0f, {TreetopLeaf{cubes[0].code, 5, 0, 1}, TreetopLeaf{cubes[2].code, 5, 2, 2}});
+    tgv_minimize_streaming(cp, hp, tp, Vec3f::Zero(), 32.0f, sp, p);
+    auto s2 = load_tgv(sp);
+    EXPECT(s2.size() == 3u);
+    if (s2.size() == 3u) EXPECT(s2[0].u > 0.0f && s2[2].u < 0.0f && s2[1].u > s2[2].u && s2[1].u < s2[0].u);
+    // the state file reads back finite
+    TGVParams q;
+    q.iters = 20;
+    write_treetop(tp, 32.0f, {TreetopLeaf{cubes[0].code, 5, 0, 2}});
+    tgv_minimize_streaming(cp, hp, tp, Vec3f::Zero(), 32.0f, sp, q);
+    for (const auto& c : load_tgv(sp)) EXPECT(std::isfinite(c.u));
+    float r = 0.0f;
+    EXPECT(load_cubes(cp, r).size() == 3u && r == 32.0f);
+    for (const auto& f : {cp, hp, tp, sp}) std::filesystem::remove(f);
+}
+
+// histogram.hpp:55-61: leaf by leaf equals all at once
+static void histograms_out_of_core() {
+    std::vector<OctreeCube> cubes;
+    for (uint32_t ix = 0; ix < 8; ++ix) {
+        OctreeCube c;
+        c.depth = 3;
+        c.r_c = 0.1f;
+        c.code = morton_encode(ix << 18, 3u << 18, 4u << 18);
+        cubes.push_back(c);
+    }
+    DepthMipmap dm;
+    dm.build(plane(64, 64, 1.0f, 32.0f, 32.0f));
+    const std::string cp = tmp_path("tgvx_h_cubes.bin"), hp = tmp_path("tgvx_h_hists.bin");
+    write_cubes(cp, 1.0f, cubes);
+    const Vec3f o(-1.0f, -1.0f, -0.5f);
+    compute_histograms_oc(cp, {TreetopLeaf{cubes[0].code, 1, 0, 4}, TreetopLeaf{cubes[5].code, 1, 5, 7}}, {dm}, o,
+                          1.0f, hp);
+    auto a = load_histograms(hp);
+    auto b = compute_histograms(cubes, {dm}, o, 1.0f);
+    EXPECT(a.size() == b.size());
+    for (size_t i = 0; i < a.size() && i < b.size(); ++i)
+        for (int k = 0; k < 8; ++k) EXPECT(a[i].bins[k] == b[i].bins[k]);
+    for (const auto& f : {cp, hp}) std::filesystem::remove(f);
+}
+
+// test_stage7.cpp:245-273: a negative cube inside positive ones is a closed
+// box of 12 triangles
+static void closed_box() {
+    std::vector<OctreeCube> cubes;
+    std::vector<TGVCube> st;
+    for (uint32_t iz = 0; iz < 3; ++iz)
+        for (uint32_t iy = 0; iy < 3; ++iy)
+            for (uint32_t ix = 0; ix < 3; ++ix) {
+                OctreeCube c;
+                c.depth = 3;
+                c.r_c = 3.0f;
+                c.code = morton_encode(ix << 18, iy << 18, iz << 18);
+                cubes.push_back(c);
+                TGVCube t;
+                t.u = (ix == 1 && iy == 1 && iz == 1) ? -1.0f : 1.0f;
+                st.push_back(t);
+            }
+    // the reference's extraction wants code order
+    std::vector<size_t> order(cubes.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cubes[a].code < cubes[b].code; });
+    std::vector<OctreeCube> cs;
+    std::vector<TGVCube> ss;
+    for (size_t i : order) {
+        cs.push_back(cubes[i]);
+        ss.push_back(st[i]);
+    }
+    Mesh m = extract_surface(cs, ss, Vec3f::Zero(), 24.0f);
+    EXPECT(m.tri_count() == 12u);
+    EXPECT(m.verts.size() == 8u);
+}
+
 int main() {
     try {
         mip_levels_of_constant_map();
@@ -186,6 +311,9 @@ int main() {
         constant_field();
         state_file_roundtrip();
         out_of_core();
+        streaming();
+        histograms_out_of_core();
+        closed_box();
     } catch (const std::exception& e) {
         std::printf("exception: %s\n", e.what());
         return 2;

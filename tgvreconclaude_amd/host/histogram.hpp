@@ -32,5 +32,12 @@ std::vector<CubeHistogram> compute_histograms(const std::vector<OctreeCube>& cub
                                               const std::vector<DepthMipmap>& depth_maps,
                                               const Vec3f& scene_origin, float r_root);
 
+// histogram.hpp:55-61: histograms of the cubes of each treetop leaf of the
+// cube file, written leaf after leaf to a 'TGVH' file
+void compute_histograms_oc(const std::string& balanced_cube_path, const std::vector<TreetopLeaf>& leaves,
+                           const std::vector<DepthMipmap>& depth_maps, const Vec3f& scene_origin, float r_root,
+                           const std::string& out_hist_path);
+
 void save_histograms(const std::string& path, const std::vector<CubeHistogram>& hists);
 std::vector<CubeHistogram> load_histograms(const std::string& path);
+std::vector<CubeHistogram> load_histograms_range(const std::string& path, uint32_t first_idx, uint32_t last_idx);

@@ -30,3 +30,11 @@ std::vector<TGVCube> tgv_minimize(const std::vector<OctreeCube>& cubes, const st
 std::vector<TGVCube> tgv_minimize_oc(const std::vector<OctreeCube>& cubes, const std::vector<CubeHistogram>& hists,
                                      const std::vector<TreetopLeaf>& leaves, const Vec3f& origin, float r_root,
                                      const TGVParams& params = TGVParams{});
+
+// tgv.hpp:57-64: inputs and result in files ('TGV1' cubes, 'TGVH'
+// histograms, 'TGVT' treetop; result 'TGVS'). The streaming semantics of the
+// reference (per-leaf lookups, level-start snapshots, zero p, q across leaf
+// borders) are kept; the whole problem is solved resident on the GPU.
+void tgv_minimize_streaming(const std::string& balanced_cube_path, const std::string& hist_path,
+                            const std::string& treetop_path, const Vec3f& origin, float r_root,
+                            const std::string& out_state_path, const TGVParams& params = TGVParams{});

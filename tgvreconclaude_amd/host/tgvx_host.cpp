@@ -12,6 +12,8 @@
 #include <stdexcept>
 #include <string>
 
+#include "marching_cubes.hpp"
+#include "octree_builder.hpp"
 #include "tgv.hpp"
 #include "tgvx.h"
 
@@ -246,4 +248,123 @@ std::vector<TGVCube> tgv_minimize_oc(const std::vector<OctreeCube>& cubes, const
                         reinterpret_cast<const tgvx_treetop_leaf*>(leaves.data()), leaves.size(), &p, state.data(),
                         nullptr));
     return state;
+}
+
+// ---- file readers of the cube and treetop files (octree_builder.cpp:99-140,
+// treetop.cpp:93-108)
+std::vector<OctreeCube> load_cubes_range(const std::string& path, uint32_t first_idx, uint32_t last_idx,
+                                         float& r_root_out) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot open cube file: " + path);
+    CubeFileHeader h;
+    in.read(reinterpret_cast<char*>(&h), sizeof(h));
+    if (std::memcmp(h.magic, "TGV1", 4) != 0) throw std::runtime_error("Bad magic in cube file: " + path);
+    r_root_out = h.r_root;
+    if (first_idx >= h.cube_count || first_idx > last_idx) return {};
+    const uint32_t end = std::min<uint64_t>((uint64_t)last_idx + 1, h.cube_count);
+    std::vector<OctreeCube> v(end - first_idx);
+    in.seekg(sizeof(CubeFileHeader) + (std::streamoff)first_idx * sizeof(OctreeCube));
+    in.read(reinterpret_cast<char*>(v.data()), v.size() * sizeof(OctreeCube));
+    return v;
+}
+
+std::vector<OctreeCube> load_cubes(const std::string& path, float& r_root_out) {
+    return load_cubes_range(path, 0, std::numeric_limits<uint32_t>::max() - 1, r_root_out);
+}
+
+std::vector<TreetopLeaf> load_treetop(const std::string& path, float& r_root_out) {
+    struct Header {
+        char magic[4];
+        uint32_t version;
+        float r_root;
+        uint32_t leaf_count;
+    } h{};
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot open treetop: " + path);
+    in.read(reinterpret_cast<char*>(&h), sizeof(h));
+    if (std::memcmp(h.magic, "TGVT", 4) != 0) throw std::runtime_error("Bad treetop magic: " + path);
+    r_root_out = h.r_root;
+    std::vector<TreetopLeaf> v(h.leaf_count);
+    if (h.leaf_count) in.read(reinterpret_cast<char*>(v.data()), h.leaf_count * sizeof(TreetopLeaf));
+    return v;
+}
+
+std::vector<CubeHistogram> load_histograms_range(const std::string& path, uint32_t first_idx, uint32_t last_idx) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot open: " + path);
+    FileHeader h{};
+    in.read(reinterpret_cast<char*>(&h), sizeof(h));
+    if (std::memcmp(h.magic, "TGVH", 4) != 0) throw std::runtime_error("Bad histogram magic");
+    if (first_idx >= h.count || first_idx > last_idx) return {};
+    const uint32_t end = std::min<uint64_t>((uint64_t)last_idx + 1, h.count);
+    std::vector<CubeHistogram> v(end - first_idx);
+    in.seekg(sizeof(FileHeader) + (std::streamoff)first_idx * sizeof(CubeHistogram));
+    in.read(reinterpret_cast<char*>(v.data()), v.size() * sizeof(CubeHistogram));
+    return v;
+}
+
+// ---- histogram.cpp:222-256: leaf after leaf, appended in leaf order
+void compute_histograms_oc(const std::string& balanced_cube_path, const std::vector<TreetopLeaf>& leaves,
+                           const std::vector<DepthMipmap>& depth_maps, const Vec3f& scene_origin, float r_root,
+                           const std::string& out_hist_path) {
+    std::ofstream out(out_hist_path, std::ios::binary);
+    if (!out) throw std::runtime_error("Cannot open: " + out_hist_path);
+    FileHeader h{{'T', 'G', 'V', 'H'}, 1u, leaves.empty() ? 0u : leaves.back().last_idx + 1};
+    out.write(reinterpret_cast<const char*>(&h), sizeof(h));
+    float r_tmp = 0.0f;
+    for (const TreetopLeaf& leaf : leaves) {
+        const auto cubes = load_cubes_range(balanced_cube_path, leaf.first_idx, leaf.last_idx, r_tmp);
+        const auto hists = compute_histograms(cubes, depth_maps, scene_origin, r_root);
+        if (!hists.empty())
+            out.write(reinterpret_cast<const char*>(hists.data()), hists.size() * sizeof(CubeHistogram));
+    }
+}
+
+// ---- tgv.cpp:802-1070
+void tgv_minimize_streaming(const std::string& balanced_cube_path, const std::string& hist_path,
+                            const std::string& treetop_path, const Vec3f& /*origin*/, float /*r_root*/,
+                            const std::string& out_state_path, const TGVParams& params) {
+    float r_tt = 0.0f, r_c = 0.0f;
+    const auto leaves = load_treetop(treetop_path, r_tt);
+    const auto cubes = load_cubes(balanced_cube_path, r_c);
+    std::vector<TGVCube> state(cubes.size());
+    if (!cubes.empty()) {
+        const auto hists = load_histograms_range(hist_path, 0, (uint32_t)cubes.size() - 1);
+        if (hists.size() != cubes.size())
+            throw std::runtime_error("tgv_minimize_streaming: the histogram file holds fewer records than cubes");
+        const tgvx_params p = solve_params(params);
+        check(tgvx_solve_streaming(ctx(), reinterpret_cast<const tgvx_cube*>(cubes.data()), cubes.size(),
+                                   reinterpret_cast<const uint32_t*>(hists.data()), 8,
+                                   reinterpret_cast<const tgvx_treetop_leaf*>(leaves.data()), leaves.size(), &p,
+                                   state.data(), nullptr));
+    }
+    save_tgv(out_state_path, state);
+}
+
+// ---- marching_cubes.cpp:63-181 on the GPU
+Mesh extract_surface(const std::vector<OctreeCube>& cubes, const std::vector<TGVCube>& state, const Vec3f& origin,
+                     float r_root, const std::vector<bool>& has_data) {
+    Mesh m;
+    if (cubes.empty()) return m;
+    if (state.size() != cubes.size()) throw std::runtime_error("extract_surface: one state per cube expected");
+    std::vector<uint32_t> nobins(cubes.size() * 8, 0u);
+    check(tgvx_problem_load(ctx(), reinterpret_cast<const tgvx_cube*>(cubes.data()), cubes.size(), nobins.data(), 8,
+                            TGVX_FP32, TGVX_OP_REF));
+    std::vector<float> u(cubes.size());
+    for (size_t i = 0; i < cubes.size(); ++i) u[i] = state[i].u;
+    std::vector<uint8_t> hd;
+    if (!has_data.empty()) {
+        if (has_data.size() != cubes.size()) throw std::runtime_error("extract_surface: has_data size mismatch");
+        hd.resize(cubes.size());
+        for (size_t i = 0; i < cubes.size(); ++i) hd[i] = has_data[i] ? 1 : 0;
+    }
+    const float o[3] = {origin.x(), origin.y(), origin.z()};
+    uint64_t nv = 0, nt = 0;
+    check(tgvx_surface_extract(ctx(), u.data(), hd.empty() ? nullptr : hd.data(), o, r_root, &nv, &nt));
+    std::vector<float> v(3 * nv);
+    m.faces.resize(3 * nt);
+    check(tgvx_surface_copy(ctx(), v.data(), m.faces.data()));
+    m.verts.resize(nv);
+    for (uint64_t i = 0; i < nv; ++i) m.verts[i] = Vec3f(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+    return m;
 }

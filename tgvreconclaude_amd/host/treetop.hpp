@@ -11,3 +11,8 @@ struct TreetopLeaf {
     uint32_t last_idx;
 };
 static constexpr uint32_t N_CUBES_PER_TREETOP_LEAF = 1u << 24;
+
+// Treetop file ('TGVT', src/treetop.hpp:15-28): leaves of the balanced cube file.
+#include <string>
+#include <vector>
+std::vector<TreetopLeaf> load_treetop(const std::string& path, float& r_root_out);
