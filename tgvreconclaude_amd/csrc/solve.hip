@@ -59,6 +59,10 @@ struct KArgs {
     // -a faces (1, 3, 5) when that neighbour lies outside the leaf (its p, q
     // read as 0); bit 6: the cube belongs to a leaf. Null otherwise.
     const uint8_t* sflag;
+    // the previous level's u_bar, v_bar as records of 4 (positions
+    // [fz_off, fz_off + fz_n)), for the fused kernel's frozen reads; or null
+    const S* fzaos;
+    uint32_t fz_off, fz_n;
 };
 
 template <class S>
@@ -546,8 +550,12 @@ __global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, S
                 src_slot[ax] = TILE + h;
             } else {
                 src_slot[ax] = -2;
+                if (a.fzaos && (uint32_t)j - a.fz_off < a.fz_n) {
+                    IO::ld4(a.fzaos, (long)((uint32_t)j - a.fz_off), frozen[ax]);
+                } else {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) frozen[ax][c] = IO::ld(a.s[F_UB + c], j);
+                    for (int c = 0; c < 4; ++c) frozen[ax][c] = IO::ld(a.s[F_UB + c], j);
+                }
             }
         }
     }
@@ -709,6 +717,16 @@ __global__ void k_stream_snapshot(KArgs<S> a, uint32_t n) {
         IO::st(a.s[F_UB + c], g, x);
         IO::st(const_cast<S*>(a.fz[c]), g, x);
     }
+}
+
+template <class S>
+__global__ void k_fz_aos(KArgs<S> a, S* __restrict__ rec) {
+    using IO = Io<S>;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.fz_n) return;
+    const long g = (long)a.fz_off + t;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::st(rec, 4l * t + c, IO::ld(a.s[F_UB + c], g));
 }
 
 // ---- initialisation
@@ -1148,6 +1166,21 @@ KArgs<S> make_args(tgvx_ctx* c, const tgvx_params* prm) {
     return a;
 }
 
+// Before level li iterates in the exact-adjoint mode: the previous level's
+// bars (the only frozen values it reads) as 4-value records.
+template <class S>
+void prepare_level(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, size_t li) {
+    Problem& P = c->prob;
+    a.fzaos = nullptr;
+    a.fz_off = a.fz_n = 0;
+    if (p->op_mode != TGVX_OP_ADJ || li == 0) return;
+    const Level& prev = P.levels[li - 1];
+    a.fz_off = prev.off;
+    a.fz_n = prev.n;
+    k_fz_aos<S><<<(prev.n + 255) / 256, 256, 0, c->stream>>>(a, P.d_fzaos.as<S>());
+    a.fzaos = P.d_fzaos.as<S>();
+}
+
 int block_of(const tgvx_params* p) {
     int b = p && p->block > 0 ? p->block : 256;
     if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
@@ -1439,6 +1472,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         a.lvl_n = L.n;
         const unsigned grid = (L.n + 255) / 256;
         int warm_kept = 0;
+        if (!leaves) prepare_level<S>(c, a, p, li);
         // the level's initialisation; with dual_eps the dual follows from (u, v)
         const bool rec = p->dual_eps > 0.0f && !leaves;
         auto recover = [&] {
@@ -1646,6 +1680,7 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
         if (l.depth == depth) L = &l;
     if (!L) throw Error(TGVX_E_ARG, "no cubes at depth " + std::to_string(depth));
     KArgs<S> a = level_args<S>(c, p, *L);
+    prepare_level<S>(c, a, p, (size_t)(L - P.levels.data()));
     const Launch cfg = launch_for(c, p, a.n, false);
     // whole iterations (graph or plain launches as configured)
     TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
@@ -1731,6 +1766,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
     int li = 0;
     for (const Level& L : P.levels) {
         KArgs<S> a = level_args<S>(c, p, L);
+        prepare_level<S>(c, a, p, (size_t)(&L - P.levels.data()));
         double best = 1e300, def = 0.0;
         Launch win = base;
         for (const Launch& cf : cand) {
@@ -2079,6 +2115,11 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
         TGVX_HIP(hipGetLastError());
     }
     for (int i = 0; i < 17; ++i) P.d_state[i].ensure(P.elem * std::max<uint64_t>(n, 1));
+    // the frozen-bar records hold one level; sized once per problem, as
+    // captured graphs keep the pointer
+    uint32_t max_level = 1;
+    for (const Level& L : P.levels) max_level = std::max(max_level, L.n);
+    P.d_fzaos.ensure(4 * P.elem * (size_t)max_level);
     for (int i = 0; i < 4; ++i) P.d_save[i].ensure(P.elem * std::max<uint64_t>(n, 1));
     // position -> input index (the frozen-border mode's in-leaf test reads it)
     up(P.d_orig, P.perm.data(), 4 * std::max<uint64_t>(n, 1));

@@ -67,6 +67,11 @@ struct Io<float> {
     __device__ static inline T ld(const float* p, long i) { return p[i]; }
     __device__ static inline void st(float* p, long i, T v) { p[i] = v; }
     __device__ static inline void stn(float* p, long i, T v) { __builtin_nontemporal_store(v, p + i); }
+    // record r of an array of 4-value records (16-byte aligned)
+    __device__ static inline void ld4(const float* p, long r, T o[4]) {
+        const float4 x = reinterpret_cast<const float4*>(p)[r];
+        o[0] = x.x, o[1] = x.y, o[2] = x.z, o[3] = x.w;
+    }
     __device__ static inline T rt(T v) { return v; }  // value after a store / load round trip
 };
 template <>
@@ -75,6 +80,10 @@ struct Io<double> {
     __device__ static inline T ld(const double* p, long i) { return p[i]; }
     __device__ static inline void st(double* p, long i, T v) { p[i] = v; }
     __device__ static inline void stn(double* p, long i, T v) { __builtin_nontemporal_store(v, p + i); }
+    __device__ static inline void ld4(const double* p, long r, T o[4]) {
+        const double2 a = reinterpret_cast<const double2*>(p)[2 * r], b = reinterpret_cast<const double2*>(p)[2 * r + 1];
+        o[0] = a.x, o[1] = a.y, o[2] = b.x, o[3] = b.y;
+    }
     __device__ static inline T rt(T v) { return v; }
 };
 template <>
@@ -91,6 +100,11 @@ struct Io<bf16s> {
         p[i].x = h;
     }
     __device__ static inline void stn(bf16s* p, long i, T v) { st(p, i, v); }
+    __device__ static inline void ld4(const bf16s* p, long r, T o[4]) {
+        const uint2 x = reinterpret_cast<const uint2*>(p)[r];
+        o[0] = __uint_as_float(x.x << 16), o[1] = __uint_as_float(x.x & 0xffff0000u);
+        o[2] = __uint_as_float(x.y << 16), o[3] = __uint_as_float(x.y & 0xffff0000u);
+    }
     __device__ static inline T rt(T v) {
         bf16s b;
         st(&b, 0, v);

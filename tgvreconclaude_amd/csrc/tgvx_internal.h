@@ -89,7 +89,8 @@ struct Problem {
     DevBuf d_state[17];              // u vx vy vz px py pz q0..q5 ub vbx vby vbz
     DevBuf d_save[4];                // scratch copy of u, v of one level
     DevBuf d_alt[13];                // second (u, v, p, q) set of the fused iteration
-    DevBuf d_snb[6], d_sflag, d_spar;  // streaming mode: per-leaf faces, flags, bisection parent
+    DevBuf d_snb[6], d_sflag, d_spar;
+    DevBuf d_fzaos;                  // the previous level's u_bar, v_bar as 4-value records  // streaming mode: per-leaf faces, flags, bisection parent
     DevBuf d_init;                   // float x 4 per position (warm start field), optional
     DevBuf d_orig;                   // uint32 position -> input index
     DevBuf d_part;                   // gap partial sums
