@@ -1408,6 +1408,21 @@ void eval_gap(tgvx_ctx* c, KArgs<S> a, const tgvx_params* p, double* Pout, doubl
     *Dout = pd[1];
 }
 
+// P, D of the level into dst (device), without waiting for them
+template <class S>
+void eval_gap_to(tgvx_ctx* c, KArgs<S> a, const tgvx_params* p, double* dst) {
+    Problem& P = c->prob;
+    a.off = a.lvl_off;
+    a.n = a.lvl_n;
+    const uint32_t blocks = (a.n + kGapBlock - 1) / kGapBlock;
+    P.d_part.ensure(sizeof(double) * 2 * std::max<uint32_t>(blocks, 1));
+    if (blocks > 0)
+        k_gap<S><<<blocks, kGapBlock, 0, c->stream>>>(a, (double)p->lambda, (double)p->alpha0, (double)p->alpha1,
+                                                      2.0, P.d_part.as<double>());
+    k_sum_pairs<<<1, kGapBlock, 0, c->stream>>>(P.d_part.as<double>(), blocks, dst);
+    TGVX_HIP(hipGetLastError());
+}
+
 double rel_gap(double P, double D) { return (P - D) / std::max(1.0, std::fabs(P)); }
 
 // Per-block relative gaps of level li of a problem holding several blocks.
@@ -1464,6 +1479,8 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
     }
     double it_ms = 0.0, gap_ms = 0.0;
     std::vector<S*> fz_store;
+    std::vector<int> deferred(std::min<size_t>(P.levels.size(), TGVX_MAX_LEVELS), 0);
+    P.d_lvlpd.ensure(sizeof(double) * 2 * TGVX_MAX_LEVELS);
     for (size_t li = 0; li < P.levels.size(); ++li) {
         const Level& L = P.levels[li];
         a.off = L.off;
@@ -1542,6 +1559,15 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             a.off = L.off;
             a.n = L.n;
             done = p->iters;
+        } else if (!to_gap && P.nblocks == 1 && li < (size_t)TGVX_MAX_LEVELS) {
+            // fixed iterations: nothing waits for the level; its time and
+            // P, D are read after the last level
+            done = P.level_iters.empty() ? p->iters : P.level_iters[li];
+            TGVX_HIP(hipEventRecord(c->ev[4 + 2 * li], c->stream));
+            run_iters<S>(c, a, p, done, false);
+            TGVX_HIP(hipEventRecord(c->ev[5 + 2 * li], c->stream));
+            eval_gap_to<S>(c, a, p, P.d_lvlpd.as<double>() + 2 * li);
+            deferred[li] = 1;
         } else if (!to_gap) {
             const int fixed = P.level_iters.empty() ? p->iters : P.level_iters[li];
             TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
@@ -1651,6 +1677,18 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         r.leaf_iters += work >= 0.0 ? work : (double)L.n * done;
     }
     TGVX_HIP(hipStreamSynchronize(c->stream));
+    if (std::find(deferred.begin(), deferred.end(), 1) != deferred.end()) {
+        std::vector<double> pd(2 * deferred.size());
+        TGVX_HIP(hipMemcpy(pd.data(), P.d_lvlpd.p, sizeof(double) * pd.size(), hipMemcpyDeviceToHost));
+        for (size_t li = 0; li < deferred.size(); ++li) {
+            if (!deferred[li]) continue;
+            r.level_energy[li] = pd[2 * li];
+            r.level_dual[li] = pd[2 * li + 1];
+            float ms = 0.f;
+            TGVX_HIP(hipEventElapsedTime(&ms, c->ev[4 + 2 * li], c->ev[5 + 2 * li]));
+            it_ms += ms;
+        }
+    }
     r.n_levels = (int)std::min<size_t>(P.levels.size(), TGVX_MAX_LEVELS);
     r.solve_ms = now_ms() - t_solve0;
     r.iterate_ms = it_ms;

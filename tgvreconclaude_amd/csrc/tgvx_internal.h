@@ -95,6 +95,7 @@ struct Problem {
     DevBuf d_orig;                   // uint32 position -> input index
     DevBuf d_part;                   // gap partial sums
     DevBuf d_pd;                     // final P, D
+    DevBuf d_lvlpd;                  // per level P, D of a fixed-iteration solve, read at its end
 };
 
 }  // namespace tgvx
@@ -110,7 +111,7 @@ struct TuneEntry {
 struct tgvx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[4 + 2 * TGVX_MAX_LEVELS] = {};  // 4 general, then a pair per level
     std::string err;
     tgvx::Problem prob;
     // scratch for histogram calls
