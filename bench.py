@@ -37,7 +37,7 @@ ELEM = {"fp32": 4, "bf16": 2, "fp64": 8}
 # figure is the work done, not the traffic.
 ALG_BYTES = {k: 34 * e + 8 + 24 for k, e in ELEM.items()}
 KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj",
-                3: "k_fused_tile (p, q staged in LDS)"}
+                3: "k_fused_tile (p, q staged in LDS)", 4: "k_fused_halo"}
 
 
 def log(msg):
@@ -57,11 +57,14 @@ def main():
     ap.add_argument("--iters", type=int, default=50, help="iterations per level in one step")
     ap.add_argument("--block", type=int, default=0, help="workgroup / tile size (0: the library default)")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3], default=1,
+    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4], default=1,
                     help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "
-                         "kernel; 3: LDS-tiled with the tile's p, q staged in LDS")
+                         "kernel; 3: LDS-tiled with the tile's p, q staged in LDS; 4: LDS-tiled with a "
+                         "tabulated halo")
     ap.add_argument("--no-fused", action="store_true", help="same as --fused 0")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start time-to-gap runs")
+    ap.add_argument("--no-octree", action="store_true",
+                    help="skip timing the octree build (depth maps -> balanced cubes) on the device and the host")
     ap.add_argument("--step-ratio", type=float, default=1024.0,
                     help="sigma / tau of the tuned to-gap runs (sigma * tau = 1/16 as the reference's 0.25 * 0.25)")
     ap.add_argument("--gap-tol", type=float, default=1e-4)
@@ -116,6 +119,33 @@ def main():
 
     ctx = T.Context(0)
     arch, cus = ctx.device_info()
+
+    # the input's stages 1-3 (per-pixel cubes, merge, 2:1 balance) on the
+    # device, against the host generator (OpenMP per-view cubes, serial merge
+    # and FIFO balance) on the same depth maps; outside the timed TGV steps
+    octree = None
+    if not args.no_octree:
+        t1 = time.time()
+        host_cubes = scene.build_octree(sc.depths, sc.cams, args.max_depth)
+        t_host = time.time() - t1
+        ctx.octree_build(sc.depths, sc.cams, sc.origin, sc.r_root, args.max_depth)  # first call allocates
+        t1 = time.time()
+        dev_cubes, orep = ctx.octree_build(sc.depths, sc.cams, sc.origin, sc.r_root, args.max_depth)
+        t_dev = time.time() - t1
+        same = len(dev_cubes) == len(host_cubes) and bool(np.array_equal(
+            np.ascontiguousarray(dev_cubes).view(np.uint8), np.ascontiguousarray(host_cubes).view(np.uint8)))
+        octree = {"gpu_ms": round(orep["total_ms"], 2), "gpu_wall_ms": round(t_dev * 1e3, 2),
+                  "host_ms": round(t_host * 1e3, 1),
+                  "host_threads": int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0)),
+                  "speedup": round(t_host / t_dev, 2), "identical_to_host": same, "pixels": orep["pixels"],
+                  "raw_cubes": orep["raw"], "merged": orep["merged"], "bridges": orep["bridges"],
+                  "balance_rounds": orep["rounds"],
+                  "stages_ms": {k: round(orep[k], 2) for k in ("upload_ms", "cubes_ms", "merge_ms", "balance_ms")}}
+        log(f"[bench] octree build: device {orep['total_ms']:.1f} ms (upload {orep['upload_ms']:.1f}, cubes "
+            f"{orep['cubes_ms']:.1f}, merge {orep['merge_ms']:.1f}, balance {orep['balance_ms']:.1f} in "
+            f"{orep['rounds']} rounds), host {t_host * 1e3:.0f} ms, identical {same}")
+        del host_cubes, dev_cubes
+
     t1 = time.time()
     hist = ctx.histograms(cubes, sc.views(), sc.origin, sc.r_root, 10)
     t_hist = time.time() - t1
@@ -233,7 +263,7 @@ def main():
     pj = os.path.join(ROOT, "profiles", "tgv_iteration_depth10_torus_box.json")
     if args.precision == "fp32" and os.path.exists(pj):
         name = KERNEL_NAMES[variant]
-        if variant in (1, 3):
+        if variant in (1, 3, 4):
             name += f", tile {tuning[-1]['block'] if tuning else (args.block or 256)}"
         rec = json.load(open(pj)).get(name)
         if rec and rec["leaves"] == n_f:
@@ -266,6 +296,7 @@ def main():
         "cpu_baseline": cpu,
         "autotune": tuning,
         "setup_s": {"scene": round(t_scene, 2), "histograms": round(t_hist, 3), "load": round(t_load, 3)},
+        "octree_build": octree,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -100,8 +100,9 @@ typedef struct {
                               primal kernel per iteration; 1 (default) one fused kernel per
                               iteration sharing a tile's primal values through LDS; 2 one fused
                               kernel recomputing every neighbour's primal values; 3 as 1 with the
-                              tile's p, q staged in LDS first. For 1 and 3, block is the tile
-                              (128, 256 = default, 512, 1024) */
+                              tile's p, q staged in LDS first; 4 as 1 with the out-of-tile
+                              neighbours tabulated once per level and tile. For 1, 3 and 4, block
+                              is the tile (128, 256 = default, 512, 1024) */
     int32_t tune;          /* 1: take each level's launch configuration from tgvx_autotune's table
                               (levels without an entry use fused / block as given) */
     float sigma, tau;      /* primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150).
@@ -208,6 +209,38 @@ int tgvx_surface_extract(tgvx_ctx* ctx, const float* u, const uint8_t* has_data,
                          float r_root, uint64_t* n_verts, uint64_t* n_tris);
 int tgvx_surface_copy(tgvx_ctx* ctx, float* verts, uint32_t* tris);
 
+/* ---- octree input build on the device (the reference's stages 1-3)
+ * tgvx_octree_build replaces build_cubes_from_depthmap (src/octree_builder.hpp:25-29,
+ * octree_builder.cpp:30-97) over every view, kway_merge (src/merge.hpp, merge.cpp:7-97)
+ * and balance_octree (src/balance.hpp:14, balance.cpp:60-150), in memory instead of
+ * through cube files; tgvx_octree_balance replaces balance_octree alone. Each pixel
+ * gives one cube (depth from its sample radius, max_depth instead of the reference's
+ * 21); equal (code, depth) cells merge with r_c averaged (a double sum in view, then
+ * pixel order); the balance adds the bridge cubes the reference's FIFO queue adds,
+ * seeded in input order, and the result is sorted by (code, depth) (input order among
+ * equal pairs). Cells are keyed by the full 63-bit code (balance.cpp:56-58 keeps 60
+ * bits). The cubes stay in the context until the next build; tgvx_octree_copy reads
+ * them (n_out records). */
+typedef struct {
+    const float* depths;    /* host, height x width row-major; valid = finite and > 0 */
+    int32_t width, height;
+    float fx, fy, cx, cy;   /* DepthMap::K (src/depth_map.hpp:9) */
+    float cam_to_world[12]; /* DepthMap::Rt (world <- camera, depth_map.hpp:10), top 3x4 rows */
+} tgvx_depth_view;
+typedef struct {
+    uint64_t pixels;   /* pixels scanned */
+    uint64_t raw;      /* one cube per valid pixel inside the root cube */
+    uint64_t merged;   /* distinct (code, depth) cells: the balance's seeds */
+    uint64_t bridges;  /* cubes the balance added */
+    int32_t rounds;    /* parallel balance rounds until the queue's result was reached */
+    double upload_ms, cubes_ms, merge_ms, balance_ms, total_ms;
+} tgvx_octree_report;
+int tgvx_octree_build(tgvx_ctx* ctx, const tgvx_depth_view* views, int32_t nviews, const float origin[3],
+                      float r_root, int32_t max_depth, uint64_t* n_out, tgvx_octree_report* report);
+int tgvx_octree_balance(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, float r_root, uint64_t* n_out,
+                        tgvx_octree_report* report);
+int tgvx_octree_copy(tgvx_ctx* ctx, tgvx_cube* out);
+
 /* ---- launch autotuner (no reference counterpart)
  * For every level of the loaded problem, times `iters` iterations of each
  * launch configuration -- kernel variant (dual + primal kernels, LDS-tiled
@@ -221,7 +254,8 @@ typedef struct {
     int32_t n_levels;
     int32_t level_depth[TGVX_MAX_LEVELS];
     uint32_t level_leaves[TGVX_MAX_LEVELS];
-    int32_t variant[TGVX_MAX_LEVELS]; /* 0 two kernels, 1 tiled fused, 2 untiled fused, 3 tiled + LDS p, q */
+    int32_t variant[TGVX_MAX_LEVELS]; /* 0 two kernels, 1 tiled fused, 2 untiled fused, 3 tiled + LDS p, q,
+                                         4 tiled + tabulated halo */
     int32_t block[TGVX_MAX_LEVELS];
     int32_t xcd[TGVX_MAX_LEVELS];
     double best_us[TGVX_MAX_LEVELS];    /* per iteration, chosen configuration */

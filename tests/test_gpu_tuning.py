@@ -19,7 +19,7 @@ def _raw(st):
 
 
 @pytest.mark.parametrize("fused,block", [(0, 64), (0, 128), (1, 128), (1, 512), (1, 1024), (2, 64), (2, 128),
-                                         (3, 128), (3, 256), (3, 1024)])
+                                         (3, 128), (3, 256), (3, 1024), (4, 128), (4, 256), (4, 512), (4, 1024)])
 def test_launch_shapes_identical(ctx, sphere, fused, block):
     sc, hist = sphere
     base = dict(alpha0=2.0, alpha1=1.0, iters=60, precision=FP32, op_mode=OP_ADJ)
@@ -36,11 +36,9 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
     levels = ctx.autotune(p, iters=10)
     assert [(l["depth"], l["leaves"]) for l in levels] == [tuple(x) for x in ctx.levels()]
     for l in levels:
-        assert l["variant"] in ((0, 1, 2, 3) if mode == OP_ADJ else (0,))
+        assert l["variant"] in ((0, 1, 2, 3, 4) if mode == OP_ADJ else (0,))
         assert l["block"] in (64, 128, 256, 512, 1024) and l["xcd"] in (0, 1)
-        if l["variant"] == 1:
-            assert l["block"] in (128, 256, 512, 1024)
-        elif l["variant"] == 3:
+        if l["variant"] in (1, 3, 4):
             assert l["block"] in (128, 256, 512, 1024)
         else:
             assert l["block"] <= 256

@@ -246,6 +246,38 @@ class Context:
         self._check(self._L.tgvx_surface_copy(self.h, N.ptr(verts), N.ptr(tris)))
         return verts, tris
 
+    def octree_build(self, depths, cams, origin, r_root, max_depth):
+        """Stages 1-3 on the device from depth maps [V, H, W] and cameras
+        (tgvs_camera records); returns (cubes sorted by (code, depth), report dict)."""
+        d = np.ascontiguousarray(depths, dtype=np.float32)
+        V, H, W = d.shape
+        views = (N.DepthView * max(1, V))()
+        for k in range(V):
+            v = views[k]
+            v.depths = d[k].ctypes.data
+            v.width, v.height = W, H
+            v.fx, v.fy, v.cx, v.cy = cams[k].fx, cams[k].fy, cams[k].cx, cams[k].cy
+            for i in range(12):
+                v.cam_to_world[i] = cams[k].cam_to_world[i]
+        o = np.ascontiguousarray(origin, dtype=np.float32)
+        n, rep = C.c_uint64(), N.OctreeReport()
+        self._check(self._L.tgvx_octree_build(self.h, views, V, N.ptr(o), float(r_root), int(max_depth),
+                                              C.byref(n), C.byref(rep)))
+        return self._octree_out(n.value), rep.as_dict()
+
+    def octree_balance(self, cubes, r_root):
+        """balance_octree on the device; returns (cubes, report dict)."""
+        c = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+        n, rep = C.c_uint64(), N.OctreeReport()
+        self._check(self._L.tgvx_octree_balance(self.h, N.ptr(c), len(c), float(r_root), C.byref(n),
+                                                C.byref(rep)))
+        return self._octree_out(n.value), rep.as_dict()
+
+    def _octree_out(self, n):
+        out = np.zeros(n, dtype=CUBE_DT)
+        self._check(self._L.tgvx_octree_copy(self.h, N.ptr(out)))
+        return out
+
     def run_levels(self, params, level_iters):
         """Solve the resident problem with a fixed iteration count per level."""
         it = np.ascontiguousarray(level_iters, dtype=np.int32)

@@ -81,6 +81,21 @@ class Camera(C.Structure):
                 ("cam_to_world", C.c_float * 12), ("world_to_cam", C.c_float * 12)]
 
 
+class DepthView(C.Structure):
+    _fields_ = [("depths", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32),
+                ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+                ("cam_to_world", C.c_float * 12)]
+
+
+class OctreeReport(C.Structure):
+    _fields_ = [("pixels", C.c_uint64), ("raw", C.c_uint64), ("merged", C.c_uint64), ("bridges", C.c_uint64),
+                ("rounds", C.c_int32), ("upload_ms", C.c_double), ("cubes_ms", C.c_double),
+                ("merge_ms", C.c_double), ("balance_ms", C.c_double), ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # exported symbols of each library, as declared in include/*.h
 TGVX_SYMBOLS = [
     "tgvx_default_params", "tgvx_create", "tgvx_destroy", "tgvx_last_error", "tgvx_device_info",
@@ -89,7 +104,8 @@ TGVX_SYMBOLS = [
     "tgvx_problem_levels", "tgvx_level_bench", "tgvx_mlp_set", "tgvx_mlp_get", "tgvx_mlp_forward",
     "tgvx_mlp_loss_grad", "tgvx_mlp_train", "tgvx_problem_features", "tgvx_problem_load_blocks",
     "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune", "tgvx_problem_solve_levels",
-    "tgvx_surface_extract", "tgvx_surface_copy", "tgvx_solve_streaming",
+    "tgvx_surface_extract", "tgvx_surface_copy", "tgvx_solve_streaming", "tgvx_octree_build",
+    "tgvx_octree_balance", "tgvx_octree_copy",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
@@ -147,6 +163,10 @@ def lib():
                                            C.POINTER(Report)]
         L.tgvx_problem_block_report.argtypes = [vp, vp, vp]
         L.tgvx_solve_batched.argtypes = [vp, vp, C.c_uint32, i32, C.POINTER(Params), vp, C.POINTER(Report)]
+        L.tgvx_octree_build.argtypes = [vp, C.POINTER(DepthView), i32, vp, C.c_float, i32, C.POINTER(u64),
+                                        C.POINTER(OctreeReport)]
+        L.tgvx_octree_balance.argtypes = [vp, vp, u64, C.c_float, C.POINTER(u64), C.POINTER(OctreeReport)]
+        L.tgvx_octree_copy.argtypes = [vp, vp]
         for name in TGVX_SYMBOLS:
             f = getattr(L, name)
             if f.restype is C.c_int:  # default

@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-I" + INCLUDE, "-I" + CSRC,
 ]
-HIP_SOURCES = ["api.hip", "hist.hip", "solve.hip", "mlp.hip", "surface.hip"]
+HIP_SOURCES = ["api.hip", "hist.hip", "solve.hip", "mlp.hip", "surface.hip", "octree.hip"]
 
 
 def _run(cmd):
@@ -54,7 +54,9 @@ def build_tgvx(jobs=8):
 
 def build_scene():
     out = os.path.join(PKG, "libtgvx_scene.so")
-    _run(["g++", "-std=c++17", "-O3", "-march=x86-64-v3", "-fPIC", "-shared", "-fopenmp",
+    # -ffp-contract=off: every float expression rounds as written, so the
+    # device octree build reproduces the generator's cubes bit for bit
+    _run(["g++", "-std=c++17", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-shared", "-fopenmp",
           "-I" + INCLUDE, "-o", out, os.path.join(CSRC, "scene.cpp")])
     return out
 

@@ -231,6 +231,20 @@ int tgvx_surface_copy(tgvx_ctx* c, float* verts, uint32_t* tris) {
     return guarded(c, [&] { tgvx::surface_copy(c, verts, tris); });
 }
 
+int tgvx_octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, const float origin[3],
+                      float r_root, int32_t max_depth, uint64_t* n_out, tgvx_octree_report* rep) {
+    return guarded(c, [&] { tgvx::octree_build(c, views, nviews, origin, r_root, max_depth, n_out, rep); });
+}
+
+int tgvx_octree_balance(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, float r_root, uint64_t* n_out,
+                        tgvx_octree_report* rep) {
+    return guarded(c, [&] { tgvx::octree_balance(c, cubes, n, r_root, n_out, rep); });
+}
+
+int tgvx_octree_copy(tgvx_ctx* c, tgvx_cube* out) {
+    return guarded(c, [&] { tgvx::octree_copy(c, out); });
+}
+
 int tgvx_solve_streaming(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
                          const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* p, void* state_out,
                          tgvx_report* rep) {

@@ -1,0 +1,666 @@
+// Octree input build on the device: one cube per depth-map pixel
+// (reference src/octree_builder.cpp:11-97), the merge of equal (code, depth)
+// cells (src/merge.cpp:57-92) and the 2:1 balance with bridge cubes
+// (src/balance.cpp:60-150).
+//
+// Stage 1, per pixel: DepthMap::unproject / sample_radius (src/depth_map.cpp:
+// 4-30) in float, the depth rule of cube_depth_for_radius, the fine-grid code.
+// Stage 2: the valid pixels, sorted by (code, depth, view, pixel) with two
+// stable radix sorts, and per run of equal (code, depth) the mean r_c, summed
+// in double in that order.
+// Stage 3: the balance is a FIFO queue: the seeds in input order, then every
+// bridge in the order it was inserted; cube k's check of face f sees the
+// cells inserted before it. The device runs it as rounds of independent
+// checks: a round evaluates every queued cube's six checks against the
+// bridges of the previous round, each bridge time-stamped with the (queue
+// position, face) of the check that inserted it, so a check at time t sees
+// exactly the bridges stamped before t. A bridge's stamp is the earliest check
+// that inserts it; its queue position is the seed count plus its rank by
+// stamp. The round's bridges become the next round's; the rounds stop when a
+// round reproduces its input. At that fixed point every check agrees with the
+// state its predecessors produce, which by induction over the queue order is
+// the sequential run's state; and each round fixes at least the earliest check
+// that was still wrong, so the rounds terminate (8-14 rounds on the scenes
+// measured).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include <rocprim/rocprim.hpp>
+
+#include "tgvx_device.h"
+#include "tgvx_internal.h"
+
+namespace tgvx {
+namespace {
+
+double ms_now() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------- stage 1
+struct PixView {
+    const float* d;
+    int W, H;
+    float fx, fy, cx, cy;
+    float M[12];
+};
+
+__device__ inline bool pix_valid(const PixView& v, int u, int w) {
+    if (u < 0 || w < 0 || u >= v.W || w >= v.H) return false;
+    const float x = v.d[(long)w * v.W + u];
+    return isfinite(x) && x > 0.0f;
+}
+// DepthMap::unproject (depth_map.cpp:4-11): camera point ((u - cx) z / fx,
+// (v - cy) z / fy, z), then Rt.
+__device__ inline void pix_at(const PixView& v, int u, int w, float out[3]) {
+    const float z = v.d[(long)w * v.W + u];
+    const float cx = ((float)u - v.cx) * z / v.fx, cy = ((float)w - v.cy) * z / v.fy;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) out[r] = ((v.M[r * 4] * cx + v.M[r * 4 + 1] * cy) + v.M[r * 4 + 2] * z) + v.M[r * 4 + 3];
+}
+// DepthMap::sample_radius (depth_map.cpp:13-30): half the mean distance to
+// the valid 4-neighbours, in the order +u, -u, +v, -v.
+__device__ inline float pix_radius(const PixView& v, int u, int w) {
+    if (!pix_valid(v, u, w)) return 0.0f;
+    float c[3];
+    pix_at(v, u, w, c);
+    const int du[4] = {1, -1, 0, 0}, dv[4] = {0, 0, 1, -1};
+    float sum = 0.0f;
+    int cnt = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int nu = u + du[i], nv = w + dv[i];
+        if (!pix_valid(v, nu, nv)) continue;
+        float p[3];
+        pix_at(v, nu, nv, p);
+        const float dx = p[0] - c[0], dy = p[1] - c[1], dz = p[2] - c[2];
+        sum += sqrtf(dx * dx + dy * dy + dz * dz);
+        ++cnt;
+    }
+    if (cnt == 0) return 0.0f;
+    return 0.5f * sum / (float)cnt;
+}
+// cube_depth_for_radius (octree_builder.cpp:11-28): the estimate rounds
+// log2(r_root / (1.125 r)) and is then moved to the one depth whose cell
+// size lies in [0.75 r, 1.5 r) (or a clamp); that depth does not depend on
+// the last bit of the logarithm.
+__device__ inline int depth_for_radius(float r_x, float r_root, int max_depth) {
+    if (r_x <= 0.0f) return max_depth;
+    int d = (int)roundf(log2f(r_root / (r_x * 1.125f)));
+    if (d < 0) d = 0;
+    if (d > max_depth) d = max_depth;
+    for (int tries = 0; tries < 3; ++tries) {
+        const float cell = r_root / (float)(1u << d);
+        if (cell >= 0.75f * r_x && cell < 1.5f * r_x) break;
+        if (cell < 0.75f * r_x) {
+            if (d > 0) --d;
+            else break;
+        } else {
+            if (d < max_depth) ++d;
+            else break;
+        }
+    }
+    return d;
+}
+
+// One thread per pixel of one view: the pixel's cube (code, depth, r) at
+// slot base + pixel, flag 1 if it yields one.
+__global__ void k_pix_cubes(PixView v, long base, float o0, float o1, float o2, float inv_2r, float r_root,
+                            int max_depth, uint64_t* __restrict__ code, uint32_t* __restrict__ depth,
+                            float* __restrict__ rad, uint8_t* __restrict__ flag) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)v.W * v.H) return;
+    const int u = (int)(i % v.W), w = (int)(i / v.W);
+    uint8_t ok = 0;
+    if (pix_valid(v, u, w)) {
+        const float r_x = pix_radius(v, u, w);
+        if (r_x > 0.0f) {
+            float p[3];
+            pix_at(v, u, w, p);
+            const float n0 = (p[0] - o0) * inv_2r, n1 = (p[1] - o1) * inv_2r, n2 = (p[2] - o2) * inv_2r;
+            if (!(n0 < 0 || n0 >= 1 || n1 < 0 || n1 >= 1 || n2 < 0 || n2 >= 1)) {
+                const int d = depth_for_radius(r_x, r_root, max_depth);
+                const uint32_t g = 1u << d;
+                uint32_t ix = (uint32_t)(n0 * (float)g), iy = (uint32_t)(n1 * (float)g), iz = (uint32_t)(n2 * (float)g);
+                if (ix >= g) ix = g - 1;
+                if (iy >= g) iy = g - 1;
+                if (iz >= g) iz = g - 1;
+                const int sh = 21 - d;
+                code[base + i] = morton(ix << sh, iy << sh, iz << sh);
+                depth[base + i] = (uint32_t)d;
+                rad[base + i] = r_x;
+                ok = 1;
+            }
+        }
+    }
+    flag[base + i] = ok;
+}
+
+// ---------------------------------------------------------------- stage 2
+template <class T>
+__global__ void k_gather(const T* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n, T* __restrict__ dst) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+// 1 where a run of equal (code, depth) starts (code sorted, depth by idx)
+__global__ void k_run_heads(const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth,
+                            const uint32_t* __restrict__ idx, uint64_t n, uint8_t* __restrict__ head) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    head[i] = i == 0 || code[i] != code[i - 1] || depth[idx[i]] != depth[idx[i - 1]];
+}
+// merge.cpp:57-92 with the sum in double over the run, in sorted order
+__global__ void k_merge_runs(const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth,
+                             const float* __restrict__ rad, const uint32_t* __restrict__ idx,
+                             const uint32_t* __restrict__ start, uint64_t nruns, uint64_t nraw,
+                             uint64_t* __restrict__ mcode, uint32_t* __restrict__ mdepth, float* __restrict__ mrc) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nruns) return;
+    const uint64_t a = start[j], b = j + 1 < nruns ? start[j + 1] : nraw;
+    double s = 0.0;
+    for (uint64_t i = a; i < b; ++i) s += (double)rad[idx[i]];
+    mcode[j] = code[a];
+    mdepth[j] = depth[idx[a]];
+    mrc[j] = (float)(s / (double)(b - a));
+}
+
+// ---------------------------------------------------------------- stage 3
+// A cell (depth d, code c) as one nonzero 64-bit key: the code's bits below
+// the cell's size are zero, so the highest of them marks the depth (d < 21);
+// a depth-21 cell sets bit 63 instead (codes use bits 0-62).
+__host__ __device__ inline uint64_t cell_key(int d, uint64_t c) {
+    return d >= 21 ? (c | (1ull << 63)) : (c | (1ull << (3 * (21 - d) - 1)));
+}
+__host__ __device__ inline void cell_of(uint64_t k, int& d, uint64_t& c) {
+    if (k >> 63) {
+        d = 21;
+        c = k & ~(1ull << 63);
+        return;
+    }
+    const int b = __builtin_ctzll(k);
+    d = 21 - (b + 1) / 3;
+    c = k & ~(1ull << b);
+}
+__host__ __device__ inline uint64_t key_hash(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Open-addressing tables (linear probing, key 0 = empty): the seeds, and the
+// current bridges with their time stamps.
+struct Table {
+    uint64_t* key;
+    uint64_t* ts;  // null for the seed table
+    uint64_t mask;
+};
+__global__ void k_table_insert(Table t, const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth,
+                               const uint64_t* __restrict__ keys, const uint64_t* __restrict__ ts, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys ? keys[i] : cell_key((int)depth[i], code[i]);
+    for (uint64_t h = key_hash(k) & t.mask;; h = (h + 1) & t.mask) {
+        const unsigned long long prev = atomicCAS((unsigned long long*)&t.key[h], 0ull, (unsigned long long)k);
+        if (prev == 0ull || prev == k) {
+            if (t.ts) t.ts[h] = ts[i];  // keys are distinct in the bridge table
+            return;
+        }
+    }
+}
+__device__ inline bool table_find(const Table& t, uint64_t k, uint64_t* ts) {
+    for (uint64_t h = key_hash(k) & t.mask;; h = (h + 1) & t.mask) {
+        const uint64_t x = t.key[h];
+        if (x == k) {
+            if (ts) *ts = t.ts[h];
+            return true;
+        }
+        if (x == 0ull) return false;
+    }
+}
+// the cell exists at time tau: a seed, or a bridge stamped before tau
+__device__ inline bool has_at(const Table& s0, const Table& br, int d, uint64_t c, uint64_t tau) {
+    const uint64_t k = cell_key(d, c);
+    if (table_find(s0, k, nullptr)) return true;
+    uint64_t ts;
+    return br.mask != 0 && table_find(br, k, &ts) && ts < tau;
+}
+
+// One thread per queued cube k (seeds, then bridges by position): the six
+// checks of balance.cpp:93-123 at times 1 + 6k + f; an insertion is recorded
+// as (cell key, time).
+__global__ void k_balance_checks(const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth, uint64_t nseed,
+                                 const uint64_t* __restrict__ bkey, uint64_t nbr, Table s0, Table br,
+                                 uint64_t* __restrict__ dkey, uint64_t* __restrict__ dts,
+                                 unsigned long long* __restrict__ count, uint64_t cap) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseed + nbr) return;
+    int d;
+    uint64_t cv;
+    if (k < nseed) {
+        d = (int)depth[k];
+        cv = code[k];
+    } else {
+        cell_of(bkey[k - nseed], d, cv);
+    }
+    if (d < 2) return;
+    for (int f = 0; f < 6; ++f) {
+        const uint64_t tau = 1 + 6 * k + (uint64_t)f;
+        const uint64_t nb = face_nb(cv, f, d);
+        if (nb == kNone) continue;
+        if (has_at(s0, br, d, nb, tau)) continue;  // same-level neighbour
+        const uint64_t par = ancestor(nb, d - 1);
+        if (has_at(s0, br, d - 1, par, tau)) continue;  // one level coarser
+        bool viol = false;
+        for (int d2 = d - 2; d2 >= 0 && !viol; --d2) viol = has_at(s0, br, d2, ancestor(nb, d2), tau);
+        if (!viol) continue;
+        const unsigned long long slot = atomicAdd(count, 1ull);
+        if (slot < cap) {
+            dkey[slot] = cell_key(d - 1, par);
+            dts[slot] = tau;
+        }
+    }
+}
+// after sorting by (key, time): 1 at the first (earliest) record of a key
+__global__ void k_key_heads(const uint64_t* __restrict__ key, uint64_t n, uint8_t* __restrict__ head) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) head[i] = i == 0 || key[i] != key[i - 1];
+}
+__global__ void k_differs(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b, const uint64_t* __restrict__ c,
+                          const uint64_t* __restrict__ e, uint64_t n, uint32_t* __restrict__ flag) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (a[i] != b[i] || c[i] != e[i])) flag[0] = 1u;
+}
+// final cells: seeds (their r_c), then bridges (r_root / 2^d, balance.cpp:118-119)
+__global__ void k_cells(const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth,
+                        const float* __restrict__ rc, uint64_t nseed, const uint64_t* __restrict__ bkey, uint64_t nbr,
+                        float r_root, uint64_t* __restrict__ ocode, uint32_t* __restrict__ odepth,
+                        float* __restrict__ orc) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseed + nbr) return;
+    if (i < nseed) {
+        ocode[i] = code[i];
+        odepth[i] = depth[i];
+        orc[i] = rc[i];
+    } else {
+        int d;
+        uint64_t cv;
+        cell_of(bkey[i - nseed], d, cv);
+        ocode[i] = cv;
+        odepth[i] = (uint32_t)d;
+        orc[i] = r_root / (float)(1u << d);
+    }
+}
+__global__ void k_iota(uint32_t* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+__global__ void k_records(const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth,
+                          const float* __restrict__ rc, const uint32_t* __restrict__ idx, uint64_t n,
+                          tgvx_cube* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = idx[i];
+    tgvx_cube c;
+    c.hi = (uint32_t)(code[j] >> 32);
+    c.lo = (uint32_t)code[j];
+    c.r_c = rc[j];
+    c.depth = (int32_t)depth[j];
+    out[i] = c;
+}
+
+unsigned grid_of(uint64_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+uint64_t pow2_at_least(uint64_t x) {
+    uint64_t p = 1024;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// Scratch buffers of the octree calls, by role.
+enum {
+    B_CODE = 0, B_DEPTH, B_RAD, B_FLAG, B_IDX, B_IDX2, B_KEY32, B_KEY32S, B_CODES, B_HEAD, B_START, B_COUNT,
+    B_TMP, B_S0, B_BRK, B_BRT, B_DKEY, B_DTS, B_DKEY2, B_DTS2, B_NKEY, B_NTS, B_PKEY, B_PTS
+};
+
+// stable sort of (keys, idx) pairs in place through the alternate buffers
+template <class K>
+void sort_pairs(tgvx_ctx* c, DevBuf& keys, DevBuf& keys_alt, DevBuf& vals, DevBuf& vals_alt, uint64_t n,
+                int end_bit) {
+    if (n == 0) return;
+    DevBuf& tmp = c->oct_tmp[B_TMP];
+    size_t bytes = 0;
+    TGVX_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys.as<K>(), keys_alt.as<K>(), vals.as<uint32_t>(),
+                                       vals_alt.as<uint32_t>(), (size_t)n, 0, end_bit, c->stream));
+    tmp.ensure(bytes);
+    TGVX_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, keys.as<K>(), keys_alt.as<K>(), vals.as<uint32_t>(),
+                                       vals_alt.as<uint32_t>(), (size_t)n, 0, end_bit, c->stream));
+}
+// stable compaction of the positions i < n with flag[i] != 0 into out; returns the count
+uint64_t select_flagged(tgvx_ctx* c, const uint8_t* flag, uint64_t n, uint32_t* out) {
+    if (n == 0) return 0;
+    DevBuf& tmp = c->oct_tmp[B_TMP];
+    DevBuf& cnt = c->oct_tmp[B_COUNT];
+    cnt.ensure(16);
+    rocprim::counting_iterator<uint32_t> it(0u);
+    size_t bytes = 0;
+    TGVX_HIP(rocprim::select(nullptr, bytes, it, flag, out, cnt.as<uint64_t>(), (size_t)n, c->stream));
+    tmp.ensure(bytes);
+    TGVX_HIP(rocprim::select(tmp.p, bytes, it, flag, out, cnt.as<uint64_t>(), (size_t)n, c->stream));
+    uint64_t m = 0;
+    TGVX_HIP(hipMemcpyAsync(&m, cnt.p, sizeof(m), hipMemcpyDeviceToHost, c->stream));
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    return m;
+}
+
+// Balance of the nseed cells (code, depth, r_c on the device, queue order)
+// into c->oct_out, sorted by (code, depth), stable. Returns the bridge count.
+uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, uint64_t nseed, float r_root,
+                        int32_t* rounds) {
+    DevBuf* T = c->oct_tmp;
+    hipStream_t st = c->stream;
+    // seed table
+    const uint64_t cap0 = pow2_at_least(2 * std::max<uint64_t>(nseed, 1));
+    T[B_S0].ensure(8 * cap0);
+    TGVX_HIP(hipMemsetAsync(T[B_S0].p, 0, 8 * cap0, st));
+    Table s0{T[B_S0].as<uint64_t>(), nullptr, cap0 - 1};
+    if (nseed)
+        k_table_insert<<<grid_of(nseed), 256, 0, st>>>(s0, code.as<uint64_t>(), depth.as<uint32_t>(), nullptr, nullptr,
+                                                        nseed);
+    uint64_t nbr = 0;       // bridges of the previous round
+    uint64_t dcap = std::max<uint64_t>(4096, nseed / 2);
+    T[B_COUNT].ensure(16);
+    T[B_HEAD].ensure(16);
+    int r = 0;
+    for (;;) {
+        if (++r > 100000) throw Error(TGVX_E_HIP, "balance rounds did not settle");
+        Table br{T[B_BRK].as<uint64_t>(), T[B_BRT].as<uint64_t>(), 0};
+        if (nbr) {
+            const uint64_t capb = pow2_at_least(2 * nbr);
+            T[B_BRK].ensure(8 * capb);
+            T[B_BRT].ensure(8 * capb);
+            TGVX_HIP(hipMemsetAsync(T[B_BRK].p, 0, 8 * capb, st));
+            br = Table{T[B_BRK].as<uint64_t>(), T[B_BRT].as<uint64_t>(), capb - 1};
+            k_table_insert<<<grid_of(nbr), 256, 0, st>>>(br, nullptr, nullptr, T[B_PKEY].as<uint64_t>(),
+                                                          T[B_PTS].as<uint64_t>(), nbr);
+        }
+        // this round's checks
+        uint64_t nd = 0;
+        for (;;) {
+            T[B_DKEY].ensure(8 * dcap);
+            T[B_DTS].ensure(8 * dcap);
+            TGVX_HIP(hipMemsetAsync(T[B_COUNT].p, 0, 8, st));
+            const uint64_t items = nseed + nbr;
+            if (items)
+                k_balance_checks<<<grid_of(items), 256, 0, st>>>(
+                    code.as<uint64_t>(), depth.as<uint32_t>(), nseed, T[B_PKEY].as<uint64_t>(), nbr, s0, br,
+                    T[B_DKEY].as<uint64_t>(), T[B_DTS].as<uint64_t>(), T[B_COUNT].as<unsigned long long>(), dcap);
+            TGVX_HIP(hipGetLastError());
+            TGVX_HIP(hipMemcpyAsync(&nd, T[B_COUNT].p, 8, hipMemcpyDeviceToHost, st));
+            TGVX_HIP(hipStreamSynchronize(st));
+            if (nd <= dcap) break;
+            dcap = nd + nd / 4;  // too many records for the buffer: again with room
+        }
+        // the new bridges: the earliest record per cell, ordered by its time
+        uint64_t nnew = 0;
+        if (nd) {
+            T[B_DKEY2].ensure(8 * nd);
+            T[B_DTS2].ensure(8 * nd);
+            DevBuf& tmp = T[B_TMP];
+            size_t bytes = 0;
+            // by time, then (stably) by key: each key's records in time order
+            TGVX_HIP(rocprim::radix_sort_pairs(nullptr, bytes, T[B_DTS].as<uint64_t>(), T[B_DTS2].as<uint64_t>(),
+                                               T[B_DKEY].as<uint64_t>(), T[B_DKEY2].as<uint64_t>(), (size_t)nd, 0,
+                                               64, st));
+            tmp.ensure(bytes);
+            TGVX_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, T[B_DTS].as<uint64_t>(), T[B_DTS2].as<uint64_t>(),
+                                               T[B_DKEY].as<uint64_t>(), T[B_DKEY2].as<uint64_t>(), (size_t)nd, 0,
+                                               64, st));
+            bytes = 0;
+            TGVX_HIP(rocprim::radix_sort_pairs(nullptr, bytes, T[B_DKEY2].as<uint64_t>(), T[B_DKEY].as<uint64_t>(),
+                                               T[B_DTS2].as<uint64_t>(), T[B_DTS].as<uint64_t>(), (size_t)nd, 0,
+                                               64, st));
+            tmp.ensure(bytes);
+            TGVX_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, T[B_DKEY2].as<uint64_t>(), T[B_DKEY].as<uint64_t>(),
+                                               T[B_DTS2].as<uint64_t>(), T[B_DTS].as<uint64_t>(), (size_t)nd, 0,
+                                               64, st));
+            T[B_HEAD].ensure(nd);
+            k_key_heads<<<grid_of(nd), 256, 0, st>>>(T[B_DKEY].as<uint64_t>(), nd, T[B_HEAD].as<uint8_t>());
+            T[B_IDX2].ensure(4 * nd);
+            nnew = select_flagged(c, T[B_HEAD].as<uint8_t>(), nd, T[B_IDX2].as<uint32_t>());
+            T[B_NKEY].ensure(8 * nnew);
+            T[B_NTS].ensure(8 * nnew);
+            T[B_DKEY2].ensure(8 * nnew);
+            T[B_DTS2].ensure(8 * nnew);
+            k_gather<uint64_t><<<grid_of(nnew), 256, 0, st>>>(T[B_DKEY].as<uint64_t>(), T[B_IDX2].as<uint32_t>(), nnew,
+                                                              T[B_DKEY2].as<uint64_t>());
+            k_gather<uint64_t><<<grid_of(nnew), 256, 0, st>>>(T[B_DTS].as<uint64_t>(), T[B_IDX2].as<uint32_t>(), nnew,
+                                                              T[B_DTS2].as<uint64_t>());
+            bytes = 0;
+            TGVX_HIP(rocprim::radix_sort_pairs(nullptr, bytes, T[B_DTS2].as<uint64_t>(), T[B_NTS].as<uint64_t>(),
+                                               T[B_DKEY2].as<uint64_t>(), T[B_NKEY].as<uint64_t>(), (size_t)nnew, 0,
+                                               64, st));
+            tmp.ensure(bytes);
+            TGVX_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, T[B_DTS2].as<uint64_t>(), T[B_NTS].as<uint64_t>(),
+                                               T[B_DKEY2].as<uint64_t>(), T[B_NKEY].as<uint64_t>(), (size_t)nnew, 0,
+                                               64, st));
+        }
+        // settled when the round reproduced the bridges it started from
+        bool same = nnew == nbr;
+        if (same && nnew) {
+            TGVX_HIP(hipMemsetAsync(T[B_COUNT].p, 0, 4, st));
+            k_differs<<<grid_of(nnew), 256, 0, st>>>(T[B_NKEY].as<uint64_t>(), T[B_PKEY].as<uint64_t>(),
+                                                     T[B_NTS].as<uint64_t>(), T[B_PTS].as<uint64_t>(), nnew,
+                                                     T[B_COUNT].as<uint32_t>());
+            uint32_t diff = 0;
+            TGVX_HIP(hipMemcpyAsync(&diff, T[B_COUNT].p, 4, hipMemcpyDeviceToHost, st));
+            TGVX_HIP(hipStreamSynchronize(st));
+            same = diff == 0;
+        }
+        if (same) break;
+        std::swap(T[B_NKEY].p, T[B_PKEY].p);
+        std::swap(T[B_NKEY].bytes, T[B_PKEY].bytes);
+        std::swap(T[B_NTS].p, T[B_PTS].p);
+        std::swap(T[B_NTS].bytes, T[B_PTS].bytes);
+        nbr = nnew;
+    }
+    if (rounds) *rounds = r;
+    // the cells, sorted by depth and then by code (both stable)
+    const uint64_t n = nseed + nbr;
+    T[B_CODES].ensure(8 * std::max<uint64_t>(n, 1));
+    T[B_KEY32].ensure(4 * std::max<uint64_t>(n, 1));
+    DevBuf cells_rc;
+    cells_rc.ensure(4 * std::max<uint64_t>(n, 1));
+    if (n)
+        k_cells<<<grid_of(n), 256, 0, st>>>(code.as<uint64_t>(), depth.as<uint32_t>(), rc.as<float>(), nseed,
+                                            T[B_PKEY].as<uint64_t>(), nbr, r_root, T[B_CODES].as<uint64_t>(),
+                                            T[B_KEY32].as<uint32_t>(), cells_rc.as<float>());
+    DevBuf dk, dk2, cc2, cc3, idx, idx2;
+    dk.ensure(4 * std::max<uint64_t>(n, 1));
+    dk2.ensure(4 * std::max<uint64_t>(n, 1));
+    cc2.ensure(8 * std::max<uint64_t>(n, 1));
+    idx.ensure(4 * std::max<uint64_t>(n, 1));
+    idx2.ensure(4 * std::max<uint64_t>(n, 1));
+    if (n) {
+        k_iota<<<grid_of(n), 256, 0, st>>>(idx.as<uint32_t>(), n);
+        TGVX_HIP(hipMemcpyAsync(dk.p, T[B_KEY32].p, 4 * n, hipMemcpyDeviceToDevice, st));
+        sort_pairs<uint32_t>(c, dk, dk2, idx, idx2, n, 5);  // idx2: by depth
+        k_gather<uint64_t><<<grid_of(n), 256, 0, st>>>(T[B_CODES].as<uint64_t>(), idx2.as<uint32_t>(), n,
+                                                       cc2.as<uint64_t>());
+        cc3.ensure(8 * n);
+        sort_pairs<uint64_t>(c, cc2, cc3, idx2, idx, n, 63);  // idx: by (code, depth)
+        c->oct_out.ensure(sizeof(tgvx_cube) * n);
+        k_records<<<grid_of(n), 256, 0, st>>>(T[B_CODES].as<uint64_t>(), T[B_KEY32].as<uint32_t>(),
+                                              cells_rc.as<float>(), idx.as<uint32_t>(), n,
+                                              c->oct_out.as<tgvx_cube>());
+        TGVX_HIP(hipGetLastError());
+    }
+    TGVX_HIP(hipStreamSynchronize(st));
+    c->oct_n = n;
+    return nbr;
+}
+
+}  // namespace
+
+void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, const float origin[3], float r_root,
+                  int32_t max_depth, uint64_t* n_out, tgvx_octree_report* rep) {
+    if (nviews < 0 || (nviews > 0 && !views) || !origin) throw Error(TGVX_E_ARG, "null pointer");
+    if (max_depth < 0 || max_depth > 21) throw Error(TGVX_E_ARG, "max_depth must be in 0..21");
+    if (!(r_root > 0.0f)) throw Error(TGVX_E_ARG, "r_root must be positive");
+    DevBuf* T = c->oct_tmp;
+    hipStream_t st = c->stream;
+    tgvx_octree_report r;
+    std::memset(&r, 0, sizeof(r));
+    const double t0 = ms_now();
+    uint64_t total = 0, maxpix = 0;
+    for (int k = 0; k < nviews; ++k) {
+        if (!views[k].depths || views[k].width <= 0 || views[k].height <= 0)
+            throw Error(TGVX_E_ARG, "view " + std::to_string(k) + ": no depth map");
+        const uint64_t p = (uint64_t)views[k].width * views[k].height;
+        total += p;
+        maxpix = std::max(maxpix, p);
+    }
+    if (total >= (1ull << 32)) throw Error(TGVX_E_ARG, "at most 2^32 - 1 pixels");
+    r.pixels = total;
+    // depth maps in, one cube per pixel
+    DevBuf dmaps;
+    dmaps.ensure(4 * std::max<uint64_t>(total, 1));
+    uint64_t off = 0;
+    for (int k = 0; k < nviews; ++k) {
+        const uint64_t p = (uint64_t)views[k].width * views[k].height;
+        TGVX_HIP(hipMemcpyAsync(dmaps.as<float>() + off, views[k].depths, 4 * p, hipMemcpyHostToDevice, st));
+        off += p;
+    }
+    TGVX_HIP(hipStreamSynchronize(st));
+    const double t1 = ms_now();
+    T[B_CODE].ensure(8 * std::max<uint64_t>(total, 1));
+    T[B_DEPTH].ensure(4 * std::max<uint64_t>(total, 1));
+    T[B_RAD].ensure(4 * std::max<uint64_t>(total, 1));
+    T[B_FLAG].ensure(std::max<uint64_t>(total, 1));
+    const float inv_2r = 1.0f / (2.0f * r_root);
+    off = 0;
+    for (int k = 0; k < nviews; ++k) {
+        PixView v;
+        v.d = dmaps.as<float>() + off;
+        v.W = views[k].width;
+        v.H = views[k].height;
+        v.fx = views[k].fx;
+        v.fy = views[k].fy;
+        v.cx = views[k].cx;
+        v.cy = views[k].cy;
+        std::memcpy(v.M, views[k].cam_to_world, sizeof(v.M));
+        const uint64_t p = (uint64_t)v.W * v.H;
+        k_pix_cubes<<<grid_of(p), 256, 0, st>>>(v, (long)off, origin[0], origin[1], origin[2], inv_2r, r_root,
+                                                max_depth, T[B_CODE].as<uint64_t>(), T[B_DEPTH].as<uint32_t>(),
+                                                T[B_RAD].as<float>(), T[B_FLAG].as<uint8_t>());
+        off += p;
+    }
+    TGVX_HIP(hipGetLastError());
+    TGVX_HIP(hipStreamSynchronize(st));  // the maps are read until here
+    dmaps.release();
+    T[B_IDX].ensure(4 * std::max<uint64_t>(total, 1));
+    const uint64_t nraw = select_flagged(c, T[B_FLAG].as<uint8_t>(), total, T[B_IDX].as<uint32_t>());
+    r.raw = nraw;
+    const double t2 = ms_now();
+    // sort by (code, depth, pixel): by depth, then stably by code
+    DevBuf dk, dk2, idx2, cs, cs2;
+    const uint64_t nr1 = std::max<uint64_t>(nraw, 1);
+    dk.ensure(4 * nr1);
+    dk2.ensure(4 * nr1);
+    idx2.ensure(4 * nr1);
+    cs.ensure(8 * nr1);
+    cs2.ensure(8 * nr1);
+    uint64_t nm = 0;
+    DevBuf mcode, mdepth, mrc;
+    if (nraw) {
+        k_gather<uint32_t><<<grid_of(nraw), 256, 0, st>>>(T[B_DEPTH].as<uint32_t>(), T[B_IDX].as<uint32_t>(), nraw,
+                                                          dk.as<uint32_t>());
+        sort_pairs<uint32_t>(c, dk, dk2, T[B_IDX], idx2, nraw, 5);
+        k_gather<uint64_t><<<grid_of(nraw), 256, 0, st>>>(T[B_CODE].as<uint64_t>(), idx2.as<uint32_t>(), nraw,
+                                                          cs.as<uint64_t>());
+        sort_pairs<uint64_t>(c, cs, cs2, idx2, T[B_IDX], nraw, 63);  // T[B_IDX]: (code, depth, pixel) order
+        T[B_HEAD].ensure(nraw);
+        k_run_heads<<<grid_of(nraw), 256, 0, st>>>(cs2.as<uint64_t>(), T[B_DEPTH].as<uint32_t>(),
+                                                   T[B_IDX].as<uint32_t>(), nraw, T[B_HEAD].as<uint8_t>());
+        T[B_START].ensure(4 * nraw);
+        nm = select_flagged(c, T[B_HEAD].as<uint8_t>(), nraw, T[B_START].as<uint32_t>());
+        mcode.ensure(8 * nm);
+        mdepth.ensure(4 * nm);
+        mrc.ensure(4 * nm);
+        k_merge_runs<<<grid_of(nm), 256, 0, st>>>(cs2.as<uint64_t>(), T[B_DEPTH].as<uint32_t>(), T[B_RAD].as<float>(),
+                                                  T[B_IDX].as<uint32_t>(), T[B_START].as<uint32_t>(), nm, nraw,
+                                                  mcode.as<uint64_t>(), mdepth.as<uint32_t>(), mrc.as<float>());
+        TGVX_HIP(hipGetLastError());
+    }
+    TGVX_HIP(hipStreamSynchronize(st));
+    dk.release();
+    dk2.release();
+    idx2.release();
+    cs.release();
+    cs2.release();
+    r.merged = nm;
+    const double t3 = ms_now();
+    mcode.ensure(8);
+    mdepth.ensure(4);
+    mrc.ensure(4);
+    r.bridges = balance_device(c, mcode, mdepth, mrc, nm, r_root, &r.rounds);
+    const double t4 = ms_now();
+    r.upload_ms = t1 - t0;
+    r.cubes_ms = t2 - t1;
+    r.merge_ms = t3 - t2;
+    r.balance_ms = t4 - t3;
+    r.total_ms = t4 - t0;
+    if (n_out) *n_out = c->oct_n;
+    if (rep) *rep = r;
+}
+
+void octree_balance(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, float r_root, uint64_t* n_out,
+                    tgvx_octree_report* rep) {
+    if (n && !cubes) throw Error(TGVX_E_ARG, "null pointer");
+    if (n >= (1ull << 32)) throw Error(TGVX_E_ARG, "at most 2^32 - 1 cubes");
+    for (uint64_t i = 0; i < n; ++i)
+        if (cubes[i].depth < 0 || cubes[i].depth > 21) throw Error(TGVX_E_ARG, "cube depth outside 0..21");
+    tgvx_octree_report r;
+    std::memset(&r, 0, sizeof(r));
+    const double t0 = ms_now();
+    std::vector<uint64_t> code(n);
+    std::vector<uint32_t> depth(n);
+    std::vector<float> rc(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        code[i] = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
+        depth[i] = (uint32_t)cubes[i].depth;
+        rc[i] = cubes[i].r_c;
+    }
+    DevBuf dcode, ddepth, drc;
+    dcode.ensure(8 * std::max<uint64_t>(n, 1));
+    ddepth.ensure(4 * std::max<uint64_t>(n, 1));
+    drc.ensure(4 * std::max<uint64_t>(n, 1));
+    if (n) {
+        TGVX_HIP(hipMemcpyAsync(dcode.p, code.data(), 8 * n, hipMemcpyHostToDevice, c->stream));
+        TGVX_HIP(hipMemcpyAsync(ddepth.p, depth.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+        TGVX_HIP(hipMemcpyAsync(drc.p, rc.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+    }
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    const double t1 = ms_now();
+    r.merged = n;
+    r.bridges = balance_device(c, dcode, ddepth, drc, n, r_root, &r.rounds);
+    const double t2 = ms_now();
+    r.upload_ms = t1 - t0;
+    r.balance_ms = t2 - t1;
+    r.total_ms = t2 - t0;
+    if (n_out) *n_out = c->oct_n;
+    if (rep) *rep = r;
+}
+
+void octree_copy(tgvx_ctx* c, tgvx_cube* out) {
+    if (c->oct_n && !out) throw Error(TGVX_E_ARG, "null pointer");
+    if (c->oct_n)
+        TGVX_HIP(hipMemcpy(out, c->oct_out.p, sizeof(tgvx_cube) * c->oct_n, hipMemcpyDeviceToHost));
+}
+
+}  // namespace tgvx

@@ -314,7 +314,10 @@ struct KeySet {
 };
 
 std::vector<tgvx_cube> balance(const std::vector<tgvx_cube>& in, float r_root) {
-    // balance.cpp:60-150, FIFO BFS seeded with the input cubes in order.
+    // balance.cpp:60-150, FIFO BFS seeded with the input cubes in order. Cells
+    // are keyed by the full 63-bit code: the reference's dc_key
+    // (balance.cpp:56-58) keeps code bits 0-59 only, which would merge cells
+    // of different root octants and clear bits 60-62 of the output codes.
     std::vector<KeySet> by_depth;
     by_depth.reserve(22);
     std::vector<size_t> per(22, 0);
@@ -349,7 +352,9 @@ std::vector<tgvx_cube> balance(const std::vector<tgvx_cube>& in, float r_root) {
             }
         }
     }
-    std::sort(out.begin(), out.end(), [](const tgvx_cube& a, const tgvx_cube& b) {
+    // by (code, depth); equal pairs (repeated input cells only: a bridge is
+    // never an existing cell) keep their input order
+    std::stable_sort(out.begin(), out.end(), [](const tgvx_cube& a, const tgvx_cube& b) {
         const uint64_t ka = code_of(a), kb = code_of(b);
         return ka != kb ? ka < kb : a.depth < b.depth;
     });

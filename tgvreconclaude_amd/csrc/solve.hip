@@ -598,6 +598,152 @@ __global__ void __launch_bounds__(TILE) k_fused_tile(KArgs<S> a, Set13<S> src, S
     for (int k = 0; k < 6; ++k) IO::stn(dst.f[F_Q0 + k], g, q[k]);
 }
 
+// ---- fused iteration with a precomputed halo (no reference counterpart).
+// The neighbour structure of a level never changes during its iterations, so
+// where each cube's +a neighbour's u_bar, v_bar come from, and which cubes of
+// the level outside a tile the tile needs, are tabulated once per launch range
+// and tile size. The kernel then computes the tile's own primal values and
+// the halo's in the same phase (no discovery pass, no atomics, one barrier).
+struct HaloTab {
+    const int32_t* slot[3];  // per position and axis: -1 the cube itself; 0..TILE-1 a
+                             // cube of the tile; TILE + k halo entry k; <= -2 the
+                             // cube -2 - v of another level (frozen u_bar, v_bar)
+    const int32_t* cnt;      // per tile: number of halo entries
+    const int32_t* list;     // per tile, 3 * TILE entries: halo positions, in order
+                             // of the referencing cube, then axis
+};
+
+// One workgroup per tile (plain order). nb = the dual step's +a neighbour
+// tables; the halo entries of a tile are numbered by an exclusive scan of the
+// per-cube counts, so the table does not depend on scheduling.
+template <int TILE>
+__global__ void __launch_bounds__(TILE) k_halo_table(const int32_t* __restrict__ nb0, const int32_t* __restrict__ nb1,
+                                                     const int32_t* __restrict__ nb2, uint32_t off, uint32_t n,
+                                                     uint32_t lvl_off, uint32_t lvl_n, int32_t* __restrict__ s0,
+                                                     int32_t* __restrict__ s1, int32_t* __restrict__ s2,
+                                                     int32_t* __restrict__ cnt, int32_t* __restrict__ list) {
+    __shared__ int32_t scan[2][TILE];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t t = tile * TILE + threadIdx.x;
+    const long tile_lo = (long)off + (long)tile * TILE;
+    const long tile_hi = (long)off + min(tile * TILE + (uint32_t)TILE, n);
+    int32_t j[3] = {-1, -1, -1};
+    int nh = 0;
+    if (t < n) {
+        j[0] = nb0[off + t];
+        j[1] = nb1[off + t];
+        j[2] = nb2[off + t];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const long v = j[ax];
+            if (v >= 0 && !(v >= tile_lo && v < tile_hi) && (uint32_t)v >= lvl_off && (uint32_t)v < lvl_off + lvl_n)
+                ++nh;
+        }
+    }
+    int cur = 0;
+    scan[0][threadIdx.x] = nh;
+    __syncthreads();
+    for (int d = 1; d < TILE; d <<= 1) {
+        const int v = scan[cur][threadIdx.x] + (threadIdx.x >= (uint32_t)d ? scan[cur][threadIdx.x - d] : 0);
+        scan[cur ^ 1][threadIdx.x] = v;
+        cur ^= 1;
+        __syncthreads();
+    }
+    const int incl = scan[cur][threadIdx.x];
+    if (threadIdx.x == TILE - 1) cnt[tile] = incl;
+    if (t >= n) return;
+    int k = incl - nh;
+    int32_t* const out[3] = {s0, s1, s2};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const long v = j[ax];
+        int32_t code;
+        if (v < 0) {
+            code = -1;
+        } else if (v >= tile_lo && v < tile_hi) {
+            code = (int32_t)(v - tile_lo);
+        } else if ((uint32_t)v >= lvl_off && (uint32_t)v < lvl_off + lvl_n) {
+            list[(long)tile * 3 * TILE + k] = (int32_t)v;
+            code = TILE + k;
+            ++k;
+        } else {
+            code = -2 - (int32_t)v;
+        }
+        out[ax][t] = code;
+    }
+}
+
+// Same values as k_fused_adj / k_fused_tile (identical expressions).
+template <class S, int TILE, bool XCD = true>
+__global__ void __launch_bounds__(TILE) k_fused_halo(KArgs<S> a, Set13<S> src, Set13<S> dst, HaloTab h) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    __shared__ T own_bar[4][TILE];
+    __shared__ T halo_bar[4][3 * TILE];
+    const uint32_t tile = (uint32_t)tile_id<XCD>();
+    const uint32_t t = tile * TILE + threadIdx.x;
+    const long g = (long)a.off + t;
+    const bool run = t < a.n && !masked(a, g);
+    T p[3], q[6], nw[4];
+    int32_t sl[3] = {-1, -1, -1};
+    T frozen[3][4];
+    if (run) {
+        T od[4];
+        primal_adj_at(a, src, g, nw, od, p, q);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) own_bar[c][threadIdx.x] = IO::rt(T(2) * nw[c] - od[c]);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            sl[ax] = __builtin_nontemporal_load(&h.slot[ax][t]);  // read once
+            if (sl[ax] <= -2) {
+                const long j = -2L - sl[ax];
+                if (a.fzaos && (uint32_t)j - a.fz_off < a.fz_n) {
+                    IO::ld4(a.fzaos, (long)((uint32_t)j - a.fz_off), frozen[ax]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) frozen[ax][c] = IO::ld(a.s[F_UB + c], j);
+                }
+            }
+        }
+    }
+    // the halo's primal values (a cube of a converged block is never read)
+    const int nh = h.cnt[tile];
+    const int32_t* hl = h.list + (long)tile * 3 * TILE;
+    for (int k = threadIdx.x; k < nh; k += TILE) {
+        const long j = hl[k];
+        if (masked(a, j)) continue;
+        T n2[4], o2[4];
+        primal_adj_at(a, src, j, n2, o2);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) halo_bar[c][k] = IO::rt(T(2) * n2[c] - o2[c]);
+    }
+    __syncthreads();
+    if (!run) return;
+    T bar[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bar[0][c] = own_bar[c][threadIdx.x];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int v = sl[ax];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            T x;
+            if (v == -1) x = bar[0][c];
+            else if (v <= -2) x = frozen[ax][c];
+            else if (v < TILE) x = own_bar[c][v];
+            else x = halo_bar[c][v - TILE];
+            bar[1 + ax][c] = x;
+        }
+    }
+    dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::stn(dst.f[F_U + c], g, nw[c]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) IO::stn(dst.f[F_PX + c], g, p[c]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) IO::stn(dst.f[F_Q0 + k], g, q[k]);
+}
+
 // ---- streaming mode (tgv_minimize_streaming, tgv.cpp:515-1070)
 // The cube file's bisection (CubeFileReader::lookup, tgv.cpp:674-683): the
 // first probe whose code matches, not necessarily the last cube with it.
@@ -1185,14 +1331,16 @@ int block_of(const tgvx_params* p) {
     int b = p && p->block > 0 ? p->block : 256;
     if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
         throw Error(TGVX_E_ARG, "block must be 64, 128, 256, 512 or 1024");
-    const bool tiled = p && p->op_mode == TGVX_OP_ADJ && (p->fused == 1 || p->fused == 3);
+    const bool tiled = p && p->op_mode == TGVX_OP_ADJ && (p->fused == 1 || p->fused == 3 || p->fused == 4);
     if (b > 256 && !tiled) throw Error(TGVX_E_ARG, "blocks above 256 exist only for the tiled fused kernel");
     return b;
 }
 
 // How one level's iterations are launched: variant 0 = dual + primal
-// kernels, 1 = LDS-tiled fused kernel, 2 = untiled fused kernel; block = the
-// workgroup size (the tile size for variant 1); xcd = XCD-aware tile order.
+// kernels, 1 = LDS-tiled fused kernel, 2 = untiled fused kernel, 3 = tiled
+// with the tile's p, q staged in LDS, 4 = tiled with a precomputed halo;
+// block = the workgroup size (the tile size for variants 1, 3, 4); xcd =
+// XCD-aware tile order.
 struct Launch {
     int variant, block, xcd;
 };
@@ -1208,11 +1356,11 @@ int size_bucket(uint32_t n) {
 
 Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool frozen) {
     const bool adj = p->op_mode == TGVX_OP_ADJ;
-    if (p->fused < 0 || p->fused > 3) throw Error(TGVX_E_ARG, "fused must be 0, 1, 2 or 3");
+    if (p->fused < 0 || p->fused > 4) throw Error(TGVX_E_ARG, "fused must be 0, 1, 2, 3 or 4");
     Launch L{adj && !frozen ? p->fused : 0, block_of(p), 1};
-    if (L.variant == 1 && p->block <= 0) L.block = kFuseTile;
-    if (L.variant == 3 && p->block <= 0) L.block = kFuseTile;
-    if ((L.variant == 1 || L.variant == 3) && L.block == 64)
+    const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4;
+    if (tiled && p->block <= 0) L.block = kFuseTile;
+    if (tiled && L.block == 64)
         throw Error(TGVX_E_ARG, "the tiled fused kernel takes block 128 or more");
     if (p->tune && !frozen) {
         auto it = c->tuned.find(std::make_tuple(c->prob.precision, p->op_mode, size_bucket(n)));
@@ -1258,10 +1406,62 @@ void launch_iter(tgvx_ctx* c, KArgs<S>& a, int op_mode, const Launch& L, bool fr
     else launch_iter_x<S, false>(c, a, op_mode, L.block, frozen);
 }
 
+// The halo table of the fused tiled kernel for a's launch range and a tile
+// size, built on first use. Callers build it before any graph capture starts.
+template <int TILE>
+void halo_table_launch(tgvx_ctx* c, const int32_t* const nb[3], uint32_t off, uint32_t n, uint32_t lvl_off,
+                       uint32_t lvl_n, HaloBufs& hb) {
+    const uint32_t ntiles = (n + TILE - 1) / TILE;
+    if (ntiles == 0) return;
+    k_halo_table<TILE><<<ntiles, TILE, 0, c->stream>>>(nb[0], nb[1], nb[2], off, n, lvl_off, lvl_n,
+                                                        hb.slot[0].as<int32_t>(), hb.slot[1].as<int32_t>(),
+                                                        hb.slot[2].as<int32_t>(), hb.cnt.as<int32_t>(),
+                                                        hb.list.as<int32_t>());
+}
+template <class S>
+HaloTab halo_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
+    Problem& P = c->prob;
+    const auto key = std::make_tuple(a.off, a.n, tile);
+    auto it = P.halo.find(key);
+    if (it == P.halo.end()) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        TGVX_HIP(hipStreamIsCapturing(c->stream, &cs));
+        if (cs != hipStreamCaptureStatusNone) throw Error(TGVX_E_STATE, "halo table built inside a graph capture");
+        auto hb = std::make_unique<HaloBufs>();
+        const uint32_t ntiles = (a.n + tile - 1) / tile;
+        for (int ax = 0; ax < 3; ++ax) hb->slot[ax].ensure(sizeof(int32_t) * std::max<uint32_t>(a.n, 1));
+        hb->cnt.ensure(sizeof(int32_t) * std::max<uint32_t>(ntiles, 1));
+        hb->list.ensure(sizeof(int32_t) * 3 * (size_t)tile * std::max<uint32_t>(ntiles, 1));
+        switch (tile) {
+            case 128: halo_table_launch<128>(c, a.nbp, a.off, a.n, a.lvl_off, a.lvl_n, *hb); break;
+            case 256: halo_table_launch<256>(c, a.nbp, a.off, a.n, a.lvl_off, a.lvl_n, *hb); break;
+            case 512: halo_table_launch<512>(c, a.nbp, a.off, a.n, a.lvl_off, a.lvl_n, *hb); break;
+            case 1024: halo_table_launch<1024>(c, a.nbp, a.off, a.n, a.lvl_off, a.lvl_n, *hb); break;
+            default: throw Error(TGVX_E_ARG, "the tiled fused kernel takes tiles of 128, 256, 512 or 1024");
+        }
+        TGVX_HIP(hipGetLastError());
+        it = P.halo.emplace(key, std::move(hb)).first;
+    }
+    const HaloBufs& hb = *it->second;
+    HaloTab h;
+    for (int ax = 0; ax < 3; ++ax) h.slot[ax] = static_cast<const int32_t*>(hb.slot[ax].p);
+    h.cnt = static_cast<const int32_t*>(hb.cnt.p);
+    h.list = static_cast<const int32_t*>(hb.list.p);
+    return h;
+}
+
 // one fused pass (primal n, dual n+1) from src into dst
 template <class S, bool X>
 void launch_fused_pass_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, const Launch& L) {
-    if (L.variant == 1 || L.variant == 3) {
+    if (L.variant == 4) {
+        const HaloTab h = halo_tab<S>(c, a, L.block);
+        switch (L.block) {
+            case 128: k_fused_halo<S, 128, X><<<(a.n + 127) / 128, 128, 0, c->stream>>>(a, src, dst, h); break;
+            case 256: k_fused_halo<S, 256, X><<<(a.n + 255) / 256, 256, 0, c->stream>>>(a, src, dst, h); break;
+            case 1024: k_fused_halo<S, 1024, X><<<(a.n + 1023) / 1024, 1024, 0, c->stream>>>(a, src, dst, h); break;
+            default: k_fused_halo<S, 512, X><<<(a.n + 511) / 512, 512, 0, c->stream>>>(a, src, dst, h); break;
+        }
+    } else if (L.variant == 1 || L.variant == 3) {
         const bool stage = L.variant == 3;
         switch (L.block) {
             case 128:
@@ -1343,6 +1543,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
     if (iters <= 0 || a.n == 0) return;
     const bool fused = L.variant != 0 && iters >= 2;
     if (fused) alt_set<S>(c);  // allocate outside any capture
+    if (fused && L.variant == 4) halo_tab<S>(c, a, L.block);
     if (p->use_graph && !frozen && iters > 1) {
         GraphKey key{a.off,
                      a.n,
@@ -1792,7 +1993,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
     for (int x = 1; x >= 0; --x) {
         for (int b : {256, 128, 64}) cand.push_back(Launch{0, b, x});
         if (p->op_mode == TGVX_OP_ADJ) {
-            for (int v : {1, 3})
+            for (int v : {1, 3, 4})
                 for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{v, b, x});
             for (int b : {256, 128}) cand.push_back(Launch{2, b, x});
         }
@@ -1965,6 +2166,7 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     g_graphs[c].clear();
     Problem& P = c->prob;
     P.loaded = false;
+    P.halo.clear();
     P.n = n;
     P.nb = nb;
     P.precision = precision;

@@ -5,6 +5,7 @@
 
 #include <cstdio>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -48,6 +49,13 @@ struct DevBuf {
     }
     template <class X>
     X* as() const { return static_cast<X*>(p); }
+};
+
+// Per launch range and tile size: the fused tiled kernel's precomputed
+// neighbour slots (three int32 per position) and per-tile lists of the
+// out-of-tile neighbours of the same level (3 * tile entries per tile).
+struct HaloBufs {
+    DevBuf slot[3], cnt, list;
 };
 
 struct Level {
@@ -96,6 +104,8 @@ struct Problem {
     DevBuf d_part;                   // gap partial sums
     DevBuf d_pd;                     // final P, D
     DevBuf d_lvlpd;                  // per level P, D of a fixed-iteration solve, read at its end
+    // halo tables of the fused tiled kernel, keyed by (first position, count, tile)
+    std::map<std::tuple<uint32_t, uint32_t, int>, std::unique_ptr<HaloBufs>> halo;
 };
 
 }  // namespace tgvx
@@ -122,6 +132,9 @@ struct tgvx_ctx {
     // last extracted surface: vertices (x, y, z) and triangles
     tgvx::DevBuf srf_verts, srf_tris, srf_tmp[8];
     uint64_t srf_nv = 0, srf_nt = 0;
+    // last octree build / balance: the cubes (tgvx_cube records) and scratch
+    tgvx::DevBuf oct_out, oct_tmp[24];
+    uint64_t oct_n = 0;
     // launch configurations chosen by tgvx_autotune
     std::map<std::tuple<int, int, int>, tgvx::TuneEntry> tuned;
 };
@@ -141,6 +154,12 @@ void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* P, double
 void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
                  double* primal_ms, double* fused_ms);
 void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* rep);
+// octree.hip
+void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, const float origin[3], float r_root,
+                  int32_t max_depth, uint64_t* n_out, tgvx_octree_report* rep);
+void octree_balance(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, float r_root, uint64_t* n_out,
+                    tgvx_octree_report* rep);
+void octree_copy(tgvx_ctx* c, tgvx_cube* out);
 // surface.hip
 void surface_extract(tgvx_ctx* c, const float* u, const uint8_t* has_data, const float origin[3], float r_root,
                      uint64_t* nv, uint64_t* nt);
