@@ -210,17 +210,21 @@ int tgvx_surface_extract(tgvx_ctx* ctx, const float* u, const uint8_t* has_data,
 int tgvx_surface_copy(tgvx_ctx* ctx, float* verts, uint32_t* tris);
 
 /* ---- octree input build on the device (the reference's stages 1-3)
- * tgvx_octree_build replaces build_cubes_from_depthmap (src/octree_builder.hpp:25-29,
- * octree_builder.cpp:30-97) over every view, kway_merge (src/merge.hpp, merge.cpp:7-97)
- * and balance_octree (src/balance.hpp:14, balance.cpp:60-150), in memory instead of
- * through cube files; tgvx_octree_balance replaces balance_octree alone. Each pixel
- * gives one cube (depth from its sample radius, max_depth instead of the reference's
- * 21); equal (code, depth) cells merge with r_c averaged (a double sum in view, then
- * pixel order); the balance adds the bridge cubes the reference's FIFO queue adds,
- * seeded in input order, and the result is sorted by (code, depth) (input order among
- * equal pairs). Cells are keyed by the full 63-bit code (balance.cpp:56-58 keeps 60
- * bits). The cubes stay in the context until the next build; tgvx_octree_copy reads
- * them (n_out records). */
+ * tgvx_octree_build runs, in memory instead of through cube files,
+ *   last_stage 1: build_cubes_from_depthmap (src/octree_builder.hpp:25-29,
+ *                 octree_builder.cpp:30-97) over every view, one list sorted by (code,
+ *                 depth), view then pixel order among equal pairs;
+ *   last_stage 2: + kway_merge (src/merge.hpp:7, merge.cpp:7-97): equal (code, depth)
+ *                 cells merged, r_c averaged (a double sum in that order);
+ *   last_stage 3: + balance_octree (src/balance.hpp:14, balance.cpp:60-150): the bridge
+ *                 cubes the reference's FIFO queue adds, seeded in (code, depth) order.
+ * Each pixel gives one cube, its depth from the sample radius (max_depth instead of the
+ * reference's 21). tgvx_octree_merge and tgvx_octree_balance run stage 2 / stage 3 alone
+ * on given cubes (kway_merge over the concatenated files; balance_octree seeded in input
+ * order, codes aligned to their depth). Results are sorted by (code, depth), input
+ * order among equal pairs. Cells are keyed by the full 63-bit code (balance.cpp:56-58
+ * keeps 60 bits). The cubes stay in the context until the next call; tgvx_octree_copy
+ * reads them (n_out records). */
 typedef struct {
     const float* depths;    /* host, height x width row-major; valid = finite and > 0 */
     int32_t width, height;
@@ -236,7 +240,10 @@ typedef struct {
     double upload_ms, cubes_ms, merge_ms, balance_ms, total_ms;
 } tgvx_octree_report;
 int tgvx_octree_build(tgvx_ctx* ctx, const tgvx_depth_view* views, int32_t nviews, const float origin[3],
-                      float r_root, int32_t max_depth, uint64_t* n_out, tgvx_octree_report* report);
+                      float r_root, int32_t max_depth, int32_t last_stage, uint64_t* n_out,
+                      tgvx_octree_report* report);
+int tgvx_octree_merge(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, uint64_t* n_out,
+                      tgvx_octree_report* report);
 int tgvx_octree_balance(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, float r_root, uint64_t* n_out,
                         tgvx_octree_report* report);
 int tgvx_octree_copy(tgvx_ctx* ctx, tgvx_cube* out);

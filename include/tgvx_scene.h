@@ -60,14 +60,19 @@ int32_t tgvs_render(const tgvs_prim* prims, int32_t nprims, const tgvs_camera* c
                     int32_t width, int32_t height, float sigma, float outlier_frac, float outlier_lo,
                     float outlier_hi, uint64_t seed, float* depths_out);
 
-/* Stages 1-3: per-pixel cubes (octree_builder.cpp:30-97, with max_depth as a
- * parameter), merge of equal (code, depth) cells averaging r_c (merge.cpp:57-92),
- * 2:1 balance with bridge cubes (balance.cpp:60-150). Output sorted by
- * (code, depth), malloc'ed into *out (release with tgvs_free). Returns the cube
- * count, or -1 on error. */
+/* Stages 1..last_stage: per-pixel cubes (octree_builder.cpp:30-97, with max_depth as a
+ * parameter; last_stage 1 returns them sorted by (code, depth), view then pixel order
+ * among equal pairs), merge of equal (code, depth) cells averaging r_c in double in that
+ * order (merge.cpp:57-92; 2), 2:1 balance with bridge cubes (balance.cpp:60-150; 3).
+ * Output sorted by (code, depth), malloc'ed into *out (release with tgvs_free). Returns
+ * the cube count, or -1 on error. */
 int64_t tgvs_build_octree(const float* depths, const tgvs_camera* cams, int32_t ncams, int32_t width,
                           int32_t height, const float origin[3], float r_root, int32_t max_depth,
-                          tgvx_cube** out);
+                          int32_t last_stage, tgvx_cube** out);
+
+/* Merge alone (merge.cpp:57-92) on an arbitrary cube list: sorted by (code, depth),
+ * input order among equal pairs, each run averaged; malloc'ed. */
+int64_t tgvs_merge(const tgvx_cube* cubes, uint64_t n, tgvx_cube** out);
 
 /* Balance alone (balance.cpp:60-150) on an arbitrary cube list; output sorted by
  * (code, depth), malloc'ed. */

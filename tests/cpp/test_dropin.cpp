@@ -10,8 +10,10 @@
 
 #include <fstream>
 
+#include "balance.hpp"
 #include "histogram.hpp"
 #include "marching_cubes.hpp"
+#include "merge.hpp"
 #include "octree_builder.hpp"
 #include "tgv.hpp"
 
@@ -301,8 +303,123 @@ static void closed_box() {
     EXPECT(m.verts.size() == 8u);
 }
 
+// ---- stages 1-3 (tests/test_stage1.cpp:84-117, tests/test_stages2_4.cpp:47-281)
+
+static void build_and_load() {
+    DepthMap dm = plane(10, 10, 5.0f, 50, 5);
+    const std::string path = tmp_path("build.cubes");
+    build_cubes_from_depthmap(dm, Vec3f{-1.0f, -1.0f, 4.0f}, 2.5f, path);
+    float rr = 0.0f;
+    const auto cubes = load_cubes(path, rr);
+    EXPECT_NEAR(rr, 2.5f, 1e-5f);
+    EXPECT(!cubes.empty());
+    bool sorted = true, rule = true;
+    for (size_t i = 1; i < cubes.size(); ++i) sorted = sorted && !(cubes[i].code < cubes[i - 1].code);
+    for (const auto& c : cubes) {
+        const float cell = 2.5f / (float)(1u << c.depth);
+        rule = rule && cell >= 0.75f * c.r_c && cell < 1.5f * c.r_c;
+        rule = rule && cube_depth_for_radius(c.r_c, 2.5f) == c.depth;
+    }
+    EXPECT(sorted);
+    EXPECT(rule);
+    std::filesystem::remove(path);
+}
+
+static void parent_and_neighbour_codes() {
+    const MortonCode code = morton_encode(5u << 11, 0, 0);  // ix 5 at depth 10
+    const MortonCode p9 = parent_code(code, 10);
+    uint32_t x, y, z;
+    morton_decode(p9, x, y, z);
+    EXPECT(x == (2u << 12) && y == 0 && z == 0);
+    EXPECT(!(code == p9));
+    morton_decode(parent_code(parent_code(p9, 9), 8), x, y, z);
+    EXPECT(x == 0 && y == 0 && z == 0);
+    const uint32_t step = 1u << 17;
+    const MortonCode c = morton_encode(step, step, step);
+    morton_decode(face_neighbor(c, 0, 4), x, y, z);
+    EXPECT(x == 2 * step && y == step && z == step);
+    morton_decode(face_neighbor(c, 1, 4), x, y, z);
+    EXPECT(x == 0);
+    EXPECT(face_neighbor(morton_encode(0, 0, 0), 1, 4).hi == UINT32_MAX);
+    EXPECT(face_neighbor(morton_encode(15 * step, 0, 0), 0, 4).hi == UINT32_MAX);
+}
+
+static void merge_two_planes() {
+    const std::string a = tmp_path("merge_a.cubes"), b = tmp_path("merge_b.cubes"), m = tmp_path("merged.cubes");
+    const Vec3f origin{-1.5f, -1.5f, 4.0f};
+    build_cubes_from_depthmap(plane(8, 8, 5.0f, 50, 4), origin, 3.0f, a);
+    build_cubes_from_depthmap(plane(8, 8, 6.0f, 50, 4), origin, 3.0f, b);
+    kway_merge({a, b}, m);
+    float rr = 0.0f, ra = 0.0f, rb = 0.0f;
+    const auto cubes = load_cubes(m, rr);
+    EXPECT(rr == 3.0f);
+    EXPECT(!cubes.empty());
+    bool sorted = true;
+    for (size_t i = 1; i < cubes.size(); ++i) sorted = sorted && !(cubes[i].code < cubes[i - 1].code);
+    EXPECT(sorted);
+    EXPECT(cubes.size() == load_cubes(a, ra).size() + load_cubes(b, rb).size());
+    for (const auto& p : {a, b, m}) std::filesystem::remove(p);
+}
+
+static void balance_cases() {
+    const std::string in = tmp_path("bal_in.cubes"), out = tmp_path("bal_out.cubes");
+    OctreeCube one;
+    one.code = morton_encode(0, 0, 0);
+    one.depth = 5;
+    one.r_c = 0.3f;
+    save_cubes(in, {one}, 10.0f);
+    balance_octree(in, out);
+    float rr = 0.0f;
+    EXPECT(load_cubes(out, rr).size() == 1u);
+    // depths 5 and 7 sharing a face: one bridge at depth 6
+    OctreeCube a, b;
+    a.depth = 5;
+    a.code = morton_encode(0, 0, 0);
+    a.r_c = 100.0f / 32;
+    b.depth = 7;
+    b.code = morton_encode(1u << 16, 0, 0);
+    b.r_c = 100.0f / 128;
+    save_cubes(in, {a, b}, 100.0f);
+    balance_octree(in, out);
+    const auto cubes = load_cubes(out, rr);
+    EXPECT(cubes.size() == 3u);
+    bool sorted = true, bridge = false;
+    for (size_t i = 1; i < cubes.size(); ++i) sorted = sorted && !(cubes[i].code < cubes[i - 1].code);
+    for (const auto& c : cubes)
+        bridge = bridge || (c.depth == 6 && c.code == morton_encode((1u << 16) - (1u << 15), 0, 0));
+    EXPECT(sorted);
+    EXPECT(bridge);
+    // no cube keeps a face neighbour two or more levels coarser
+    bool violation = false;
+    for (const auto& c : cubes) {
+        if (c.depth < 2) continue;
+        for (int f = 0; f < 6; ++f) {
+            const MortonCode nb = face_neighbor(c.code, f, c.depth);
+            if (nb.hi == UINT32_MAX) continue;
+            auto has = [&](int d, const MortonCode& k) {
+                for (const auto& e : cubes)
+                    if (e.depth == d && e.code == k) return true;
+                return false;
+            };
+            if (has(c.depth, nb) || has(c.depth - 1, parent_code(nb, c.depth))) continue;
+            MortonCode anc = parent_code(nb, c.depth);
+            for (int d2 = c.depth - 2; d2 >= 0; --d2) {
+                anc = parent_code(anc, d2 + 1);
+                violation = violation || has(d2, anc);
+            }
+        }
+    }
+    EXPECT(!violation);
+    std::filesystem::remove(in);
+    std::filesystem::remove(out);
+}
+
 int main() {
     try {
+        build_and_load();
+        parent_and_neighbour_codes();
+        merge_two_planes();
+        balance_cases();
         mip_levels_of_constant_map();
         projection_cases();
         histogram_statistics();

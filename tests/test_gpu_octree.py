@@ -55,6 +55,17 @@ def test_balance_repeated_cells(ctx):
     assert np.array_equal(_raw(got), _raw(scene.balance(inp, 1.5)))
 
 
+def test_balance_coarse_and_unaligned_inputs(ctx):
+    # depth-0 / depth-1 cells only (the code sort has no bits to sort on)
+    inp = B.to_cubes([(0, 0, np.float32(1.5)), (1 << 60, 1, np.float32(0.75)), (0, 1, np.float32(0.75))])
+    got, _ = ctx.octree_balance(inp, 1.5)
+    assert np.array_equal(_raw(got), _raw(scene.balance(inp, 1.5)))
+    # a code with bits below its cell size is not a cube the builder writes
+    bad = B.to_cubes([(B.interleave(1, 0, 0), 19, np.float32(0.1))])
+    with pytest.raises(RuntimeError, match="below its cell size"):
+        ctx.octree_balance(bad, 1.5)
+
+
 def test_balance_thinned_scene(ctx):
     # a real octree with 40 % of its cubes dropped: many violations at once
     c = scene.sphere_scene().cubes
@@ -80,9 +91,41 @@ def test_build_equals_host(ctx, which):
     assert rep["raw"] >= rep["merged"] and rep["merged"] + rep["bridges"] == len(got)
 
 
+@pytest.mark.parametrize("stage", [1, 2])
+def test_build_stages_equal_host(ctx, stage):
+    sc = scene.sphere_scene()
+    got, rep = ctx.octree_build(sc.depths, sc.cams, sc.origin, sc.r_root, 7, last_stage=stage)
+    want = scene.build_octree(sc.depths, sc.cams, 7, last_stage=stage)
+    assert np.array_equal(_raw(got), _raw(want))
+    assert len(got) == (rep["raw"] if stage == 1 else rep["merged"])
+
+
+def test_reference_build_and_merge_cases(ctx):
+    from test_octree_pins import _build, reference_build_case, reference_merge_case
+    for case, stages in ((reference_build_case, (1, 3)), (reference_merge_case, (1, 2, 3))):
+        depths, cams, origin, r_root = case()
+        for stage in stages:
+            got, _ = ctx.octree_build(depths, cams, origin, r_root, 21, last_stage=stage)
+            assert np.array_equal(_raw(got), _raw(_build(depths, cams, origin, r_root, 21, stage)))
+
+
+def test_merge_equals_host(ctx):
+    rng = np.random.default_rng(9)
+    cells = B.random_cells(12, 300)
+    inp = B.to_cubes([cells[i] for i in rng.integers(0, len(cells), 3000)])
+    inp["r_c"] = rng.uniform(0.01, 0.2, len(inp)).astype(np.float32)
+    got, rep = ctx.octree_merge(inp)
+    assert np.array_equal(_raw(got), _raw(scene.merge(inp)))
+    assert rep["raw"] == len(inp) and rep["merged"] == len(got)
+    got, _ = ctx.octree_merge(inp[:0])
+    assert len(got) == 0
+
+
 def test_build_rejects_bad_arguments(ctx):
     sc = scene.sphere_scene(views=2)
     with pytest.raises(RuntimeError, match="max_depth"):
         ctx.octree_build(sc.depths, sc.cams, sc.origin, sc.r_root, 22)
     with pytest.raises(RuntimeError, match="r_root"):
         ctx.octree_build(sc.depths, sc.cams, sc.origin, 0.0, 7)
+    with pytest.raises(RuntimeError, match="last_stage"):
+        ctx.octree_build(sc.depths, sc.cams, sc.origin, sc.r_root, 7, last_stage=4)

@@ -86,22 +86,73 @@ def _identity_camera(fx, cx, cy):
     return cam
 
 
-def test_build_reference_case():
-    # tests/test_stage1.cpp:84-117: a 10 x 10 map at depth 5, K = (50, 50, 5, 5),
-    # Rt = I, origin (-1, -1, 4), r_root 2.5 -> sorted cubes obeying the cell rule
-    cams = (N.Camera * 1)(_identity_camera(50.0, 5.0, 5.0))
-    depths = np.full((1, 10, 10), 5.0, dtype=np.float32)
+def _build(depths, cams, origin, r_root, max_depth, last_stage):
     S = N.scene_lib()
     p = C.c_void_p()
-    o = np.array([-1.0, -1.0, 4.0], dtype=np.float32)
-    n = S.tgvs_build_octree(N.ptr(depths), cams, 1, 10, 10, N.ptr(o), 2.5, 21, C.byref(p))
-    assert n > 0
+    o = np.asarray(origin, dtype=np.float32)
+    V, H, W = depths.shape
+    n = S.tgvs_build_octree(N.ptr(depths), cams, V, W, H, N.ptr(o), r_root, max_depth, last_stage, C.byref(p))
+    assert n >= 0
     c = np.empty(n, dtype=N.CUBE_DT)
-    C.memmove(c.ctypes.data, p, n * N.CUBE_DT.itemsize)
+    if n:
+        C.memmove(c.ctypes.data, p, n * N.CUBE_DT.itemsize)
     S.tgvs_free(p)
-    assert _sorted_unique(c)
-    cell = np.float32(2.5) / (np.float32(2.0) ** c["depth"].astype(np.float32))
-    assert np.all(cell >= np.float32(0.75) * c["r_c"]) and np.all(cell < np.float32(1.5) * c["r_c"])
+    return c
+
+
+def reference_build_case():
+    """tests/test_stage1.cpp:84-117: one 10 x 10 map at depth 5, K = (50, 50, 5, 5),
+    Rt = I, origin (-1, -1, 4), r_root 2.5, max depth 21."""
+    cams = (N.Camera * 1)(_identity_camera(50.0, 5.0, 5.0))
+    return np.full((1, 10, 10), 5.0, dtype=np.float32), cams, (-1.0, -1.0, 4.0), 2.5
+
+
+def reference_merge_case():
+    """tests/test_stages2_4.cpp:29-44,104-133: two 8 x 8 maps at depths 5 and 6,
+    K = (50, 50, 4, 4), Rt = I, origin (-1.5, -1.5, 4), r_root 3."""
+    cams = (N.Camera * 2)(_identity_camera(50.0, 4.0, 4.0), _identity_camera(50.0, 4.0, 4.0))
+    d = np.stack([np.full((8, 8), 5.0, np.float32), np.full((8, 8), 6.0, np.float32)])
+    return d, cams, (-1.5, -1.5, 4.0), 3.0
+
+
+def test_build_reference_case():
+    # the builder's own output (stage 1): non-empty, sorted, every cube obeys the
+    # cell size rule 0.75 r <= r_root / 2^d < 1.5 r
+    depths, cams, origin, r_root = reference_build_case()
+    for stage in (1, 3):
+        c = _build(depths, cams, origin, r_root, 21, stage)
+        assert len(c) > 0
+        k = B.codes(c)
+        assert np.all(k[1:] >= k[:-1])
+        cell = np.float32(2.5) / (np.float32(2.0) ** c["depth"].astype(np.float32))
+        assert np.all(cell >= np.float32(0.75) * c["r_c"]) and np.all(cell < np.float32(1.5) * c["r_c"])
+
+
+def test_merge_reference_case():
+    # two disjoint planes: the merged list is sorted and holds both maps' cubes
+    depths, cams, origin, r_root = reference_merge_case()
+    a = _build(depths[:1], cams, origin, r_root, 21, 1)
+    b = _build(depths[1:], (N.Camera * 1)(cams[1]), origin, r_root, 21, 1)
+    m = _build(depths, cams, origin, r_root, 21, 2)
+    assert len(a) and len(b) and len(m) == len(a) + len(b)
+    assert _sorted_unique(m)
+
+
+def test_merge_equals_python_restatement():
+    # runs of equal (code, depth) in stable (code, depth) order, mean r_c summed
+    # in double in that order (merge.cpp:57-92 with the tie order fixed)
+    rng = np.random.default_rng(5)
+    cells = B.random_cells(11, 120)
+    pick = rng.integers(0, len(cells), 600)
+    inp = B.to_cubes([cells[i] for i in pick])
+    inp["r_c"] = rng.uniform(0.01, 0.2, len(inp)).astype(np.float32)
+    got = scene.merge(inp)
+    order = sorted(range(len(inp)), key=lambda i: (int(B.codes(inp[i:i + 1])[0]), int(inp["depth"][i])))
+    runs = {}
+    for i in order:
+        runs.setdefault((int(B.codes(inp[i:i + 1])[0]), int(inp["depth"][i])), []).append(float(inp["r_c"][i]))
+    want = B.to_cubes([(k[0], k[1], np.float32(sum(v) / len(v))) for k, v in runs.items()])
+    assert np.array_equal(_raw(got), _raw(want))
 
 
 def test_build_sphere_properties():

@@ -246,9 +246,10 @@ class Context:
         self._check(self._L.tgvx_surface_copy(self.h, N.ptr(verts), N.ptr(tris)))
         return verts, tris
 
-    def octree_build(self, depths, cams, origin, r_root, max_depth):
-        """Stages 1-3 on the device from depth maps [V, H, W] and cameras
-        (tgvs_camera records); returns (cubes sorted by (code, depth), report dict)."""
+    def octree_build(self, depths, cams, origin, r_root, max_depth, last_stage=3):
+        """Stages 1..last_stage on the device from depth maps [V, H, W] and
+        cameras (tgvs_camera records); returns (cubes sorted by (code, depth),
+        report dict)."""
         d = np.ascontiguousarray(depths, dtype=np.float32)
         V, H, W = d.shape
         views = (N.DepthView * max(1, V))()
@@ -262,7 +263,14 @@ class Context:
         o = np.ascontiguousarray(origin, dtype=np.float32)
         n, rep = C.c_uint64(), N.OctreeReport()
         self._check(self._L.tgvx_octree_build(self.h, views, V, N.ptr(o), float(r_root), int(max_depth),
-                                              C.byref(n), C.byref(rep)))
+                                              int(last_stage), C.byref(n), C.byref(rep)))
+        return self._octree_out(n.value), rep.as_dict()
+
+    def octree_merge(self, cubes):
+        """kway_merge on the device over the concatenated cube lists."""
+        c = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+        n, rep = C.c_uint64(), N.OctreeReport()
+        self._check(self._L.tgvx_octree_merge(self.h, N.ptr(c), len(c), C.byref(n), C.byref(rep)))
         return self._octree_out(n.value), rep.as_dict()
 
     def octree_balance(self, cubes, r_root):

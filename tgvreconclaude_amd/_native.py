@@ -105,11 +105,11 @@ TGVX_SYMBOLS = [
     "tgvx_mlp_loss_grad", "tgvx_mlp_train", "tgvx_problem_features", "tgvx_problem_load_blocks",
     "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune", "tgvx_problem_solve_levels",
     "tgvx_surface_extract", "tgvx_surface_copy", "tgvx_solve_streaming", "tgvx_octree_build",
-    "tgvx_octree_balance", "tgvx_octree_copy",
+    "tgvx_octree_balance", "tgvx_octree_copy", "tgvx_octree_merge",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
-                 "tgvs_balance", "tgvs_treetop", "tgvs_free"]
+                 "tgvs_merge", "tgvs_balance", "tgvs_treetop", "tgvs_free"]
 
 _lib = None
 _scene = None
@@ -163,8 +163,9 @@ def lib():
                                            C.POINTER(Report)]
         L.tgvx_problem_block_report.argtypes = [vp, vp, vp]
         L.tgvx_solve_batched.argtypes = [vp, vp, C.c_uint32, i32, C.POINTER(Params), vp, C.POINTER(Report)]
-        L.tgvx_octree_build.argtypes = [vp, C.POINTER(DepthView), i32, vp, C.c_float, i32, C.POINTER(u64),
+        L.tgvx_octree_build.argtypes = [vp, C.POINTER(DepthView), i32, vp, C.c_float, i32, i32, C.POINTER(u64),
                                         C.POINTER(OctreeReport)]
+        L.tgvx_octree_merge.argtypes = [vp, vp, u64, C.POINTER(u64), C.POINTER(OctreeReport)]
         L.tgvx_octree_balance.argtypes = [vp, vp, u64, C.c_float, C.POINTER(u64), C.POINTER(OctreeReport)]
         L.tgvx_octree_copy.argtypes = [vp, vp]
         for name in TGVX_SYMBOLS:
@@ -183,7 +184,9 @@ def scene_lib():
         L.tgvs_ring_cameras.argtypes = [i32, f32, i32, i32, f32, vp]
         L.tgvs_random_cameras.argtypes = [i32, f32, i32, i32, f32, C.c_uint64, vp]
         L.tgvs_render.argtypes = [vp, i32, vp, i32, i32, i32, f32, f32, f32, f32, C.c_uint64, vp]
-        L.tgvs_build_octree.argtypes = [vp, vp, i32, i32, i32, vp, f32, i32, C.POINTER(vp)]
+        L.tgvs_build_octree.argtypes = [vp, vp, i32, i32, i32, vp, f32, i32, i32, C.POINTER(vp)]
+        L.tgvs_merge.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+        L.tgvs_merge.restype = C.c_int64
         L.tgvs_build_octree.restype = C.c_int64
         L.tgvs_balance.argtypes = [vp, C.c_uint64, f32, C.POINTER(vp)]
         L.tgvs_balance.restype = C.c_int64

@@ -232,8 +232,14 @@ int tgvx_surface_copy(tgvx_ctx* c, float* verts, uint32_t* tris) {
 }
 
 int tgvx_octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, const float origin[3],
-                      float r_root, int32_t max_depth, uint64_t* n_out, tgvx_octree_report* rep) {
-    return guarded(c, [&] { tgvx::octree_build(c, views, nviews, origin, r_root, max_depth, n_out, rep); });
+                      float r_root, int32_t max_depth, int32_t last_stage, uint64_t* n_out, tgvx_octree_report* rep) {
+    return guarded(c, [&] {
+        tgvx::octree_build(c, views, nviews, origin, r_root, max_depth, last_stage, n_out, rep);
+    });
+}
+
+int tgvx_octree_merge(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, uint64_t* n_out, tgvx_octree_report* rep) {
+    return guarded(c, [&] { tgvx::octree_merge(c, cubes, n, n_out, rep); });
 }
 
 int tgvx_octree_balance(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, float r_root, uint64_t* n_out,

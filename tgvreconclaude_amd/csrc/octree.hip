@@ -154,16 +154,27 @@ __global__ void k_run_heads(const uint64_t* __restrict__ code, const uint32_t* _
     if (i >= n) return;
     head[i] = i == 0 || code[i] != code[i - 1] || depth[idx[i]] != depth[idx[i - 1]];
 }
-// merge.cpp:57-92 with the sum in double over the run, in sorted order
+// merge.cpp:57-92 with the sum in double over the run, in sorted order.
+// rs = the radii already in sorted order (contiguous per run). Runs of the
+// coarsest cells collect up to millions of pixels (outliers), so the loads
+// of a run are issued eight at a time; the additions keep their order.
 __global__ void k_merge_runs(const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth,
-                             const float* __restrict__ rad, const uint32_t* __restrict__ idx,
+                             const float* __restrict__ rs, const uint32_t* __restrict__ idx,
                              const uint32_t* __restrict__ start, uint64_t nruns, uint64_t nraw,
                              uint64_t* __restrict__ mcode, uint32_t* __restrict__ mdepth, float* __restrict__ mrc) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nruns) return;
     const uint64_t a = start[j], b = j + 1 < nruns ? start[j + 1] : nraw;
     double s = 0.0;
-    for (uint64_t i = a; i < b; ++i) s += (double)rad[idx[i]];
+    uint64_t i = a;
+    for (; i + 8 <= b; i += 8) {
+        float x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = rs[i + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += (double)x[k];
+    }
+    for (; i < b; ++i) s += (double)rs[i];
     mcode[j] = code[a];
     mdepth[j] = depth[idx[a]];
     mrc[j] = (float)(s / (double)(b - a));
@@ -329,17 +340,23 @@ enum {
 };
 
 // stable sort of (keys, idx) pairs in place through the alternate buffers
+// (bits [begin_bit, end_bit) of the keys)
 template <class K>
 void sort_pairs(tgvx_ctx* c, DevBuf& keys, DevBuf& keys_alt, DevBuf& vals, DevBuf& vals_alt, uint64_t n,
-                int end_bit) {
+                int begin_bit, int end_bit) {
     if (n == 0) return;
+    if (begin_bit >= end_bit) {  // nothing to sort by: the order stays
+        TGVX_HIP(hipMemcpyAsync(keys_alt.p, keys.p, sizeof(K) * n, hipMemcpyDeviceToDevice, c->stream));
+        TGVX_HIP(hipMemcpyAsync(vals_alt.p, vals.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        return;
+    }
     DevBuf& tmp = c->oct_tmp[B_TMP];
     size_t bytes = 0;
     TGVX_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys.as<K>(), keys_alt.as<K>(), vals.as<uint32_t>(),
-                                       vals_alt.as<uint32_t>(), (size_t)n, 0, end_bit, c->stream));
+                                       vals_alt.as<uint32_t>(), (size_t)n, begin_bit, end_bit, c->stream));
     tmp.ensure(bytes);
     TGVX_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, keys.as<K>(), keys_alt.as<K>(), vals.as<uint32_t>(),
-                                       vals_alt.as<uint32_t>(), (size_t)n, 0, end_bit, c->stream));
+                                       vals_alt.as<uint32_t>(), (size_t)n, begin_bit, end_bit, c->stream));
 }
 // stable compaction of the positions i < n with flag[i] != 0 into out; returns the count
 uint64_t select_flagged(tgvx_ctx* c, const uint8_t* flag, uint64_t n, uint32_t* out) {
@@ -359,9 +376,10 @@ uint64_t select_flagged(tgvx_ctx* c, const uint8_t* flag, uint64_t n, uint32_t* 
 }
 
 // Balance of the nseed cells (code, depth, r_c on the device, queue order)
-// into c->oct_out, sorted by (code, depth), stable. Returns the bridge count.
+// into c->oct_out, sorted by (code, depth), stable. Every code (seeds and
+// bridges) has its bits below low_bit zero. Returns the bridge count.
 uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, uint64_t nseed, float r_root,
-                        int32_t* rounds) {
+                        int low_bit, int32_t* rounds) {
     DevBuf* T = c->oct_tmp;
     hipStream_t st = c->stream;
     // seed table
@@ -489,11 +507,11 @@ uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, ui
     if (n) {
         k_iota<<<grid_of(n), 256, 0, st>>>(idx.as<uint32_t>(), n);
         TGVX_HIP(hipMemcpyAsync(dk.p, T[B_KEY32].p, 4 * n, hipMemcpyDeviceToDevice, st));
-        sort_pairs<uint32_t>(c, dk, dk2, idx, idx2, n, 5);  // idx2: by depth
+        sort_pairs<uint32_t>(c, dk, dk2, idx, idx2, n, 0, 5);  // idx2: by depth
         k_gather<uint64_t><<<grid_of(n), 256, 0, st>>>(T[B_CODES].as<uint64_t>(), idx2.as<uint32_t>(), n,
                                                        cc2.as<uint64_t>());
         cc3.ensure(8 * n);
-        sort_pairs<uint64_t>(c, cc2, cc3, idx2, idx, n, 63);  // idx: by (code, depth)
+        sort_pairs<uint64_t>(c, cc2, cc3, idx2, idx, n, low_bit, 63);  // idx: by (code, depth)
         c->oct_out.ensure(sizeof(tgvx_cube) * n);
         k_records<<<grid_of(n), 256, 0, st>>>(T[B_CODES].as<uint64_t>(), T[B_KEY32].as<uint32_t>(),
                                               cells_rc.as<float>(), idx.as<uint32_t>(), n,
@@ -505,12 +523,92 @@ uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, ui
     return nbr;
 }
 
+// Stage 2 over n entries: idx lists them in tie order (ascending pixel or
+// input index) over the arrays code / depth / rad. Sorts idx by (code,
+// depth), stably; with merge, writes one cell per run of equal (code, depth)
+// to mcode / mdepth / mrc (mean r_c, double sum in sorted order) and returns
+// the run count, else returns n with idx sorted. Codes have their bits
+// below low_bit zero.
+uint64_t sort_merge(tgvx_ctx* c, const uint64_t* code, const uint32_t* depth, const float* rad, DevBuf& idx, uint64_t n,
+                    int low_bit, bool merge, DevBuf& mcode, DevBuf& mdepth, DevBuf& mrc) {
+    DevBuf* T = c->oct_tmp;
+    hipStream_t st = c->stream;
+    if (n == 0) return 0;
+    DevBuf dk, dk2, idx2, cs, cs2, rs;  // rs: radii in sorted order
+    dk.ensure(4 * n);
+    dk2.ensure(4 * n);
+    idx2.ensure(4 * n);
+    cs.ensure(8 * n);
+    cs2.ensure(8 * n);
+    // by depth, then stably by code
+    k_gather<uint32_t><<<grid_of(n), 256, 0, st>>>(depth, idx.as<uint32_t>(), n, dk.as<uint32_t>());
+    sort_pairs<uint32_t>(c, dk, dk2, idx, idx2, n, 0, 5);
+    k_gather<uint64_t><<<grid_of(n), 256, 0, st>>>(code, idx2.as<uint32_t>(), n, cs.as<uint64_t>());
+    sort_pairs<uint64_t>(c, cs, cs2, idx2, idx, n, low_bit, 63);
+    uint64_t nm = n;
+    if (merge) {
+        T[B_HEAD].ensure(n);
+        k_run_heads<<<grid_of(n), 256, 0, st>>>(cs2.as<uint64_t>(), depth, idx.as<uint32_t>(), n,
+                                                T[B_HEAD].as<uint8_t>());
+        T[B_START].ensure(4 * n);
+        nm = select_flagged(c, T[B_HEAD].as<uint8_t>(), n, T[B_START].as<uint32_t>());
+        mcode.ensure(8 * nm);
+        mdepth.ensure(4 * nm);
+        mrc.ensure(4 * nm);
+        rs.ensure(4 * n);
+        k_gather<float><<<grid_of(n), 256, 0, st>>>(rad, idx.as<uint32_t>(), n, rs.as<float>());
+        k_merge_runs<<<grid_of(nm), 256, 0, st>>>(cs2.as<uint64_t>(), depth, rs.as<float>(), idx.as<uint32_t>(),
+                                                  T[B_START].as<uint32_t>(), nm, n, mcode.as<uint64_t>(),
+                                                  mdepth.as<uint32_t>(), mrc.as<float>());
+    }
+    TGVX_HIP(hipGetLastError());
+    TGVX_HIP(hipStreamSynchronize(st));  // before the scratch buffers go
+    return nm;
+}
+
+// cells (code, depth, r_c) listed by idx -> c->oct_out
+void write_records(tgvx_ctx* c, const uint64_t* code, const uint32_t* depth, const float* rc, const uint32_t* idx,
+                   uint64_t n) {
+    c->oct_n = n;
+    if (n == 0) return;
+    c->oct_out.ensure(sizeof(tgvx_cube) * n);
+    k_records<<<grid_of(n), 256, 0, c->stream>>>(code, depth, rc, idx, n, c->oct_out.as<tgvx_cube>());
+    TGVX_HIP(hipGetLastError());
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+}
+
+// host cubes -> device code / depth / r_c arrays; returns the lowest set bit
+// over the codes (63 when all are zero)
+int upload_cubes(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, DevBuf& dcode, DevBuf& ddepth, DevBuf& drc) {
+    std::vector<uint64_t> code(n);
+    std::vector<uint32_t> depth(n);
+    std::vector<float> rc(n);
+    int low_bit = 63;
+    for (uint64_t i = 0; i < n; ++i) {
+        code[i] = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
+        depth[i] = (uint32_t)cubes[i].depth;
+        rc[i] = cubes[i].r_c;
+        if (code[i]) low_bit = std::min(low_bit, __builtin_ctzll(code[i]));
+    }
+    dcode.ensure(8 * std::max<uint64_t>(n, 1));
+    ddepth.ensure(4 * std::max<uint64_t>(n, 1));
+    drc.ensure(4 * std::max<uint64_t>(n, 1));
+    if (n) {
+        TGVX_HIP(hipMemcpyAsync(dcode.p, code.data(), 8 * n, hipMemcpyHostToDevice, c->stream));
+        TGVX_HIP(hipMemcpyAsync(ddepth.p, depth.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+        TGVX_HIP(hipMemcpyAsync(drc.p, rc.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+    }
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    return low_bit;
+}
+
 }  // namespace
 
 void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, const float origin[3], float r_root,
-                  int32_t max_depth, uint64_t* n_out, tgvx_octree_report* rep) {
+                  int32_t max_depth, int32_t last_stage, uint64_t* n_out, tgvx_octree_report* rep) {
     if (nviews < 0 || (nviews > 0 && !views) || !origin) throw Error(TGVX_E_ARG, "null pointer");
     if (max_depth < 0 || max_depth > 21) throw Error(TGVX_E_ARG, "max_depth must be in 0..21");
+    if (last_stage < 1 || last_stage > 3) throw Error(TGVX_E_ARG, "last_stage must be 1, 2 or 3");
     if (!(r_root > 0.0f)) throw Error(TGVX_E_ARG, "r_root must be positive");
     DevBuf* T = c->oct_tmp;
     hipStream_t st = c->stream;
@@ -567,48 +665,28 @@ void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, con
     const uint64_t nraw = select_flagged(c, T[B_FLAG].as<uint8_t>(), total, T[B_IDX].as<uint32_t>());
     r.raw = nraw;
     const double t2 = ms_now();
-    // sort by (code, depth, pixel): by depth, then stably by code
-    DevBuf dk, dk2, idx2, cs, cs2;
-    const uint64_t nr1 = std::max<uint64_t>(nraw, 1);
-    dk.ensure(4 * nr1);
-    dk2.ensure(4 * nr1);
-    idx2.ensure(4 * nr1);
-    cs.ensure(8 * nr1);
-    cs2.ensure(8 * nr1);
-    uint64_t nm = 0;
+    // per-pixel codes are aligned to their depth (<= max_depth); so is a
+    // bridge, an ancestor of a neighbour at a coarser depth
+    const int low_bit = 3 * (21 - max_depth);
     DevBuf mcode, mdepth, mrc;
-    if (nraw) {
-        k_gather<uint32_t><<<grid_of(nraw), 256, 0, st>>>(T[B_DEPTH].as<uint32_t>(), T[B_IDX].as<uint32_t>(), nraw,
-                                                          dk.as<uint32_t>());
-        sort_pairs<uint32_t>(c, dk, dk2, T[B_IDX], idx2, nraw, 5);
-        k_gather<uint64_t><<<grid_of(nraw), 256, 0, st>>>(T[B_CODE].as<uint64_t>(), idx2.as<uint32_t>(), nraw,
-                                                          cs.as<uint64_t>());
-        sort_pairs<uint64_t>(c, cs, cs2, idx2, T[B_IDX], nraw, 63);  // T[B_IDX]: (code, depth, pixel) order
-        T[B_HEAD].ensure(nraw);
-        k_run_heads<<<grid_of(nraw), 256, 0, st>>>(cs2.as<uint64_t>(), T[B_DEPTH].as<uint32_t>(),
-                                                   T[B_IDX].as<uint32_t>(), nraw, T[B_HEAD].as<uint8_t>());
-        T[B_START].ensure(4 * nraw);
-        nm = select_flagged(c, T[B_HEAD].as<uint8_t>(), nraw, T[B_START].as<uint32_t>());
-        mcode.ensure(8 * nm);
-        mdepth.ensure(4 * nm);
-        mrc.ensure(4 * nm);
-        k_merge_runs<<<grid_of(nm), 256, 0, st>>>(cs2.as<uint64_t>(), T[B_DEPTH].as<uint32_t>(), T[B_RAD].as<float>(),
-                                                  T[B_IDX].as<uint32_t>(), T[B_START].as<uint32_t>(), nm, nraw,
-                                                  mcode.as<uint64_t>(), mdepth.as<uint32_t>(), mrc.as<float>());
-        TGVX_HIP(hipGetLastError());
-    }
-    TGVX_HIP(hipStreamSynchronize(st));
-    dk.release();
-    dk2.release();
-    idx2.release();
-    cs.release();
-    cs2.release();
-    r.merged = nm;
+    const uint64_t nm = sort_merge(c, T[B_CODE].as<uint64_t>(), T[B_DEPTH].as<uint32_t>(), T[B_RAD].as<float>(),
+                                   T[B_IDX], nraw, low_bit, last_stage >= 2, mcode, mdepth, mrc);
+    if (last_stage == 1)
+        write_records(c, T[B_CODE].as<uint64_t>(), T[B_DEPTH].as<uint32_t>(), T[B_RAD].as<float>(),
+                      T[B_IDX].as<uint32_t>(), nraw);
+    r.merged = last_stage >= 2 ? nm : 0;
     const double t3 = ms_now();
-    mcode.ensure(8);
-    mdepth.ensure(4);
-    mrc.ensure(4);
-    r.bridges = balance_device(c, mcode, mdepth, mrc, nm, r_root, &r.rounds);
+    if (last_stage == 2) {
+        DevBuf id;
+        id.ensure(4 * std::max<uint64_t>(nm, 1));
+        if (nm) k_iota<<<grid_of(nm), 256, 0, st>>>(id.as<uint32_t>(), nm);
+        write_records(c, mcode.as<uint64_t>(), mdepth.as<uint32_t>(), mrc.as<float>(), id.as<uint32_t>(), nm);
+    } else if (last_stage == 3) {
+        mcode.ensure(8);
+        mdepth.ensure(4);
+        mrc.ensure(4);
+        r.bridges = balance_device(c, mcode, mdepth, mrc, nm, r_root, low_bit, &r.rounds);
+    }
     const double t4 = ms_now();
     r.upload_ms = t1 - t0;
     r.cubes_ms = t2 - t1;
@@ -623,35 +701,58 @@ void octree_balance(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, float r_roo
                     tgvx_octree_report* rep) {
     if (n && !cubes) throw Error(TGVX_E_ARG, "null pointer");
     if (n >= (1ull << 32)) throw Error(TGVX_E_ARG, "at most 2^32 - 1 cubes");
+    for (uint64_t i = 0; i < n; ++i) {
+        if (cubes[i].depth < 0 || cubes[i].depth > 21) throw Error(TGVX_E_ARG, "cube depth outside 0..21");
+        // a cube's code is its cell's lower corner (octree_builder.cpp:70-71):
+        // the bits below the cell size are zero, which the cell keys rely on
+        const uint64_t cv = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
+        if (cv & ((1ull << (3 * (21 - cubes[i].depth))) - 1ull))
+            throw Error(TGVX_E_ARG, "cube " + std::to_string(i) + ": code has bits below its cell size");
+    }
+    int max_depth = 0;
+    for (uint64_t i = 0; i < n; ++i) max_depth = std::max(max_depth, (int)cubes[i].depth);
+    tgvx_octree_report r;
+    std::memset(&r, 0, sizeof(r));
+    const double t0 = ms_now();
+    DevBuf dcode, ddepth, drc;
+    // the sort's lowest key bit: every seed code's lowest set bit, and the
+    // alignment of a bridge (one level above the deepest seed at most)
+    int low_bit = upload_cubes(c, cubes, n, dcode, ddepth, drc);
+    const double t1 = ms_now();
+    r.merged = n;
+    low_bit = std::min(low_bit, 3 * (21 - max_depth));
+    r.bridges = balance_device(c, dcode, ddepth, drc, n, r_root, low_bit, &r.rounds);
+    const double t2 = ms_now();
+    r.upload_ms = t1 - t0;
+    r.balance_ms = t2 - t1;
+    r.total_ms = t2 - t0;
+    if (n_out) *n_out = c->oct_n;
+    if (rep) *rep = r;
+}
+
+void octree_merge(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, uint64_t* n_out, tgvx_octree_report* rep) {
+    if (n && !cubes) throw Error(TGVX_E_ARG, "null pointer");
+    if (n >= (1ull << 32)) throw Error(TGVX_E_ARG, "at most 2^32 - 1 cubes");
     for (uint64_t i = 0; i < n; ++i)
         if (cubes[i].depth < 0 || cubes[i].depth > 21) throw Error(TGVX_E_ARG, "cube depth outside 0..21");
     tgvx_octree_report r;
     std::memset(&r, 0, sizeof(r));
     const double t0 = ms_now();
-    std::vector<uint64_t> code(n);
-    std::vector<uint32_t> depth(n);
-    std::vector<float> rc(n);
-    for (uint64_t i = 0; i < n; ++i) {
-        code[i] = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
-        depth[i] = (uint32_t)cubes[i].depth;
-        rc[i] = cubes[i].r_c;
-    }
-    DevBuf dcode, ddepth, drc;
-    dcode.ensure(8 * std::max<uint64_t>(n, 1));
-    ddepth.ensure(4 * std::max<uint64_t>(n, 1));
-    drc.ensure(4 * std::max<uint64_t>(n, 1));
-    if (n) {
-        TGVX_HIP(hipMemcpyAsync(dcode.p, code.data(), 8 * n, hipMemcpyHostToDevice, c->stream));
-        TGVX_HIP(hipMemcpyAsync(ddepth.p, depth.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
-        TGVX_HIP(hipMemcpyAsync(drc.p, rc.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
-    }
-    TGVX_HIP(hipStreamSynchronize(c->stream));
+    DevBuf dcode, ddepth, drc, idx, mcode, mdepth, mrc, id;
+    const int low_bit = upload_cubes(c, cubes, n, dcode, ddepth, drc);
     const double t1 = ms_now();
-    r.merged = n;
-    r.bridges = balance_device(c, dcode, ddepth, drc, n, r_root, &r.rounds);
+    idx.ensure(4 * std::max<uint64_t>(n, 1));
+    if (n) k_iota<<<grid_of(n), 256, 0, c->stream>>>(idx.as<uint32_t>(), n);  // ties: input order
+    const uint64_t nm = sort_merge(c, dcode.as<uint64_t>(), ddepth.as<uint32_t>(), drc.as<float>(), idx, n, low_bit,
+                                   true, mcode, mdepth, mrc);
+    id.ensure(4 * std::max<uint64_t>(nm, 1));
+    if (nm) k_iota<<<grid_of(nm), 256, 0, c->stream>>>(id.as<uint32_t>(), nm);
+    write_records(c, mcode.as<uint64_t>(), mdepth.as<uint32_t>(), mrc.as<float>(), id.as<uint32_t>(), nm);
     const double t2 = ms_now();
+    r.raw = n;
+    r.merged = nm;
     r.upload_ms = t1 - t0;
-    r.balance_ms = t2 - t1;
+    r.merge_ms = t2 - t1;
     r.total_ms = t2 - t0;
     if (n_out) *n_out = c->oct_n;
     if (rep) *rep = r;

@@ -361,6 +361,49 @@ std::vector<tgvx_cube> balance(const std::vector<tgvx_cube>& in, float r_root) {
     return out;
 }
 
+tgvx_cube cube_of(const Raw& r) {
+    tgvx_cube c;
+    c.hi = (uint32_t)(r.key >> 32);
+    c.lo = (uint32_t)r.key;
+    c.depth = r.depth;
+    c.r_c = r.r;
+    return c;
+}
+
+// positions of `all` sorted by (code, depth), position order among equal pairs
+std::vector<uint32_t> sorted_order(const std::vector<Raw>& all) {
+    std::vector<uint32_t> ord(all.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (uint32_t)i;
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
+        const Raw &x = all[a], &y = all[b];
+        if (x.key != y.key) return x.key < y.key;
+        if (x.depth != y.depth) return x.depth < y.depth;
+        return a < b;
+    });
+    return ord;
+}
+
+// merge.cpp:57-92: one cell per run of equal (code, depth) in sorted order,
+// r_c the mean of the run (double sum in that order)
+std::vector<tgvx_cube> merge_sorted(const std::vector<Raw>& all, const std::vector<uint32_t>& ord) {
+    std::vector<tgvx_cube> merged;
+    merged.reserve(all.size() / 2 + 16);
+    for (size_t i = 0; i < ord.size();) {
+        size_t j = i;
+        double s = 0;
+        const Raw& r0 = all[ord[i]];
+        while (j < ord.size() && all[ord[j]].key == r0.key && all[ord[j]].depth == r0.depth) {
+            s += all[ord[j]].r;
+            ++j;
+        }
+        tgvx_cube c = cube_of(r0);
+        c.r_c = (float)(s / (double)(j - i));
+        merged.push_back(c);
+        i = j;
+    }
+    return merged;
+}
+
 void treetop_rec(const tgvx_cube* c, uint32_t first, uint32_t last, uint64_t node, int depth, uint32_t cap,
                  std::vector<tgvx_treetop_leaf>& out) {
     const uint32_t count = last - first;
@@ -452,8 +495,9 @@ int32_t tgvs_render(const tgvs_prim* prims, int32_t np, const tgvs_camera* cams,
 }
 
 int64_t tgvs_build_octree(const float* depths, const tgvs_camera* cams, int32_t nc, int32_t W, int32_t H,
-                          const float origin[3], float r_root, int32_t max_depth, tgvx_cube** out) {
-    if (!depths || !cams || !out || max_depth < 0 || max_depth > 21) return -1;
+                          const float origin[3], float r_root, int32_t max_depth, int32_t last_stage,
+                          tgvx_cube** out) {
+    if (!depths || !cams || !out || max_depth < 0 || max_depth > 21 || last_stage < 1 || last_stage > 3) return -1;
     const float inv_2r = 1.0f / (2.0f * r_root);
     // stage 1 per view (octree_builder.cpp:30-97), then stage 2: one global
     // merge of equal (code, depth) cells.
@@ -498,38 +542,32 @@ int64_t tgvs_build_octree(const float* depths, const tgvs_camera* cams, int32_t 
         src.insert(src.end(), per[c].size(), (uint32_t)c);
         std::vector<Raw>().swap(per[c]);
     }
-    std::vector<uint32_t> ord(all.size());
-    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (uint32_t)i;
-    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
-        const Raw &x = all[a], &y = all[b];
-        if (x.key != y.key) return x.key < y.key;
-        if (x.depth != y.depth) return x.depth < y.depth;
-        return a < b;
-    });
-    std::vector<tgvx_cube> merged;
-    merged.reserve(all.size() / 2 + 16);
-    for (size_t i = 0; i < ord.size();) {
-        size_t j = i;
-        double s = 0;
-        const Raw& r0 = all[ord[i]];
-        while (j < ord.size() && all[ord[j]].key == r0.key && all[ord[j]].depth == r0.depth) {
-            s += all[ord[j]].r;
-            ++j;
-        }
-        tgvx_cube c;
-        c.hi = (uint32_t)(r0.key >> 32);
-        c.lo = (uint32_t)r0.key;
-        c.depth = r0.depth;
-        c.r_c = (float)(s / (double)(j - i));
-        merged.push_back(c);
-        i = j;
+    const std::vector<uint32_t> ord = sorted_order(all);
+    std::vector<tgvx_cube> res;
+    if (last_stage == 1) {
+        res.resize(ord.size());
+        for (size_t i = 0; i < ord.size(); ++i) res[i] = cube_of(all[ord[i]]);
+    } else {
+        std::vector<tgvx_cube> merged = merge_sorted(all, ord);
+        std::vector<Raw>().swap(all);
+        res = last_stage == 2 ? std::move(merged) : balance(merged, r_root);
     }
-    std::vector<Raw>().swap(all);
-    std::vector<tgvx_cube> bal = balance(merged, r_root);
+    const std::vector<tgvx_cube>& bal = res;
     *out = (tgvx_cube*)std::malloc(std::max<size_t>(1, bal.size()) * sizeof(tgvx_cube));
     if (!*out) return -1;
     std::memcpy(*out, bal.data(), bal.size() * sizeof(tgvx_cube));
     return (int64_t)bal.size();
+}
+
+int64_t tgvs_merge(const tgvx_cube* cubes, uint64_t n, tgvx_cube** out) {
+    if ((n && !cubes) || !out) return -1;
+    std::vector<Raw> all(n);
+    for (uint64_t i = 0; i < n; ++i) all[i] = {code_of(cubes[i]), cubes[i].depth, cubes[i].r_c};
+    const std::vector<tgvx_cube> m = merge_sorted(all, sorted_order(all));
+    *out = (tgvx_cube*)std::malloc(std::max<size_t>(1, m.size()) * sizeof(tgvx_cube));
+    if (!*out) return -1;
+    std::memcpy(*out, m.data(), m.size() * sizeof(tgvx_cube));
+    return (int64_t)m.size();
 }
 
 int64_t tgvs_balance(const tgvx_cube* cubes, uint64_t n, float r_root, tgvx_cube** out) {

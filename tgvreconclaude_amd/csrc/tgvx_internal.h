@@ -156,7 +156,8 @@ void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double
 void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* rep);
 // octree.hip
 void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, const float origin[3], float r_root,
-                  int32_t max_depth, uint64_t* n_out, tgvx_octree_report* rep);
+                  int32_t max_depth, int32_t last_stage, uint64_t* n_out, tgvx_octree_report* rep);
+void octree_merge(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, uint64_t* n_out, tgvx_octree_report* rep);
 void octree_balance(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, float r_root, uint64_t* n_out,
                     tgvx_octree_report* rep);
 void octree_copy(tgvx_ctx* c, tgvx_cube* out);

@@ -12,7 +12,9 @@
 #include <stdexcept>
 #include <string>
 
+#include "balance.hpp"
 #include "marching_cubes.hpp"
+#include "merge.hpp"
 #include "octree_builder.hpp"
 #include "tgv.hpp"
 #include "tgvx.h"
@@ -270,6 +272,104 @@ std::vector<OctreeCube> load_cubes_range(const std::string& path, uint32_t first
 
 std::vector<OctreeCube> load_cubes(const std::string& path, float& r_root_out) {
     return load_cubes_range(path, 0, std::numeric_limits<uint32_t>::max() - 1, r_root_out);
+}
+
+void save_cubes(const std::string& path, const std::vector<OctreeCube>& cubes, float r_root) {
+    std::ofstream out(path, std::ios::binary);
+    if (!out) throw std::runtime_error("Cannot open output: " + path);
+    CubeFileHeader h;
+    h.r_root = r_root;
+    h.cube_count = (uint32_t)cubes.size();
+    out.write(reinterpret_cast<const char*>(&h), sizeof(h));
+    if (!cubes.empty()) out.write(reinterpret_cast<const char*>(cubes.data()), cubes.size() * sizeof(OctreeCube));
+}
+
+namespace {
+std::vector<OctreeCube> octree_result(uint64_t n) {
+    std::vector<OctreeCube> v(n);
+    check(tgvx_octree_copy(ctx(), reinterpret_cast<tgvx_cube*>(v.data())));
+    return v;
+}
+}  // namespace
+
+// ---- stages 1-3 (octree_builder.cpp:11-97, merge.cpp:7-97, balance.cpp:22-150)
+int cube_depth_for_radius(float r_x, float r_root, int max_depth) {
+    if (r_x <= 0.0f) return max_depth;
+    int d = (int)std::round(std::log2(r_root / (r_x * 1.125f)));
+    d = std::min(std::max(d, 0), max_depth);
+    // the one depth whose cell size lies in [0.75 r_x, 1.5 r_x), three steps at most
+    for (int step = 0; step < 3; ++step) {
+        const float cell = r_root / (float)(1u << d);
+        if (cell < 0.75f * r_x && d > 0) --d;
+        else if (cell >= 1.5f * r_x && d < max_depth) ++d;
+        else break;
+    }
+    return d;
+}
+
+void build_cubes_from_depthmap(const DepthMap& dm, const Vec3f& scene_origin, float r_root,
+                               const std::string& out_path) {
+    if ((int64_t)dm.depths.size() < (int64_t)dm.width * dm.height) throw std::runtime_error("depth map too small");
+    tgvx_depth_view v;
+    v.depths = dm.depths.data();
+    v.width = dm.width;
+    v.height = dm.height;
+    v.fx = dm.K(0, 0);
+    v.fy = dm.K(1, 1);
+    v.cx = dm.K(0, 2);
+    v.cy = dm.K(1, 2);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) v.cam_to_world[r * 4 + c] = dm.Rt(r, c);
+    const float origin[3] = {scene_origin.x(), scene_origin.y(), scene_origin.z()};
+    uint64_t n = 0;
+    check(tgvx_octree_build(ctx(), &v, 1, origin, r_root, 21, 1, &n, nullptr));
+    save_cubes(out_path, octree_result(n), r_root);
+}
+
+void kway_merge(const std::vector<std::string>& in_paths, const std::string& out_path) {
+    std::vector<OctreeCube> all;
+    float r_root = 0.0f;
+    for (size_t i = 0; i < in_paths.size(); ++i) {
+        float r = 0.0f;
+        std::vector<OctreeCube> v = load_cubes(in_paths[i], r);
+        if (i == 0) r_root = r;
+        all.insert(all.end(), v.begin(), v.end());
+    }
+    uint64_t n = 0;
+    check(tgvx_octree_merge(ctx(), reinterpret_cast<const tgvx_cube*>(all.data()), all.size(), &n, nullptr));
+    save_cubes(out_path, octree_result(n), r_root);
+}
+
+void balance_octree(const std::string& in_path, const std::string& out_path) {
+    float r_root = 0.0f;
+    const std::vector<OctreeCube> in = load_cubes(in_path, r_root);
+    uint64_t n = 0;
+    check(tgvx_octree_balance(ctx(), reinterpret_cast<const tgvx_cube*>(in.data()), in.size(), r_root, &n, nullptr));
+    save_cubes(out_path, octree_result(n), r_root);
+}
+
+MortonCode parent_code(const MortonCode& code, int current_depth) {
+    const uint64_t c = ((uint64_t)code.hi << 32) | code.lo;
+    const unsigned s = 3u * (21u - (unsigned)(current_depth - 1));
+    const uint64_t p = s >= 64u ? 0ull : c & ~((1ull << s) - 1ull);
+    return {(uint32_t)(p >> 32), (uint32_t)p};
+}
+
+MortonCode face_neighbor(const MortonCode& code, int face, int depth) {
+    const MortonCode none{UINT32_MAX, UINT32_MAX};
+    if (face < 0 || face > 5) return none;
+    uint32_t a[3];
+    morton_decode(code, a[0], a[1], a[2]);
+    const uint32_t step = 1u << (21 - depth);
+    uint32_t& x = a[face / 2];
+    if (face % 2 == 0) {
+        if (x + step >= (1u << 21)) return none;
+        x += step;
+    } else {
+        if (x < step) return none;
+        x -= step;
+    }
+    return morton_encode(a[0], a[1], a[2]);
 }
 
 std::vector<TreetopLeaf> load_treetop(const std::string& path, float& r_root_out) {

@@ -86,12 +86,12 @@ def render(prims, cams, W, H, sigma, outlier_frac, seed, lo=0.5, hi=6.0):
     return out
 
 
-def build_octree(depths, cams, max_depth, origin=ORIGIN, r_root=R_ROOT):
+def build_octree(depths, cams, max_depth, origin=ORIGIN, r_root=R_ROOT, last_stage=3):
     S = N.scene_lib()
     o = np.asarray(origin, dtype=np.float32)
     p = C.c_void_p()
     V, H, W = depths.shape
-    n = S.tgvs_build_octree(N.ptr(depths), cams, V, W, H, N.ptr(o), r_root, max_depth, C.byref(p))
+    n = S.tgvs_build_octree(N.ptr(depths), cams, V, W, H, N.ptr(o), r_root, max_depth, last_stage, C.byref(p))
     if n < 0:
         raise RuntimeError("tgvs_build_octree failed")
     cubes = np.empty(n, dtype=N.CUBE_DT)
@@ -99,6 +99,20 @@ def build_octree(depths, cams, max_depth, origin=ORIGIN, r_root=R_ROOT):
         C.memmove(cubes.ctypes.data, p, n * N.CUBE_DT.itemsize)
     S.tgvs_free(p)
     return cubes
+
+
+def merge(cubes):
+    S = N.scene_lib()
+    c = np.ascontiguousarray(cubes, dtype=N.CUBE_DT)
+    p = C.c_void_p()
+    n = S.tgvs_merge(N.ptr(c), len(c), C.byref(p))
+    if n < 0:
+        raise RuntimeError("tgvs_merge failed")
+    out = np.empty(n, dtype=N.CUBE_DT)
+    if n:
+        C.memmove(out.ctypes.data, p, n * N.CUBE_DT.itemsize)
+    S.tgvs_free(p)
+    return out
 
 
 def balance(cubes, r_root=R_ROOT):
