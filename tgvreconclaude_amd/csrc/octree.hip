@@ -212,9 +212,11 @@ struct Table {
     uint64_t mask;
 };
 __global__ void k_table_insert(Table t, const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth,
-                               const uint64_t* __restrict__ keys, const uint64_t* __restrict__ ts, uint64_t n) {
+                               const uint64_t* __restrict__ keys, const uint64_t* __restrict__ ts, uint64_t n,
+                               int min_depth) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (!keys && (int)depth[i] < min_depth) return;
     const uint64_t k = keys ? keys[i] : cell_key((int)depth[i], code[i]);
     for (uint64_t h = key_hash(k) & t.mask;; h = (h + 1) & t.mask) {
         const unsigned long long prev = atomicCAS((unsigned long long*)&t.key[h], 0ull, (unsigned long long)k);
@@ -234,48 +236,129 @@ __device__ inline bool table_find(const Table& t, uint64_t k, uint64_t* ts) {
         if (x == 0ull) return false;
     }
 }
-// the cell exists at time tau: a seed, or a bridge stamped before tau
-__device__ inline bool has_at(const Table& s0, const Table& br, int d, uint64_t c, uint64_t tau) {
-    const uint64_t k = cell_key(d, c);
-    if (table_find(s0, k, nullptr)) return true;
+// Cells of depth <= kBitDepth are also kept as bits, one per cell of their
+// level (Morton index), levels packed coarse to fine (19 MB): the check's
+// walk over the coarser ancestors reads bits instead of probing the tables,
+// and the seed table holds only the deeper cells.
+constexpr int kBitDepth = 9;
+constexpr uint64_t kBitWords = ((((1ull << (3 * (kBitDepth + 1))) - 1) / 7) + 31) / 32;
+__host__ __device__ inline uint64_t bit_of(int d, uint64_t c) {
+    return ((1ull << (3 * d)) - 1) / 7 + (c >> (3 * (21 - d)));
+}
+__device__ inline bool bit_set(const uint32_t* bits, uint64_t b) { return (bits[b >> 5] >> (b & 31)) & 1u; }
+// bits of the n cells (code, depth) or keys with depth <= kBitDepth
+__global__ void k_bits_set(uint32_t* __restrict__ bits, const uint64_t* __restrict__ code,
+                           const uint32_t* __restrict__ depth, const uint64_t* __restrict__ keys, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d;
+    uint64_t c;
+    if (keys) {
+        cell_of(keys[i], d, c);
+    } else {
+        d = (int)depth[i];
+        c = code[i];
+    }
+    if (d > kBitDepth) return;
+    const uint64_t b = bit_of(d, c);
+    atomicOr(&bits[b >> 5], 1u << (b & 31));
+}
+struct Cells {
+    Table s0, br;               // seeds deeper than kBitDepth; all bridges with their stamps
+    const uint32_t* s0bits;     // seeds of depth <= kBitDepth
+    const uint32_t* brbits;     // bridges of depth <= kBitDepth
+};
+// the cell is a seed
+__device__ inline bool seed_has(const Cells& t, int d, uint64_t c) {
+    if (d <= kBitDepth) return bit_set(t.s0bits, bit_of(d, c));
+    return table_find(t.s0, cell_key(d, c), nullptr);
+}
+// the cell is a bridge stamped before tau
+__device__ inline bool bridge_has(const Cells& t, int d, uint64_t c, uint64_t tau) {
+    if (t.br.mask == 0) return false;
+    if (d <= kBitDepth && !bit_set(t.brbits, bit_of(d, c))) return false;
     uint64_t ts;
-    return br.mask != 0 && table_find(br, k, &ts) && ts < tau;
+    return table_find(t.br, cell_key(d, c), &ts) && ts < tau;
 }
 
 // One thread per queued cube k (seeds, then bridges by position): the six
 // checks of balance.cpp:93-123 at times 1 + 6k + f; an insertion is recorded
-// as (cell key, time).
-__global__ void k_balance_checks(const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth, uint64_t nseed,
-                                 const uint64_t* __restrict__ bkey, uint64_t nbr, Table s0, Table br,
-                                 uint64_t* __restrict__ dkey, uint64_t* __restrict__ dts,
-                                 unsigned long long* __restrict__ count, uint64_t cap) {
+// as (cell key, time). A check inserts iff neither the same-level neighbour
+// (A) nor its parent one level coarser (B) exists and a cell two or more
+// levels coarser covers the neighbour (C), each "exists" meaning a seed or an
+// earlier bridge. The seeds' part never changes: the first round stores it
+// per seed and face in seed_bits (4 bits per face: no neighbour, A, B, C by
+// seeds) and later rounds only look the bridges up.
+__global__ void __launch_bounds__(256) k_balance_checks(
+    const uint64_t* __restrict__ code, const uint32_t* __restrict__ depth, uint64_t nseed,
+    const uint64_t* __restrict__ bkey, uint64_t nbr, Cells cells, uint32_t* __restrict__ seed_bits, int first_round,
+    uint64_t* __restrict__ dkey, uint64_t* __restrict__ dts, unsigned long long* __restrict__ count, uint64_t cap) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nseed + nbr) return;
-    int d;
-    uint64_t cv;
-    if (k < nseed) {
+    int d = 0;
+    uint64_t cv = 0;
+    const bool seed = k < nseed;
+    if (seed) {
         d = (int)depth[k];
         cv = code[k];
-    } else {
+    } else if (k < nseed + nbr) {
         cell_of(bkey[k - nseed], d, cv);
     }
-    if (d < 2) return;
-    for (int f = 0; f < 6; ++f) {
-        const uint64_t tau = 1 + 6 * k + (uint64_t)f;
-        const uint64_t nb = face_nb(cv, f, d);
-        if (nb == kNone) continue;
-        if (has_at(s0, br, d, nb, tau)) continue;  // same-level neighbour
-        const uint64_t par = ancestor(nb, d - 1);
-        if (has_at(s0, br, d - 1, par, tau)) continue;  // one level coarser
-        bool viol = false;
-        for (int d2 = d - 2; d2 >= 0 && !viol; --d2) viol = has_at(s0, br, d2, ancestor(nb, d2), tau);
-        if (!viol) continue;
-        const unsigned long long slot = atomicAdd(count, 1ull);
-        if (slot < cap) {
-            dkey[slot] = cell_key(d - 1, par);
-            dts[slot] = tau;
+    // this cube's insertions (at most one per face), appended below with one
+    // atomic per wavefront: a single counter hit by every lane serialises
+    uint64_t ins_key[6], ins_ts[6];
+    int nins = 0;
+    if (d >= 2) {
+        const bool known = seed && !first_round;
+        const uint32_t old_bits = known ? seed_bits[k] : 0u;
+        uint32_t bits = 0;
+        for (int f = 0; f < 6; ++f) {
+            uint32_t m;  // bit 0 no neighbour, 1 A, 2 B, 3 C, by seeds
+            const uint64_t nb = face_nb(cv, f, d);
+            const uint64_t par = nb == kNone ? 0 : ancestor(nb, d - 1);
+            if (known) {
+                m = (old_bits >> (4 * f)) & 15u;
+            } else {
+                m = nb == kNone ? 1u : 0u;
+                if (!m && seed_has(cells, d, nb)) m |= 2u;
+                if (!m && seed_has(cells, d - 1, par)) m |= 4u;
+                if (!m)
+                    for (int d2 = d - 2; d2 >= 0; --d2)
+                        if (seed_has(cells, d2, ancestor(nb, d2))) {
+                            m |= 8u;
+                            break;
+                        }
+            }
+            bits |= m << (4 * f);
+            if (m & 7u) continue;  // no neighbour, or A / B by a seed: never an insertion
+            const uint64_t tau = 1 + 6 * k + (uint64_t)f;
+            if (bridge_has(cells, d, nb, tau) || bridge_has(cells, d - 1, par, tau)) continue;
+            bool viol = m & 8u;
+            for (int d2 = d - 2; d2 >= 0 && !viol; --d2) viol = bridge_has(cells, d2, ancestor(nb, d2), tau);
+            if (!viol) continue;
+            ins_key[nins] = cell_key(d - 1, par);
+            ins_ts[nins] = tau;
+            ++nins;
         }
+        if (seed && first_round) seed_bits[k] = bits;
     }
+    // wavefront prefix sum of the counts, one atomic per wavefront
+    const int lane = (int)(threadIdx.x & 63u);
+    int incl = nins;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    const int total = __shfl(incl, 63, 64);
+    unsigned long long base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(count, (unsigned long long)total);
+    base = __shfl(base, 63, 64);
+    const unsigned long long first = base + (unsigned long long)(incl - nins);
+    for (int i = 0; i < nins; ++i)
+        if (first + i < cap) {
+            dkey[first + i] = ins_key[i];
+            dts[first + i] = ins_ts[i];
+        }
 }
 // after sorting by (key, time): 1 at the first (earliest) record of a key
 __global__ void k_key_heads(const uint64_t* __restrict__ key, uint64_t n, uint8_t* __restrict__ head) {
@@ -336,7 +419,8 @@ uint64_t pow2_at_least(uint64_t x) {
 // Scratch buffers of the octree calls, by role.
 enum {
     B_CODE = 0, B_DEPTH, B_RAD, B_FLAG, B_IDX, B_IDX2, B_KEY32, B_KEY32S, B_CODES, B_HEAD, B_START, B_COUNT,
-    B_TMP, B_S0, B_BRK, B_BRT, B_DKEY, B_DTS, B_DKEY2, B_DTS2, B_NKEY, B_NTS, B_PKEY, B_PTS
+    B_TMP, B_S0, B_BRK, B_BRT, B_DKEY, B_DTS, B_DKEY2, B_DTS2, B_NKEY, B_NTS, B_PKEY, B_PTS, B_S0BITS, B_BRBITS,
+    B_SEEDBITS
 };
 
 // stable sort of (keys, idx) pairs in place through the alternate buffers
@@ -387,9 +471,16 @@ uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, ui
     T[B_S0].ensure(8 * cap0);
     TGVX_HIP(hipMemsetAsync(T[B_S0].p, 0, 8 * cap0, st));
     Table s0{T[B_S0].as<uint64_t>(), nullptr, cap0 - 1};
-    if (nseed)
+    T[B_SEEDBITS].ensure(4 * std::max<uint64_t>(nseed, 1));
+    T[B_S0BITS].ensure(4 * kBitWords);
+    T[B_BRBITS].ensure(4 * kBitWords);
+    TGVX_HIP(hipMemsetAsync(T[B_S0BITS].p, 0, 4 * kBitWords, st));
+    if (nseed) {
         k_table_insert<<<grid_of(nseed), 256, 0, st>>>(s0, code.as<uint64_t>(), depth.as<uint32_t>(), nullptr, nullptr,
-                                                        nseed);
+                                                        nseed, kBitDepth + 1);
+        k_bits_set<<<grid_of(nseed), 256, 0, st>>>(T[B_S0BITS].as<uint32_t>(), code.as<uint64_t>(),
+                                                   depth.as<uint32_t>(), nullptr, nseed);
+    }
     uint64_t nbr = 0;       // bridges of the previous round
     uint64_t dcap = std::max<uint64_t>(4096, nseed / 2);
     T[B_COUNT].ensure(16);
@@ -398,6 +489,7 @@ uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, ui
     for (;;) {
         if (++r > 100000) throw Error(TGVX_E_HIP, "balance rounds did not settle");
         Table br{T[B_BRK].as<uint64_t>(), T[B_BRT].as<uint64_t>(), 0};
+        TGVX_HIP(hipMemsetAsync(T[B_BRBITS].p, 0, 4 * kBitWords, st));
         if (nbr) {
             const uint64_t capb = pow2_at_least(2 * nbr);
             T[B_BRK].ensure(8 * capb);
@@ -405,10 +497,14 @@ uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, ui
             TGVX_HIP(hipMemsetAsync(T[B_BRK].p, 0, 8 * capb, st));
             br = Table{T[B_BRK].as<uint64_t>(), T[B_BRT].as<uint64_t>(), capb - 1};
             k_table_insert<<<grid_of(nbr), 256, 0, st>>>(br, nullptr, nullptr, T[B_PKEY].as<uint64_t>(),
-                                                          T[B_PTS].as<uint64_t>(), nbr);
+                                                          T[B_PTS].as<uint64_t>(), nbr, 0);
+            k_bits_set<<<grid_of(nbr), 256, 0, st>>>(T[B_BRBITS].as<uint32_t>(), nullptr, nullptr,
+                                                     T[B_PKEY].as<uint64_t>(), nbr);
         }
-        // this round's checks
+        const Cells cells{s0, br, T[B_S0BITS].as<uint32_t>(), T[B_BRBITS].as<uint32_t>()};
+        // this round's checks (a repeat of the first round stores the seed bits again)
         uint64_t nd = 0;
+        const int first_round = r == 1;
         for (;;) {
             T[B_DKEY].ensure(8 * dcap);
             T[B_DTS].ensure(8 * dcap);
@@ -416,7 +512,8 @@ uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, ui
             const uint64_t items = nseed + nbr;
             if (items)
                 k_balance_checks<<<grid_of(items), 256, 0, st>>>(
-                    code.as<uint64_t>(), depth.as<uint32_t>(), nseed, T[B_PKEY].as<uint64_t>(), nbr, s0, br,
+                    code.as<uint64_t>(), depth.as<uint32_t>(), nseed, T[B_PKEY].as<uint64_t>(), nbr, cells,
+                    T[B_SEEDBITS].as<uint32_t>(), first_round,
                     T[B_DKEY].as<uint64_t>(), T[B_DTS].as<uint64_t>(), T[B_COUNT].as<unsigned long long>(), dcap);
             TGVX_HIP(hipGetLastError());
             TGVX_HIP(hipMemcpyAsync(&nd, T[B_COUNT].p, 8, hipMemcpyDeviceToHost, st));
@@ -431,14 +528,17 @@ uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, ui
             T[B_DTS2].ensure(8 * nd);
             DevBuf& tmp = T[B_TMP];
             size_t bytes = 0;
+            // times are below 6 (nseed + nbr) + 7: only their low bits are sorted
+            int ts_bits = 1;
+            while (ts_bits < 64 && (1ull << ts_bits) <= 6 * (nseed + nbr) + 7) ++ts_bits;
             // by time, then (stably) by key: each key's records in time order
             TGVX_HIP(rocprim::radix_sort_pairs(nullptr, bytes, T[B_DTS].as<uint64_t>(), T[B_DTS2].as<uint64_t>(),
                                                T[B_DKEY].as<uint64_t>(), T[B_DKEY2].as<uint64_t>(), (size_t)nd, 0,
-                                               64, st));
+                                               ts_bits, st));
             tmp.ensure(bytes);
             TGVX_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, T[B_DTS].as<uint64_t>(), T[B_DTS2].as<uint64_t>(),
                                                T[B_DKEY].as<uint64_t>(), T[B_DKEY2].as<uint64_t>(), (size_t)nd, 0,
-                                               64, st));
+                                               ts_bits, st));
             bytes = 0;
             TGVX_HIP(rocprim::radix_sort_pairs(nullptr, bytes, T[B_DKEY2].as<uint64_t>(), T[B_DKEY].as<uint64_t>(),
                                                T[B_DTS2].as<uint64_t>(), T[B_DTS].as<uint64_t>(), (size_t)nd, 0,
@@ -462,11 +562,11 @@ uint64_t balance_device(tgvx_ctx* c, DevBuf& code, DevBuf& depth, DevBuf& rc, ui
             bytes = 0;
             TGVX_HIP(rocprim::radix_sort_pairs(nullptr, bytes, T[B_DTS2].as<uint64_t>(), T[B_NTS].as<uint64_t>(),
                                                T[B_DKEY2].as<uint64_t>(), T[B_NKEY].as<uint64_t>(), (size_t)nnew, 0,
-                                               64, st));
+                                               ts_bits, st));
             tmp.ensure(bytes);
             TGVX_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, T[B_DTS2].as<uint64_t>(), T[B_NTS].as<uint64_t>(),
                                                T[B_DKEY2].as<uint64_t>(), T[B_NKEY].as<uint64_t>(), (size_t)nnew, 0,
-                                               64, st));
+                                               ts_bits, st));
         }
         // settled when the round reproduced the bridges it started from
         bool same = nnew == nbr;

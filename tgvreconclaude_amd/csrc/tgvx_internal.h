@@ -133,7 +133,7 @@ struct tgvx_ctx {
     tgvx::DevBuf srf_verts, srf_tris, srf_tmp[8];
     uint64_t srf_nv = 0, srf_nt = 0;
     // last octree build / balance: the cubes (tgvx_cube records) and scratch
-    tgvx::DevBuf oct_out, oct_tmp[24];
+    tgvx::DevBuf oct_out, oct_tmp[32];
     uint64_t oct_n = 0;
     // launch configurations chosen by tgvx_autotune
     std::map<std::tuple<int, int, int>, tgvx::TuneEntry> tuned;
