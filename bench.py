@@ -37,7 +37,7 @@ ELEM = {"fp32": 4, "bf16": 2, "fp64": 8}
 # figure is the work done, not the traffic.
 ALG_BYTES = {k: 34 * e + 8 + 24 for k, e in ELEM.items()}
 KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj",
-                3: "k_fused_tile (p, q staged in LDS)", 4: "k_fused_halo"}
+                3: "k_fused_tile (p, q staged in LDS)", 4: "k_fused_halo", 5: "k_fused2"}
 
 
 def log(msg):
@@ -57,10 +57,10 @@ def main():
     ap.add_argument("--iters", type=int, default=50, help="iterations per level in one step")
     ap.add_argument("--block", type=int, default=0, help="workgroup / tile size (0: the library default)")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4], default=1,
+    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5], default=1,
                     help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "
                          "kernel; 3: LDS-tiled with the tile's p, q staged in LDS; 4: LDS-tiled with a "
-                         "tabulated halo")
+                         "tabulated halo; 5: two iterations per launch over tabulated tile sets")
     ap.add_argument("--no-fused", action="store_true", help="same as --fused 0")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start time-to-gap runs")
     ap.add_argument("--no-octree", action="store_true",
@@ -188,8 +188,13 @@ def main():
     if tuning:
         variant = tuning[-1]["variant"]
     fused = variant != 0
+    # fused_ms is per iteration; the two-iteration kernel's launch is two
+    two = variant == 5 and ctx.tile_sets(finest, tuning[-1]["block"] if tuning else (args.block or 256))[0]
+    if variant == 5 and not two:
+        variant = 1  # the level's tile sets do not fit: one-iteration tiled passes
     it_ms = fused_ms if fused else dual_ms + primal_ms
-    kernel = f"{KERNEL_NAMES[variant]}, depth {finest}, {n_f} leaves" + (" (one launch = one iteration)" if fused else "")
+    kernel = f"{KERNEL_NAMES[variant]}, depth {finest}, {n_f} leaves" + (
+        " (one launch = two iterations)" if two else (" (one launch = one iteration)" if fused else ""))
     if tuning:
         kernel += f", workgroup {tuning[-1]['block']}, {'XCD-contiguous' if tuning[-1]['xcd'] else 'plain'} tile order"
     ab = ALG_BYTES[args.precision]
@@ -263,7 +268,7 @@ def main():
     pj = os.path.join(ROOT, "profiles", "tgv_iteration_depth10_torus_box.json")
     if args.precision == "fp32" and os.path.exists(pj):
         name = KERNEL_NAMES[variant]
-        if variant in (1, 3, 4):
+        if variant in (1, 3, 4, 5):
             name += f", tile {tuning[-1]['block'] if tuning else (args.block or 256)}"
         rec = json.load(open(pj)).get(name)
         if rec and rec["leaves"] == n_f:

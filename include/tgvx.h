@@ -101,8 +101,10 @@ typedef struct {
                               iteration sharing a tile's primal values through LDS; 2 one fused
                               kernel recomputing every neighbour's primal values; 3 as 1 with the
                               tile's p, q staged in LDS first; 4 as 1 with the out-of-tile
-                              neighbours tabulated once per level and tile. For 1, 3 and 4, block
-                              is the tile (128, 256 = default, 512, 1024) */
+                              neighbours tabulated once per level and tile; 5 two iterations
+                              per launch over tile sets tabulated once per level and tile (a
+                              level whose sets do not fit runs as 1). For 1, 3, 4 and 5, block
+                              is the tile (128, 256 = default, 512, and 1024 except for 5) */
     int32_t tune;          /* 1: take each level's launch configuration from tgvx_autotune's table
                               (levels without an entry use fused / block as given) */
     float sigma, tau;      /* primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150).
@@ -197,6 +199,17 @@ int tgvx_problem_levels(tgvx_ctx* ctx, int32_t* depths, uint32_t* counts, int32_
 int tgvx_level_bench(tgvx_ctx* ctx, int32_t depth, const tgvx_params* params, int32_t iters,
                      double* total_ms, double* dual_ms, double* primal_ms, double* fused_ms);
 
+/* Tile sets of the two-iteration kernel (params.fused = 5; no reference
+ * counterpart) for the level at `depth` of the resident problem (exact-adjoint
+ * mode) and a tile of 128, 256 or 512 cubes, built if not yet built: usable = 1
+ * when every tile's sets fit (otherwise that level runs one iteration per
+ * launch), max_sets = the largest PA, DB, PC over the tiles (the cubes the
+ * launch's first primal step, its inner dual step and its second primal step
+ * cover), lds_bytes = the kernel's LDS per workgroup. With params.fused = 5,
+ * fused_ms of tgvx_level_bench is per iteration (half a launch). */
+int tgvx_tile_sets(tgvx_ctx* ctx, int32_t depth, int32_t tile, int32_t* usable, int32_t* max_sets,
+                   int64_t* lds_bytes);
+
 /* ---- iso-surface extraction (marching_cubes.cpp:63-181 extract_surface), on the device
  * Over the resident problem's cubes with u = the given field (n floats in input
  * order) or, when u is NULL, the problem's current state; has_data NULL or n
@@ -250,9 +263,9 @@ int tgvx_octree_copy(tgvx_ctx* ctx, tgvx_cube* out);
 
 /* ---- launch autotuner (no reference counterpart)
  * For every level of the loaded problem, times `iters` iterations of each
- * launch configuration -- kernel variant (dual + primal kernels, LDS-tiled
- * fused, untiled fused; exact-adjoint mode only for the fused ones),
- * workgroup / tile size (64, 128, 256) and tile order (XCD-contiguous or
+ * launch configuration -- kernel variant (dual + primal kernels, the fused
+ * variants 1-5 of tgvx_params.fused; exact-adjoint mode only for the fused
+ * ones), workgroup / tile size (64 to 1024) and tile order (XCD-contiguous or
  * plain) -- and stores the fastest in the context, per (precision, operator
  * mode, floor(log2 leaves)); solves with params.tune = 1 use it. Every
  * configuration gives identical results. The problem's state is overwritten
@@ -262,7 +275,7 @@ typedef struct {
     int32_t level_depth[TGVX_MAX_LEVELS];
     uint32_t level_leaves[TGVX_MAX_LEVELS];
     int32_t variant[TGVX_MAX_LEVELS]; /* 0 two kernels, 1 tiled fused, 2 untiled fused, 3 tiled + LDS p, q,
-                                         4 tiled + tabulated halo */
+                                         4 tiled + tabulated halo, 5 two iterations per launch */
     int32_t block[TGVX_MAX_LEVELS];
     int32_t xcd[TGVX_MAX_LEVELS];
     double best_us[TGVX_MAX_LEVELS];    /* per iteration, chosen configuration */

@@ -144,7 +144,7 @@ def test_cold_dual_recovery_to_gap(ctx, sphere):
     assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("prec", [FP32, FP64])
 def test_fused_iteration_identical_to_two_kernels(ctx, sphere, prec, variant):
     # exact-adjoint mode: one fused kernel per iteration (LDS-tiled or not) ==
@@ -159,7 +159,7 @@ def test_fused_iteration_identical_to_two_kernels(ctx, sphere, prec, variant):
                           b.view(np.uint32 if b.dtype == np.float32 else np.uint64))
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 def test_fused_bf16_matches_two_kernels(ctx, sphere, variant):
     sc, hist = sphere
     a, _ = ctx.solve(sc.cubes, hist, TGVParams(alpha0=2.0, alpha1=1.0, iters=40, precision=BF16, op_mode=OP_ADJ,
@@ -189,7 +189,7 @@ def test_parameter_change_on_resident_problem(ctx, sphere):
     assert np.array_equal(_state13(ctx.state()).view(np.uint32), _state13(first).view(np.uint32))
 
 
-@pytest.mark.parametrize("fused", [0, 1, 4])
+@pytest.mark.parametrize("fused", [0, 1, 4, 5])
 def test_to_gap_adj_large_step_ratio(ctx, sphere, fused):
     # sigma / tau = 1024 (sigma = 8, tau = 1/128, sigma * tau = 1/16): the
     # same iteration with other steps, bit-exact with the oracle, counts equal

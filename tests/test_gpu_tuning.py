@@ -19,7 +19,8 @@ def _raw(st):
 
 
 @pytest.mark.parametrize("fused,block", [(0, 64), (0, 128), (1, 128), (1, 512), (1, 1024), (2, 64), (2, 128),
-                                         (3, 128), (3, 256), (3, 1024), (4, 128), (4, 256), (4, 512), (4, 1024)])
+                                         (3, 128), (3, 256), (3, 1024), (4, 128), (4, 256), (4, 512), (4, 1024),
+                                         (5, 128), (5, 256), (5, 512)])
 def test_launch_shapes_identical(ctx, sphere, fused, block):
     sc, hist = sphere
     base = dict(alpha0=2.0, alpha1=1.0, iters=60, precision=FP32, op_mode=OP_ADJ)
@@ -36,10 +37,12 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
     levels = ctx.autotune(p, iters=10)
     assert [(l["depth"], l["leaves"]) for l in levels] == [tuple(x) for x in ctx.levels()]
     for l in levels:
-        assert l["variant"] in ((0, 1, 2, 3, 4) if mode == OP_ADJ else (0,))
+        assert l["variant"] in ((0, 1, 2, 3, 4, 5) if mode == OP_ADJ else (0,))
         assert l["block"] in (64, 128, 256, 512, 1024) and l["xcd"] in (0, 1)
         if l["variant"] in (1, 3, 4):
             assert l["block"] in (128, 256, 512, 1024)
+        elif l["variant"] == 5:
+            assert l["block"] in (128, 256, 512)
         else:
             assert l["block"] <= 256
         assert 0 < l["best_us"] <= l["default_us"]
@@ -49,3 +52,40 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
     ctx.run(p)
     tuned = ctx.state()
     assert np.array_equal(_raw(tuned), _raw(plain))
+
+
+@pytest.mark.parametrize("prec,tile,graph", [(FP32, 128, 1), (FP32, 256, 1), (FP32, 512, 1), (FP32, 256, 0),
+                                             (BF16, 256, 1), (FP64, 128, 1)])
+def test_two_iteration_kernel(ctx, sphere, prec, tile, graph):
+    # fused = 5: two iterations per launch over the tabulated tile sets; the
+    # finest level's sets fit (so the kernel runs there), and the state equals
+    # the two-kernel iteration's bit for bit, for odd and even pass counts
+    sc, hist = sphere
+    ctx.load(sc.cubes, hist, precision=prec, op_mode=OP_ADJ)
+    for depth, n in ctx.levels():
+        ok, (sa, sd, sp), lds = ctx.tile_sets(depth, tile)
+        if ok:
+            assert sa >= sd >= sp >= tile and sa <= 4 * tile and 0 < lds <= 160 * 1024
+    assert ctx.tile_sets(ctx.levels()[-1][0], tile)[0]
+    for iters in (61, 60):
+        base = dict(alpha0=2.0, alpha1=1.0, iters=iters, precision=prec, op_mode=OP_ADJ, use_graph=graph)
+        ref, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=0, block=256, **base))
+        got, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=5, block=tile, **base))
+        assert np.array_equal(_raw(got), _raw(ref))
+
+
+def test_two_iteration_level_bench(ctx, sphere):
+    # the level bench times the two-iteration launch per iteration and leaves
+    # the state where the same number of iterations of any variant leaves it
+    sc, hist = sphere
+    depth = None
+    outs = []
+    for fused in (5, 0):
+        ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+        p = TGVParams(alpha0=2.0, alpha1=1.0, iters=40, precision=FP32, op_mode=OP_ADJ, fused=fused, block=256)
+        ctx.run(p)
+        depth = ctx.levels()[-1][0]
+        total, dual, primal, fz = ctx.level_bench(depth, p, 20)
+        assert total > 0 and dual > 0 and primal > 0 and fz > 0
+        outs.append(ctx.state())
+    assert np.array_equal(_raw(outs[0]), _raw(outs[1]))

@@ -361,6 +361,14 @@ class Context:
                                              C.byref(p), C.byref(f)))
         return t.value, d.value, p.value, f.value
 
+    def tile_sets(self, depth, tile):
+        """The two-iteration kernel's tile sets at `depth` (exact-adjoint problem):
+        (usable, (largest PA, DB, PC), LDS bytes per workgroup)."""
+        ok, lds = C.c_int32(), C.c_int64()
+        sets = (C.c_int32 * 3)()
+        self._check(self._L.tgvx_tile_sets(self.h, depth, tile, C.byref(ok), sets, C.byref(lds)))
+        return bool(ok.value), tuple(sets), lds.value
+
 
 _default_ctx = None
 

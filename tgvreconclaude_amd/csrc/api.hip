@@ -155,6 +155,14 @@ int tgvx_level_bench(tgvx_ctx* c, int32_t depth, const tgvx_params* p, int32_t i
     });
 }
 
+int tgvx_tile_sets(tgvx_ctx* c, int32_t depth, int32_t tile, int32_t* usable, int32_t* max_sets,
+                   int64_t* lds_bytes) {
+    return guarded(c, [&] {
+        if (!usable || !max_sets || !lds_bytes) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        tgvx::tile_sets(c, depth, tile, usable, max_sets, lds_bytes);
+    });
+}
+
 int tgvx_mlp_set(tgvx_ctx* c, const float* w) {
     return guarded(c, [&] {
         if (!w) throw tgvx::Error(TGVX_E_ARG, "null pointer");

@@ -744,6 +744,443 @@ __global__ void __launch_bounds__(TILE) k_fused_halo(KArgs<S> a, Set13<S> src, S
     for (int k = 0; k < 6; ++k) IO::stn(dst.f[F_Q0 + k], g, q[k]);
 }
 
+// ---- two iterations per launch (no reference counterpart). A tile T of
+// consecutive positions advances by two exact-adjoint iterations in one
+// launch: primal n+1 over PA, dual n+2 over DB, primal n+2 over PC and dual
+// n+3 over T, where, within the level,
+//   PC = T and the +a neighbours of T (the u_bar, v_bar the last dual reads),
+//   DB = PC and the cubes whose +a neighbour lies in PC (the p, q that PC's
+//        primal step reads),
+//   PA = DB and the +a neighbours of DB.
+// Every intermediate value stays in LDS, so a cube's own state is read and
+// written once per two iterations. The sets are tabulated once per level and
+// tile size (k_halo2_table): slot s < TILE is the tile's cube s, slots from
+// TILE on are the other members, numbered so that PC, DB and PA are the
+// prefixes [0, nPC), [0, nDB) and [0, nPA) of the slots. Each value is
+// computed by the expressions of k_primal_adj and k_dual, so the results are
+// bit-identical to every other launch variant.
+constexpr int kHalo2Cap = 4;  // slots per tile: at most kHalo2Cap * TILE
+constexpr int kHalo2Lds = 160 * 1024;  // LDS budget of k_fused2 per workgroup (gfx950: 160 KiB)
+
+__host__ __device__ constexpr int log2_ceil(int x) {
+    int l = 0;
+    while ((1 << l) < x) ++l;
+    return l;
+}
+
+struct Halo2Tab {
+    const int32_t* cnt;    // per tile: nPC, nDB, nPA, failed (the tile's sets exceed the cap)
+    const int32_t* list;   // per tile, (kHalo2Cap - 1) * TILE: position of slot TILE + k
+    const int32_t* bslot;  // per tile, 3 x kHalo2Cap * TILE: the +a source of slot s < nDB:
+                           // -1 the cube itself, >= 0 a slot, <= -2 the cube -2 - v of
+                           // another level (its frozen u_bar, v_bar)
+    const int32_t* cslot;  // per tile, 3 x kHalo2Cap * TILE: the cube whose +a neighbour
+                           // slot s < nPA is: -1 none, >= 0 its slot, <= -2 the position
+                           // -2 - v outside the tile's sets (never for s < nPC)
+};
+
+// One workgroup per tile. nba = depth-aware +a neighbours, nbt = their
+// transpose within the level; [off, off + n) must be the whole level. Members
+// outside the tile are collected in an LDS hash set (a member's slot is its
+// insertion rank, so the numbering depends on scheduling; the results do not).
+template <int TILE>
+__global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict__ nba0,
+                                                      const int32_t* __restrict__ nba1,
+                                                      const int32_t* __restrict__ nba2,
+                                                      const int32_t* __restrict__ nbt0,
+                                                      const int32_t* __restrict__ nbt1,
+                                                      const int32_t* __restrict__ nbt2, uint32_t off, uint32_t n,
+                                                      int32_t* __restrict__ cnt, int32_t* __restrict__ list,
+                                                      int32_t* __restrict__ bslot, int32_t* __restrict__ cslot) {
+    constexpr int CA = kHalo2Cap * TILE, NH = CA - TILE;
+    constexpr int LH = log2_ceil(2 * NH), H = 1 << LH;
+    __shared__ int32_t hkey[H], hval[H], lst[NH];
+    __shared__ int32_t nh, failed, cP, cD;
+    const int32_t* const nba[3] = {nba0, nba1, nba2};
+    const int32_t* const nbt[3] = {nbt0, nbt1, nbt2};
+    const uint32_t tile = blockIdx.x;
+    const int tid = (int)threadIdx.x;
+    const long lo = (long)off + (long)tile * TILE;
+    const int cntT = (int)min((uint32_t)TILE, n - tile * TILE);
+    const long hi = lo + cntT, lvl_hi = (long)off + n;
+    for (int i = tid; i < H; i += TILE) hkey[i] = -1;
+    if (tid == 0) {
+        nh = 0;
+        failed = 0;
+    }
+    __syncthreads();
+    auto in_tile = [&](long j) { return j >= lo && j < hi; };
+    auto in_level = [&](long j) { return j >= (long)off && j < lvl_hi; };
+    auto hash = [](int32_t j) { return ((uint32_t)j * 0x9E3779B1u) >> (32 - LH); };
+    auto insert = [&](int32_t j) {
+        uint32_t hp = hash(j);
+        for (int it = 0; it < H; ++it) {
+            const int32_t prev = atomicCAS(&hkey[hp], -1, j);
+            if (prev == -1) {
+                const int k = atomicAdd(&nh, 1);
+                if (k < NH) {
+                    lst[k] = j;
+                    hval[hp] = TILE + k;
+                } else {
+                    hval[hp] = -1;
+                    failed = 1;
+                }
+                return;
+            }
+            if (prev == j) return;
+            hp = (hp + 1u) & (uint32_t)(H - 1);
+        }
+        failed = 1;
+    };
+    auto find = [&](int32_t j) -> int32_t {
+        uint32_t hp = hash(j);
+        for (int it = 0; it < H; ++it) {
+            const int32_t k = hkey[hp];
+            if (k == j) return hval[hp];
+            if (k == -1) break;
+            hp = (hp + 1u) & (uint32_t)(H - 1);
+        }
+        return -1;
+    };
+    auto lookup = [&](int32_t j) -> int32_t {
+        const int32_t v = find(j);
+        if (v < 0) failed = 1;
+        return v;
+    };
+    auto pos = [&](int s) -> long { return s < TILE ? lo + s : (long)lst[s - TILE]; };
+    auto valid = [&](int s) { return !(s >= cntT && s < TILE); };
+    // PC: the tile's +a neighbours in the level
+    if (tid < cntT) {
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int32_t j = nba[ax][lo + tid];
+            if (j >= 0 && in_level(j) && !in_tile(j)) insert(j);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) cP = TILE + min((int)nh, NH);
+    __syncthreads();
+    const int nPC = cP;
+    // DB: the cubes whose +a neighbour is in PC (always in the level)
+    for (int s = tid; s < nPC; s += TILE) {
+        if (!valid(s)) continue;
+        const long g = pos(s);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int32_t k = nbt[ax][g];
+            if (k >= 0 && !in_level(k)) failed = 1;
+            else if (k >= 0 && !in_tile(k)) insert(k);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) cD = TILE + min((int)nh, NH);
+    __syncthreads();
+    const int nDB = cD;
+    // PA: the +a neighbours of DB in the level
+    for (int s = tid; s < nDB; s += TILE) {
+        if (!valid(s)) continue;
+        const long g = pos(s);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int32_t j = nba[ax][g];
+            if (j >= 0 && in_level(j) && !in_tile(j)) insert(j);
+        }
+    }
+    __syncthreads();
+    const int nPA = TILE + min((int)nh, NH);
+    int32_t* const L = list + (long)tile * NH;
+    for (int k = tid; k < nPA - TILE; k += TILE) L[k] = lst[k];
+    int32_t* const B = bslot + (long)tile * 3 * CA;
+    int32_t* const C = cslot + (long)tile * 3 * CA;
+    for (int s = tid; s < nPA; s += TILE) {
+        if (!valid(s)) continue;
+        const long g = pos(s);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (s < nDB) {
+                const int32_t j = nba[ax][g];
+                int32_t v;
+                if (j < 0) v = -1;
+                else if (in_tile(j)) v = (int32_t)(j - lo);
+                else if (in_level(j)) v = lookup(j);
+                else v = -2 - j;
+                B[ax * CA + s] = v;
+            }
+            const int32_t k = nbt[ax][g];
+            int32_t v;
+            if (k < 0) v = -1;
+            else if (in_tile(k)) v = (int32_t)(k - lo);
+            else if (s < nPC) v = lookup(k);
+            else {
+                v = find(k);
+                if (v < 0) v = -2 - k;
+            }
+            C[ax * CA + s] = v;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        cnt[4 * tile] = nPC;
+        cnt[4 * tile + 1] = nDB;
+        cnt[4 * tile + 2] = nPA;
+        cnt[4 * tile + 3] = (failed || nh > NH) ? 1 : 0;
+    }
+}
+
+// The dynamic LDS of k_fused2: u_bar, v_bar and p, q of PA (sa slots), and
+// u, v of the PC slots from 2 * TILE on (sx slots; the first two slots of
+// each thread keep theirs in registers, as they keep their table entries,
+// info words and frozen neighbours).
+extern __shared__ uint4 k_fused2_lds[];
+
+// tiles: the tile ids this launch runs (the tiles of one LDS class).
+template <class S, int TILE, bool XCD = true>
+__global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13<S> dst, Halo2Tab h,
+                                                 const int32_t* __restrict__ tiles, int sa, int sx) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    constexpr int CA = kHalo2Cap * TILE, NH = CA - TILE;
+    T* const ub = reinterpret_cast<T*>(k_fused2_lds);  // [4][sa]
+    T* const pq = ub + 4 * sa;                         // [9][sa]: p[3], q[6]
+    T* const uvx = pq + 9 * sa;                        // [4][sx]
+    const uint32_t tile = (uint32_t)tiles[tile_id<XCD>()];
+    const int tid = (int)threadIdx.x;
+    const long lo = (long)a.off + (long)tile * TILE;
+    const int cntT = (int)min((uint32_t)TILE, a.n - tile * TILE);
+    const int nP = h.cnt[4 * tile], nD = h.cnt[4 * tile + 1], nA = h.cnt[4 * tile + 2];
+    const int32_t* const hl = h.list + (long)tile * NH;
+    const int32_t* const bs = h.bslot + (long)tile * 3 * CA;
+    const int32_t* const cs = h.cslot + (long)tile * 3 * CA;
+    auto live = [&](int s, long& g) {
+        if (s >= cntT && s < TILE) return false;
+        g = s < TILE ? lo + s : (long)hl[s - TILE];
+        return !masked(a, g);
+    };
+    auto frozen = [&](int v, T o[4]) {
+        const long j = -2L - v;
+        if (a.fzaos && (uint32_t)j - a.fz_off < a.fz_n) {
+            IO::ld4(a.fzaos, (long)((uint32_t)j - a.fz_off), o);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) o[c] = IO::ld(a.s[F_UB + c], j);
+        }
+    };
+    // own p, q of slot s into LDS
+    auto stage = [&](int s, long g) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pq[c * sa + s] = IO::ld(src.f[F_PX + c], g);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pq[(3 + k) * sa + s] = IO::ld(src.f[F_Q0 + k], g);
+    };
+    // passes 3-5 of slot s (the expressions of primal_adj_at): own p, q and
+    // the incoming ones from LDS (cv >= 0) or from src (cv <= -2, p, q of the
+    // first iteration only); u_bar, v_bar to LDS, the new u, v in nw
+    auto primal = [&](int s, uint32_t w, const T od[4], const int cv[3], T nw[4]) {
+        T p[3], qs[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = pq[c * sa + s];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qs[k] = pq[(3 + k) * sa + s];
+        T dp = T(0);
+        T dq[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const bool hs = InfoWord::has(w, ax);
+            dp = hs ? dp + p[ax] : dp;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T w2 = (c == ax) ? T(1) : T(0.5);
+                dq[c] = hs ? dq[c] + w2 * qs[qidx(c, ax) - F_Q0] : dq[c];
+            }
+        }
+        // incoming edge of axis ax: p_ax and the q entries of row ax of eps
+        constexpr int kF[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
+        T in[3][4];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int v = cv[ax];
+            if (v >= 0) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) in[ax][e] = pq[(kF[ax][e] - F_PX) * sa + v];
+            } else {
+                const long j = v <= -2 ? -2L - v : 0L;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) in[ax][e] = v <= -2 ? IO::ld(src.f[kF[ax][e]], j) : T(0);
+            }
+        }
+        if (cv[0] != -1) {
+            dp -= in[0][0];
+            dq[0] -= T(1) * in[0][1];
+            dq[1] -= T(0.5) * in[0][2];
+            dq[2] -= T(0.5) * in[0][3];
+        }
+        if (cv[1] != -1) {
+            dp -= in[1][0];
+            dq[0] -= T(0.5) * in[1][1];
+            dq[1] -= T(1) * in[1][2];
+            dq[2] -= T(0.5) * in[1][3];
+        }
+        if (cv[2] != -1) {
+            dp -= in[2][0];
+            dq[0] -= T(0.5) * in[2][1];
+            dq[1] -= T(0.5) * in[2][2];
+            dq[2] -= T(1) * in[2][3];
+        }
+        nw[0] = prox(od[0] + a.tau * dp, w, a.nbins, a.tl);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nw[1 + c] = od[1 + c] + a.tau * (p[c] + dq[c]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ub[c * sa + s] = IO::rt(T(2) * nw[c] - od[c]);
+    };
+    // dual n+2 of slot s; fz: the frozen neighbours already read (have_fz)
+    auto dual2 = [&](int s, const int bv[3], bool have_fz, const T (&fz)[3][4]) {
+        T bar[4][4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bar[0][c] = ub[c * sa + s];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int v = bv[ax];
+            if (v <= -2) {
+                if (have_fz) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) bar[1 + ax][c] = fz[ax][c];
+                } else {
+                    frozen(v, bar[1 + ax]);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) bar[1 + ax][c] = v == -1 ? bar[0][c] : ub[c * sa + v];
+            }
+        }
+        T p[3], q[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = pq[c * sa + s];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q[k] = pq[(3 + k) * sa + s];
+        dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pq[c * sa + s] = IO::rt(p[c]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pq[(3 + k) * sa + s] = IO::rt(q[k]);
+    };
+    auto load_uv = [&](long g, T o[4]) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) o[c] = IO::ld(src.f[F_U + c], g);
+    };
+    // ---- stage p, q of PA. Slots tid (0) and tid + TILE (1) keep their
+    // table entries, info words, u, v and (slot 0) frozen neighbours in
+    // registers.
+    const int s1 = tid + TILE;
+    long g0 = 0, g1 = 0;
+    const bool l0 = live(tid, g0);
+    const bool l1 = s1 < nA && live(s1, g1);
+    int b0[3] = {-1, -1, -1}, c0[3] = {-1, -1, -1}, b1[3] = {-1, -1, -1}, c1[3] = {-1, -1, -1};
+    uint32_t w0 = 0u, w1 = 0u;
+    T od0[4] = {T(0), T(0), T(0), T(0)}, od1[4] = {T(0), T(0), T(0), T(0)};
+    T fz0[3][4];
+    if (l0) {
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            b0[ax] = bs[ax * CA + tid];
+            c0[ax] = cs[ax * CA + tid];
+        }
+        w0 = a.info[g0];
+        load_uv(g0, od0);
+        stage(tid, g0);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+            if (b0[ax] <= -2) frozen(b0[ax], fz0[ax]);
+    }
+    if (l1) {
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) c1[ax] = cs[ax * CA + s1];
+        if (s1 < nD) {
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) b1[ax] = bs[ax * CA + s1];
+        }
+        w1 = a.info[g1];
+        load_uv(g1, od1);
+        stage(s1, g1);
+    }
+    for (int s = tid + 2 * TILE; s < nA; s += TILE) {
+        long g;
+        if (live(s, g)) stage(s, g);
+    }
+    __syncthreads();
+    // ---- primal n+1 over PA
+    T nw0[4] = {T(0), T(0), T(0), T(0)}, nw1[4] = {T(0), T(0), T(0), T(0)};
+    if (l0) primal(tid, w0, od0, c0, nw0);
+    if (l1) primal(s1, w1, od1, c1, nw1);
+    for (int s = tid + 2 * TILE; s < nA; s += TILE) {
+        long g;
+        if (!live(s, g)) continue;
+        const int cv[3] = {cs[s], cs[CA + s], cs[2 * CA + s]};
+        T od[4], nw[4];
+        load_uv(g, od);
+        primal(s, a.info[g], od, cv, nw);
+        if (s < nP) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) uvx[c * sx + s - 2 * TILE] = IO::rt(nw[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        nw0[c] = IO::rt(nw0[c]);  // the stored u, v of the first iteration
+        nw1[c] = IO::rt(nw1[c]);
+    }
+    __syncthreads();
+    // ---- dual n+2 over DB
+    if (l0) dual2(tid, b0, true, fz0);
+    if (l1 && s1 < nD) dual2(s1, b1, false, fz0);
+    for (int s = tid + 2 * TILE; s < nD; s += TILE) {
+        long g;
+        if (!live(s, g)) continue;
+        const int bv[3] = {bs[s], bs[CA + s], bs[2 * CA + s]};
+        dual2(s, bv, false, fz0);
+    }
+    __syncthreads();
+    // ---- primal n+2 over PC
+    T own[4] = {T(0), T(0), T(0), T(0)};
+    if (l0) primal(tid, w0, nw0, c0, own);
+    if (l1 && s1 < nP) {
+        T x[4];
+        primal(s1, w1, nw1, c1, x);
+    }
+    for (int s = tid + 2 * TILE; s < nP; s += TILE) {
+        long g;
+        if (!live(s, g)) continue;
+        const int cv[3] = {cs[s], cs[CA + s], cs[2 * CA + s]};
+        T od[4], x[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) od[c] = uvx[c * sx + s - 2 * TILE];
+        primal(s, a.info[g], od, cv, x);
+    }
+    __syncthreads();
+    // ---- dual n+3 over the tile
+    if (!l0) return;
+    T bar[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bar[0][c] = ub[c * sa + tid];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int v = b0[ax];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bar[1 + ax][c] = v <= -2 ? fz0[ax][c] : (v == -1 ? bar[0][c] : ub[c * sa + v]);
+    }
+    T p[3], q[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p[c] = pq[c * sa + tid];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = pq[(3 + k) * sa + tid];
+    dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::stn(dst.f[F_U + c], g0, own[c]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) IO::stn(dst.f[F_PX + c], g0, p[c]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) IO::stn(dst.f[F_Q0 + k], g0, q[k]);
+}
+
 // ---- streaming mode (tgv_minimize_streaming, tgv.cpp:515-1070)
 // The cube file's bisection (CubeFileReader::lookup, tgv.cpp:674-683): the
 // first probe whose code matches, not necessarily the last cube with it.
@@ -1331,14 +1768,18 @@ int block_of(const tgvx_params* p) {
     int b = p && p->block > 0 ? p->block : 256;
     if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
         throw Error(TGVX_E_ARG, "block must be 64, 128, 256, 512 or 1024");
-    const bool tiled = p && p->op_mode == TGVX_OP_ADJ && (p->fused == 1 || p->fused == 3 || p->fused == 4);
-    if (b > 256 && !tiled) throw Error(TGVX_E_ARG, "blocks above 256 exist only for the tiled fused kernel");
+    const bool tiled =
+        p && p->op_mode == TGVX_OP_ADJ && (p->fused == 1 || p->fused == 3 || p->fused == 4 || p->fused == 5);
+    if (b > 256 && !tiled) throw Error(TGVX_E_ARG, "blocks above 256 exist only for the tiled fused kernels");
+    if (p && p->op_mode == TGVX_OP_ADJ && p->fused == 5 && b == 1024)
+        throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256 or 512");
     return b;
 }
 
 // How one level's iterations are launched: variant 0 = dual + primal
 // kernels, 1 = LDS-tiled fused kernel, 2 = untiled fused kernel, 3 = tiled
-// with the tile's p, q staged in LDS, 4 = tiled with a precomputed halo;
+// with the tile's p, q staged in LDS, 4 = tiled with a precomputed halo,
+// 5 = two iterations per launch over tabulated tile sets;
 // block = the workgroup size (the tile size for variants 1, 3, 4); xcd =
 // XCD-aware tile order.
 struct Launch {
@@ -1356,9 +1797,9 @@ int size_bucket(uint32_t n) {
 
 Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool frozen) {
     const bool adj = p->op_mode == TGVX_OP_ADJ;
-    if (p->fused < 0 || p->fused > 4) throw Error(TGVX_E_ARG, "fused must be 0, 1, 2, 3 or 4");
+    if (p->fused < 0 || p->fused > 5) throw Error(TGVX_E_ARG, "fused must be 0, 1, 2, 3, 4 or 5");
     Launch L{adj && !frozen ? p->fused : 0, block_of(p), 1};
-    const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4;
+    const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4 || L.variant == 5;
     if (tiled && p->block <= 0) L.block = kFuseTile;
     if (tiled && L.block == 64)
         throw Error(TGVX_E_ARG, "the tiled fused kernel takes block 128 or more");
@@ -1450,9 +1891,139 @@ HaloTab halo_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
     return h;
 }
 
+// LDS per workgroup the two-iteration kernel may use: the device's limit
+// (160 KiB on gfx950), at most kHalo2Lds.
+inline size_t lds_budget(const tgvx_ctx* c) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess || v <= 0)
+        v = 64 * 1024;
+    return std::min<size_t>((size_t)v, (size_t)kHalo2Lds);
+}
+
+// The two-iteration kernel's tile sets for a's level and a tile size, built
+// on first use (callers build them before any graph capture starts).
+template <int TILE>
+void halo2_table_launch(tgvx_ctx* c, const int32_t* const nba[3], const int32_t* const nbt[3], uint32_t off,
+                        uint32_t n, Halo2Bufs& hb) {
+    const uint32_t ntiles = (n + TILE - 1) / TILE;
+    if (ntiles == 0) return;
+    k_halo2_table<TILE><<<ntiles, TILE, 0, c->stream>>>(nba[0], nba[1], nba[2], nbt[0], nbt[1], nbt[2], off, n,
+                                                         hb.cnt.as<int32_t>(), hb.list.as<int32_t>(),
+                                                         hb.bslot.as<int32_t>(), hb.cslot.as<int32_t>());
+}
+template <class S>
+const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
+    Problem& P = c->prob;
+    const auto key = std::make_tuple(a.off, a.n, tile, (int)sizeof(S));
+    auto it = P.halo2.find(key);
+    if (it != P.halo2.end()) return *it->second;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    TGVX_HIP(hipStreamIsCapturing(c->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) throw Error(TGVX_E_STATE, "tile sets built inside a graph capture");
+    auto hb = std::make_unique<Halo2Bufs>();
+    const uint32_t ntiles = (a.n + tile - 1) / tile;
+    const size_t ca = (size_t)kHalo2Cap * tile;
+    // the kernel needs the whole level as its launch range (the sets are
+    // closed under the level's neighbour relation)
+    const bool whole = a.off == a.lvl_off && a.n == a.lvl_n;
+    if (whole && ntiles > 0) {
+        hb->cnt.ensure(sizeof(int32_t) * 4 * ntiles);
+        hb->list.ensure(sizeof(int32_t) * (ca - tile) * ntiles);
+        hb->bslot.ensure(sizeof(int32_t) * 3 * ca * ntiles);
+        hb->cslot.ensure(sizeof(int32_t) * 3 * ca * ntiles);
+        switch (tile) {
+            case 128: halo2_table_launch<128>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
+            case 256: halo2_table_launch<256>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
+            case 512: halo2_table_launch<512>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
+            default: throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256 or 512");
+        }
+        TGVX_HIP(hipGetLastError());
+        std::vector<int32_t> cnt(4 * (size_t)ntiles);
+        TGVX_HIP(hipMemcpyAsync(cnt.data(), hb->cnt.p, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost,
+                                c->stream));
+        TGVX_HIP(hipStreamSynchronize(c->stream));
+        bool failed = false;
+        const size_t el = sizeof(typename Io<S>::T);
+        auto need = [&](uint32_t t) {
+            return el * (13 * (size_t)cnt[4 * t + 2] + 4 * (size_t)std::max(0, cnt[4 * t] - 2 * tile));
+        };
+        std::vector<size_t> needs(ntiles);
+        for (uint32_t t = 0; t < ntiles; ++t) {
+            hb->sp = std::max(hb->sp, cnt[4 * t]);
+            hb->sd = std::max(hb->sd, cnt[4 * t + 1]);
+            hb->sa = std::max(hb->sa, cnt[4 * t + 2]);
+            failed = failed || cnt[4 * t + 3] != 0;
+            needs[t] = need(t);
+        }
+        std::vector<size_t> sorted = needs;
+        std::nth_element(sorted.begin(), sorted.begin() + (ntiles * 9) / 10, sorted.end());
+        const size_t thr = sorted[(ntiles * 9) / 10];
+        std::vector<int32_t> order;
+        order.reserve(ntiles);
+        for (int k = 0; k < 2; ++k) {
+            Halo2Bufs::Class& cl = hb->cls[k];
+            cl.first = (uint32_t)order.size();
+            for (uint32_t t = 0; t < ntiles; ++t) {
+                if ((needs[t] <= thr) != (k == 0)) continue;
+                order.push_back((int32_t)t);
+                cl.sa = std::max(cl.sa, cnt[4 * t + 2]);
+                cl.sd = std::max(cl.sd, cnt[4 * t + 1]);
+                cl.sx = std::max(cl.sx, cnt[4 * t] - 2 * tile);
+            }
+            cl.count = (uint32_t)order.size() - cl.first;
+            cl.lds = el * (13 * (size_t)cl.sa + 4 * (size_t)cl.sx);
+            hb->lds = std::max(hb->lds, cl.lds);
+        }
+        hb->ok = !failed && hb->lds <= lds_budget(c);
+        if (hb->ok) {
+            hb->tiles.ensure(sizeof(int32_t) * order.size());
+            TGVX_HIP(hipMemcpyAsync(hb->tiles.p, order.data(), sizeof(int32_t) * order.size(),
+                                    hipMemcpyHostToDevice, c->stream));
+            TGVX_HIP(hipStreamSynchronize(c->stream));
+        } else {  // the level falls back to one iteration per launch
+            hb->cnt.release();
+            hb->list.release();
+            hb->bslot.release();
+            hb->cslot.release();
+        }
+    }
+    return *P.halo2.emplace(key, std::move(hb)).first->second;
+}
+
+// two iterations (primal n+1 ... dual n+3) from src into dst; false when the
+// level's tile sets do not fit (nothing launched)
+template <class S, bool X>
+bool launch_fused2_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, int tile) {
+    const Halo2Bufs& hb = halo2_tab<S>(c, a, tile);
+    if (!hb.ok) return false;
+    Halo2Tab h;
+    h.cnt = hb.cnt.as<int32_t>();
+    h.list = hb.list.as<int32_t>();
+    h.bslot = hb.bslot.as<int32_t>();
+    h.cslot = hb.cslot.as<int32_t>();
+    for (const Halo2Bufs::Class& cl : hb.cls) {
+        if (cl.count == 0) continue;
+        const int32_t* tl = hb.tiles.as<int32_t>() + cl.first;
+        switch (tile) {
+            case 128:
+                k_fused2<S, 128, X><<<cl.count, 128, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                break;
+            case 256:
+                k_fused2<S, 256, X><<<cl.count, 256, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                break;
+            default:
+                k_fused2<S, 512, X><<<cl.count, 512, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                break;
+        }
+    }
+    return true;
+}
+
 // one fused pass (primal n, dual n+1) from src into dst
 template <class S, bool X>
-void launch_fused_pass_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, const Launch& L) {
+void launch_fused_pass_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, const Launch& L0) {
+    // variant 5 advances one iteration with the tiled kernel of the same tile
+    const Launch L = L0.variant == 5 ? Launch{1, L0.block, L0.xcd} : L0;
     if (L.variant == 4) {
         const HaloTab h = halo_tab<S>(c, a, L.block);
         switch (L.block) {
@@ -1501,7 +2072,14 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
     Set13<S> cur = main_set(a), oth = alt_set<S>(c);
     const Set13<S> mainset = cur;
     k_dual<S, 0, X><<<grid, wg, 0, c->stream>>>(a);
-    for (int i = 1; i < iters; ++i) {
+    int i = 1;
+    if (L.variant == 5) {
+        for (; i + 1 < iters; i += 2) {
+            if (!launch_fused2_x<S, X>(c, a, cur, oth, L.block)) break;
+            std::swap(cur, oth);
+        }
+    }
+    for (; i < iters; ++i) {
         launch_fused_pass_x<S, X>(c, a, cur, oth, L);
         std::swap(cur, oth);
     }
@@ -1544,6 +2122,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
     const bool fused = L.variant != 0 && iters >= 2;
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (fused && L.variant == 4) halo_tab<S>(c, a, L.block);
+    if (fused && L.variant == 5) halo2_tab<S>(c, a, L.block);
     if (p->use_graph && !frozen && iters > 1) {
         GraphKey key{a.off,
                      a.n,
@@ -1931,7 +2510,7 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     *total_ms = ms;
     // dual and primal kernels, each bracketed by its own events (the
     // configuration's workgroup size and tile order)
-    const int k = std::max(1, std::min(iters, 20));
+    const int k = std::max(2, std::min(iters, 20) & ~1);  // even: the fused passes below end on the main set
     Launch split = cfg;
     split.variant = 0;
     if (split.block > 256) split.block = 256;
@@ -1966,11 +2545,23 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     if (p->op_mode == TGVX_OP_ADJ) {
         Launch fz = cfg;
         if (fz.variant == 0) fz = Launch{1, kFuseTile, cfg.xcd};
+        // the two-iteration kernel: per launch / 2 (one iteration per
+        // launch where the level's tile sets do not fit)
+        int per = 1;
+        if (fz.variant == 5) {
+            if (halo2_tab<S>(c, a, fz.block).ok) per = 2;
+            else fz.variant = 1;
+        }
         Set13<S> cur = main_set(a), oth = alt_set<S>(c);
-        const int kf = 2 * k;
+        const int kf = 2 * k / per;  // 2k iterations whatever the variant
         for (int i = 0; i < kf; ++i) {
             TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-            launch_fused_pass<S>(c, a, cur, oth, fz);
+            if (per == 2) {
+                if (fz.xcd) launch_fused2_x<S, true>(c, a, cur, oth, fz.block);
+                else launch_fused2_x<S, false>(c, a, cur, oth, fz.block);
+            } else {
+                launch_fused_pass<S>(c, a, cur, oth, fz);
+            }
             TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
             TGVX_HIP(hipEventSynchronize(c->ev[1]));
             float m1 = 0.f;
@@ -1978,7 +2569,7 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
             sf += m1;
             std::swap(cur, oth);
         }
-        sf /= kf;
+        sf /= (double)kf * per;
     }
     if (fused_ms) *fused_ms = sf;
 }
@@ -1996,6 +2587,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
             for (int v : {1, 3, 4})
                 for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{v, b, x});
             for (int b : {256, 128}) cand.push_back(Launch{2, b, x});
+            for (int b : {256, 128, 512}) cand.push_back(Launch{5, b, x});
         }
     }
     tgvx_params q = *p;
@@ -2167,6 +2759,7 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     Problem& P = c->prob;
     P.loaded = false;
     P.halo.clear();
+    P.halo2.clear();
     P.n = n;
     P.nb = nb;
     P.precision = precision;
@@ -2484,6 +3077,38 @@ void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double
         case TGVX_FP64: bench_t<double>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
         case TGVX_FP32: bench_t<float>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
         default: bench_t<bf16s>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
+    }
+}
+
+template <class S>
+void tile_sets_t(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_sets, int64_t* lds_bytes) {
+    Problem& P = c->prob;
+    const Level* L = nullptr;
+    for (const Level& l : P.levels)
+        if (l.depth == depth) L = &l;
+    if (!L) throw Error(TGVX_E_ARG, "no cubes at depth " + std::to_string(depth));
+    if (tile != 128 && tile != 256 && tile != 512)
+        throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256 or 512");
+    tgvx_params q;
+    tgvx_default_params(&q);
+    q.op_mode = TGVX_OP_ADJ;
+    const KArgs<S> a = level_args<S>(c, &q, *L);
+    const Halo2Bufs& hb = halo2_tab<S>(c, a, tile);
+    *usable = hb.ok ? 1 : 0;
+    max_sets[0] = hb.sa;
+    max_sets[1] = hb.sd;
+    max_sets[2] = hb.sp;
+    *lds_bytes = (int64_t)hb.lds;
+}
+
+void tile_sets(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_sets, int64_t* lds_bytes) {
+    Problem& P = c->prob;
+    if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
+    if (P.op_mode != TGVX_OP_ADJ) throw Error(TGVX_E_STATE, "the two-iteration kernel needs the exact-adjoint tables");
+    switch (P.precision) {
+        case TGVX_FP64: tile_sets_t<double>(c, depth, tile, usable, max_sets, lds_bytes); break;
+        case TGVX_FP32: tile_sets_t<float>(c, depth, tile, usable, max_sets, lds_bytes); break;
+        default: tile_sets_t<bf16s>(c, depth, tile, usable, max_sets, lds_bytes); break;
     }
 }
 

@@ -58,6 +58,26 @@ struct HaloBufs {
     DevBuf slot[3], cnt, list;
 };
 
+// Per level and tile size: the two-iteration kernel's tile sets (per tile:
+// sizes, out-of-tile members, +a source slots, incoming-edge slots). ok is
+// false when a tile's sets exceed the cap or its LDS need exceeds the budget;
+// the level then runs one-iteration passes. sa, sd, sp: the largest PA, DB,
+// PC over the tiles; lds: the largest dynamic LDS of a launch. The tiles run
+// in two launches (LDS classes): those that need at most the 90th percentile
+// of LDS, then the rest, each launch sized for its largest tile; tiles lists
+// their ids class by class.
+struct Halo2Bufs {
+    DevBuf cnt, list, bslot, cslot, tiles;
+    bool ok = false;
+    int sa = 0, sd = 0, sp = 0;
+    size_t lds = 0;
+    struct Class {
+        uint32_t first = 0, count = 0;
+        int sa = 0, sd = 0, sx = 0;
+        size_t lds = 0;
+    } cls[2];
+};
+
 struct Level {
     int depth;
     uint32_t off, n;  // contiguous range of permuted positions
@@ -106,6 +126,8 @@ struct Problem {
     DevBuf d_lvlpd;                  // per level P, D of a fixed-iteration solve, read at its end
     // halo tables of the fused tiled kernel, keyed by (first position, count, tile)
     std::map<std::tuple<uint32_t, uint32_t, int>, std::unique_ptr<HaloBufs>> halo;
+    // tile sets of the two-iteration kernel, keyed likewise (and by the state's element size)
+    std::map<std::tuple<uint32_t, uint32_t, int, int>, std::unique_ptr<Halo2Bufs>> halo2;
 };
 
 }  // namespace tgvx
@@ -154,6 +176,7 @@ void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* P, double
 void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
                  double* primal_ms, double* fused_ms);
 void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* rep);
+void tile_sets(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_sets, int64_t* lds_bytes);
 // octree.hip
 void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, const float origin[3], float r_root,
                   int32_t max_depth, int32_t last_stage, uint64_t* n_out, tgvx_octree_report* rep);
