@@ -1900,6 +1900,30 @@ inline size_t lds_budget(const tgvx_ctx* c) {
     return std::min<size_t>((size_t)v, (size_t)kHalo2Lds);
 }
 
+// LDS of one CU (160 KiB on gfx950).
+inline size_t lds_per_cu(const tgvx_ctx* c) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) != hipSuccess ||
+        v <= 0)
+        v = 64 * 1024;
+    return (size_t)v;
+}
+// Workgroups of k_fused2 (tile threads) that its VGPRs let a CU hold: four
+// SIMDs of 512 VGPRs per lane, allocated in granules of 8, at most 8 waves each.
+template <class S>
+size_t fused2_wg_per_cu(int tile) {
+    hipFuncAttributes fa;
+    hipError_t e;
+    switch (tile) {
+        case 128: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 128, true>)); break;
+        case 256: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 256, true>)); break;
+        default: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 512, true>)); break;
+    }
+    if (e != hipSuccess || fa.numRegs <= 0) return 1;
+    const int waves = std::min(8, 512 / ((fa.numRegs + 7) / 8 * 8));
+    return (size_t)std::max(1, 4 * waves / std::max(1, tile / 64));
+}
+
 // The two-iteration kernel's tile sets for a's level and a tile size, built
 // on first use (callers build them before any graph capture starts).
 template <int TILE>
@@ -1955,9 +1979,15 @@ const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
             failed = failed || cnt[4 * t + 3] != 0;
             needs[t] = need(t);
         }
+        // the main launch takes every tile that keeps the occupancy (workgroups
+        // per CU, from the kernel's VGPRs and the CU's LDS) of the tile at the
+        // 90th percentile of LDS need; the rest run in a second launch
         std::vector<size_t> sorted = needs;
         std::nth_element(sorted.begin(), sorted.begin() + (ntiles * 9) / 10, sorted.end());
-        const size_t thr = sorted[(ntiles * 9) / 10];
+        const size_t p90 = std::max<size_t>(sorted[(ntiles * 9) / 10], 1);
+        const size_t cu_lds = lds_per_cu(c);
+        const size_t wg = std::max<size_t>(1, std::min(fused2_wg_per_cu<S>(tile), cu_lds / p90));
+        const size_t thr = std::max(p90, cu_lds / wg);
         std::vector<int32_t> order;
         order.reserve(ntiles);
         for (int k = 0; k < 2; ++k) {

@@ -63,9 +63,9 @@ struct HaloBufs {
 // false when a tile's sets exceed the cap or its LDS need exceeds the budget;
 // the level then runs one-iteration passes. sa, sd, sp: the largest PA, DB,
 // PC over the tiles; lds: the largest dynamic LDS of a launch. The tiles run
-// in two launches (LDS classes): those that need at most the 90th percentile
-// of LDS, then the rest, each launch sized for its largest tile; tiles lists
-// their ids class by class.
+// in at most two launches (LDS classes): those whose LDS need keeps the
+// occupancy of the tile at the 90th percentile, then the rest, each launch
+// sized for its largest tile; tiles lists their ids class by class.
 struct Halo2Bufs {
     DevBuf cnt, list, bslot, cslot, tiles;
     bool ok = false;
