@@ -55,9 +55,11 @@ def main():
     ap.add_argument("--height", type=int, default=960)
     ap.add_argument("--max-depth", type=int, default=10)
     ap.add_argument("--iters", type=int, default=50, help="iterations per level in one step")
-    ap.add_argument("--block", type=int, default=0, help="workgroup / tile size (0: the library default)")
+    ap.add_argument("--block", type=int, default=0,
+                    help="workgroup / tile size (0: 512 for the tiled fused variants, else 256); with the "
+                         "variant, the autotuner's base configuration")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5], default=1,
+    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5], default=5,
                     help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "
                          "kernel; 3: LDS-tiled with the tile's p, q staged in LDS; 4: LDS-tiled with a "
                          "tabulated halo; 5: two iterations per launch over tabulated tile sets")
@@ -79,6 +81,10 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.no_fused:
+        args.fused = 0
+    if not args.block:
+        args.block = 512 if args.fused in (1, 3, 4, 5) else 256
 
     if args.calibrate_copy:
         import ctypes
@@ -162,7 +168,7 @@ def main():
                       use_graph=0 if args.no_graph else 1, block=args.block, fused=variant)
     tuning = None
     if args.tune:
-        tuning = ctx.autotune(prm, iters=20)
+        tuning = ctx.autotune(prm, iters=args.iters)  # the step's own chunk of iterations per level
         prm.tune = 1
         log(f"[bench] autotuned (depth, variant, block, xcd, us): "
             f"{[(l['depth'], l['variant'], l['block'], l['xcd'], round(l['best_us'], 1)) for l in tuning]}")
@@ -189,7 +195,7 @@ def main():
         variant = tuning[-1]["variant"]
     fused = variant != 0
     # fused_ms is per iteration; the two-iteration kernel's launch is two
-    two = variant == 5 and ctx.tile_sets(finest, tuning[-1]["block"] if tuning else (args.block or 256))[0]
+    two = variant == 5 and ctx.tile_sets(finest, tuning[-1]["block"] if tuning else args.block)[0]
     if variant == 5 and not two:
         variant = 1  # the level's tile sets do not fit: one-iteration tiled passes
     it_ms = fused_ms if fused else dual_ms + primal_ms
@@ -269,7 +275,7 @@ def main():
     if args.precision == "fp32" and os.path.exists(pj):
         name = KERNEL_NAMES[variant]
         if variant in (1, 3, 4, 5):
-            name += f", tile {tuning[-1]['block'] if tuning else (args.block or 256)}"
+            name += f", tile {tuning[-1]['block'] if tuning else args.block}"
         rec = json.load(open(pj)).get(name)
         if rec and rec["leaves"] == n_f:
             traffic = rec["traffic_bytes_per_iteration"]
@@ -330,11 +336,11 @@ def batched(args, T, scene, prec):
         f"max {counts.max()}) in {time.time() - t0:.0f} s")
 
     prm = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, iters=args.iters,
-                      use_graph=0 if args.no_graph else 1, fused=0 if args.no_fused else args.fused)
+                      use_graph=0 if args.no_graph else 1, fused=args.fused, block=args.block)
     ctx.load_blocks(blocks, precision=prec, op_mode=T.OP_ADJ)
     tuning = None
     if args.tune:
-        tuning = ctx.autotune(prm, iters=20)
+        tuning = ctx.autotune(prm, iters=args.iters)
         prm.tune = 1
         log(f"[bench] autotuned: {[(l['depth'], l['variant'], l['block'], l['xcd']) for l in tuning]}")
     for _ in range(args.warmup):
@@ -361,7 +367,8 @@ def batched(args, T, scene, prec):
     gap = None
     if args.batched_gap > 0:
         pg = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, gap_tol=args.batched_gap,
-                         chunk=50, max_iters=20000, use_graph=prm.use_graph, fused=prm.fused, tune=prm.tune)
+                         chunk=50, max_iters=20000, use_graph=prm.use_graph, fused=prm.fused, block=prm.block,
+                         tune=prm.tune)
         ctx.load_blocks(blocks, precision=prec, op_mode=T.OP_ADJ)
         ctx.run(pg)
         t1 = time.perf_counter()

@@ -263,11 +263,13 @@ int tgvx_octree_copy(tgvx_ctx* ctx, tgvx_cube* out);
 
 /* ---- launch autotuner (no reference counterpart)
  * For every level of the loaded problem, times `iters` iterations of each
- * launch configuration -- kernel variant (dual + primal kernels, the fused
+ * launch configuration (the better of two timings, after a warm-up; pass the
+ * solve's chunk, e.g. 50) -- kernel variant (dual + primal kernels, the fused
  * variants 1-5 of tgvx_params.fused; exact-adjoint mode only for the fused
  * ones), workgroup / tile size (64 to 1024) and tile order (XCD-contiguous or
  * plain) -- and stores the fastest in the context, per (precision, operator
- * mode, floor(log2 leaves)); solves with params.tune = 1 use it. Every
+ * mode, floor(log2 leaves)), unless it beats the params' own configuration
+ * by less than 3 % (then that one); solves with params.tune = 1 use it. Every
  * configuration gives identical results. The problem's state is overwritten
  * (a later solve re-initialises it). */
 typedef struct {

@@ -2632,19 +2632,29 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
         Launch win = base;
         for (const Launch& cf : cand) {
             run_launch<S>(c, a, &q, iters, false, cf);  // warm-up (and graph build)
-            TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-            run_launch<S>(c, a, &q, iters, false, cf);
-            TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
-            TGVX_HIP(hipEventSynchronize(c->ev[1]));
-            float ms = 0.f;
-            TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-            const double us = 1e3 * ms / iters;
+            double us = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {  // the better of two timings
+                TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+                run_launch<S>(c, a, &q, iters, false, cf);
+                TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+                TGVX_HIP(hipEventSynchronize(c->ev[1]));
+                float ms = 0.f;
+                TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                us = std::min(us, 1e3 * ms / iters);
+            }
             const bool is_base = cf.variant == base.variant && cf.block == base.block && cf.xcd == base.xcd;
             if (is_base) def = us;
             if (us < best) {
                 best = us;
                 win = cf;
             }
+        }
+        // keep the caller's configuration unless a candidate beats it by 3 %:
+        // a level timed alone runs warmer than inside a sweep, and within that
+        // margin the isolated ranking did not hold for whole sweeps
+        if (def > 0.0 && best > 0.97 * def) {
+            best = def;
+            win = base;
         }
         c->tuned[std::make_tuple(P.precision, p->op_mode, size_bucket(L.n))] = TuneEntry{win.variant, win.block,
                                                                                          win.xcd, best};
