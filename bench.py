@@ -37,7 +37,7 @@ ELEM = {"fp32": 4, "bf16": 2, "fp64": 8}
 # figure is the work done, not the traffic.
 ALG_BYTES = {k: 34 * e + 8 + 24 for k, e in ELEM.items()}
 KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj",
-                3: "k_fused_tile (p, q staged in LDS)", 4: "k_fused_halo", 5: "k_fused2"}
+                3: "k_fused_tile (p, q staged in LDS)", 4: "k_fused_halo", 5: "k_fused2", 6: "k_level1"}
 
 
 def log(msg):
@@ -59,10 +59,11 @@ def main():
                     help="workgroup / tile size (0: 512 for the tiled fused variants, else 256); with the "
                          "variant, the autotuner's base configuration")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5], default=5,
+    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5, 6], default=5,
                     help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "
                          "kernel; 3: LDS-tiled with the tile's p, q staged in LDS; 4: LDS-tiled with a "
-                         "tabulated halo; 5: two iterations per launch over tabulated tile sets")
+                         "tabulated halo; 5: two iterations per launch over tabulated tile sets; 6: levels of at "
+                         "most 1024 cubes in one workgroup for all their iterations, larger ones as 5")
     ap.add_argument("--no-fused", action="store_true", help="same as --fused 0")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start time-to-gap runs")
     ap.add_argument("--no-octree", action="store_true",

@@ -103,8 +103,10 @@ typedef struct {
                               tile's p, q staged in LDS first; 4 as 1 with the out-of-tile
                               neighbours tabulated once per level and tile; 5 two iterations
                               per launch over tile sets tabulated once per level and tile (a
-                              level whose sets do not fit runs as 1). For 1, 3, 4 and 5, block
-                              is the tile (128, 256 = default, 512, and 1024 except for 5) */
+                              level whose sets do not fit runs as 1); 6 a level of at most 1024
+                              cubes in one workgroup for all its iterations (larger levels run
+                              as 5 with tiles of 512). For 1, 3, 4 and 5, block is the tile
+                              (128, 256 = default, 512, and 1024 except for 5) */
     int32_t tune;          /* 1: take each level's launch configuration from tgvx_autotune's table
                               (levels without an entry use fused / block as given) */
     float sigma, tau;      /* primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150).
@@ -126,6 +128,8 @@ typedef struct {
     double iterate_ms;    /* time in the iteration kernels alone (events around chunks) */
     double gap_ms;        /* time in the energy / gap evaluations */
     double setup_ms;      /* neighbour tables, permutation and per-leaf constants */
+    double level_ms[TGVX_MAX_LEVELS]; /* iteration time of each level (fixed-iteration solves
+                                         of one block: events around the level's launches) */
 } tgvx_report;
 
 void tgvx_default_params(tgvx_params* p);
@@ -265,11 +269,13 @@ int tgvx_octree_copy(tgvx_ctx* ctx, tgvx_cube* out);
  * For every level of the loaded problem, times `iters` iterations of each
  * launch configuration (the better of two timings, after a warm-up; pass the
  * solve's chunk, e.g. 50) -- kernel variant (dual + primal kernels, the fused
- * variants 1-5 of tgvx_params.fused; exact-adjoint mode only for the fused
+ * variants 1-6 of tgvx_params.fused; exact-adjoint mode only for the fused
  * ones), workgroup / tile size (64 to 1024) and tile order (XCD-contiguous or
  * plain) -- and stores the fastest in the context, per (precision, operator
- * mode, floor(log2 leaves)), unless it beats the params' own configuration
- * by less than 3 % (then that one); solves with params.tune = 1 use it. Every
+ * mode, floor(log2 leaves)), counting 1 us per kernel launch and iteration on
+ * top of the measured time (what a launch costs more inside a coarse-to-fine
+ * sweep than alone), unless it beats the params' own configuration by less
+ * than 3 % (then that one); solves with params.tune = 1 use it. Every
  * configuration gives identical results. The problem's state is overwritten
  * (a later solve re-initialises it). */
 typedef struct {
@@ -277,7 +283,8 @@ typedef struct {
     int32_t level_depth[TGVX_MAX_LEVELS];
     uint32_t level_leaves[TGVX_MAX_LEVELS];
     int32_t variant[TGVX_MAX_LEVELS]; /* 0 two kernels, 1 tiled fused, 2 untiled fused, 3 tiled + LDS p, q,
-                                         4 tiled + tabulated halo, 5 two iterations per launch */
+                                         4 tiled + tabulated halo, 5 two iterations per launch,
+                                         6 the whole level in one workgroup */
     int32_t block[TGVX_MAX_LEVELS];
     int32_t xcd[TGVX_MAX_LEVELS];
     double best_us[TGVX_MAX_LEVELS];    /* per iteration, chosen configuration */

@@ -46,7 +46,7 @@ def test_solve_batched_entry_matches_resident(ctx, blocks):
         assert np.array_equal(_raw(got), _raw(alone))
 
 
-@pytest.mark.parametrize("fused", [1, 4, 5])
+@pytest.mark.parametrize("fused", [1, 4, 5, 6])
 def test_batched_to_gap_per_block(ctx, blocks, fused):
     p = TGVParams(alpha0=2.0, alpha1=1.0, precision=FP32, op_mode=OP_ADJ, gap_tol=1e-3, chunk=50,
                   max_iters=20000, fused=fused)

@@ -189,7 +189,7 @@ def test_parameter_change_on_resident_problem(ctx, sphere):
     assert np.array_equal(_state13(ctx.state()).view(np.uint32), _state13(first).view(np.uint32))
 
 
-@pytest.mark.parametrize("fused", [0, 1, 4, 5])
+@pytest.mark.parametrize("fused", [0, 1, 4, 5, 6])
 def test_to_gap_adj_large_step_ratio(ctx, sphere, fused):
     # sigma / tau = 1024 (sigma = 8, tau = 1/128, sigma * tau = 1/16): the
     # same iteration with other steps, bit-exact with the oracle, counts equal

@@ -20,7 +20,7 @@ def _raw(st):
 
 @pytest.mark.parametrize("fused,block", [(0, 64), (0, 128), (1, 128), (1, 512), (1, 1024), (2, 64), (2, 128),
                                          (3, 128), (3, 256), (3, 1024), (4, 128), (4, 256), (4, 512), (4, 1024),
-                                         (5, 128), (5, 256), (5, 512)])
+                                         (5, 128), (5, 256), (5, 512), (6, 0)])
 def test_launch_shapes_identical(ctx, sphere, fused, block):
     sc, hist = sphere
     base = dict(alpha0=2.0, alpha1=1.0, iters=60, precision=FP32, op_mode=OP_ADJ)
@@ -34,18 +34,30 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
     sc, hist = sphere
     p = TGVParams(alpha0=2.0, alpha1=1.0, iters=60, precision=prec, op_mode=mode)
     ctx.load(sc.cubes, hist, precision=prec, op_mode=mode)
-    levels = ctx.autotune(p, iters=10)
+    iters = 10
+    levels = ctx.autotune(p, iters=iters)
     assert [(l["depth"], l["leaves"]) for l in levels] == [tuple(x) for x in ctx.levels()]
     for l in levels:
-        assert l["variant"] in ((0, 1, 2, 3, 4, 5) if mode == OP_ADJ else (0,))
+        assert l["variant"] in ((0, 1, 2, 3, 4, 5, 6) if mode == OP_ADJ else (0,))
         assert l["block"] in (64, 128, 256, 512, 1024) and l["xcd"] in (0, 1)
         if l["variant"] in (1, 3, 4):
             assert l["block"] in (128, 256, 512, 1024)
         elif l["variant"] == 5:
             assert l["block"] in (128, 256, 512)
+        elif l["variant"] == 6:
+            assert l["block"] == 1024 and l["leaves"] <= 1024
         else:
             assert l["block"] <= 256
-        assert 0 < l["best_us"] <= l["default_us"]
+        # candidates are ranked by time + 1 us per kernel launch and iteration
+        def launches(v, n):
+            if v == 0:
+                return 2.0
+            if v == 5 or (v == 6 and n > 1024):
+                return (2.0 + (iters - 1) / 2.0) / iters
+            return 1.0 / iters if v == 6 else (iters + 1.0) / iters
+        own = 0 if mode == OP_REF else p.fused
+        assert 0 < l["best_us"] and 0 < l["default_us"]
+        assert l["best_us"] + launches(l["variant"], l["leaves"]) <= l["default_us"] + launches(own, l["leaves"]) + 1e-9
     ctx.run(p)
     plain = ctx.state()
     p.tune = 1
@@ -89,3 +101,17 @@ def test_two_iteration_level_bench(ctx, sphere):
         assert total > 0 and dual > 0 and primal > 0 and fz > 0
         outs.append(ctx.state())
     assert np.array_equal(_raw(outs[0]), _raw(outs[1]))
+
+
+@pytest.mark.parametrize("prec", [FP32, BF16, FP64])
+def test_whole_level_kernel(ctx, sphere, prec):
+    # fused = 6: the levels of at most 1024 cubes run all their iterations in
+    # one workgroup (the larger ones as fused = 5); the state equals the
+    # two-kernel iteration's bit for bit, with and without graphs
+    sc, hist = sphere
+    for graph in (1, 0):
+        base = dict(alpha0=2.0, alpha1=1.0, iters=57, precision=prec, op_mode=OP_ADJ, use_graph=graph)
+        ref, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=0, block=256, **base))
+        got, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=6, **base))
+        assert np.array_equal(_raw(got), _raw(ref))
+    assert min(n for _, n in ctx.levels()) <= 1024  # some level ran in one workgroup

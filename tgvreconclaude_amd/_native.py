@@ -42,7 +42,7 @@ class Report(C.Structure):
                 ("level_iters", C.c_int32 * MAX_LEVELS), ("level_warm", C.c_int32 * MAX_LEVELS),
                 ("level_energy", C.c_double * MAX_LEVELS), ("level_dual", C.c_double * MAX_LEVELS),
                 ("leaf_iters", C.c_double), ("solve_ms", C.c_double), ("iterate_ms", C.c_double),
-                ("gap_ms", C.c_double), ("setup_ms", C.c_double)]
+                ("gap_ms", C.c_double), ("setup_ms", C.c_double), ("level_ms", C.c_double * MAX_LEVELS)]
 
     def levels(self):
         out = []
@@ -50,7 +50,7 @@ class Report(C.Structure):
             P, D = self.level_energy[i], self.level_dual[i]
             out.append(dict(depth=self.level_depth[i], leaves=self.level_leaves[i],
                             iters=self.level_iters[i], warm=self.level_warm[i], P=P, D=D,
-                            rel_gap=(P - D) / max(1.0, abs(P))))
+                            rel_gap=(P - D) / max(1.0, abs(P)), ms=self.level_ms[i]))
         return out
 
 

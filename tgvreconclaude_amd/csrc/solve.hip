@@ -1181,6 +1181,140 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     for (int k = 0; k < 6; ++k) IO::stn(dst.f[F_Q0 + k], g0, q[k]);
 }
 
+// ---- a whole level in one workgroup (no reference counterpart), for levels
+// of at most kLevel1 cubes: `iters` iterations in one launch. Thread t owns
+// the level's cube t and keeps its u, v, p, q, u_bar, v_bar, table entries and
+// frozen neighbours in registers; each iteration publishes u_bar, v_bar, then
+// p, q through LDS (two barriers). The dual and primal steps are k_dual's and
+// primal_adj_at's expressions on the main set, so the state after the launch
+// is that of `iters` two-kernel iterations, bit for bit.
+constexpr int kLevel1 = 1024;
+
+template <class S>
+__global__ void __launch_bounds__(kLevel1) k_level1(KArgs<S> a, int iters) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    __shared__ T lub[4][kLevel1];
+    __shared__ T lpq[9][kLevel1];
+    const int t = (int)threadIdx.x;
+    const long g = (long)a.off + (t < (int)a.n ? t : 0);
+    const bool run = t < (int)a.n && !masked(a, g);
+    auto in_level = [&](int j) { return j >= 0 && (uint32_t)j >= a.off && (uint32_t)j < a.off + a.n; };
+    T u = IO::ld(a.s[F_U], g), v[3], p[3], q[6], ub[4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        v[c] = IO::ld(a.s[F_VX + c], g);
+        p[c] = IO::ld(a.s[F_PX + c], g);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = IO::ld(a.s[F_Q0 + k], g);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ub[c] = IO::ld(a.s[F_UB + c], g);
+    const uint32_t w = a.info[g];
+    // +a sources: >= 0 an LDS slot, -1 this cube, -2 frozen (values in fz);
+    // incoming edges: the LDS slot or -1
+    int src[3], inc[3];
+    T fz[3][4];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int j = a.nbp[ax][g];
+        src[ax] = j < 0 ? -1 : (in_level(j) ? j - (int)a.off : -2);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) fz[ax][c] = IO::ld(a.s[F_UB + c], j >= 0 ? (long)j : g);
+        const int k = a.nbt[ax][g];
+        inc[ax] = k < 0 ? -1 : k - (int)a.off;
+    }
+    for (int it = 0; it < iters; ++it) {
+        // passes 1-2 (k_dual)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) lub[c][t] = ub[c];
+        __syncthreads();
+        if (run) {
+            T bar[4][4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bar[0][c] = ub[c];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    bar[1 + ax][c] = src[ax] == -1 ? ub[c] : (src[ax] == -2 ? fz[ax][c] : lub[c][src[ax]]);
+            }
+            dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p[c] = IO::rt(p[c]);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) q[k] = IO::rt(q[k]);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) lpq[c][t] = p[c];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) lpq[3 + k][t] = q[k];
+        __syncthreads();
+        // passes 3-5 (primal_adj_at, then the extrapolation)
+        if (run) {
+            T dp = T(0);
+            T dq[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const bool hs = InfoWord::has(w, ax);
+                dp = hs ? dp + p[ax] : dp;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T w2 = (c == ax) ? T(1) : T(0.5);
+                    dq[c] = hs ? dq[c] + w2 * q[qidx(c, ax) - F_Q0] : dq[c];
+                }
+            }
+            constexpr int kF[3][4] = {{0, 3, 6, 7}, {1, 6, 4, 8}, {2, 7, 8, 5}};  // p_ax, q row ax in lpq
+            T in[3][4];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const int k = inc[ax] >= 0 ? inc[ax] : t;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) in[ax][e] = lpq[kF[ax][e]][k];
+            }
+            if (inc[0] >= 0) {
+                dp -= in[0][0];
+                dq[0] -= T(1) * in[0][1];
+                dq[1] -= T(0.5) * in[0][2];
+                dq[2] -= T(0.5) * in[0][3];
+            }
+            if (inc[1] >= 0) {
+                dp -= in[1][0];
+                dq[0] -= T(0.5) * in[1][1];
+                dq[1] -= T(1) * in[1][2];
+                dq[2] -= T(0.5) * in[1][3];
+            }
+            if (inc[2] >= 0) {
+                dp -= in[2][0];
+                dq[0] -= T(0.5) * in[2][1];
+                dq[1] -= T(0.5) * in[2][2];
+                dq[2] -= T(1) * in[2][3];
+            }
+            T nw[4];
+            nw[0] = prox(u + a.tau * dp, w, a.nbins, a.tl);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nw[1 + c] = v[c] + a.tau * (p[c] + dq[c]);
+            ub[0] = IO::rt(T(2) * nw[0] - u);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ub[1 + c] = IO::rt(T(2) * nw[1 + c] - v[c]);
+            u = IO::rt(nw[0]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = IO::rt(nw[1 + c]);
+        }
+    }
+    if (!run) return;
+    IO::st(a.s[F_U], g, u);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        IO::st(a.s[F_VX + c], g, v[c]);
+        IO::st(a.s[F_PX + c], g, p[c]);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) IO::st(a.s[F_Q0 + k], g, q[k]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::st(a.s[F_UB + c], g, ub[c]);
+}
+
 // ---- streaming mode (tgv_minimize_streaming, tgv.cpp:515-1070)
 // The cube file's bisection (CubeFileReader::lookup, tgv.cpp:674-683): the
 // first probe whose code matches, not necessarily the last cube with it.
@@ -1768,8 +1902,8 @@ int block_of(const tgvx_params* p) {
     int b = p && p->block > 0 ? p->block : 256;
     if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
         throw Error(TGVX_E_ARG, "block must be 64, 128, 256, 512 or 1024");
-    const bool tiled =
-        p && p->op_mode == TGVX_OP_ADJ && (p->fused == 1 || p->fused == 3 || p->fused == 4 || p->fused == 5);
+    const bool tiled = p && p->op_mode == TGVX_OP_ADJ &&
+                       (p->fused == 1 || p->fused == 3 || p->fused == 4 || p->fused == 5 || p->fused == 6);
     if (b > 256 && !tiled) throw Error(TGVX_E_ARG, "blocks above 256 exist only for the tiled fused kernels");
     if (p && p->op_mode == TGVX_OP_ADJ && p->fused == 5 && b == 1024)
         throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256 or 512");
@@ -1779,7 +1913,9 @@ int block_of(const tgvx_params* p) {
 // How one level's iterations are launched: variant 0 = dual + primal
 // kernels, 1 = LDS-tiled fused kernel, 2 = untiled fused kernel, 3 = tiled
 // with the tile's p, q staged in LDS, 4 = tiled with a precomputed halo,
-// 5 = two iterations per launch over tabulated tile sets;
+// 5 = two iterations per launch over tabulated tile sets, 6 = a level of at
+// most kLevel1 cubes in one workgroup for all its iterations (larger levels
+// run variant 5 with tiles of 512);
 // block = the workgroup size (the tile size for variants 1, 3, 4); xcd =
 // XCD-aware tile order.
 struct Launch {
@@ -1797,10 +1933,11 @@ int size_bucket(uint32_t n) {
 
 Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool frozen) {
     const bool adj = p->op_mode == TGVX_OP_ADJ;
-    if (p->fused < 0 || p->fused > 5) throw Error(TGVX_E_ARG, "fused must be 0, 1, 2, 3, 4 or 5");
+    if (p->fused < 0 || p->fused > 6) throw Error(TGVX_E_ARG, "fused must be 0 to 6");
     Launch L{adj && !frozen ? p->fused : 0, block_of(p), 1};
     const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4 || L.variant == 5;
     if (tiled && p->block <= 0) L.block = kFuseTile;
+    if (L.variant == 6) L = Launch{6, kLevel1, 0};
     if (tiled && L.block == 64)
         throw Error(TGVX_E_ARG, "the tiled fused kernel takes block 128 or more");
     if (p->tune && !frozen) {
@@ -2147,8 +2284,17 @@ struct GraphCache {
 std::map<const tgvx_ctx*, GraphCache> g_graphs;
 
 template <class S>
-void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, const Launch& L) {
+void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, const Launch& L0) {
     if (iters <= 0 || a.n == 0) return;
+    Launch L = L0;
+    if (L.variant == 6) {
+        if (!frozen && a.n <= (uint32_t)kLevel1 && a.off == a.lvl_off && a.n == a.lvl_n) {
+            k_level1<S><<<1, kLevel1, 0, c->stream>>>(a, iters);
+            TGVX_HIP(hipGetLastError());
+            return;
+        }
+        L = Launch{5, 512, 1};
+    }
     const bool fused = L.variant != 0 && iters >= 2;
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (fused && L.variant == 4) halo_tab<S>(c, a, L.block);
@@ -2497,6 +2643,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             float ms = 0.f;
             TGVX_HIP(hipEventElapsedTime(&ms, c->ev[4 + 2 * li], c->ev[5 + 2 * li]));
             it_ms += ms;
+            r.level_ms[li] = ms;
         }
     }
     r.n_levels = (int)std::min<size_t>(P.levels.size(), TGVX_MAX_LEVELS);
@@ -2575,6 +2722,7 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     if (p->op_mode == TGVX_OP_ADJ) {
         Launch fz = cfg;
         if (fz.variant == 0) fz = Launch{1, kFuseTile, cfg.xcd};
+        if (fz.variant == 6) fz = Launch{5, 512, 1};
         // the two-iteration kernel: per launch / 2 (one iteration per
         // launch where the level's tile sets do not fit)
         int per = 1;
@@ -2620,6 +2768,23 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
             for (int b : {256, 128, 512}) cand.push_back(Launch{5, b, x});
         }
     }
+    if (p->op_mode == TGVX_OP_ADJ) cand.push_back(Launch{6, kLevel1, 0});
+    // Kernel launches per iteration of a candidate over `iters` iterations of
+    // a level of n cubes. Inside a sweep every launch costs about 1 us more
+    // than when one level's graph is timed alone (measured at depths 4-6 of
+    // the depth-10 workload: two kernels per iteration 7.5-7.8 us in the
+    // sweep against 5.0-5.5 us alone), so candidates are compared by their
+    // time plus kSweepLaunchUs per launch and iteration.
+    constexpr double kSweepLaunchUs = 1.0;
+    auto launches = [&](const Launch& cf, uint32_t n) -> double {
+        const double k = (double)std::max(iters, 1);
+        switch (cf.variant) {
+            case 0: return 2.0;
+            case 5: return (2.0 + (k - 1.0) / 2.0) / k;
+            case 6: return n <= (uint32_t)kLevel1 ? 1.0 / k : (2.0 + (k - 1.0) / 2.0) / k;
+            default: return (k + 1.0) / k;
+        }
+    };
     tgvx_params q = *p;
     q.tune = 0;
     const Launch base = launch_for(c, &q, 1u << 20, false);
@@ -2628,9 +2793,10 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
     for (const Level& L : P.levels) {
         KArgs<S> a = level_args<S>(c, p, L);
         prepare_level<S>(c, a, p, (size_t)(&L - P.levels.data()));
-        double best = 1e300, def = 0.0;
+        double best = 1e300, def = 0.0, best_score = 1e300, def_score = 0.0;
         Launch win = base;
         for (const Launch& cf : cand) {
+            if (cf.variant == 6 && L.n > (uint32_t)kLevel1) continue;  // would run as 5
             run_launch<S>(c, a, &q, iters, false, cf);  // warm-up (and graph build)
             double us = 1e300;
             for (int rep = 0; rep < 2; ++rep) {  // the better of two timings
@@ -2642,17 +2808,20 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
                 TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
                 us = std::min(us, 1e3 * ms / iters);
             }
+            const double score = us + kSweepLaunchUs * launches(cf, L.n);
             const bool is_base = cf.variant == base.variant && cf.block == base.block && cf.xcd == base.xcd;
-            if (is_base) def = us;
-            if (us < best) {
+            if (is_base) {
+                def = us;
+                def_score = score;
+            }
+            if (score < best_score) {
+                best_score = score;
                 best = us;
                 win = cf;
             }
         }
-        // keep the caller's configuration unless a candidate beats it by 3 %:
-        // a level timed alone runs warmer than inside a sweep, and within that
-        // margin the isolated ranking did not hold for whole sweeps
-        if (def > 0.0 && best > 0.97 * def) {
+        // keep the caller's configuration unless a candidate beats it by 3 %
+        if (def > 0.0 && best_score > 0.97 * def_score) {
             best = def;
             win = base;
         }
