@@ -40,9 +40,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+USE_GRAPH = 1  # 0 (--no-graph): plain launches, e.g. under a counter-collecting profiler
+
+
 def params(**kw):
     base = dict(alpha0=ALPHA0, alpha1=ALPHA1, precision=FP32, op_mode=OP_ADJ, gap_tol=GAP, chunk=50,
-                max_iters=40000, sigma=SIGMA, tau=TAU)
+                max_iters=40000, sigma=SIGMA, tau=TAU, use_graph=USE_GRAPH)
     base.update(kw)
     return TGVParams(**base)
 
@@ -175,7 +178,10 @@ def main():
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--out", default=None, help="JSON results file")
     ap.add_argument("--diagnostics", action="store_true", help="also run without fallback and from the solution")
+    ap.add_argument("--no-graph", action="store_true", help="launch iteration chunks without HIP graphs")
     args = ap.parse_args()
+    global USE_GRAPH
+    USE_GRAPH = 0 if args.no_graph else 1
     ctx = Context(0)
     result = {}
     if args.cmd in ("train", "both"):
