@@ -14,6 +14,14 @@ is taken between the mesh vertex sets, both ways. The FP64 solve to the same
 gap is also run, for its iteration counts.
 
     python tests/accuracy.py --out gpurun_out/accuracy_depth10.json   # the depth-10 torus+box scene
+
+With --oracle the FP64 reference is the CPU oracle itself (the restatement of
+tgv.cpp's iteration in FP64, OpenMP over the host cores), solved to the same gap
+with steps sigma / tau = 1024: the GPU FP64 solve must equal it bit for bit,
+with the same per-level iteration counts, and the GPU FP32 solve is compared
+with it (u and surfaces).
+
+    python tests/accuracy.py --oracle --out gpurun_out/accuracy_oracle_depth10.json
 """
 import argparse
 import json
@@ -104,11 +112,55 @@ def fp64_to_gap(ctx, sc, hist, tol=1e-4):
             "reached": all(l["rel_gap"] <= tol for l in rep.levels()), "solve_ms": rep.solve_ms}
 
 
+def against_oracle(ctx, sc, hist, tol, sigma, tau, threads):
+    """GPU FP32 and FP64 solves to the gap against the FP64 CPU oracle's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    has = (hist.sum(axis=1) > 0).astype(np.uint8)
+    leaf = 2.0 * sc.r_root / 2 ** int(sc.cubes["depth"].max())
+    kw = dict(gap_tol=tol, sigma=sigma, tau=tau)
+    gpu = {}
+    for prec in (FP32, FP64):
+        ctx.load(sc.cubes, hist, precision=prec, op_mode=OP_ADJ)
+        rep = ctx.run(_params(prec, **kw))
+        st = ctx.state()
+        gpu[prec] = (st, [l["iters"] for l in rep.levels()], rep.solve_ms)
+        log(f"gpu {NAMES[prec]} to gap: {rep.solve_ms:.0f} ms, iterations {gpu[prec][1]}")
+    t0 = time.time()
+    log(f"oracle fp64 to gap, {threads} host threads ...")
+    ref, _, its, _ = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, op_mode=1, gap_tol=tol, chunk=50,
+                                                             max_iters=20000, threads=threads, sigma=sigma, tau=tau),
+                                    f64=True)
+    t_or = time.time() - t0
+    log(f"oracle fp64: {t_or:.1f} s, iterations {its.tolist()}")
+    st64 = gpu[FP64][0]
+    g64 = (np.concatenate([st64["u"][:, None], st64["v"], st64["p"], st64["q"]], axis=1) if st64.dtype.names
+           else np.ascontiguousarray(st64[:, :13], dtype=np.float64))
+    bit_exact = bool(np.array_equal(g64.view(np.uint64), np.ascontiguousarray(ref).view(np.uint64)))
+    u32 = u_of(gpu[FP32][0])
+    d = np.abs(u32 - ref[:, 0])
+    ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+    va, _ = surface(ctx, sc, u32, has)
+    vb, _ = surface(ctx, sc, ref[:, 0], has)
+    h = hausdorff(va, vb)
+    log(f"fp64 gpu == oracle bit for bit: {bit_exact}; fp32 linf u {d.max():.3g}; Hausdorff {h / leaf:.3g} leaf")
+    return {"sigma": sigma, "tau": tau, "gap_tol": tol, "oracle_threads": threads, "oracle_s": round(t_or, 1),
+            "oracle_iters_per_level": its.tolist(), "gpu_fp64_iters_per_level": gpu[FP64][1],
+            "gpu_fp32_iters_per_level": gpu[FP32][1], "gpu_fp64_ms": gpu[FP64][2], "gpu_fp32_ms": gpu[FP32][2],
+            "fp64_state_bit_exact_with_oracle": bit_exact,
+            "fp64_iters_equal": gpu[FP64][1] == its.tolist(),
+            "fp32_linf_u": float(d.max()), "fp32_linf_u_voted": float(d[has > 0].max()), "fp32_mean_abs_u": float(d.mean()),
+            "hausdorff": h, "leaf_edge": leaf, "hausdorff_leaf": h / leaf,
+            "surface_vertices": [int(len(va)), int(len(vb))]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--bf16-max-iters", type=int, default=6000, help="per-level cap of the BF16 to-gap attempt")
+    ap.add_argument("--oracle", action="store_true", help="FP64 reference = the CPU oracle (steps sigma/tau = 1024)")
+    ap.add_argument("--threads", type=int, default=0, help="oracle host threads (0: the cores available, at most 16)")
     args = ap.parse_args()
     from tgvreconclaude_amd import Context, scene
     t0 = time.time()
@@ -116,6 +168,13 @@ def main():
     print(f"[accuracy] scene: {len(sc.cubes)} cubes ({time.time() - t0:.0f} s)", flush=True)
     ctx = Context(0)
     hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
+    if args.oracle:
+        threads = args.threads or min(16, len(os.sched_getaffinity(0)))
+        res = against_oracle(ctx, sc, hist, args.tol, 8.0, 1.0 / 128, threads)
+        with open(args.out, "w") as f:
+            json.dump({"workload": "torus_box_depth10", "cubes": int(len(sc.cubes)), "oracle": res}, f, indent=1)
+        ctx.close()
+        return
     out = {"workload": "torus_box_depth10", "cubes": int(len(sc.cubes)), "results": []}
     # FP32 to the gap; BF16 for the FP32 solve's iterations; BF16 to the gap
     r32 = compare(ctx, sc, hist, FP32, args.tol)
