@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <set>
 #include <vector>
 
 #include "tgvx_device.h"
@@ -2840,6 +2841,23 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
     }
     if (rep) rep->n_levels = std::min(li, TGVX_MAX_LEVELS);
     release_graphs(c);  // the candidates' graphs
+    // keep only the tables the chosen configurations use (the candidates'
+    // tables of other tile sizes hold about 100 B per cube each)
+    std::set<std::tuple<uint32_t, uint32_t, int>> keep4, keep5;
+    for (const Level& L : P.levels) {
+        auto it = c->tuned.find(std::make_tuple(P.precision, p->op_mode, size_bucket(L.n)));
+        if (it == c->tuned.end()) continue;
+        const TuneEntry& e = it->second;
+        if (e.variant == 4) keep4.insert(std::make_tuple(L.off, L.n, e.block));
+        if (e.variant == 5) keep5.insert(std::make_tuple(L.off, L.n, e.block));
+        if (e.variant == 6 && L.n > (uint32_t)kLevel1) keep5.insert(std::make_tuple(L.off, L.n, 512));
+    }
+    for (auto it = P.halo.begin(); it != P.halo.end();)
+        it = keep4.count(it->first) ? std::next(it) : P.halo.erase(it);
+    for (auto it = P.halo2.begin(); it != P.halo2.end();) {
+        const auto k = std::make_tuple(std::get<0>(it->first), std::get<1>(it->first), std::get<2>(it->first));
+        it = keep5.count(k) ? std::next(it) : P.halo2.erase(it);
+    }
 }
 
 // tgv_minimize_streaming (tgv.cpp:802-1070) with every treetop leaf of a
