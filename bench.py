@@ -218,12 +218,20 @@ def main():
     wpath = os.path.join(ROOT, "tgvreconclaude_amd", "weights", "warmstart_mlp.npy")
     if not args.no_gap_run:
         sg, ta = 0.25 * args.step_ratio ** 0.5, 0.25 / args.step_ratio ** 0.5
+        # per-depth steps: sigma / tau = 256 up to four levels above the finest,
+        # 1024 for the next three, 4096 at the finest (sigma * tau = 1/16)
+        top = max(d for d, _ in levels)
+        ratios = [256 if d <= top - 4 else (1024 if d < top else 4096) for d in range(top + 1)]
+        sd, td = T.step_schedule(ratios)
+        sched = dict(sigma=sg, tau=ta, sigma_depth=sd, tau_depth=td)
         runs = [("cold_reference_steps", {}), ("cold", dict(sigma=sg, tau=ta)),
-                ("cold_dual", dict(sigma=sg, tau=ta, dual_eps=1e-4))]
+                ("cold_dual", dict(sigma=sg, tau=ta, dual_eps=1e-4)),
+                ("cold_sched", dict(sched)), ("cold_dual_sched", dict(sched, dual_eps=1e-4))]
         if not args.no_warm and os.path.exists(wpath):
             ctx.mlp_set(np.load(wpath))
             runs.append(("warm", dict(sigma=sg, tau=ta, dual_eps=1e-4, warm_start=1, warm_fallback=1)))
-        gap = {"tol": args.gap_tol, "step_ratio": args.step_ratio}
+            runs.append(("warm_sched", dict(sched, dual_eps=1e-4, warm_start=1, warm_fallback=1)))
+        gap = {"tol": args.gap_tol, "step_ratio": args.step_ratio, "step_ratio_per_depth": ratios}
         for name, kw in runs:
             pw = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, gap_tol=args.gap_tol,
                              chunk=50, max_iters=20000, use_graph=prm.use_graph, block=args.block, fused=prm.fused,
@@ -246,6 +254,11 @@ def main():
             warm = {"vs_cold_reference_steps": gap["warm"]["speedup_vs_cold_reference_steps"],
                     "vs_cold_same_steps": gap["warm"]["speedup_vs_cold"],
                     "vs_cold_dual_same_steps": round(gap["cold_dual"]["wall_ms"] / gap["warm"]["wall_ms"], 3)}
+        if "warm_sched" in gap:
+            warm["per_depth_steps"] = {
+                "vs_cold_reference_steps": gap["warm_sched"]["speedup_vs_cold_reference_steps"],
+                "vs_cold_same_steps": round(gap["cold_sched"]["wall_ms"] / gap["warm_sched"]["wall_ms"], 3),
+                "vs_cold_dual_same_steps": round(gap["cold_dual_sched"]["wall_ms"] / gap["warm_sched"]["wall_ms"], 3)}
 
     cpu = None
     if args.cpu_seconds > 0:

@@ -73,6 +73,8 @@ enum {
     TGVX_E_NUMERIC = 4 /* non-finite energy or gap */
 };
 
+#define TGVX_MAX_LEVELS 22
+
 /* TGVParams (src/tgv.hpp:24-28) plus the controls the reference hard-codes or
  * lacks. Defaults (tgvx_default_params) reproduce the reference. */
 typedef struct {
@@ -111,9 +113,12 @@ typedef struct {
                               (levels without an entry use fused / block as given) */
     float sigma, tau;      /* primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150).
                               Exact-adjoint mode converges for sigma * tau <= 1/16 */
+    /* per octree depth: > 0 replaces sigma (tau) for the level of that depth in tgvx_solve,
+       tgvx_solve_oc and the resident / batched solves (not the streaming solve); 0 = sigma
+       (tau). No reference counterpart. */
+    float sigma_depth[TGVX_MAX_LEVELS];
+    float tau_depth[TGVX_MAX_LEVELS];
 } tgvx_params;
-
-#define TGVX_MAX_LEVELS 22
 
 typedef struct {
     int32_t n_levels;

@@ -25,7 +25,8 @@ class OParams(C.Structure):
                 ("iters", C.c_int32), ("op_mode", C.c_int32), ("chunk", C.c_int32),
                 ("max_iters", C.c_int32), ("gap_tol", C.c_double), ("threads", C.c_int32),
                 ("warm_fallback", C.c_int32), ("dual_eps", C.c_float), ("warm_dual", C.c_int32),
-                ("sigma", C.c_float), ("tau", C.c_float)]
+                ("sigma", C.c_float), ("tau", C.c_float), ("sigma_depth", C.c_float * 22),
+                ("tau_depth", C.c_float * 22)]
 
 
 _L = None
@@ -80,9 +81,13 @@ def _p(a):
 
 
 def params(alpha0=0.02, alpha1=0.04, lam=1.0, iters=200, op_mode=0, chunk=50, max_iters=20000, gap_tol=0.0,
-           threads=1, warm_fallback=1, dual_eps=0.0, warm_dual=0, sigma=0.0, tau=0.0):
+           threads=1, warm_fallback=1, dual_eps=0.0, warm_dual=0, sigma=0.0, tau=0.0, sigma_depth=None,
+           tau_depth=None):
+    """sigma_depth / tau_depth: per octree depth (22 entries, > 0 replaces sigma / tau at that depth)."""
+    sd = (C.c_float * 22)(*(sigma_depth or [0.0] * 22))
+    td = (C.c_float * 22)(*(tau_depth or [0.0] * 22))
     return OParams(alpha0, alpha1, lam, iters, op_mode, chunk, max_iters, gap_tol, threads, warm_fallback, dual_eps,
-                   warm_dual, sigma, tau)
+                   warm_dual, sigma, tau, sd, td)
 
 
 def view(depths, fx, fy, cx, cy, m, lidar=False):

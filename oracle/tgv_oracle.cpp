@@ -267,6 +267,8 @@ struct Params {
     float dual_eps;         // with an init field: > 0 recovers p, q from (u, v), see dual_recover
     int32_t warm_dual;      // with an init field: 1 keeps the dual recovered from the cold init
     float sigma, tau;       // primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150)
+    // per octree depth: > 0 replaces sigma, tau for that depth's level (project-defined)
+    float sigma_depth[22], tau_depth[22];
 };
 
 using CodeMap = std::unordered_map<uint64_t, uint32_t>;
@@ -599,7 +601,10 @@ struct Solver {
     // level-start snapshots of run_level_frozen (tgv.cpp:263-298), or null.
     void iterate(const Level& L, int iters, std::vector<T>& un, std::vector<T>* vn, const T* fu,
                  const T* const* fv, uint32_t leaf_first, uint32_t leaf_last) {
-        const T sigma = prm.sigma > 0.f ? (T)prm.sigma : T(0.25), tau = prm.tau > 0.f ? (T)prm.tau : T(0.25);
+        const int dl = L.active.empty() ? 0 : (int)cubes[L.active[0]].depth;
+        const float sg = prm.sigma_depth[dl] > 0.f ? prm.sigma_depth[dl] : prm.sigma;
+        const float ta = prm.tau_depth[dl] > 0.f ? prm.tau_depth[dl] : prm.tau;
+        const T sigma = sg > 0.f ? (T)sg : T(0.25), tau = ta > 0.f ? (T)ta : T(0.25);
         const T a0 = (T)prm.alpha0, a1 = (T)prm.alpha1;
         const T tl = tau * (T)prm.lambda;
         const long na = (long)L.active.size();
@@ -1136,6 +1141,7 @@ typedef struct {
     float dual_eps;
     int32_t warm_dual;
     float sigma, tau;
+    float sigma_depth[22], tau_depth[22];
 } orc_params;
 
 void orc_morton_encode(uint32_t x, uint32_t y, uint32_t z, uint32_t* hi, uint32_t* lo) {
@@ -1195,6 +1201,10 @@ static orc::Params to_params(const orc_params* p) {
     q.max_iters = p->max_iters; q.gap_tol = p->gap_tol;
     q.threads = p->threads; q.warm_fallback = p->warm_fallback; q.dual_eps = p->dual_eps; q.warm_dual = p->warm_dual;
     q.sigma = p->sigma; q.tau = p->tau;
+    for (int d = 0; d < 22; ++d) {
+        q.sigma_depth[d] = p->sigma_depth[d];
+        q.tau_depth[d] = p->tau_depth[d];
+    }
     return q;
 }
 

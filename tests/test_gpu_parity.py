@@ -144,7 +144,7 @@ def test_cold_dual_recovery_to_gap(ctx, sphere):
     assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("prec", [FP32, FP64])
 def test_fused_iteration_identical_to_two_kernels(ctx, sphere, prec, variant):
     # exact-adjoint mode: one fused kernel per iteration (LDS-tiled or not) ==
@@ -159,7 +159,7 @@ def test_fused_iteration_identical_to_two_kernels(ctx, sphere, prec, variant):
                           b.view(np.uint32 if b.dtype == np.float32 else np.uint64))
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
 def test_fused_bf16_matches_two_kernels(ctx, sphere, variant):
     sc, hist = sphere
     a, _ = ctx.solve(sc.cubes, hist, TGVParams(alpha0=2.0, alpha1=1.0, iters=40, precision=BF16, op_mode=OP_ADJ,
@@ -202,3 +202,23 @@ def test_to_gap_adj_large_step_ratio(ctx, sphere, fused):
     assert [l["iters"] for l in rep.levels()] == it.tolist()
     assert all(l["rel_gap"] <= 1e-4 for l in rep.levels())
     assert np.array_equal(_state13(st).view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("prec", [FP32, FP64])
+def test_to_gap_with_depth_schedule(ctx, sphere, prec):
+    # per-depth steps (sigma / tau = 256 up to depth 5, 1024 at 6, 4096 at 7;
+    # sigma * tau = 1/16): bit-exact with the oracle under the same schedule,
+    # iteration counts equal, every level reaches the gap
+    from tgvreconclaude_amd import step_schedule
+    sc, hist = sphere
+    sd, td = step_schedule([256] * 6 + [1024, 4096])
+    prm = TGVParams(alpha0=2.0, alpha1=1.0, op_mode=OP_ADJ, gap_tol=1e-4, chunk=50, max_iters=8000,
+                    sigma=8.0, tau=1.0 / 128, precision=prec, fused=5, block=512, sigma_depth=sd, tau_depth=td)
+    st, rep = ctx.solve(sc.cubes, hist, prm)
+    ref, _, it, pd = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, op_mode=1, gap_tol=1e-4,
+                                                             chunk=50, max_iters=8000, sigma=8.0, tau=1.0 / 128,
+                                                             sigma_depth=sd, tau_depth=td), f64=prec == FP64)
+    assert [l["iters"] for l in rep.levels()] == it.tolist()
+    assert all(l["rel_gap"] <= 1e-4 for l in rep.levels())
+    w = np.uint64 if prec == FP64 else np.uint32
+    assert np.array_equal(np.ascontiguousarray(_state13(st)).view(w), ref.view(w))

@@ -613,3 +613,54 @@ def test_adam_step_closed_form():
     O.mlp_adam(w, m, v, g, 1, 1e-3)
     # first step with zero moments: w -= lr * sign(g) * |g| / (|g| + eps)
     np.testing.assert_allclose(w, w0 - 1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- per-depth steps
+
+def _two_level_octree(seed):
+    """A random level at depth 3 and its children-level neighbours at depth 4."""
+    rng = np.random.default_rng(seed)
+    coarse = sorted({tuple(p) for p in rng.integers(0, 6, (40, 3))})
+    fine = sorted({tuple(p) for p in rng.integers(0, 12, (120, 3))} - {(2 * x + i, 2 * y + j, 2 * z + k)
+                                                                         for x, y, z in coarse for i in (0, 1)
+                                                                         for j in (0, 1) for k in (0, 1)})
+    cs = cubes_of([cube(*p, 3) for p in coarse] + [cube(*p, 4) for p in fine])
+    order = np.lexsort((cs["depth"], cs["lo"], cs["hi"]))
+    cs = cs[order]
+    h = (rng.integers(0, 3, (len(cs), 10)) * (rng.random((len(cs), 1)) < 0.8)).astype(np.uint32)
+    return cs, h
+
+
+def test_depth_steps_equal_to_scalar_steps_bitwise():
+    cs, h = _two_level_octree(3)
+    base = dict(alpha0=2.0, alpha1=1.0, iters=120, op_mode=1)
+    a, _, _, _ = O.tgv_minimize(cs, h, O.params(sigma=2.0, tau=1.0 / 32, **base), f64=True)
+    b, _, _, _ = O.tgv_minimize(cs, h, O.params(sigma_depth=[2.0] * 22, tau_depth=[1.0 / 32] * 22, **base), f64=True)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_depth_steps_act_on_their_level_only():
+    # steps given for depth 4 change the depth-4 cubes and leave the depth-3
+    # level (iterated before, never written afterwards) bit for bit as it was
+    cs, h = _two_level_octree(4)
+    base = dict(alpha0=2.0, alpha1=1.0, iters=120, op_mode=1)
+    sd, td = [0.0] * 22, [0.0] * 22
+    sd[4], td[4] = 2.0, 1.0 / 32
+    a, _, _, _ = O.tgv_minimize(cs, h, O.params(**base), f64=True)
+    b, _, _, _ = O.tgv_minimize(cs, h, O.params(sigma_depth=sd, tau_depth=td, **base), f64=True)
+    c3 = cs["depth"] == 3
+    assert np.array_equal(a[c3].view(np.uint64), b[c3].view(np.uint64))
+    assert not np.array_equal(a[~c3].view(np.uint64), b[~c3].view(np.uint64))
+
+
+def test_depth_steps_gap_closes():
+    # sigma * tau = 1/16 at every depth (ratios 64 and 4096): exact PDHG, the gap closes
+    cs, h = _two_level_octree(5)
+    sd, td = [0.0] * 22, [0.0] * 22
+    sd[3], td[3] = 2.0, 1.0 / 32
+    sd[4], td[4] = 16.0, 1.0 / 256
+    _, _, it, pd = O.tgv_minimize(cs, h, O.params(alpha0=2.0, alpha1=1.0, op_mode=1, gap_tol=1e-6, chunk=50,
+                                                  max_iters=40000, sigma_depth=sd, tau_depth=td), f64=True)
+    for P, D in pd:
+        assert -1e-9 <= (P - D) / max(1.0, abs(P)) <= 1e-6
+    assert all(0 < k < 40000 for k in it)

@@ -21,7 +21,7 @@ from ._native import BF16, CUBE_DT, FP32, FP64, LEAF_DT, OP_ADJ, OP_REF, STATE_D
 
 __all__ = [
     "Context", "TGVParams", "compute_histograms", "tgv_minimize", "tgv_minimize_oc", "median_indicator",
-    "CUBE_DT", "STATE_DT", "LEAF_DT", "FP64", "FP32", "BF16", "OP_REF", "OP_ADJ", "make_view",
+    "CUBE_DT", "STATE_DT", "LEAF_DT", "FP64", "FP32", "BF16", "OP_REF", "OP_ADJ", "make_view", "step_schedule",
 ]
 
 
@@ -38,8 +38,21 @@ def TGVParams(**kw):
             k = "lambda_"
         if not hasattr(p, k):
             raise TypeError(f"unknown parameter {k}")
+        if k in ("sigma_depth", "tau_depth"):  # per octree depth, 22 entries (0 = sigma / tau)
+            v = (C.c_float * 22)(*[float(x) for x in v])
         setattr(p, k, v)
     return p
+
+
+def step_schedule(ratios, product=1.0 / 16):
+    """Per-depth steps (sigma_depth, tau_depth) with sigma / tau = ratios[depth] and
+    sigma * tau = product, for TGVParams(sigma_depth=..., tau_depth=...); depths
+    without a ratio (None or 0) keep the scalar steps."""
+    sd, td = [0.0] * 22, [0.0] * 22
+    for d, r in enumerate(ratios):
+        if r:
+            sd[d], td[d] = (product * r) ** 0.5, (product / r) ** 0.5
+    return sd, td
 
 
 def make_view(depths, fx, fy, cx, cy, world_to_cam, lidar=False):

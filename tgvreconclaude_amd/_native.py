@@ -33,7 +33,8 @@ class Params(C.Structure):
                 ("warm_fallback", C.c_int32), ("use_graph", C.c_int32), ("block", C.c_int32),
                 ("dual_eps", C.c_float), ("warm_start", C.c_int32), ("warm_dual", C.c_int32),
                 ("fused", C.c_int32), ("tune", C.c_int32),
-                ("sigma", C.c_float), ("tau", C.c_float)]
+                ("sigma", C.c_float), ("tau", C.c_float),
+                ("sigma_depth", C.c_float * 22), ("tau_depth", C.c_float * 22)]
 
 
 class Report(C.Structure):

@@ -1884,6 +1884,19 @@ KArgs<S> make_args(tgvx_ctx* c, const tgvx_params* prm) {
     return a;
 }
 
+// The level of octree depth `depth` iterates with the per-depth steps where
+// the parameters give them (sigma_depth, tau_depth > 0), else sigma, tau.
+template <class S>
+void set_steps(KArgs<S>& a, const tgvx_params* p, int depth) {
+    using T = typename Io<S>::T;
+    const bool in = depth >= 0 && depth < TGVX_MAX_LEVELS;
+    const float sg = in && p->sigma_depth[depth] > 0.f ? p->sigma_depth[depth] : p->sigma;
+    const float ta = in && p->tau_depth[depth] > 0.f ? p->tau_depth[depth] : p->tau;
+    a.sigma = sg > 0.f ? (T)sg : T(0.25);
+    a.tau = ta > 0.f ? (T)ta : T(0.25);
+    a.tl = a.tau * (T)p->lambda;
+}
+
 // Before level li iterates in the exact-adjoint mode: the previous level's
 // bars (the only frozen values it reads) as 4-value records.
 template <class S>
@@ -2444,6 +2457,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         a.n = L.n;
         a.lvl_off = L.off;
         a.lvl_n = L.n;
+        set_steps<S>(a, p, L.depth);
         const unsigned grid = (L.n + 255) / 256;
         int warm_kept = 0;
         if (!leaves) prepare_level<S>(c, a, p, li);
