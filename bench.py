@@ -284,7 +284,8 @@ def main():
 
     # HBM traffic per iteration from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     # summary of this workload (counters cannot be read from inside the run)
-    traffic = None
+    traffic = traffic_it = None
+    its_launch = 2 if two else 1
     pj = os.path.join(ROOT, "profiles", "tgv_iteration_depth10_torus_box.json")
     if args.precision == "fp32" and os.path.exists(pj):
         name = KERNEL_NAMES[variant]
@@ -292,7 +293,8 @@ def main():
             name += f", tile {tuning[-1]['block'] if tuning else args.block}"
         rec = json.load(open(pj)).get(name)
         if rec and rec["leaves"] == n_f:
-            traffic = rec["traffic_bytes_per_iteration"]
+            traffic_it = rec["traffic_bytes_per_iteration"]
+            traffic = traffic_it * rec.get("iterations_per_launch", 1)
 
     out = {
         "metric": "TGV leaf-iter/s + time-to-gap<=1e-4 (learned warm start vs cold), 1 GPU",
@@ -313,7 +315,11 @@ def main():
                    "parallelism": "single-gpu", "device": arch, "cus": cus},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_unit": "HBM bytes per iteration at the finest level, rocprofv3 FETCH_SIZE/WRITE_SIZE",
+                     "traffic_unit": "HBM bytes per launch of the kernel at the finest level (rocprofv3 "
+                                     "FETCH_SIZE x 2 + WRITE_SIZE, committed profile)",
+                     "traffic_per_iteration": traffic_it, "iterations_per_launch": its_launch,
+                     "launch_us": (fused_ms if fused else dual_ms + primal_ms) * 1e3 * its_launch,
+                     "alg_bytes_per_launch": ab * n_f * its_launch,
                      "kernel": kernel, "fused_us": fused_ms * 1e3,
                      "alg_bytes_per_leaf_iter": ab, "dual_us": dual_ms * 1e3, "primal_us": primal_ms * 1e3},
         "time_to_gap": gap,
