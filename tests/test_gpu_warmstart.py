@@ -23,7 +23,7 @@ def test_trained_network_on_held_out_scene(ctx):
     assert w.shape == (6404,) and np.isfinite(w).all()
     sc = scene.csg_scene(912345, views=24, W=640, H=480, max_depth=9)
     r = warmstart.solve_variants(ctx, sc, w)
-    for v in ("cold_reference_steps", "cold", "cold_dual", "warm"):
+    for v in ("cold_reference_steps", "cold", "cold_dual", "warm", "cold_sched", "cold_dual_sched", "warm_sched"):
         assert r[v]["reached"], v
     print({v: (round(r[v]["ms"], 1), sum(r[v]["iters"])) for v in ("cold_reference_steps", "cold", "cold_dual",
                                                                       "warm")})

@@ -44,8 +44,10 @@ USE_GRAPH = 1  # 0 (--no-graph): plain launches, e.g. under a counter-collecting
 
 
 def params(**kw):
+    # launches: levels of at most 1024 cubes in one workgroup, the others two
+    # iterations per launch with tiles of 512
     base = dict(alpha0=ALPHA0, alpha1=ALPHA1, precision=FP32, op_mode=OP_ADJ, gap_tol=GAP, chunk=50,
-                max_iters=40000, sigma=SIGMA, tau=TAU, use_graph=USE_GRAPH)
+                max_iters=40000, sigma=SIGMA, tau=TAU, use_graph=USE_GRAPH, fused=6)
     base.update(kw)
     return TGVParams(**base)
 
@@ -104,13 +106,25 @@ def initial_weights(seed):
     return w
 
 
+def depth_schedule(max_depth):
+    """Per-depth steps: sigma / tau = 256 up to four levels above the finest,
+    1024 for the next three, 4096 at the finest (sigma * tau = 1/16)."""
+    from . import step_schedule
+    sd, td = step_schedule([256 if d <= max_depth - 4 else (1024 if d < max_depth else 4096)
+                            for d in range(max_depth + 1)])
+    return dict(sigma_depth=sd, tau_depth=td)
+
+
 def solve_variants(ctx, sc, weights, diagnostics=False):
     hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
     ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
     out = {"cubes": int(len(sc.cubes))}
+    sched = depth_schedule(int(sc.cubes["depth"].max()))
     runs = {"cold_reference_steps": params(**REFERENCE_STEPS), "cold": params(),
             "cold_dual": params(dual_eps=DUAL_EPS),
-            "warm": params(dual_eps=DUAL_EPS, warm_start=1, warm_fallback=1)}
+            "warm": params(dual_eps=DUAL_EPS, warm_start=1, warm_fallback=1),
+            "cold_sched": params(**sched), "cold_dual_sched": params(dual_eps=DUAL_EPS, **sched),
+            "warm_sched": params(dual_eps=DUAL_EPS, warm_start=1, warm_fallback=1, **sched)}
     if diagnostics:
         # the network without the fallback rule, and the converged solution
         # itself as the initial (u, v): what a perfect predictor could gain
@@ -165,6 +179,19 @@ def evaluate(ctx, weights, n_scenes, seed0=900000, diagnostics=False):
         if extra in res[0]:
             x = np.array([r[extra]["ms"] for r in res])
             summary[f"speedup_{extra}_vs_cold_total"] = float(cold.sum() / x.sum())
+    # per-depth step schedule (cold, cold + dual, warm), against the same references
+    cs = np.array([r["cold_sched"]["ms"] for r in res])
+    cds = np.array([r["cold_dual_sched"]["ms"] for r in res])
+    ws = np.array([r["warm_sched"]["ms"] for r in res])
+    summary["schedule"] = {
+        "reached_per_variant": {v: int(sum(r[v]["reached"] for r in res))
+                                for v in ("cold_sched", "cold_dual_sched", "warm_sched")},
+        "cold_ms_total": float(cs.sum()), "cold_dual_ms_total": float(cds.sum()), "warm_ms_total": float(ws.sum()),
+        "speedup_warm_vs_cold_reference_steps_total": float(ref.sum() / ws.sum()),
+        "speedup_warm_vs_cold_reference_steps_median": float(np.median(ref / ws)),
+        "speedup_cold_dual_vs_cold_reference_steps_total": float(ref.sum() / cds.sum()),
+        "speedup_warm_vs_cold_same_steps_total": float(cs.sum() / ws.sum()),
+        "speedup_warm_vs_cold_dual_same_steps_total": float(cds.sum() / ws.sum())}
     return summary, res
 
 
