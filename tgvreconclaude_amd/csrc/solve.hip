@@ -824,14 +824,14 @@ __global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict_
                     hval[hp] = TILE + k;
                 } else {
                     hval[hp] = -1;
-                    failed = 1;
+                    atomicOr(&failed, 1);
                 }
                 return;
             }
             if (prev == j) return;
             hp = (hp + 1u) & (uint32_t)(H - 1);
         }
-        failed = 1;
+        atomicOr(&failed, 1);
     };
     auto find = [&](int32_t j) -> int32_t {
         uint32_t hp = hash(j);
@@ -845,7 +845,7 @@ __global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict_
     };
     auto lookup = [&](int32_t j) -> int32_t {
         const int32_t v = find(j);
-        if (v < 0) failed = 1;
+        if (v < 0) atomicOr(&failed, 1);
         return v;
     };
     auto pos = [&](int s) -> long { return s < TILE ? lo + s : (long)lst[s - TILE]; };
@@ -869,7 +869,7 @@ __global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict_
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             const int32_t k = nbt[ax][g];
-            if (k >= 0 && !in_level(k)) failed = 1;
+            if (k >= 0 && !in_level(k)) atomicOr(&failed, 1);
             else if (k >= 0 && !in_tile(k)) insert(k);
         }
     }
