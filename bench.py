@@ -56,8 +56,8 @@ def main():
     ap.add_argument("--max-depth", type=int, default=10)
     ap.add_argument("--iters", type=int, default=50, help="iterations per level in one step")
     ap.add_argument("--block", type=int, default=0,
-                    help="workgroup / tile size (0: 512 for the tiled fused variants, else 256); with the "
-                         "variant, the autotuner's base configuration")
+                    help="workgroup / tile size (0: 1024 for the two-iteration kernel, 512 for the other tiled "
+                         "fused variants, else 256); with the variant, the autotuner's base configuration")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5, 6], default=5,
                     help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "
@@ -85,7 +85,7 @@ def main():
     if args.no_fused:
         args.fused = 0
     if not args.block:
-        args.block = 512 if args.fused in (1, 3, 4, 5) else 256
+        args.block = {5: 1024, 1: 512, 3: 512, 4: 512}.get(args.fused, 256)
 
     if args.calibrate_copy:
         import ctypes

@@ -107,8 +107,8 @@ typedef struct {
                               per launch over tile sets tabulated once per level and tile (a
                               level whose sets do not fit runs as 1); 6 a level of at most 1024
                               cubes in one workgroup for all its iterations (larger levels run
-                              as 5 with tiles of 512). For 1, 3, 4 and 5, block is the tile
-                              (128, 256 = default, 512, and 1024 except for 5) */
+                              as 5, tiles of 1024 from 2^18 cubes on, else 512). For 1, 3, 4
+                              and 5, block is the tile (128, 256 = default, 512, 1024) */
     int32_t tune;          /* 1: take each level's launch configuration from tgvx_autotune's table
                               (levels without an entry use fused / block as given) */
     float sigma, tau;      /* primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150).

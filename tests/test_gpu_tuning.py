@@ -20,7 +20,7 @@ def _raw(st):
 
 @pytest.mark.parametrize("fused,block", [(0, 64), (0, 128), (1, 128), (1, 512), (1, 1024), (2, 64), (2, 128),
                                          (3, 128), (3, 256), (3, 1024), (4, 128), (4, 256), (4, 512), (4, 1024),
-                                         (5, 128), (5, 256), (5, 512), (6, 0)])
+                                         (5, 128), (5, 256), (5, 512), (5, 1024), (6, 0)])
 def test_launch_shapes_identical(ctx, sphere, fused, block):
     sc, hist = sphere
     base = dict(alpha0=2.0, alpha1=1.0, iters=60, precision=FP32, op_mode=OP_ADJ)
@@ -43,7 +43,7 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
         if l["variant"] in (1, 3, 4):
             assert l["block"] in (128, 256, 512, 1024)
         elif l["variant"] == 5:
-            assert l["block"] in (128, 256, 512)
+            assert l["block"] in (128, 256, 512, 1024)
         elif l["variant"] == 6:
             assert l["block"] == 1024 and l["leaves"] <= 1024
         else:
@@ -66,7 +66,8 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
     assert np.array_equal(_raw(tuned), _raw(plain))
 
 
-@pytest.mark.parametrize("prec,tile,graph", [(FP32, 128, 1), (FP32, 256, 1), (FP32, 512, 1), (FP32, 256, 0),
+@pytest.mark.parametrize("prec,tile,graph", [(FP32, 128, 1), (FP32, 256, 1), (FP32, 512, 1), (FP32, 1024, 1),
+                                             (FP32, 256, 0),
                                              (BF16, 256, 1), (FP64, 128, 1)])
 def test_two_iteration_kernel(ctx, sphere, prec, tile, graph):
     # fused = 5: two iterations per launch over the tabulated tile sets; the

@@ -1919,8 +1919,6 @@ int block_of(const tgvx_params* p) {
     const bool tiled = p && p->op_mode == TGVX_OP_ADJ &&
                        (p->fused == 1 || p->fused == 3 || p->fused == 4 || p->fused == 5 || p->fused == 6);
     if (b > 256 && !tiled) throw Error(TGVX_E_ARG, "blocks above 256 exist only for the tiled fused kernels");
-    if (p && p->op_mode == TGVX_OP_ADJ && p->fused == 5 && b == 1024)
-        throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256 or 512");
     return b;
 }
 
@@ -2068,6 +2066,7 @@ size_t fused2_wg_per_cu(int tile) {
     switch (tile) {
         case 128: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 128, true>)); break;
         case 256: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 256, true>)); break;
+        case 1024: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 1024, true>)); break;
         default: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 512, true>)); break;
     }
     if (e != hipSuccess || fa.numRegs <= 0) return 1;
@@ -2110,7 +2109,8 @@ const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
             case 128: halo2_table_launch<128>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
             case 256: halo2_table_launch<256>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
             case 512: halo2_table_launch<512>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
-            default: throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256 or 512");
+            case 1024: halo2_table_launch<1024>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
+            default: throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256, 512 or 1024");
         }
         TGVX_HIP(hipGetLastError());
         std::vector<int32_t> cnt(4 * (size_t)ntiles);
@@ -2191,6 +2191,9 @@ bool launch_fused2_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<
                 break;
             case 256:
                 k_fused2<S, 256, X><<<cl.count, 256, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                break;
+            case 1024:
+                k_fused2<S, 1024, X><<<cl.count, 1024, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
                 break;
             default:
                 k_fused2<S, 512, X><<<cl.count, 512, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
@@ -2297,6 +2300,11 @@ struct GraphCache {
 };
 std::map<const tgvx_ctx*, GraphCache> g_graphs;
 
+// Variant 6 on a level larger than kLevel1: the two-iteration kernel, tiles of
+// 1024 once the level has one per CU (2^18 cubes; measured 2.6-6 % faster than
+// 512 at 0.38-4 M cubes, 12 % slower at 97 k), else 512.
+inline Launch level1_fallback(uint32_t n) { return Launch{5, n >= (1u << 18) ? 1024 : 512, 1}; }
+
 template <class S>
 void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, const Launch& L0) {
     if (iters <= 0 || a.n == 0) return;
@@ -2307,7 +2315,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
             TGVX_HIP(hipGetLastError());
             return;
         }
-        L = Launch{5, 512, 1};
+        L = level1_fallback(a.n);
     }
     const bool fused = L.variant != 0 && iters >= 2;
     if (fused) alt_set<S>(c);  // allocate outside any capture
@@ -2737,7 +2745,7 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     if (p->op_mode == TGVX_OP_ADJ) {
         Launch fz = cfg;
         if (fz.variant == 0) fz = Launch{1, kFuseTile, cfg.xcd};
-        if (fz.variant == 6) fz = Launch{5, 512, 1};
+        if (fz.variant == 6) fz = level1_fallback(a.n);
         // the two-iteration kernel: per launch / 2 (one iteration per
         // launch where the level's tile sets do not fit)
         int per = 1;
@@ -2780,7 +2788,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
             for (int v : {1, 3, 4})
                 for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{v, b, x});
             for (int b : {256, 128}) cand.push_back(Launch{2, b, x});
-            for (int b : {256, 128, 512}) cand.push_back(Launch{5, b, x});
+            for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{5, b, x});
         }
     }
     if (p->op_mode == TGVX_OP_ADJ) cand.push_back(Launch{6, kLevel1, 0});
@@ -2864,7 +2872,8 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
         const TuneEntry& e = it->second;
         if (e.variant == 4) keep4.insert(std::make_tuple(L.off, L.n, e.block));
         if (e.variant == 5) keep5.insert(std::make_tuple(L.off, L.n, e.block));
-        if (e.variant == 6 && L.n > (uint32_t)kLevel1) keep5.insert(std::make_tuple(L.off, L.n, 512));
+        if (e.variant == 6 && L.n > (uint32_t)kLevel1)
+            keep5.insert(std::make_tuple(L.off, L.n, level1_fallback(L.n).block));
     }
     for (auto it = P.halo.begin(); it != P.halo.end();)
         it = keep4.count(it->first) ? std::next(it) : P.halo.erase(it);
@@ -3328,8 +3337,8 @@ void tile_sets_t(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max
     for (const Level& l : P.levels)
         if (l.depth == depth) L = &l;
     if (!L) throw Error(TGVX_E_ARG, "no cubes at depth " + std::to_string(depth));
-    if (tile != 128 && tile != 256 && tile != 512)
-        throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256 or 512");
+    if (tile != 128 && tile != 256 && tile != 512 && tile != 1024)
+        throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256, 512 or 1024");
     tgvx_params q;
     tgvx_default_params(&q);
     q.op_mode = TGVX_OP_ADJ;
