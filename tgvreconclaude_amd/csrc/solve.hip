@@ -782,8 +782,9 @@ struct Halo2Tab {
 
 // One workgroup per tile. nba = depth-aware +a neighbours, nbt = their
 // transpose within the level; [off, off + n) must be the whole level. Members
-// outside the tile are collected in an LDS hash set (a member's slot is its
-// insertion rank, so the numbering depends on scheduling; the results do not).
+// outside the tile are collected in an LDS hash set, then the members of
+// PC \ T, DB \ PC and PA \ DB are each numbered in position order, so that
+// neighbouring lanes of k_fused2 read neighbouring addresses of the halo.
 template <int TILE>
 __global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict__ nba0,
                                                       const int32_t* __restrict__ nba1,
@@ -795,7 +796,7 @@ __global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict_
                                                       int32_t* __restrict__ bslot, int32_t* __restrict__ cslot) {
     constexpr int CA = kHalo2Cap * TILE, NH = CA - TILE;
     constexpr int LH = log2_ceil(2 * NH), H = 1 << LH;
-    __shared__ int32_t hkey[H], hval[H], lst[NH];
+    __shared__ int32_t hkey[H], hval[H], lst[NH], rk[NH];
     __shared__ int32_t nh, failed, cP, cD;
     const int32_t* const nba[3] = {nba0, nba1, nba2};
     const int32_t* const nbt[3] = {nbt0, nbt1, nbt2};
@@ -889,6 +890,34 @@ __global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict_
     }
     __syncthreads();
     const int nPA = TILE + min((int)nh, NH);
+    // position order within each class: rank among the class's members
+    const int nX = nPA - TILE;
+    for (int k = tid; k < nX; k += TILE) {
+        const int sl = TILE + k;
+        const int k0 = (sl < nPC ? TILE : sl < nDB ? nPC : nDB) - TILE;
+        const int k1 = (sl < nPC ? nPC : sl < nDB ? nDB : nPA) - TILE;
+        const int32_t v = lst[k];
+        int r = k0;
+        for (int m = k0; m < k1; ++m) r += lst[m] < v ? 1 : 0;
+        rk[k] = r;
+    }
+    int32_t mv[kHalo2Cap - 1];
+#pragma unroll
+    for (int i = 0; i < kHalo2Cap - 1; ++i) {
+        const int k = tid + i * TILE;
+        mv[i] = k < nX ? lst[k] : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kHalo2Cap - 1; ++i) {
+        const int k = tid + i * TILE;
+        if (k < nX) lst[rk[k]] = mv[i];
+    }
+    for (int i = tid; i < H; i += TILE) {
+        const int32_t sv = hval[i];
+        if (hkey[i] != -1 && sv >= TILE) hval[i] = TILE + rk[sv - TILE];
+    }
+    __syncthreads();
     int32_t* const L = list + (long)tile * NH;
     for (int k = tid; k < nPA - TILE; k += TILE) L[k] = lst[k];
     int32_t* const B = bslot + (long)tile * 3 * CA;
@@ -973,10 +1002,14 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
 #pragma unroll
         for (int k = 0; k < 6; ++k) pq[(3 + k) * sa + s] = IO::ld(src.f[F_Q0 + k], g);
     };
+    // incoming edge of axis ax: p_ax and the q entries of row ax of eps
+    constexpr int kF[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
     // passes 3-5 of slot s (the expressions of primal_adj_at): own p, q and
     // the incoming ones from LDS (cv >= 0) or from src (cv <= -2, p, q of the
-    // first iteration only); u_bar, v_bar to LDS, the new u, v in nw
-    auto primal = [&](int s, uint32_t w, const T od[4], const int cv[3], T nw[4]) {
+    // first iteration only; pin holds them if have_in); u_bar, v_bar to LDS,
+    // the new u, v in nw
+    auto primal = [&](int s, uint32_t w, const T od[4], const int cv[3], T nw[4], bool have_in,
+                      const T (&pin)[3][4]) {
         T p[3], qs[6];
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = pq[c * sa + s];
@@ -994,8 +1027,6 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
                 dq[c] = hs ? dq[c] + w2 * qs[qidx(c, ax) - F_Q0] : dq[c];
             }
         }
-        // incoming edge of axis ax: p_ax and the q entries of row ax of eps
-        constexpr int kF[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
         T in[3][4];
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
@@ -1003,6 +1034,9 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
             if (v >= 0) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) in[ax][e] = pq[(kF[ax][e] - F_PX) * sa + v];
+            } else if (have_in) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) in[ax][e] = v <= -2 ? pin[ax][e] : T(0);
             } else {
                 const long j = v <= -2 ? -2L - v : 0L;
 #pragma unroll
@@ -1070,7 +1104,12 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     };
     // ---- stage p, q of PA. Slots tid (0) and tid + TILE (1) keep their
     // table entries, info words, u, v and (slot 0) frozen neighbours in
-    // registers.
+    // registers. With tiles of 1024 a CU holds one workgroup whatever the
+    // VGPRs up to 128, so in single precision slot 1 also keeps its frozen
+    // neighbours and its incoming p, q from outside the sets (the global
+    // reads of the first primal and the dual step), which leaves those phases
+    // LDS-only (in double precision these registers would spill).
+    constexpr bool PF = TILE >= 1024 && sizeof(T) == 4;
     const int s1 = tid + TILE;
     long g0 = 0, g1 = 0;
     const bool l0 = live(tid, g0);
@@ -1078,7 +1117,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     int b0[3] = {-1, -1, -1}, c0[3] = {-1, -1, -1}, b1[3] = {-1, -1, -1}, c1[3] = {-1, -1, -1};
     uint32_t w0 = 0u, w1 = 0u;
     T od0[4] = {T(0), T(0), T(0), T(0)}, od1[4] = {T(0), T(0), T(0), T(0)};
-    T fz0[3][4];
+    T fz0[3][4], fz1[3][4], in1[3][4];
     if (l0) {
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
@@ -1102,6 +1141,17 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         w1 = a.info[g1];
         load_uv(g1, od1);
         stage(s1, g1);
+        if constexpr (PF) {
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                if (b1[ax] <= -2) frozen(b1[ax], fz1[ax]);
+                if (c1[ax] <= -2) {
+                    const long j = -2L - c1[ax];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) in1[ax][e] = IO::ld(src.f[kF[ax][e]], j);
+                }
+            }
+        }
     }
     for (int s = tid + 2 * TILE; s < nA; s += TILE) {
         long g;
@@ -1110,15 +1160,15 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     __syncthreads();
     // ---- primal n+1 over PA
     T nw0[4] = {T(0), T(0), T(0), T(0)}, nw1[4] = {T(0), T(0), T(0), T(0)};
-    if (l0) primal(tid, w0, od0, c0, nw0);
-    if (l1) primal(s1, w1, od1, c1, nw1);
+    if (l0) primal(tid, w0, od0, c0, nw0, false, in1);
+    if (l1) primal(s1, w1, od1, c1, nw1, PF, in1);
     for (int s = tid + 2 * TILE; s < nA; s += TILE) {
         long g;
         if (!live(s, g)) continue;
         const int cv[3] = {cs[s], cs[CA + s], cs[2 * CA + s]};
         T od[4], nw[4];
         load_uv(g, od);
-        primal(s, a.info[g], od, cv, nw);
+        primal(s, a.info[g], od, cv, nw, false, in1);
         if (s < nP) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) uvx[c * sx + s - 2 * TILE] = IO::rt(nw[c]);
@@ -1132,7 +1182,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     __syncthreads();
     // ---- dual n+2 over DB
     if (l0) dual2(tid, b0, true, fz0);
-    if (l1 && s1 < nD) dual2(s1, b1, false, fz0);
+    if (l1 && s1 < nD) dual2(s1, b1, PF, fz1);
     for (int s = tid + 2 * TILE; s < nD; s += TILE) {
         long g;
         if (!live(s, g)) continue;
@@ -1142,10 +1192,10 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     __syncthreads();
     // ---- primal n+2 over PC
     T own[4] = {T(0), T(0), T(0), T(0)};
-    if (l0) primal(tid, w0, nw0, c0, own);
+    if (l0) primal(tid, w0, nw0, c0, own, false, in1);
     if (l1 && s1 < nP) {
         T x[4];
-        primal(s1, w1, nw1, c1, x);
+        primal(s1, w1, nw1, c1, x, false, in1);
     }
     for (int s = tid + 2 * TILE; s < nP; s += TILE) {
         long g;
@@ -1154,7 +1204,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         T od[4], x[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) od[c] = uvx[c * sx + s - 2 * TILE];
-        primal(s, a.info[g], od, cv, x);
+        primal(s, a.info[g], od, cv, x, false, in1);
     }
     __syncthreads();
     // ---- dual n+3 over the tile
