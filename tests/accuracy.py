@@ -112,20 +112,34 @@ def fp64_to_gap(ctx, sc, hist, tol=1e-4):
             "reached": all(l["rel_gap"] <= tol for l in rep.levels()), "solve_ms": rep.solve_ms}
 
 
-def against_oracle(ctx, sc, hist, tol, sigma, tau, threads):
-    """GPU FP32 and FP64 solves to the gap against the FP64 CPU oracle's."""
+def against_oracle(ctx, sc, hist, tol, sigma, tau, threads, fused=None, block=None):
+    """GPU FP32 and FP64 solves to the gap against the FP64 CPU oracle's;
+    fused / block select the GPU's launch variant (default: the library's)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     has = (hist.sum(axis=1) > 0).astype(np.uint8)
     leaf = 2.0 * sc.r_root / 2 ** int(sc.cubes["depth"].max())
     kw = dict(gap_tol=tol, sigma=sigma, tau=tau)
+    launch = {}
+    if fused is not None:
+        launch["fused"] = fused
+    if block is not None:
+        launch["block"] = block
     gpu = {}
     for prec in (FP32, FP64):
         ctx.load(sc.cubes, hist, precision=prec, op_mode=OP_ADJ)
-        rep = ctx.run(_params(prec, **kw))
+        rep = ctx.run(_params(prec, **kw, **launch))
         st = ctx.state()
         gpu[prec] = (st, [l["iters"] for l in rep.levels()], rep.solve_ms)
-        log(f"gpu {NAMES[prec]} to gap: {rep.solve_ms:.0f} ms, iterations {gpu[prec][1]}")
+        log(f"gpu {NAMES[prec]} to gap ({launch or 'default launch'}): {rep.solve_ms:.0f} ms, "
+            f"iterations {gpu[prec][1]}")
+    same32 = None
+    if launch:  # the FP32 state of the chosen launch against the default launch's
+        ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+        ctx.run(_params(FP32, **kw))
+        a, b = np.asarray(ctx.state()), np.asarray(gpu[FP32][0])
+        same32 = bool(a.tobytes() == b.tobytes())
+        log(f"fp32 state equal to the default launch's bit for bit: {same32}")
     t0 = time.time()
     log(f"oracle fp64 to gap, {threads} host threads ...")
     ref, _, its, _ = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, op_mode=1, gap_tol=tol, chunk=50,
@@ -144,7 +158,9 @@ def against_oracle(ctx, sc, hist, tol, sigma, tau, threads):
     vb, _ = surface(ctx, sc, ref[:, 0], has)
     h = hausdorff(va, vb)
     log(f"fp64 gpu == oracle bit for bit: {bit_exact}; fp32 linf u {d.max():.3g}; Hausdorff {h / leaf:.3g} leaf")
-    return {"sigma": sigma, "tau": tau, "gap_tol": tol, "oracle_threads": threads, "oracle_s": round(t_or, 1),
+    return {"sigma": sigma, "tau": tau, "gap_tol": tol, "launch": launch or "default",
+            "fp32_state_equal_to_default_launch": same32,
+            "oracle_threads": threads, "oracle_s": round(t_or, 1),
             "oracle_iters_per_level": its.tolist(), "gpu_fp64_iters_per_level": gpu[FP64][1],
             "gpu_fp32_iters_per_level": gpu[FP32][1], "gpu_fp64_ms": gpu[FP64][2], "gpu_fp32_ms": gpu[FP32][2],
             "fp64_state_bit_exact_with_oracle": bit_exact,
@@ -161,6 +177,8 @@ def main():
     ap.add_argument("--bf16-max-iters", type=int, default=6000, help="per-level cap of the BF16 to-gap attempt")
     ap.add_argument("--oracle", action="store_true", help="FP64 reference = the CPU oracle (steps sigma/tau = 1024)")
     ap.add_argument("--threads", type=int, default=0, help="oracle host threads (0: the cores available, at most 16)")
+    ap.add_argument("--fused", type=int, default=None, help="with --oracle: the GPU's launch variant (0-6)")
+    ap.add_argument("--block", type=int, default=None, help="with --oracle: the GPU's workgroup / tile size")
     args = ap.parse_args()
     from tgvreconclaude_amd import Context, scene
     t0 = time.time()
@@ -170,7 +188,7 @@ def main():
     hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
     if args.oracle:
         threads = args.threads or min(16, len(os.sched_getaffinity(0)))
-        res = against_oracle(ctx, sc, hist, args.tol, 8.0, 1.0 / 128, threads)
+        res = against_oracle(ctx, sc, hist, args.tol, 8.0, 1.0 / 128, threads, args.fused, args.block)
         with open(args.out, "w") as f:
             json.dump({"workload": "torus_box_depth10", "cubes": int(len(sc.cubes)), "oracle": res}, f, indent=1)
         ctx.close()
