@@ -101,7 +101,8 @@ def compare(ctx, sc, hist, prec, tol=1e-4, level_iters=None, max_iters=20000):
     return {"precision": NAMES[prec], "gap_tol": tol if level_iters is None else None, "reached": reached,
             "iters_per_level": its, "final_rel_gap": gaps,
             "solve_ms": rep.solve_ms, "linf_u": float(d.max()), "linf_u_voted": float(d[has > 0].max()),
-            "mean_abs_u": float(d.mean()), "hausdorff": h, "leaf_edge": leaf, "hausdorff_leaf": h / leaf,
+            "mean_abs_u": float(d.mean()), "rms_u": float(np.sqrt(np.mean(d * d))),
+            "l2_u": float(np.linalg.norm(d)), "hausdorff": h, "leaf_edge": leaf, "hausdorff_leaf": h / leaf,
             "surface_vertices": [int(len(va)), int(len(vb))], "wall_s": round(time.time() - t0, 2)}
 
 
@@ -166,6 +167,7 @@ def against_oracle(ctx, sc, hist, tol, sigma, tau, threads, fused=None, block=No
             "fp64_state_bit_exact_with_oracle": bit_exact,
             "fp64_iters_equal": gpu[FP64][1] == its.tolist(),
             "fp32_linf_u": float(d.max()), "fp32_linf_u_voted": float(d[has > 0].max()), "fp32_mean_abs_u": float(d.mean()),
+            "fp32_rms_u": float(np.sqrt(np.mean(d * d))), "fp32_l2_u": float(np.linalg.norm(d)),
             "hausdorff": h, "leaf_edge": leaf, "hausdorff_leaf": h / leaf,
             "surface_vertices": [int(len(va)), int(len(vb))]}
 
