@@ -35,7 +35,9 @@ _L = None
 def lib():
     global _L
     if _L is None:
-        p = os.path.join(HERE, "liboracle.so")
+        # ORACLE_LIB: another build of the same source (the tests that check
+        # the pins against deliberately altered copies of the oracle use it)
+        p = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(p):
             raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
         L = C.CDLL(p)
