@@ -150,6 +150,24 @@ int tgvx_device_info(tgvx_ctx* ctx, char* name, int32_t name_len, int32_t* cus);
 int tgvx_histograms(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views,
                     int32_t nviews, const float origin[3], float r_root, int32_t nb, uint32_t* bins_out);
 
+/* Allocation-free compute_histograms. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). cubes,
+ * bins (n*nb counts, as tgvx_histograms) and every view's depths are DEVICE
+ * memory of the context's device; the views array itself is host memory.
+ * Everything else -- the view table and the pyramid levels above each base
+ * level -- lives in one fixed device array of TGVX_NOALLOC_SCRATCH_BYTES per
+ * device that the library holds until the process ends, so the call makes no
+ * allocation. Limits: at most TGVX_NOALLOC_MAX_VIEWS views, and their
+ * pyramid levels above the base (about a third of the base pixels) within the
+ * array less the view table (e.g. 40 views of 1280x960); larger inputs are
+ * rejected with TGVX_E_ARG. Results equal tgvx_histograms' bit for bit. Calls
+ * on one device share the array: calls that overlap on one device race. */
+#define TGVX_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define TGVX_NOALLOC_MAX_VIEWS 1024
+int tgvx_histograms_noalloc(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views,
+                            int32_t nviews, const float origin[3], float r_root, int32_t nb, uint32_t* bins);
+
 /* DepthMipmap::build of one view: all levels concatenated into out (size
  * >= width*height*4/3 + 16), (w, h) per level into dims; level count in *nlevels. */
 int tgvx_mip_build(tgvx_ctx* ctx, const tgvx_view* view, float* out, int32_t* dims, int32_t* nlevels);
@@ -165,7 +183,9 @@ int tgvx_solve(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint32_t
                const tgvx_params* params, const float* init_uv, void* state_out, tgvx_report* report);
 
 /* tgv_minimize_oc: per treetop leaf, out-of-leaf u_bar / v_bar read from the
- * level-start snapshot. All leaves of a level are solved in one launch. */
+ * level-start snapshot. The leaves of a level run one after another, in the
+ * order given, as in the reference (a later leaf reads the final p, q of the
+ * earlier ones, tgv.cpp:489-507). */
 int tgvx_solve_oc(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
                   const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* params,
                   void* state_out, tgvx_report* report);

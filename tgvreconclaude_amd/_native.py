@@ -106,7 +106,7 @@ TGVX_SYMBOLS = [
     "tgvx_mlp_loss_grad", "tgvx_mlp_train", "tgvx_problem_features", "tgvx_problem_load_blocks",
     "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune", "tgvx_problem_solve_levels",
     "tgvx_surface_extract", "tgvx_surface_copy", "tgvx_solve_streaming", "tgvx_octree_build",
-    "tgvx_octree_balance", "tgvx_octree_copy", "tgvx_octree_merge", "tgvx_tile_sets",
+    "tgvx_octree_balance", "tgvx_octree_copy", "tgvx_octree_merge", "tgvx_tile_sets", "tgvx_histograms_noalloc",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
@@ -138,6 +138,7 @@ def lib():
         L.tgvx_last_error.restype = C.c_char_p
         L.tgvx_device_info.argtypes = [vp, C.c_char_p, i32, C.POINTER(i32)]
         L.tgvx_histograms.argtypes = [vp, vp, u64, C.POINTER(View), i32, vp, C.c_float, i32, vp]
+        L.tgvx_histograms_noalloc.argtypes = [vp, vp, u64, C.POINTER(View), i32, vp, C.c_float, i32, vp]
         L.tgvx_mip_build.argtypes = [vp, C.POINTER(View), vp, vp, C.POINTER(i32)]
         L.tgvx_median_init.argtypes = [vp, vp, u64, i32, vp]
         L.tgvx_solve.argtypes = [vp, vp, u64, vp, i32, C.POINTER(Params), vp, vp, C.POINTER(Report)]

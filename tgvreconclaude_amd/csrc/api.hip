@@ -68,6 +68,7 @@ int tgvx_create(tgvx_ctx** out, int32_t device) {
             return TGVX_E_HIP;
         }
     }
+    tgvx::noalloc_setup(device);
     *out = c;
     return TGVX_OK;
 }
@@ -100,6 +101,11 @@ int tgvx_device_info(tgvx_ctx* c, char* name, int32_t name_len, int32_t* cus) {
 int tgvx_histograms(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views, int32_t nviews,
                     const float origin[3], float r_root, int32_t nb, uint32_t* bins_out) {
     return guarded(c, [&] { tgvx::histograms(c, cubes, n, views, nviews, origin, r_root, nb, bins_out); });
+}
+
+int tgvx_histograms_noalloc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views,
+                            int32_t nviews, const float origin[3], float r_root, int32_t nb, uint32_t* bins) {
+    return guarded(c, [&] { tgvx::histograms_noalloc(c, cubes, n, views, nviews, origin, r_root, nb, bins); });
 }
 
 int tgvx_mip_build(tgvx_ctx* c, const tgvx_view* view, float* out, int32_t* dims, int32_t* nlevels) {

@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "tgvx_device.h"
@@ -26,7 +27,7 @@ struct ViewMeta {
     float fx, fy, cx, cy;
     int32_t lidar, nlev, pad0, pad1;
     int32_t w[kMaxLevels], h[kMaxLevels];
-    uint64_t off[kMaxLevels];
+    const float* lv[kMaxLevels];  // device address of each pyramid level
 };
 
 struct Log2Thr {
@@ -79,7 +80,6 @@ struct VoteArgs {
     uint64_t n;
     const ViewMeta* views;
     int32_t nv, nb;
-    const float* pyr;
     float ox, oy, oz, r_root;
     Log2Thr thr;
     uint32_t* bins;
@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(256) k_vote(VoteArgs a) {
                 const int uu = min(max(u0 >> lvl, 0), V.w[lvl] - 1);
                 const int vv = min(max(v0 >> lvl, 0), V.h[lvl] - 1);
                 // depth_at + vote (histogram.cpp:148-158)
-                const float d = a.pyr[V.off[lvl] + (uint64_t)vv * V.w[lvl] + uu];
+                const float d = V.lv[lvl][(uint64_t)vv * V.w[lvl] + uu];
                 if (!isfinite(d) || d <= 0.0f) continue;
                 float s = d - z;
                 if (s < -eta) continue;
@@ -190,38 +190,47 @@ std::vector<std::pair<int, int>> mip_dims(int w, int h) {
     return d;
 }
 
+// Fills the view table (pyramid levels at base + off) and returns the floats
+// all levels need.
+uint64_t view_meta(const tgvx_view& in, ViewMeta& m, uint64_t off) {
+    if (in.width <= 0 || in.height <= 0 || !in.depths) throw Error(TGVX_E_ARG, "view without depths");
+    std::memset(&m, 0, sizeof(m));
+    std::memcpy(m.m, in.world_to_cam, sizeof(m.m));
+    m.fx = in.fx; m.fy = in.fy; m.cx = in.cx; m.cy = in.cy;
+    m.lidar = in.lidar;
+    const auto d = mip_dims(in.width, in.height);
+    m.nlev = (int)d.size();
+    for (int l = 0; l < m.nlev; ++l) {
+        m.w[l] = d[l].first;
+        m.h[l] = d[l].second;
+        m.lv[l] = reinterpret_cast<const float*>(off * sizeof(float));  // offset until placed
+        off += (uint64_t)d[l].first * d[l].second;
+    }
+    return off;
+}
+
+void launch_mips(const ViewMeta& m, hipStream_t s) {
+    for (int l = 1; l < m.nlev; ++l) {
+        const int cnt = m.w[l] * m.h[l];
+        k_mip<<<(cnt + 255) / 256, 256, 0, s>>>(m.lv[l - 1], m.w[l - 1], m.h[l - 1], const_cast<float*>(m.lv[l]),
+                                               m.w[l], m.h[l]);
+        TGVX_HIP(hipGetLastError());
+    }
+}
+
 // Uploads the base levels and builds every pyramid on the device.
 void build_pyramids(tgvx_ctx* c, const tgvx_view* views, int nv, std::vector<ViewMeta>& meta) {
     meta.assign(nv, ViewMeta{});
     uint64_t total = 0;
-    for (int v = 0; v < nv; ++v) {
-        const tgvx_view& in = views[v];
-        if (in.width <= 0 || in.height <= 0 || !in.depths) throw Error(TGVX_E_ARG, "view without depths");
-        ViewMeta& m = meta[v];
-        std::memcpy(m.m, in.world_to_cam, sizeof(m.m));
-        m.fx = in.fx; m.fy = in.fy; m.cx = in.cx; m.cy = in.cy;
-        m.lidar = in.lidar;
-        const auto d = mip_dims(in.width, in.height);
-        m.nlev = (int)d.size();
-        for (int l = 0; l < m.nlev; ++l) {
-            m.w[l] = d[l].first;
-            m.h[l] = d[l].second;
-            m.off[l] = total;
-            total += (uint64_t)d[l].first * d[l].second;
-        }
-    }
+    for (int v = 0; v < nv; ++v) total = view_meta(views[v], meta[v], total);
     c->h_pyr.ensure(total * sizeof(float));
-    float* pyr = c->h_pyr.as<float>();
+    char* pyr = c->h_pyr.as<char>();
     for (int v = 0; v < nv; ++v) {
-        const ViewMeta& m = meta[v];
-        TGVX_HIP(hipMemcpyAsync(pyr + m.off[0], views[v].depths, sizeof(float) * (size_t)m.w[0] * m.h[0],
+        ViewMeta& m = meta[v];
+        for (int l = 0; l < m.nlev; ++l) m.lv[l] = reinterpret_cast<const float*>(pyr + (uintptr_t)m.lv[l]);
+        TGVX_HIP(hipMemcpyAsync(const_cast<float*>(m.lv[0]), views[v].depths, sizeof(float) * (size_t)m.w[0] * m.h[0],
                                 hipMemcpyHostToDevice, c->stream));
-        for (int l = 1; l < m.nlev; ++l) {
-            const int cnt = m.w[l] * m.h[l];
-            k_mip<<<(cnt + 255) / 256, 256, 0, c->stream>>>(pyr + m.off[l - 1], m.w[l - 1], m.h[l - 1],
-                                                            pyr + m.off[l], m.w[l], m.h[l]);
-            TGVX_HIP(hipGetLastError());
-        }
+        launch_mips(m, c->stream);
     }
 }
 }  // namespace
@@ -247,7 +256,6 @@ void histograms(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view
     a.views = c->h_meta.as<ViewMeta>();
     a.nv = nv;
     a.nb = nb;
-    a.pyr = c->h_pyr.as<float>();
     a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2];
     a.r_root = r_root;
     a.thr = thr;
@@ -273,7 +281,7 @@ void mip_build(tgvx_ctx* c, const tgvx_view* v, float* out, int32_t* dims, int32
         total += (uint64_t)m.w[l] * m.h[l];
     }
     *nlevels = m.nlev;
-    TGVX_HIP(hipMemcpyAsync(out, c->h_pyr.p, sizeof(float) * total, hipMemcpyDeviceToHost, c->stream));
+    TGVX_HIP(hipMemcpyAsync(out, m.lv[0], sizeof(float) * total, hipMemcpyDeviceToHost, c->stream));
     TGVX_HIP(hipStreamSynchronize(c->stream));
 }
 
@@ -287,6 +295,123 @@ void median_init(tgvx_ctx* c, const uint32_t* bins, uint64_t n, int nb, float* o
                                                                    c->h_pyr.as<float>());
     TGVX_HIP(hipGetLastError());
     TGVX_HIP(hipMemcpyAsync(out, c->h_pyr.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+}
+
+// ---- allocation-free histogram voting (no reference counterpart)
+// For callers that cannot allocate device memory (code under graph capture,
+// tight serving loops): inputs and output are the caller's device buffers, and
+// everything else the vote needs -- the view table and the pyramid levels above
+// each caller-held base level -- lives in one file-scope device array. Every
+// device holds its own copy until the process ends; host code never uses the
+// array's name, only the address hipGetSymbolAddress gives on each device.
+constexpr uint64_t kScratchBytes = (uint64_t)TGVX_NOALLOC_SCRATCH_BYTES;
+constexpr uint64_t kMetaBytes = (sizeof(ViewMeta) * TGVX_NOALLOC_MAX_VIEWS + 255) / 256 * 256;
+static_assert(kMetaBytes < kScratchBytes, "scratch too small for the view table");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+void* scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) throw Error(TGVX_E_STATE, "device index beyond the scratch table");
+    if (!g_scratch[device])
+        throw Error(TGVX_E_STATE, std::string("allocation-free scratch has no address on this device: ") +
+                                      hipGetErrorString(g_scratch_err[device]));
+    return g_scratch[device];
+}
+
+// The caller's buffer must be device memory of the context's device.
+void require_device(const void* p, int device, const char* what) {
+    hipPointerAttribute_t at;
+    if (!p || hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(TGVX_E_ARG, std::string(what) + " is not device memory");
+    }
+    if ((at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged) || at.device != device)
+        throw Error(TGVX_E_ARG, std::string(what) + " is not memory of the context's device");
+}
+}  // namespace
+
+// Once per device, with that device current (tgvx_create): the address of the
+// device's copy of scratch. A failure is kept and reported by the entry point.
+void noalloc_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices || g_scratch[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && ptr) {
+        g_scratch[device] = ptr;
+    } else {
+        g_scratch_err[device] = e;
+        (void)hipGetLastError();
+    }
+}
+
+void histograms_noalloc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views, int nv,
+                        const float origin[3], float r_root, int nb, uint32_t* bins) {
+    if (nb <= 0 || nb > kMaxBins) throw Error(TGVX_E_ARG, "nb must be in 1..32");
+    if (nv < 0 || (nv > 0 && !views)) throw Error(TGVX_E_ARG, "bad views");
+    if (nv > TGVX_NOALLOC_MAX_VIEWS)
+        throw Error(TGVX_E_ARG, "the allocation-free call takes at most " + std::to_string(TGVX_NOALLOC_MAX_VIEWS) +
+                                    " views");
+    if (n == 0) return;
+    if (!origin) throw Error(TGVX_E_ARG, "null pointer");
+    const uint64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffffull) throw Error(TGVX_E_ARG, "too many cubes");
+    require_device(cubes, c->device, "cubes");
+    require_device(bins, c->device, "bins");
+    // pyramid levels above the base, as DepthMipmap::build makes them
+    uint64_t need = 0;
+    for (int v = 0; v < nv; ++v) {
+        ViewMeta m;
+        need += view_meta(views[v], m, 0) - (uint64_t)m.w[0] * m.h[0];
+        require_device(views[v].depths, c->device, "view depths");
+    }
+    if (need * sizeof(float) > kScratchBytes - kMetaBytes)
+        throw Error(TGVX_E_ARG, "the views' pyramid levels need " + std::to_string(need * sizeof(float)) +
+                                    " bytes; the allocation-free call holds at most " +
+                                    std::to_string(kScratchBytes - kMetaBytes));
+    char* base = static_cast<char*>(scratch_of(c->device));
+    ViewMeta* meta_dev = reinterpret_cast<ViewMeta*>(base);
+    float* mip = reinterpret_cast<float*>(base + kMetaBytes);
+    static const Log2Thr thr = log2_thresholds();
+    // the view table in chunks from the stack: each chunk is copied and its
+    // pyramids launched, then the stream drains before the chunk is reused
+    constexpr int kChunk = 16;
+    ViewMeta chunk[kChunk];
+    uint64_t off = 0;
+    for (int v0 = 0; v0 < nv; v0 += kChunk) {
+        const int k = std::min(kChunk, nv - v0);
+        for (int j = 0; j < k; ++j) {
+            ViewMeta& m = chunk[j];
+            view_meta(views[v0 + j], m, 0);
+            m.lv[0] = views[v0 + j].depths;
+            for (int l = 1; l < m.nlev; ++l) {
+                m.lv[l] = mip + off;
+                off += (uint64_t)m.w[l] * m.h[l];
+            }
+            launch_mips(m, c->stream);
+        }
+        TGVX_HIP(hipMemcpyAsync(meta_dev + v0, chunk, sizeof(ViewMeta) * k, hipMemcpyHostToDevice, c->stream));
+        TGVX_HIP(hipStreamSynchronize(c->stream));
+    }
+    VoteArgs a;
+    a.cubes = cubes;
+    a.n = n;
+    a.views = meta_dev;
+    a.nv = nv;
+    a.nb = nb;
+    a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2];
+    a.r_root = r_root;
+    a.thr = thr;
+    a.bins = bins;
+    k_vote<<<(unsigned)blocks, 256, sizeof(uint32_t) * nb * 256, c->stream>>>(a);
+    TGVX_HIP(hipGetLastError());
     TGVX_HIP(hipStreamSynchronize(c->stream));
 }
 

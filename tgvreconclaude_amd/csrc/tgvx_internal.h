@@ -203,4 +203,7 @@ void histograms(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view
                 const float origin[3], float r_root, int nb, uint32_t* bins_out);
 void mip_build(tgvx_ctx* c, const tgvx_view* v, float* out, int32_t* dims, int32_t* nlevels);
 void median_init(tgvx_ctx* c, const uint32_t* bins, uint64_t n, int nb, float* out);
+void noalloc_setup(int device);
+void histograms_noalloc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view* views, int nv,
+                        const float origin[3], float r_root, int nb, uint32_t* bins);
 }  // namespace tgvx
