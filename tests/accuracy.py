@@ -72,25 +72,25 @@ def u_of(st):
     return st["u"].astype(np.float64) if st.dtype.names else st[:, 0].copy()
 
 
-def compare(ctx, sc, hist, prec, tol=1e-4, level_iters=None, max_iters=20000):
+def compare(ctx, sc, hist, prec, tol=1e-4, level_iters=None, max_iters=20000, **steps):
     """Report for one storage precision against FP64 run for the same
     iterations: solved to relative gap tol, or for the given per-level
-    iteration counts."""
+    iteration counts. steps: sigma, tau, sigma_depth, tau_depth for both runs."""
     has = (hist.sum(axis=1) > 0).astype(np.uint8)
     leaf = 2.0 * sc.r_root / 2 ** int(sc.cubes["depth"].max())
     ctx.load(sc.cubes, hist, precision=prec, op_mode=OP_ADJ)
     t0 = time.time()
     if level_iters is None:
-        rep = ctx.run(_params(prec, gap_tol=tol, max_iters=max_iters))
+        rep = ctx.run(_params(prec, gap_tol=tol, max_iters=max_iters, **steps))
     else:
-        rep = ctx.run_levels(_params(prec), level_iters)
+        rep = ctx.run_levels(_params(prec, **steps), level_iters)
     its = [l["iters"] for l in rep.levels()]
     gaps = [l["rel_gap"] for l in rep.levels()]
     reached = all(g <= tol for g in gaps)
     u = u_of(ctx.state())
     log(f"{NAMES[prec]}: {rep.solve_ms:.0f} ms, iterations {its}")
     ctx.load(sc.cubes, hist, precision=FP64, op_mode=OP_ADJ)
-    r64 = ctx.run_levels(_params(FP64), its)
+    r64 = ctx.run_levels(_params(FP64, **steps), its)
     u64 = u_of(ctx.state())
     log(f"fp64 for the same iterations: {r64.solve_ms:.0f} ms")
     d = np.abs(u - u64)
