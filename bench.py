@@ -72,7 +72,8 @@ def main():
                     help="sigma / tau of the tuned to-gap runs (sigma * tau = 1/16 as the reference's 0.25 * 0.25)")
     ap.add_argument("--gap-tol", type=float, default=1e-4)
     ap.add_argument("--no-gap-run", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="CPU baseline budget over its three timings (0 = skip)")
     ap.add_argument("--workload", choices=["depth10", "batched"], default="depth10",
                     help="batched: 256 independent blocks of log-uniform size in one problem vs one problem per block")
     ap.add_argument("--blocks", type=int, default=256)
@@ -271,23 +272,42 @@ def main():
     if args.cpu_seconds > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        op = O.params(alpha0=2.0, alpha1=1.0, op_mode=1, threads=1)
-        t_one, na = O.time_level(cubes, hist, op, finest, 1)
-        k = max(1, int(args.cpu_seconds / max(t_one, 1e-3)))
-        t_k, na = O.time_level(cubes, hist, op, finest, k)
-        cpu = {"value": na * k / t_k, "unit": "leaf-iter/s", "cores": 1, "kind": "port",
-               "sample": f"{k} iterations of the depth-{finest} level ({na} leaves) of this workload, "
-                         f"FP32 restatement of run_level (tgv.cpp:137-232), 1 thread, {t_k:.1f} s"}
-        log(f"[bench] CPU baseline: {cpu['value']:.4g} leaf-iter/s ({k} its, {t_k:.1f} s)")
-        # the same loops over all host cores (OpenMP over cubes, Jacobi passes)
-        ncpu = min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0)))
-        opm = O.params(alpha0=2.0, alpha1=1.0, op_mode=1, threads=ncpu)
-        t_m1, _ = O.time_level(cubes, hist, opm, finest, 1)
-        km = max(1, int(args.cpu_seconds / max(t_m1, 1e-3)))
-        t_km, _ = O.time_level(cubes, hist, opm, finest, km)
-        cpu["all_cores"] = {"value": na * km / t_km, "cores": ncpu,
-                            "sample": f"{km} iterations of the same level, {ncpu} OpenMP threads, {t_km:.1f} s"}
-        log(f"[bench] CPU, {ncpu} threads: {cpu['all_cores']['value']:.4g} leaf-iter/s")
+        model, _ = O.host_cpu()
+        _, march = O.timing_lib()
+        # the process's CPU share: the cores it may run on, as the job's
+        # thread limit caps them (OMP_NUM_THREADS)
+        share = len(os.sched_getaffinity(0))
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        ncpu = min(share, omp) if omp > 0 else share
+
+        def timed(op_mode, threads, budget):
+            op = O.params(alpha0=2.0, alpha1=1.0, op_mode=op_mode, threads=threads)
+            t1, na = O.time_level(cubes, hist, op, finest, 1)
+            k = max(1, int(budget / max(t1, 1e-3)))
+            tk, na = O.time_level(cubes, hist, op, finest, k)
+            return na * k / tk, k, tk, na
+
+        budget = args.cpu_seconds / 3
+        v1, k1, t1, na = timed(0, 1, budget)
+        cpu = {"value": v1, "unit": "leaf-iter/s", "cores": 1, "kind": "port",
+               "sample": f"{k1} iterations of the depth-{finest} level ({na} leaves) of this workload: the CPU "
+                         f"restatement of run_level (tgv.cpp:137-232) with the reference's update rules, AoS "
+                         f"TGVCube, active[] indirection, the median recomputed in prox_data, FP32, "
+                         f"-O3 -march={march}, 1 thread, {t1:.1f} s",
+               "host_cpu": model, "nproc": os.cpu_count(), "cpu_share": ncpu, "march": march}
+        log(f"[bench] CPU baseline (reference rules, 1 thread): {v1:.4g} leaf-iter/s ({k1} its, {t1:.1f} s)")
+        # the same loops over the process's cores (OpenMP over cubes, Jacobi passes)
+        vm, km, tm, _ = timed(0, ncpu, budget)
+        cpu["all_cores"] = {"value": vm, "cores": ncpu,
+                            "sample": f"{km} iterations of the same level, reference rules, {ncpu} OpenMP threads "
+                                      f"(the process's CPU share of {share} cores), {tm:.1f} s"}
+        log(f"[bench] CPU, reference rules, {ncpu} threads: {vm:.4g} leaf-iter/s")
+        # the exact-adjoint iteration the GPU sweep runs, same loops, one thread
+        va, ka, ta, _ = timed(1, 1, budget)
+        cpu["adjoint_1_thread"] = {"value": va, "cores": 1,
+                                   "sample": f"{ka} iterations of the same level, exact-adjoint rules (the "
+                                             f"GPU sweep's iteration), 1 thread, {ta:.1f} s"}
+        log(f"[bench] CPU, exact-adjoint rules, 1 thread: {va:.4g} leaf-iter/s")
 
     # HBM traffic per iteration from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     # summary of this workload (counters cannot be read from inside the run)

@@ -257,12 +257,49 @@ def log2_rule_mismatches(lo, hi, thr):
     return lib().orc_log2_rule_mismatches(lo, hi, _p(t), len(t))
 
 
+def host_cpu():
+    """(model name, has the AVX-512 subsets of x86-64-v4) from /proc/cpuinfo."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return model, {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+
+
+_T = {}
+
+
+def timing_lib():
+    """(library, -march) for timing on this host: the x86-64-v4 build where the
+    host has AVX-512, else the x86-64-v3 one (same source, same results)."""
+    if "lib" not in _T:
+        v4 = os.path.join(HERE, "liboracle_v4.so")
+        if host_cpu()[1] and os.path.exists(v4):
+            L, march = C.CDLL(v4), "x86-64-v4"
+        else:
+            L, march = lib(), "x86-64-v3"
+        L.orc_time_level.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int32, C.POINTER(OParams), C.c_int32,
+                                     C.c_int32, C.POINTER(C.c_uint64)]
+        L.orc_time_level.restype = C.c_double
+        _T["lib"], _T["march"] = L, march
+    return _T["lib"], _T["march"]
+
+
 def time_level(cubes, hists, prm, depth, iters):
-    """Seconds for `iters` iterations of one level of the CPU restatement, and its cube count."""
+    """Seconds for `iters` iterations of one level of the CPU restatement (the
+    reference's AoS loops; prm.op_mode 0 = its own update rules), and the
+    level's cube count."""
     cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
     h = np.ascontiguousarray(hists, dtype=np.uint32)
     na = C.c_uint64()
-    t = lib().orc_time_level(_p(cubes), len(cubes), _p(h), h.shape[1], C.byref(prm), depth, iters, C.byref(na))
+    L, _ = timing_lib()
+    t = L.orc_time_level(_p(cubes), len(cubes), _p(h), h.shape[1], C.byref(prm), depth, iters, C.byref(na))
     return t, na.value
 
 

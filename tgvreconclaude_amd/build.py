@@ -76,7 +76,7 @@ def build_host():
 
 
 def build_oracle():
-    _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
+    _run(["make", "-s", "-B", "-j2", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
