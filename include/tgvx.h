@@ -207,6 +207,13 @@ int tgvx_problem_load(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const u
 /* Solve the resident problem from its cold (or given) initialisation. */
 int tgvx_problem_solve(tgvx_ctx* ctx, const tgvx_params* params, const float* init_uv,
                        tgvx_report* report);
+/* tgvx_problem_solve starting every level from a given state (no reference
+ * counterpart): n TGVCube records (save_tgv's payload, tgv.cpp:17-46) in
+ * input order; each level's u, v, p, q are taken from it and u_bar = u,
+ * v_bar = v, instead of the median / parent-copy initialisation (resuming a
+ * saved solve, or a warm start that includes the dual). With warm_fallback a
+ * level keeps its cold start where that starts from a smaller gap. */
+int tgvx_problem_solve_from(tgvx_ctx* ctx, const tgvx_params* params, const tgvx_state* init, tgvx_report* report);
 /* tgvx_problem_solve with a fixed iteration count per level (n_levels = the
  * number of depths present, as tgvx_problem_levels; gap_tol must be 0): runs
  * another precision for exactly the iterations a to-gap solve took */

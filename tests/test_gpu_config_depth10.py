@@ -56,7 +56,9 @@ def _flat(st):
 @pytest.mark.parametrize("prec,mode", [(FP64, OP_ADJ), (FP32, OP_REF)])
 def test_one_chunk_per_level_equals_oracle(ctx, torus_box, prec, mode):
     sc, hist = torus_box
-    p = TGVParams(alpha0=2.0, alpha1=1.0, iters=50, precision=prec, op_mode=mode, fused=6 if mode == OP_ADJ else 1)
+    # the benchmark's launches: two iterations per launch (the smallest levels
+    # in one workgroup in the exact-adjoint mode)
+    p = TGVParams(alpha0=2.0, alpha1=1.0, iters=50, precision=prec, op_mode=mode, fused=6)
     ctx.load(sc.cubes, hist, precision=prec, op_mode=mode)
     rep = ctx.run(p)
     got = _flat(ctx.state())

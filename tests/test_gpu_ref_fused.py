@@ -1,0 +1,81 @@
+"""The reference's update rules (tgv.cpp:137-232, the drop-in tgv_minimize)
+with two iterations per launch over tabulated tile sets: bit for bit the
+two-kernel iteration and the CPU oracle, for every tile size, odd and even
+iteration counts, FP32 and FP64, inputs where octant-0 children share their
+parents' codes, and several blocks in one problem."""
+import numpy as np
+import pytest
+
+import oracle as O
+import test_ref_rules_pins as pins
+from tgvreconclaude_amd import FP32, FP64, OP_REF, TGVParams, scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _flat(st):
+    if st.dtype.names:
+        return np.concatenate([st["u"][:, None], st["v"], st["p"], st["q"]], axis=1)
+    return np.ascontiguousarray(st)
+
+
+def _bits(a):
+    return a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+
+
+@pytest.mark.parametrize("fused,block,iters", [(5, 128, 60), (5, 256, 61), (5, 512, 60), (5, 1024, 61),
+                                               (6, 0, 60), (6, 0, 1), (6, 0, 2), (6, 0, 3)])
+def test_ref_two_iterations_per_launch_equal_oracle(ctx, sphere, fused, block, iters):
+    sc, hist = sphere
+    p = TGVParams(alpha0=2.0, alpha1=1.0, iters=iters, precision=FP32, op_mode=OP_REF, fused=fused, block=block)
+    st, _ = ctx.solve(sc.cubes, hist, p)
+    ref = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, iters=iters, op_mode=0))[0]
+    bad = np.flatnonzero(np.any(_bits(_flat(st)) != _bits(ref), axis=1))
+    assert bad.size == 0, f"{bad.size} cubes differ"
+
+
+def test_ref_fp64_equals_oracle(ctx, sphere):
+    sc, hist = sphere
+    p = TGVParams(alpha0=2.0, alpha1=1.0, iters=100, precision=FP64, op_mode=OP_REF, fused=6)
+    st, _ = ctx.solve(sc.cubes, hist, p)
+    ref = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, iters=100, op_mode=0), f64=True)[0]
+    assert np.array_equal(_bits(_flat(st)), _bits(ref))
+
+
+@pytest.mark.parametrize("seed", pins.SEEDS)
+@pytest.mark.parametrize("block", [128, 512])
+def test_ref_code_collisions(ctx, seed, block):
+    # octant-0 children sharing codes with coarser cubes, in both orders,
+    # coarser fallbacks and neighbours of other depths (read frozen or stale)
+    cs, h = pins.scene(seed)
+    p = TGVParams(alpha0=0.5, alpha1=0.3, iters=20, precision=FP32, op_mode=OP_REF, fused=5, block=block)
+    st, _ = ctx.solve(cs, h, p)
+    ref = O.tgv_minimize(cs, h, O.params(alpha0=0.5, alpha1=0.3, iters=20, op_mode=0))[0]
+    assert np.array_equal(_bits(_flat(st)), _bits(ref))
+
+
+def test_ref_csg_scene_tuned(ctx):
+    sc = scene.csg_scene(21, views=24, W=640, H=480, max_depth=8)
+    hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
+    p = TGVParams(alpha0=2.0, alpha1=1.0, iters=50, precision=FP32, op_mode=OP_REF, fused=6)
+    ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_REF)
+    tuned = ctx.autotune(p, iters=50)
+    assert all(l["variant"] in (0, 5) for l in tuned)
+    p.tune = 1
+    ctx.run(p)
+    got = _flat(ctx.state())
+    ref = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, iters=50, op_mode=0, threads=8))[0]
+    assert np.array_equal(_bits(got), _bits(ref))
+
+
+def test_ref_batched_blocks(ctx):
+    blocks = []
+    for seed, depth in ((31, 7), (32, 6), (33, 7)):
+        sc = scene.csg_scene(seed, views=12, W=320, H=240, max_depth=depth)
+        blocks.append((sc.cubes, ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)))
+    p = TGVParams(alpha0=2.0, alpha1=1.0, iters=40, precision=FP32, op_mode=OP_REF, fused=6)
+    ctx.load_blocks(blocks, precision=FP32, op_mode=OP_REF)
+    ctx.run(p)
+    for (cubes, hist), got in zip(blocks, ctx.split_blocks(ctx.state())):
+        ref = O.tgv_minimize(cubes, hist, O.params(alpha0=2.0, alpha1=1.0, iters=40, op_mode=0))[0]
+        assert np.array_equal(_bits(_flat(got)), _bits(ref))

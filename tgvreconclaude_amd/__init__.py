@@ -240,8 +240,17 @@ class Context:
             o += n
         return out
 
-    def run(self, params, init_uv=None):
+    def run(self, params, init_uv=None, init_state=None):
+        """Solve the resident problem; init_uv (n x 4: u, v) or init_state (n
+        records of 13 FP32 values: u, v, p, q, as STATE_DT) replaces each
+        level's cold initialisation."""
         rep = Report()
+        if init_state is not None:
+            st = np.ascontiguousarray(init_state)
+            st = st.view(np.float32).reshape(-1, 13) if st.dtype.names else np.ascontiguousarray(st, np.float32)
+            assert st.shape == (self._n, 13)
+            self._check(self._L.tgvx_problem_solve_from(self.h, C.byref(params), N.ptr(st), C.byref(rep)))
+            return rep
         init = None if init_uv is None else np.ascontiguousarray(init_uv, dtype=np.float32)
         self._check(self._L.tgvx_problem_solve(self.h, C.byref(params), N.ptr(init), C.byref(rep)))
         return rep

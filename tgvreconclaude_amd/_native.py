@@ -107,6 +107,7 @@ TGVX_SYMBOLS = [
     "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune", "tgvx_problem_solve_levels",
     "tgvx_surface_extract", "tgvx_surface_copy", "tgvx_solve_streaming", "tgvx_octree_build",
     "tgvx_octree_balance", "tgvx_octree_copy", "tgvx_octree_merge", "tgvx_tile_sets", "tgvx_histograms_noalloc",
+    "tgvx_problem_solve_from",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
@@ -145,6 +146,7 @@ def lib():
         L.tgvx_solve_oc.argtypes = [vp, vp, u64, vp, i32, vp, u64, C.POINTER(Params), vp, C.POINTER(Report)]
         L.tgvx_problem_load.argtypes = [vp, vp, u64, vp, i32, i32, i32]
         L.tgvx_problem_solve.argtypes = [vp, C.POINTER(Params), vp, C.POINTER(Report)]
+        L.tgvx_problem_solve_from.argtypes = [vp, C.POINTER(Params), vp, C.POINTER(Report)]
         L.tgvx_problem_state.argtypes = [vp, vp, vp]
         L.tgvx_problem_gap.argtypes = [vp, i32, C.POINTER(Params), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tgvx_problem_levels.argtypes = [vp, vp, vp, C.POINTER(i32)]

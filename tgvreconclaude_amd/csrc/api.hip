@@ -125,6 +125,13 @@ int tgvx_problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, 
     return guarded(c, [&] { tgvx::problem_solve(c, p, init_uv, rep, nullptr, 0, nullptr); });
 }
 
+int tgvx_problem_solve_from(tgvx_ctx* c, const tgvx_params* p, const tgvx_state* init, tgvx_report* rep) {
+    return guarded(c, [&] {
+        if (!init) throw tgvx::Error(TGVX_E_ARG, "null initial state");
+        tgvx::problem_solve(c, p, reinterpret_cast<const float*>(init), rep, nullptr, 0, nullptr, 13);
+    });
+}
+
 int tgvx_problem_solve_levels(tgvx_ctx* c, const tgvx_params* p, const int32_t* level_iters, int32_t n_levels,
                               tgvx_report* rep) {
     return guarded(c, [&] { tgvx::problem_solve_levels(c, p, level_iters, n_levels, rep); });

@@ -48,7 +48,8 @@ struct KArgs {
     const S* fz[4];
     const uint32_t* orig;  // position -> input index
     uint32_t leaf_first, leaf_last;
-    const float* init;     // warm start field, 4 per position
+    const float* init;     // initial field: init_stride values per position (4: u, v; 13: u, v, p, q)
+    int32_t init_stride;
     S* save[4];
     // several blocks iterated to a gap: a cube is skipped while its block's
     // byte in bact is 0 (the block has converged); null: every cube runs
@@ -259,6 +260,60 @@ template <class S>
 struct Set13 {
     S* f[13];
 };
+
+// Passes 3-5 with the reference's rules (as k_primal_ref) reading (u, v, p, q)
+// from src -- a cube of another depth from the main set, which the level
+// never writes -- and writing the new u, v to dst and u_bar, v_bar to the main
+// set; copy_dual also moves p, q from src to dst. Ends a run of
+// reference-rule iterations that alternate between two sets. Same
+// expressions, in the same order, as k_primal_ref.
+template <class S, bool XCD = true>
+__global__ void __launch_bounds__(256) k_primal_ref_set(KArgs<S> a, Set13<S> src, Set13<S> dst, int copy_dual) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = (uint32_t)tile_id<XCD>() * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const long g = (long)a.off + t;
+    if (masked(a, g)) return;
+    const int m0 = a.nbm[0][g], m2 = a.nbm[1][g], m4 = a.nbm[2][g];
+    auto rd = [&](int f, int m) -> T {
+        if (m < 0) return T(0);
+        const bool live = (uint32_t)m - a.lvl_off < a.lvl_n;
+        return live ? IO::ld(src.f[f], m) : IO::ld(a.s[f], m);
+    };
+    const T px = IO::ld(src.f[F_PX], g), py = IO::ld(src.f[F_PY], g), pz = IO::ld(src.f[F_PZ], g);
+    const T div_p = (px - rd(F_PX, m0)) + (py - rd(F_PY, m2)) + (pz - rd(F_PZ, m4));
+    const T u = IO::ld(src.f[F_U], g);
+    const T un = prox(u + a.tau * div_p, a.info[g], a.nbins, a.tl);
+    const T q0 = IO::ld(src.f[F_Q0], g), q1 = IO::ld(src.f[F_Q1], g), q2 = IO::ld(src.f[F_Q2], g),
+            q3 = IO::ld(src.f[F_Q3], g), q4 = IO::ld(src.f[F_Q4], g), q5 = IO::ld(src.f[F_Q5], g);
+    const T dqx = (q0 - rd(F_Q0, m0)) + (q3 - rd(F_Q3, m2)) + (q4 - rd(F_Q4, m4));
+    const T dqy = (q3 - rd(F_Q3, m0)) + (q1 - rd(F_Q1, m2)) + (q5 - rd(F_Q5, m4));
+    const T dqz = (q4 - rd(F_Q4, m0)) + (q5 - rd(F_Q5, m2)) + (q2 - rd(F_Q2, m4));
+    const T vx = IO::ld(src.f[F_VX], g), vy = IO::ld(src.f[F_VY], g), vz = IO::ld(src.f[F_VZ], g);
+    const T vxn = vx + a.tau * (px + dqx);
+    const T vyn = vy + a.tau * (py + dqy);
+    const T vzn = vz + a.tau * (pz + dqz);
+    IO::st(a.s[F_UB], g, T(2) * un - u);
+    IO::st(a.s[F_VBX], g, T(2) * vxn - vx);
+    IO::st(a.s[F_VBY], g, T(2) * vyn - vy);
+    IO::st(a.s[F_VBZ], g, T(2) * vzn - vz);
+    IO::st(dst.f[F_U], g, un);
+    IO::st(dst.f[F_VX], g, vxn);
+    IO::st(dst.f[F_VY], g, vyn);
+    IO::st(dst.f[F_VZ], g, vzn);
+    if (copy_dual) {
+        IO::st(dst.f[F_PX], g, px);
+        IO::st(dst.f[F_PY], g, py);
+        IO::st(dst.f[F_PZ], g, pz);
+        IO::st(dst.f[F_Q0], g, q0);
+        IO::st(dst.f[F_Q1], g, q1);
+        IO::st(dst.f[F_Q2], g, q2);
+        IO::st(dst.f[F_Q3], g, q3);
+        IO::st(dst.f[F_Q4], g, q4);
+        IO::st(dst.f[F_Q5], g, q5);
+    }
+}
 
 // Passes 3-5 of cube g in the exact-adjoint mode, from the values in src:
 // the new (u, v) in nw, the old ones in od.
@@ -776,8 +831,9 @@ struct Halo2Tab {
                            // -1 the cube itself, >= 0 a slot, <= -2 the cube -2 - v of
                            // another level (its frozen u_bar, v_bar)
     const int32_t* cslot;  // per tile, 3 x kHalo2Cap * TILE: the cube whose +a neighbour
-                           // slot s < nPA is: -1 none, >= 0 its slot, <= -2 the position
-                           // -2 - v outside the tile's sets (never for s < nPC)
+                           // slot s < nPA is (reference rules: slot s's -a neighbour):
+                           // -1 none, >= 0 its slot, <= -2 the position -2 - v outside
+                           // the tile's sets (for s < nPC only outside the level)
 };
 
 // One workgroup per tile. nba = depth-aware +a neighbours, nbt = their
@@ -785,7 +841,11 @@ struct Halo2Tab {
 // outside the tile are collected in an LDS hash set, then the members of
 // PC \ T, DB \ PC and PA \ DB are each numbered in position order, so that
 // neighbouring lanes of k_fused2 read neighbouring addresses of the halo.
-template <int TILE>
+// REF (the reference's rules): nba = the reference table's +a neighbours and
+// nbt its -a neighbours (tgv.cpp:97-133), whose p, q the primal step reads;
+// both may point outside the level (cubes of other depths, constant during
+// the level): such a -a source is stored as -2 - position.
+template <int TILE, bool REF = false>
 __global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict__ nba0,
                                                       const int32_t* __restrict__ nba1,
                                                       const int32_t* __restrict__ nba2,
@@ -870,8 +930,13 @@ __global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict_
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             const int32_t k = nbt[ax][g];
-            if (k >= 0 && !in_level(k)) atomicOr(&failed, 1);
-            else if (k >= 0 && !in_tile(k)) insert(k);
+            if (REF) {
+                if (k >= 0 && in_level(k) && !in_tile(k)) insert(k);
+            } else if (k >= 0 && !in_level(k)) {
+                atomicOr(&failed, 1);
+            } else if (k >= 0 && !in_tile(k)) {
+                insert(k);
+            }
         }
     }
     __syncthreads();
@@ -940,6 +1005,7 @@ __global__ void __launch_bounds__(TILE) k_halo2_table(const int32_t* __restrict_
             int32_t v;
             if (k < 0) v = -1;
             else if (in_tile(k)) v = (int32_t)(k - lo);
+            else if (REF && !in_level(k)) v = -2 - k;
             else if (s < nPC) v = lookup(k);
             else {
                 v = find(k);
@@ -972,7 +1038,11 @@ __device__ inline void lds_dma4(const float* g, float* l_base) {
 extern __shared__ uint4 k_fused2_lds[];
 
 // tiles: the tile ids this launch runs (the tiles of one LDS class).
-template <class S, int TILE, bool XCD = true>
+// REF: the reference's update rules (k_dual and k_primal_ref's expressions):
+// the primal step reads p, q of the -a neighbours of the reference table and
+// always its own; a -a neighbour of another depth gives its p, q from the
+// main set, which the level never writes.
+template <class S, int TILE, bool XCD = true, bool REF = false>
 __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13<S> dst, Halo2Tab h,
                                                  const int32_t* __restrict__ tiles, int sa, int sx) {
     using IO = Io<S>;
@@ -1025,6 +1095,12 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     };
     // incoming edge of axis ax: p_ax and the q entries of row ax of eps
     constexpr int kF[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
+    // field f of the cube at position j outside the tile's sets: in the
+    // level from src, else (reference rules only) from the main set
+    auto ld_out = [&](int f, long j) -> T {
+        if (REF && !((uint32_t)j - a.lvl_off < a.lvl_n)) return IO::ld(a.s[f], j);
+        return IO::ld(src.f[f], j);
+    };
     // passes 3-5 of slot s (the expressions of primal_adj_at): own p, q and
     // the incoming ones from LDS (cv >= 0) or from src (cv <= -2, p, q of the
     // first iteration only; pin holds them if have_in); u_bar, v_bar to LDS,
@@ -1061,8 +1137,22 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
             } else {
                 const long j = v <= -2 ? -2L - v : 0L;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) in[ax][e] = v <= -2 ? IO::ld(src.f[kF[ax][e]], j) : T(0);
+                for (int e = 0; e < 4; ++e) in[ax][e] = v <= -2 ? ld_out(kF[ax][e], j) : T(0);
             }
+        }
+        if constexpr (REF) {
+            // tgv.cpp:192-217 (k_primal_ref): a missing -a neighbour gives 0
+            const T div_p = (p[0] - in[0][0]) + (p[1] - in[1][0]) + (p[2] - in[2][0]);
+            nw[0] = prox(od[0] + a.tau * div_p, w, a.nbins, a.tl);
+            const T dqx = (qs[0] - in[0][1]) + (qs[3] - in[1][1]) + (qs[4] - in[2][1]);
+            const T dqy = (qs[3] - in[0][2]) + (qs[1] - in[1][2]) + (qs[5] - in[2][2]);
+            const T dqz = (qs[4] - in[0][3]) + (qs[5] - in[1][3]) + (qs[2] - in[2][3]);
+            nw[1] = od[1] + a.tau * (p[0] + dqx);
+            nw[2] = od[2] + a.tau * (p[1] + dqy);
+            nw[3] = od[3] + a.tau * (p[2] + dqz);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) ub[c * sa + s] = IO::rt(T(2) * nw[c] - od[c]);
+            return;
         }
         if (cv[0] != -1) {
             dp -= in[0][0];
@@ -1169,7 +1259,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
                 if (c1[ax] <= -2) {
                     const long j = -2L - c1[ax];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) in1[ax][e] = IO::ld(src.f[kF[ax][e]], j);
+                    for (int e = 0; e < 4; ++e) in1[ax][e] = ld_out(kF[ax][e], j);
                 }
             }
         }
@@ -1572,6 +1662,7 @@ __global__ void k_field_init(KArgs<S> a, int mode, int keep_dual) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const long g = (long)a.off + t;
+    const int st = a.init_stride;
     T x[4];
     if (mode == 2) {
 #pragma unroll
@@ -1582,11 +1673,15 @@ __global__ void k_field_init(KArgs<S> a, int mode, int keep_dual) {
             for (int c = 0; c < 4; ++c) IO::st(a.save[c], t, IO::ld(a.s[F_U + c], g));
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) x[c] = (T)a.init[g * 4 + c];
+        for (int c = 0; c < 4; ++c) x[c] = (T)a.init[g * st + c];
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) IO::st(a.s[F_U + c], g, x[c]);
-    if (!keep_dual) {
+    if (mode != 2 && st == 13) {
+        // a whole state (u, v, p, q): its dual too
+#pragma unroll
+        for (int f = F_PX; f <= F_Q5; ++f) IO::st(a.s[f], g, (T)a.init[g * 13 + (f - F_U)]);
+    } else if (!keep_dual) {
 #pragma unroll
         for (int f = F_PX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
     }
@@ -1945,6 +2040,7 @@ KArgs<S> make_args(tgvx_ctx* c, const tgvx_params* prm) {
     a.parent = adj ? P.d_parent_a.as<int32_t>() : P.d_parent.as<int32_t>();
     for (int i = 0; i < 4; ++i) a.save[i] = P.d_save[i].as<S>();
     a.init = P.d_init.as<float>();
+    a.init_stride = 4;
     if (prm) {
         a.sigma = prm->sigma > 0.f ? (T)prm->sigma : T(0.25);
         a.tau = prm->tau > 0.f ? (T)prm->tau : T(0.25);
@@ -1987,8 +2083,8 @@ int block_of(const tgvx_params* p) {
     int b = p && p->block > 0 ? p->block : 256;
     if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
         throw Error(TGVX_E_ARG, "block must be 64, 128, 256, 512 or 1024");
-    const bool tiled = p && p->op_mode == TGVX_OP_ADJ &&
-                       (p->fused == 1 || p->fused == 3 || p->fused == 4 || p->fused == 5 || p->fused == 6);
+    const bool tiled = p && ((p->op_mode == TGVX_OP_ADJ && (p->fused == 1 || p->fused == 3 || p->fused == 4)) ||
+                             p->fused == 5 || p->fused == 6);
     if (b > 256 && !tiled) throw Error(TGVX_E_ARG, "blocks above 256 exist only for the tiled fused kernels");
     return b;
 }
@@ -2017,7 +2113,11 @@ int size_bucket(uint32_t n) {
 Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool frozen) {
     const bool adj = p->op_mode == TGVX_OP_ADJ;
     if (p->fused < 0 || p->fused > 6) throw Error(TGVX_E_ARG, "fused must be 0 to 6");
-    Launch L{adj && !frozen ? p->fused : 0, block_of(p), 1};
+    // the reference's rules have the two-kernel iteration and the two-iteration
+    // kernel (5; 6 runs every level as 5, its small-level kernel being the
+    // exact-adjoint one's); the frozen-border solve has the two kernels only
+    const int var = frozen ? 0 : adj ? p->fused : (p->fused == 5 || p->fused == 6 ? p->fused : 0);
+    Launch L{var, block_of(p), 1};
     const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4 || L.variant == 5;
     if (tiled && p->block <= 0) L.block = kFuseTile;
     if (L.variant == 6) L = Launch{6, kLevel1, 0};
@@ -2130,15 +2230,15 @@ inline size_t lds_per_cu(const tgvx_ctx* c) {
 }
 // Workgroups of k_fused2 (tile threads) that its VGPRs let a CU hold: four
 // SIMDs of 512 VGPRs per lane, allocated in granules of 8, at most 8 waves each.
-template <class S>
+template <class S, bool REF>
 size_t fused2_wg_per_cu(int tile) {
     hipFuncAttributes fa;
     hipError_t e;
     switch (tile) {
-        case 128: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 128, true>)); break;
-        case 256: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 256, true>)); break;
-        case 1024: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 1024, true>)); break;
-        default: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 512, true>)); break;
+        case 128: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 128, true, REF>)); break;
+        case 256: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 256, true, REF>)); break;
+        case 1024: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 1024, true, REF>)); break;
+        default: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_fused2<S, 512, true, REF>)); break;
     }
     if (e != hipSuccess || fa.numRegs <= 0) return 1;
     const int waves = std::min(8, 512 / ((fa.numRegs + 7) / 8 * 8));
@@ -2147,19 +2247,20 @@ size_t fused2_wg_per_cu(int tile) {
 
 // The two-iteration kernel's tile sets for a's level and a tile size, built
 // on first use (callers build them before any graph capture starts).
-template <int TILE>
+template <int TILE, bool REF>
 void halo2_table_launch(tgvx_ctx* c, const int32_t* const nba[3], const int32_t* const nbt[3], uint32_t off,
                         uint32_t n, Halo2Bufs& hb) {
     const uint32_t ntiles = (n + TILE - 1) / TILE;
     if (ntiles == 0) return;
-    k_halo2_table<TILE><<<ntiles, TILE, 0, c->stream>>>(nba[0], nba[1], nba[2], nbt[0], nbt[1], nbt[2], off, n,
-                                                         hb.cnt.as<int32_t>(), hb.list.as<int32_t>(),
-                                                         hb.bslot.as<int32_t>(), hb.cslot.as<int32_t>());
+    k_halo2_table<TILE, REF><<<ntiles, TILE, 0, c->stream>>>(nba[0], nba[1], nba[2], nbt[0], nbt[1], nbt[2], off, n,
+                                                              hb.cnt.as<int32_t>(), hb.list.as<int32_t>(),
+                                                              hb.bslot.as<int32_t>(), hb.cslot.as<int32_t>());
 }
-template <class S>
+// REF: the tile sets of the reference's rules (its own +a / -a table).
+template <class S, bool REF = false>
 const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
     Problem& P = c->prob;
-    const auto key = std::make_tuple(a.off, a.n, tile, (int)sizeof(S));
+    const auto key = std::make_tuple(a.off, a.n, tile, (int)sizeof(S) + (REF ? 1000 : 0));
     auto it = P.halo2.find(key);
     if (it != P.halo2.end()) return *it->second;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -2176,11 +2277,13 @@ const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
         hb->list.ensure(sizeof(int32_t) * (ca - tile) * ntiles);
         hb->bslot.ensure(sizeof(int32_t) * 3 * ca * ntiles);
         hb->cslot.ensure(sizeof(int32_t) * 3 * ca * ntiles);
+        const int32_t* const* nb_plus = REF ? a.nbp : a.nba;
+        const int32_t* const* nb_in = REF ? a.nbm : a.nbt;
         switch (tile) {
-            case 128: halo2_table_launch<128>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
-            case 256: halo2_table_launch<256>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
-            case 512: halo2_table_launch<512>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
-            case 1024: halo2_table_launch<1024>(c, a.nba, a.nbt, a.off, a.n, *hb); break;
+            case 128: halo2_table_launch<128, REF>(c, nb_plus, nb_in, a.off, a.n, *hb); break;
+            case 256: halo2_table_launch<256, REF>(c, nb_plus, nb_in, a.off, a.n, *hb); break;
+            case 512: halo2_table_launch<512, REF>(c, nb_plus, nb_in, a.off, a.n, *hb); break;
+            case 1024: halo2_table_launch<1024, REF>(c, nb_plus, nb_in, a.off, a.n, *hb); break;
             default: throw Error(TGVX_E_ARG, "the two-iteration kernel takes tiles of 128, 256, 512 or 1024");
         }
         TGVX_HIP(hipGetLastError());
@@ -2208,7 +2311,7 @@ const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
         std::nth_element(sorted.begin(), sorted.begin() + (ntiles * 9) / 10, sorted.end());
         const size_t p90 = std::max<size_t>(sorted[(ntiles * 9) / 10], 1);
         const size_t cu_lds = lds_per_cu(c);
-        const size_t wg = std::max<size_t>(1, std::min(fused2_wg_per_cu<S>(tile), cu_lds / p90));
+        const size_t wg = std::max<size_t>(1, std::min(fused2_wg_per_cu<S, REF>(tile), cu_lds / p90));
         const size_t thr = std::max(p90, cu_lds / wg);
         std::vector<int32_t> order;
         order.reserve(ntiles);
@@ -2244,9 +2347,9 @@ const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
 
 // two iterations (primal n+1 ... dual n+3) from src into dst; false when the
 // level's tile sets do not fit (nothing launched)
-template <class S, bool X>
+template <class S, bool X, bool REF = false>
 bool launch_fused2_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, int tile) {
-    const Halo2Bufs& hb = halo2_tab<S>(c, a, tile);
+    const Halo2Bufs& hb = halo2_tab<S, REF>(c, a, tile);
     if (!hb.ok) return false;
     Halo2Tab h;
     h.cnt = hb.cnt.as<int32_t>();
@@ -2258,16 +2361,16 @@ bool launch_fused2_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<
         const int32_t* tl = hb.tiles.as<int32_t>() + cl.first;
         switch (tile) {
             case 128:
-                k_fused2<S, 128, X><<<cl.count, 128, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 128, X, REF><<<cl.count, 128, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
                 break;
             case 256:
-                k_fused2<S, 256, X><<<cl.count, 256, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 256, X, REF><<<cl.count, 256, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
                 break;
             case 1024:
-                k_fused2<S, 1024, X><<<cl.count, 1024, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 1024, X, REF><<<cl.count, 1024, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
                 break;
             default:
-                k_fused2<S, 512, X><<<cl.count, 512, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 512, X, REF><<<cl.count, 512, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
                 break;
         }
     }
@@ -2321,13 +2424,28 @@ void launch_fused_pass(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set1
 // fused (primal n, dual n+1) passes alternating between the main and the
 // alternate set, primal of the last back into the main set.
 template <class S, bool X>
-void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
+void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode) {
     const int wg = L.block > 256 ? 256 : L.block;  // the dual / primal kernels take at most 256
     const unsigned grid = (a.n + wg - 1) / wg;
     Set13<S> cur = main_set(a), oth = alt_set<S>(c);
     const Set13<S> mainset = cur;
     k_dual<S, 0, X><<<grid, wg, 0, c->stream>>>(a);
     int i = 1;
+    if (op_mode == TGVX_OP_REF) {
+        // the reference's rules: two iterations per launch while two remain,
+        // then primal and dual kernels on the main set
+        for (; i + 1 < iters; i += 2) {
+            if (!launch_fused2_x<S, X, true>(c, a, cur, oth, L.block)) break;
+            std::swap(cur, oth);
+        }
+        for (; i < iters; ++i) {
+            k_primal_ref_set<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, cur.f[0] != mainset.f[0]);
+            cur = mainset;
+            k_dual<S, 0, X><<<grid, wg, 0, c->stream>>>(a);
+        }
+        k_primal_ref_set<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, cur.f[0] != mainset.f[0]);
+        return;
+    }
     if (L.variant == 5) {
         for (; i + 1 < iters; i += 2) {
             if (!launch_fused2_x<S, X>(c, a, cur, oth, L.block)) break;
@@ -2342,9 +2460,9 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
     k_primal_adj<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, copy);
 }
 template <class S>
-void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L) {
-    if (L.xcd) launch_fused_x<S, true>(c, a, iters, L);
-    else launch_fused_x<S, false>(c, a, iters, L);
+void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode) {
+    if (L.xcd) launch_fused_x<S, true>(c, a, iters, L, op_mode);
+    else launch_fused_x<S, false>(c, a, iters, L, op_mode);
 }
 
 // Cached graph of `iters` iterations of one level.
@@ -2381,7 +2499,8 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
     if (iters <= 0 || a.n == 0) return;
     Launch L = L0;
     if (L.variant == 6) {
-        if (!frozen && a.n <= (uint32_t)kLevel1 && a.off == a.lvl_off && a.n == a.lvl_n) {
+        if (!frozen && p->op_mode == TGVX_OP_ADJ && a.n <= (uint32_t)kLevel1 && a.off == a.lvl_off &&
+            a.n == a.lvl_n) {
             k_level1<S><<<1, kLevel1, 0, c->stream>>>(a, iters);
             TGVX_HIP(hipGetLastError());
             return;
@@ -2391,7 +2510,10 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
     const bool fused = L.variant != 0 && iters >= 2;
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (fused && L.variant == 4) halo_tab<S>(c, a, L.block);
-    if (fused && L.variant == 5) halo2_tab<S>(c, a, L.block);
+    if (fused && L.variant == 5) {
+        if (p->op_mode == TGVX_OP_REF) halo2_tab<S, true>(c, a, L.block);
+        else halo2_tab<S>(c, a, L.block);
+    }
     if (p->use_graph && !frozen && iters > 1) {
         GraphKey key{a.off,
                      a.n,
@@ -2414,7 +2536,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         if (it == gc.m.end()) {
             hipGraph_t graph;
             TGVX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            if (fused) launch_fused<S>(c, a, iters, L);
+            if (fused) launch_fused<S>(c, a, iters, L, p->op_mode);
             else
                 for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, false);
             TGVX_HIP(hipStreamEndCapture(c->stream, &graph));
@@ -2426,7 +2548,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         TGVX_HIP(hipGraphLaunch(it->second, c->stream));
         return;
     }
-    if (fused) launch_fused<S>(c, a, iters, L);
+    if (fused) launch_fused<S>(c, a, iters, L, p->op_mode);
     else
         for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, frozen);
     TGVX_HIP(hipGetLastError());
@@ -2497,7 +2619,7 @@ void eval_gap_blocks(tgvx_ctx* c, KArgs<S> a, const tgvx_params* p, size_t li, s
 
 template <class S>
 void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
-             const tgvx_treetop_leaf* leaves, uint64_t nleaves) {
+             const tgvx_treetop_leaf* leaves, uint64_t nleaves, int init_stride) {
     Problem& P = c->prob;
     KArgs<S> a = make_args<S>(c, p);
     const bool to_gap = p->gap_tol > 0.0;
@@ -2512,14 +2634,19 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         throw Error(TGVX_E_ARG, "a problem of several blocks takes neither warm starts nor treetop leaves");
     P.block_iters.assign(P.levels.size() * P.nblocks, -1);
     P.block_gap.assign(P.levels.size() * P.nblocks, 0.0);
+    a.init_stride = 4;
+    const bool full = init_uv && init_stride == 13;  // a whole state: no dual recovery after it
     if (init_uv) {
-        P.d_init.ensure(sizeof(float) * 4 * P.n);
-        std::vector<float> perm_init(4 * P.n);
-        for (uint64_t g = 0; g < P.n; ++g) std::memcpy(&perm_init[4 * g], init_uv + 4 * (size_t)P.perm[g], 16);
-        TGVX_HIP(hipMemcpyAsync(P.d_init.p, perm_init.data(), sizeof(float) * 4 * P.n, hipMemcpyHostToDevice,
+        const size_t w = (size_t)init_stride;
+        P.d_init.ensure(sizeof(float) * w * P.n);
+        std::vector<float> perm_init(w * P.n);
+        for (uint64_t g = 0; g < P.n; ++g)
+            std::memcpy(&perm_init[w * g], init_uv + w * (size_t)P.perm[g], sizeof(float) * w);
+        TGVX_HIP(hipMemcpyAsync(P.d_init.p, perm_init.data(), sizeof(float) * w * P.n, hipMemcpyHostToDevice,
                                 c->stream));
         TGVX_HIP(hipStreamSynchronize(c->stream));
         a.init = P.d_init.as<float>();
+        a.init_stride = init_stride;
     }
     const double t_solve0 = now_ms();
     if (P.n > 0) {
@@ -2561,7 +2688,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                 double Pc, Dc, Pw, Dw;
                 eval_gap<S>(c, a, p, &Pc, &Dc);
                 k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 1, keep);
-                if (!keep) recover();
+                if (!keep && !full) recover();
                 eval_gap<S>(c, a, p, &Pw, &Dw);
                 if (Pw - Dw > Pc - Dc) {
                     k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 2, keep);
@@ -2575,7 +2702,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                     recover();
                 }
                 k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 0, keep);
-                if (!keep) recover();
+                if (!keep && !full) recover();
                 warm_kept = 1;
             }
         } else {
@@ -2813,7 +2940,12 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     // the fused pass alone (the configuration's variant, the tiled kernel if
     // it has none; an even count, so the main set ends current)
     double sf = 0.0;
-    if (p->op_mode == TGVX_OP_ADJ) {
+    const bool ref = p->op_mode != TGVX_OP_ADJ;
+    // the reference's rules: the two-iteration kernel when configured and the
+    // level's tile sets fit (they have no one-iteration fused kernel)
+    const bool ref2 = ref && (cfg.variant == 5 || cfg.variant == 6) &&
+                      halo2_tab<S, true>(c, a, cfg.variant == 6 ? level1_fallback(a.n).block : cfg.block).ok;
+    if (!ref || ref2) {
         Launch fz = cfg;
         if (fz.variant == 0) fz = Launch{1, kFuseTile, cfg.xcd};
         if (fz.variant == 6) fz = level1_fallback(a.n);
@@ -2821,14 +2953,17 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
         // launch where the level's tile sets do not fit)
         int per = 1;
         if (fz.variant == 5) {
-            if (halo2_tab<S>(c, a, fz.block).ok) per = 2;
+            if (ref2 || halo2_tab<S>(c, a, fz.block).ok) per = 2;
             else fz.variant = 1;
         }
         Set13<S> cur = main_set(a), oth = alt_set<S>(c);
         const int kf = 2 * k / per;  // 2k iterations whatever the variant
         for (int i = 0; i < kf; ++i) {
             TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-            if (per == 2) {
+            if (per == 2 && ref) {
+                if (fz.xcd) launch_fused2_x<S, true, true>(c, a, cur, oth, fz.block);
+                else launch_fused2_x<S, false, true>(c, a, cur, oth, fz.block);
+            } else if (per == 2) {
                 if (fz.xcd) launch_fused2_x<S, true>(c, a, cur, oth, fz.block);
                 else launch_fused2_x<S, false>(c, a, cur, oth, fz.block);
             } else {
@@ -2859,8 +2994,8 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
             for (int v : {1, 3, 4})
                 for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{v, b, x});
             for (int b : {256, 128}) cand.push_back(Launch{2, b, x});
-            for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{5, b, x});
         }
+        for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{5, b, x});
     }
     if (p->op_mode == TGVX_OP_ADJ) cand.push_back(Launch{6, kLevel1, 0});
     // Kernel launches per iteration of a candidate over `iters` iterations of
@@ -3283,7 +3418,7 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
 }
 
 void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
-                   const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_cube* cubes) {
+                   const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_cube* cubes, int init_stride) {
     Problem& P = c->prob;
     if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
     if (!p) throw Error(TGVX_E_ARG, "null params");
@@ -3291,10 +3426,12 @@ void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx
         throw Error(TGVX_E_ARG, "bad op_mode");
     if (leaves && init_uv) throw Error(TGVX_E_ARG, "the frozen-border solve takes no initial field");
     (void)cubes;
+    if (init_stride != 4 && init_stride != 13) throw Error(TGVX_E_ARG, "bad initial field layout");
+    if (init_stride == 13 && p->warm_start) throw Error(TGVX_E_ARG, "warm_start excludes an initial state");
     switch (P.precision) {
-        case TGVX_FP64: solve_t<double>(c, p, init_uv, rep, leaves, nleaves); break;
-        case TGVX_FP32: solve_t<float>(c, p, init_uv, rep, leaves, nleaves); break;
-        default: solve_t<bf16s>(c, p, init_uv, rep, leaves, nleaves); break;
+        case TGVX_FP64: solve_t<double>(c, p, init_uv, rep, leaves, nleaves, init_stride); break;
+        case TGVX_FP32: solve_t<float>(c, p, init_uv, rep, leaves, nleaves, init_stride); break;
+        default: solve_t<bf16s>(c, p, init_uv, rep, leaves, nleaves, init_stride); break;
     }
 }
 

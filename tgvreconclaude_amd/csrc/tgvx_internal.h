@@ -166,7 +166,7 @@ namespace tgvx {
 void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb, int precision,
                   int op_mode, const uint64_t* blk_start = nullptr, uint32_t nblocks = 1);
 void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
-                   const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_cube* cubes);
+                   const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_cube* cubes, int init_stride = 4);
 void problem_solve_levels(tgvx_ctx* c, const tgvx_params* p, const int32_t* level_iters, int32_t n_levels,
                           tgvx_report* rep);
 void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out);

@@ -1,7 +1,7 @@
 """Learned warm start: training-data generation, training and evaluation.
 
 Training pairs come from this solver itself: seeded CSG scenes (24 ring views
-of 640x480, octree depth <= 9) are solved to relative gap 1e-4 on the GPU; for
+of 320x240, octree depth <= 9, as the small sphere workload) are solved to relative gap 1e-4 on the GPU; for
 every level, the features of each cube (computed with the parent level's final
 state, exactly as at inference) are paired with the cube's converged (u, v)
 minus its cold initialisation. The network is trained on the GPU with the
@@ -14,6 +14,7 @@ cold); each must reach the gap.
 
     python -m tgvreconclaude_amd.warmstart train --scenes 150 --out weights.npy
     python -m tgvreconclaude_amd.warmstart eval --scenes 50 --weights weights.npy
+    python -m tgvreconclaude_amd.warmstart bound --eval-scenes 5   # what any predictor could gain
 """
 import argparse
 import json
@@ -52,7 +53,7 @@ def params(**kw):
     return TGVParams(**base)
 
 
-def train_scene(seed, max_depth=9, W=640, H=480):
+def train_scene(seed, max_depth=9, W=320, H=240):
     return scene.csg_scene(seed, views=24, W=W, H=H, max_depth=max_depth)
 
 
@@ -195,10 +196,80 @@ def evaluate(ctx, weights, n_scenes, seed0=900000, diagnostics=False):
     return summary, res
 
 
+def perturbed(st, e_uv, e_pq, rng):
+    """The state st (u, v, p, q) with Gaussian noise of RMS e_uv on u, v and
+    e_pq times the ball radius on p, q; p, q then projected back onto their
+    balls (alpha1, alpha0), as the dual step would."""
+    x = np.concatenate([st["u"][:, None], st["v"], st["p"], st["q"]], axis=1).astype(np.float64)
+    x[:, 0:4] += rng.normal(0.0, e_uv, (len(x), 4)) if e_uv > 0 else 0.0
+    if e_pq > 0:
+        x[:, 4:7] += rng.normal(0.0, e_pq * ALPHA1, (len(x), 3))
+        x[:, 7:13] += rng.normal(0.0, e_pq * ALPHA0, (len(x), 6))
+    for sl, r in ((slice(4, 7), ALPHA1), (slice(7, 13), ALPHA0)):
+        n = np.sqrt((x[:, sl] ** 2).sum(1))
+        x[:, sl] *= np.where(n > r, r / np.maximum(n, 1e-30), 1.0)[:, None]
+    return np.ascontiguousarray(x, dtype=np.float32)
+
+
+def bound(ctx, n_scenes, seed0=900000, e_uv_list=(0.0, 0.018), e_pq_list=(0.0, 0.01, 0.03, 0.1, 0.3),
+          weights=None):
+    """What any per-cube predictor of the initial state could gain: each
+    held-out scene is solved cold to the gap (per-depth steps); then every
+    level is started from that converged (u, v, p, q) with noise added (e_uv:
+    RMS on u and v, the trained network's residual is about 0.018; e_pq: RMS
+    on p and q as a fraction of their ball radius) and solved to the same gap
+    again. The converged primal with the dual recovered from it (what a
+    perfect (u, v) network gives) is run beside it, and, with weights, the
+    trained network's (u, v) (from the features of the cold solve's states)
+    with the converged dual and with the recovered one."""
+    rng = np.random.default_rng(5)
+    rows = []
+    for k in range(n_scenes):
+        sc = eval_scene(seed0 + k)
+        hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
+        ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+        sched = depth_schedule(int(sc.cubes["depth"].max()))
+        pc = params(**sched)
+        ctx.run(pc)  # untimed: graphs
+        cold = ctx.run(pc)
+        st = ctx.state()
+        uv = np.ascontiguousarray(np.concatenate([st["u"][:, None], st["v"]], axis=1))
+        row = {"seed": seed0 + k, "cubes": int(len(sc.cubes)), "cold_ms": cold.solve_ms,
+               "cold_iters": [l["iters"] for l in cold.levels()]}
+        r = ctx.run(params(dual_eps=DUAL_EPS, warm_fallback=0, **sched), init_uv=uv)
+        row["primal_solution_dual_recovered"] = {"ms": r.solve_ms, "iters": [l["iters"] for l in r.levels()]}
+        if weights is not None:
+            ctx.mlp_set(weights)
+            pred = uv.copy()
+            for depth, _ in ctx.levels():
+                x, _, idx = ctx.features(depth)
+                pred[idx] = x[:, 20:24] + ctx.mlp_forward(x)
+            row["network_uv_rms"] = float(np.sqrt(np.mean((pred - uv) ** 2)))
+            full = perturbed(st, 0.0, 0.0, rng)
+            full[:, 0:4] = pred
+            r = ctx.run(params(warm_fallback=0, **sched), init_state=full)
+            row["network_uv_exact_dual"] = {"ms": r.solve_ms, "iters": [l["iters"] for l in r.levels()]}
+            r = ctx.run(params(dual_eps=DUAL_EPS, warm_fallback=0, **sched), init_uv=pred)
+            row["network_uv_dual_recovered"] = {"ms": r.solve_ms, "iters": [l["iters"] for l in r.levels()]}
+        for e_uv in e_uv_list:
+            for e_pq in e_pq_list:
+                r = ctx.run(params(warm_fallback=0, **sched), init_state=perturbed(st, e_uv, e_pq, rng))
+                row[f"uv{e_uv}_pq{e_pq}"] = {"ms": r.solve_ms, "iters": [l["iters"] for l in r.levels()],
+                                             "reached": bool(all(l["rel_gap"] <= GAP for l in r.levels()))}
+        rows.append(row)
+        log(f"[warmstart] bound scene {k}: cold {row['cold_ms']:.0f} ms; " + ", ".join(
+            f"{key} {v['ms']:.0f}" for key, v in row.items() if isinstance(v, dict)))
+    cold = sum(r["cold_ms"] for r in rows)
+    keys = [k for k in rows[0] if isinstance(rows[0][k], dict)]
+    summary = {"scenes": n_scenes, "cold_ms_total": cold,
+               "speedup_vs_cold_total": {k: cold / sum(r[k]["ms"] for r in rows) for k in keys}}
+    return summary, rows
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train", "eval", "both"])
-    ap.add_argument("--scenes", type=int, default=150)
+    ap.add_argument("cmd", choices=["train", "eval", "both", "bound"])
+    ap.add_argument("--scenes", type=int, default=2000)
     ap.add_argument("--eval-scenes", type=int, default=50)
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--cap", type=int, default=4000, help="training pairs kept per level and scene")
@@ -226,6 +297,12 @@ def main():
         result["train"] = {"scenes": args.scenes, "pairs": int(len(X)), "epochs": args.epochs,
                            "loss_curve": curve.tolist(), "zero_correction_loss": base,
                            "data_s": t_data, "train_s": t_train}
+    if args.cmd == "bound":
+        summary, rows = bound(ctx, args.eval_scenes,
+                              weights=np.load(args.weights) if os.path.exists(args.weights) else None)
+        log(f"[warmstart] {json.dumps(summary)}")
+        result["bound"] = summary
+        result["bound_scenes"] = rows
     if args.cmd in ("eval", "both"):
         w = np.load(args.weights)
         summary, res = evaluate(ctx, w, args.eval_scenes, diagnostics=args.diagnostics)
@@ -235,7 +312,7 @@ def main():
     if args.out:
         with open(args.out, "w") as f:
             json.dump(result, f, indent=1)
-    print(json.dumps({k: v for k, v in result.items() if k != "eval_scenes"}))
+    print(json.dumps({k: v for k, v in result.items() if k not in ("eval_scenes", "bound_scenes")}))
 
 
 if __name__ == "__main__":
