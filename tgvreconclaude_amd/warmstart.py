@@ -141,7 +141,7 @@ def solve_variants(ctx, sc, weights, diagnostics=False):
     for name, p in runs.items():
         rep = ctx.run(p, init_uv=sol if name == "solution_init" else None)
         lv = rep.levels()
-        out[name] = {"ms": rep.solve_ms, "iters": [l["iters"] for l in lv],
+        out[name] = {"ms": rep.solve_ms, "iters": [l["iters"] for l in lv], "leaf_iters": rep.leaf_iters,
                      "reached": bool(all(l["rel_gap"] <= GAP for l in lv)),
                      "warm_levels": [l["warm"] for l in lv]}
     return out
@@ -175,7 +175,12 @@ def evaluate(ctx, weights, n_scenes, seed0=900000, diagnostics=False):
                "speedup_cold_dual_vs_cold_total": float(cold.sum() / cd.sum()),
                "speedup_warm_vs_cold_dual_total": float(cd.sum() / warm.sum()),
                "cold_ms_total": float(cold.sum()), "cold_dual_ms_total": float(cd.sum()),
-               "warm_ms_total": float(warm.sum())}
+               "warm_ms_total": float(warm.sum()),
+               "leaf_iters_total": {v: float(sum(r[v]["leaf_iters"] for r in res))
+                                    for v in ("cold", "cold_dual", "warm", "cold_sched", "cold_dual_sched",
+                                              "warm_sched")},
+               "scenes_warm_fewer_leaf_iters_than_cold_dual": int(sum(
+                   r["warm"]["leaf_iters"] <= r["cold_dual"]["leaf_iters"] for r in res))}
     for extra in ("warm_no_fallback", "warm_cold_dual", "warm_cold_dual_no_fallback", "solution_init"):
         if extra in res[0]:
             x = np.array([r[extra]["ms"] for r in res])
