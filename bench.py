@@ -38,6 +38,8 @@ ELEM = {"fp32": 4, "bf16": 2, "fp64": 8}
 # bytes than this (26 state values, a 4 B info word), so for them the
 # figure is the work done, not the traffic.
 ALG_BYTES = {k: 34 * e + 8 + 24 for k, e in ELEM.items()}
+# p, q in BF16, u, v and the extrapolations in FP32: 16 values of 4 B and 18 of 2 B
+ALG_BYTES["bf16-dual"] = 16 * 4 + 18 * 2 + 8 + 24
 KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj",
                 3: "k_fused_tile (p, q staged in LDS)", 4: "k_fused_halo", 5: "k_fused2", 6: "k_level1"}
 
@@ -51,7 +53,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp32", "bf16", "fp64"], default="fp32")
+    ap.add_argument("--precision", choices=["fp32", "bf16", "bf16-dual", "fp64"], default="fp32")
     ap.add_argument("--views", type=int, default=96)
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=960)
@@ -117,7 +119,7 @@ def main():
     import tgvreconclaude_amd as T
     from tgvreconclaude_amd import scene
 
-    prec = {"fp32": T.FP32, "bf16": T.BF16, "fp64": T.FP64}[args.precision]
+    prec = {"fp32": T.FP32, "bf16": T.BF16, "bf16-dual": T.BF16_DUAL, "fp64": T.FP64}[args.precision]
     if args.workload == "batched":
         return batched(args, T, scene, prec)
     t0 = time.time()
@@ -190,6 +192,9 @@ def main():
         leaf_iters += rep.leaf_iters
     hip.hipDeviceSynchronize()
     dt = time.perf_counter() - ts
+    # where a step's time goes: each level's iterations (events around its
+    # launches inside the last timed step)
+    level_ms = [(l["depth"], round(l["ms"], 4)) for l in rep.levels()]
     value = leaf_iters / dt
     log(f"[bench] {args.steps} steps in {dt:.3f} s: {value:.4g} leaf-iter/s")
     # the same leaf-iterations with the one-time setup counted in: histogram
@@ -378,7 +383,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "bf16": "bf16-storage/f32", "fp64": "f64"}[args.precision],
+        "dtype": {"fp32": "f32", "bf16": "bf16-storage/f32", "bf16-dual": "bf16-dual-storage/f32",
+                  "fp64": "f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": "torus_box_depth10", "leaves": n, "views": args.views,
                    "image": [args.width, args.height], "max_depth": args.max_depth, "bins": 10,
@@ -396,6 +402,7 @@ def main():
         "time_to_gap": gap,
         "warm_start_speedup": warm,
         "reference_rules_sweep": ref_sweep,
+        "level_ms_per_step": level_ms,
         "cpu_baseline": cpu,
         "autotune": tuning,
         "setup_s": {"scene": round(t_scene, 2), "histograms": round(t_hist, 3), "load": round(t_load, 3)},
@@ -484,7 +491,8 @@ def batched(args, T, scene, prec):
         "metric": "TGV leaf-iter/s, 256 independent blocks batched, 1 GPU",
         "value": value, "unit": "leaf-iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp32": "f32", "bf16": "bf16-storage/f32", "fp64": "f64"}[args.precision], "data": "synthetic",
+        "dtype": {"fp32": "f32", "bf16": "bf16-storage/f32", "bf16-dual": "bf16-dual-storage/f32",
+                  "fp64": "f64"}[args.precision], "data": "synthetic",
         "config": {"workload": f"batched_{len(blocks)}_blocks", "cubes": n, "blocks": len(blocks),
                    "cubes_min_median_max": [int(counts.min()), int(np.median(counts)), int(counts.max())],
                    "iters_per_level_per_step": args.iters, "op_mode": "adjoint", "bins": 10,

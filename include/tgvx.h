@@ -63,7 +63,10 @@ typedef struct {
     int32_t lidar; /* vote 5 instead of 1 (histogram.cpp:158) */
 } tgvx_view;
 
-enum { TGVX_FP64 = 0, TGVX_FP32 = 1, TGVX_BF16 = 2 };
+/* storage of the iterated state (arithmetic is FP32 except for TGVX_FP64):
+ * TGVX_BF16 stores u, v, p, q (and the extrapolations) in BF16;
+ * TGVX_BF16_DUAL stores p, q in BF16 and u, v, u_bar, v_bar in FP32 */
+enum { TGVX_FP64 = 0, TGVX_FP32 = 1, TGVX_BF16 = 2, TGVX_BF16_DUAL = 3 };
 enum { TGVX_OP_REF = 0, TGVX_OP_ADJ = 1 };
 enum {
     TGVX_OK = 0,
@@ -82,7 +85,8 @@ typedef struct {
     float alpha1;      /* weight on |grad u - v| (reference default 0.04) */
     float lambda;      /* data weight (reference: 1) */
     int32_t iters;     /* iterations per level when gap_tol <= 0 (reference default 200) */
-    int32_t precision; /* TGVX_FP64 / TGVX_FP32 / TGVX_BF16 (state storage; BF16 computes in FP32) */
+    int32_t precision; /* TGVX_FP64 / TGVX_FP32 / TGVX_BF16 / TGVX_BF16_DUAL (state storage; the BF16 ones
+                          compute in FP32) */
     int32_t op_mode;   /* TGVX_OP_REF: the reference's update rules bit for bit;
                           TGVX_OP_ADJ: primal step with the exact adjoint of the dual step */
     int32_t chunk;     /* iterations between gap evaluations (default 50) */

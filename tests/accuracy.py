@@ -22,6 +22,12 @@ with the same per-level iteration counts, and the GPU FP32 solve is compared
 with it (u and surfaces).
 
     python tests/accuracy.py --oracle --out gpurun_out/accuracy_oracle_depth10.json
+
+With --config3 the sphere workload (24 ring views of 320x240, depth <= 7) is
+solved to the gap in FP32, BF16-dual (p, q in BF16) and BF16 storage and each
+is compared with the FP64 oracle's solution to the same gap.
+
+    python tests/accuracy.py --config3 --out gpurun_out/accuracy_sphere_precisions.json
 """
 import argparse
 import json
@@ -35,9 +41,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 sys.path.insert(0, ROOT)
 
-from tgvreconclaude_amd import BF16, FP32, FP64, OP_ADJ, TGVParams  # noqa: E402
+from tgvreconclaude_amd import BF16, BF16_DUAL, FP32, FP64, OP_ADJ, TGVParams  # noqa: E402
 
-NAMES = {FP32: "fp32", BF16: "bf16", FP64: "fp64"}
+NAMES = {FP32: "fp32", BF16: "bf16", FP64: "fp64", BF16_DUAL: "bf16-dual"}
 
 
 def _params(prec, **kw):
@@ -172,17 +178,80 @@ def against_oracle(ctx, sc, hist, tol, sigma, tau, threads, fused=None, block=No
             "surface_vertices": [int(len(va)), int(len(vb))]}
 
 
+def config3(ctx, sc, hist, threads, cap=6000, tol=1e-4):
+    """Storage precisions on the sphere workload against the FP64 CPU oracle:
+    each precision is solved to relative gap tol (at most cap iterations per
+    level) with the per-depth steps, the oracle in FP64 to the same gap with
+    the same steps; L_inf / RMS of u and the surfaces' Hausdorff distance
+    against the oracle's solution."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from tgvreconclaude_amd import step_schedule
+    has = (hist.sum(axis=1) > 0).astype(np.uint8)
+    d = int(sc.cubes["depth"].max())
+    leaf = 2.0 * sc.r_root / 2 ** d
+    ratios = [256 if k <= d - 4 else (1024 if k < d else 4096) for k in range(d + 1)]
+    sd, td = step_schedule(ratios)
+    steps = dict(sigma=8.0, tau=1.0 / 128, sigma_depth=sd, tau_depth=td)
+    t0 = time.time()
+    ref, _, its, _ = O.tgv_minimize(sc.cubes, hist, O.params(alpha0=2.0, alpha1=1.0, op_mode=1, gap_tol=tol, chunk=50,
+                                                             max_iters=cap, threads=threads, sigma=8.0, tau=1.0 / 128,
+                                                             sigma_depth=sd, tau_depth=td), f64=True)
+    log(f"oracle fp64 to gap: {time.time() - t0:.1f} s, iterations {its.tolist()}")
+    ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+    vr, _ = surface(ctx, sc, ref[:, 0], has)
+    out = {"gap_tol": tol, "cap_per_level": cap, "oracle_iters_per_level": its.tolist(), "leaf_edge": leaf,
+           "results": []}
+    for prec in (FP32, BF16_DUAL, BF16):
+        ctx.load(sc.cubes, hist, precision=prec, op_mode=OP_ADJ)
+        rep = ctx.run(_params(prec, gap_tol=tol, max_iters=cap, **steps))
+        lv = rep.levels()
+        u = u_of(ctx.state())
+        e = np.abs(u - ref[:, 0])
+        # the storage error alone: FP64 (the oracle's iteration, bit for bit)
+        # run for the same per-level iteration counts
+        its_p = [l["iters"] for l in lv]
+        ctx.load(sc.cubes, hist, precision=FP64, op_mode=OP_ADJ)
+        ctx.run_levels(_params(FP64, **steps), its_p)
+        e_same = np.abs(u - u_of(ctx.state()))
+        ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+        va, _ = surface(ctx, sc, u, has)
+        h = hausdorff(va, vr)
+        r = {"precision": NAMES[prec], "reached": all(l["rel_gap"] <= tol for l in lv),
+             "iters_per_level": its_p, "final_rel_gap": [l["rel_gap"] for l in lv],
+             "linf_u": float(e.max()), "rms_u": float(np.sqrt(np.mean(e * e))), "hausdorff_leaf": h / leaf,
+             "linf_u_same_iters": float(e_same.max()), "rms_u_same_iters": float(np.sqrt(np.mean(e_same ** 2))),
+             "bytes_per_leaf_iter": {FP32: 168, BF16_DUAL: 132, BF16: 100}[prec]}
+        log(f"{NAMES[prec]}: reached {r['reached']}, iterations {r['iters_per_level']}, max gap "
+            f"{max(r['final_rel_gap']):.3g}, linf u {r['linf_u']:.3g} (same iterations in FP64: "
+            f"{r['linf_u_same_iters']:.3g}), Hausdorff {r['hausdorff_leaf']:.3g} leaf")
+        out["results"].append(r)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--bf16-max-iters", type=int, default=6000, help="per-level cap of the BF16 to-gap attempt")
     ap.add_argument("--oracle", action="store_true", help="FP64 reference = the CPU oracle (steps sigma/tau = 1024)")
+    ap.add_argument("--config3", action="store_true",
+                    help="storage precisions on the sphere workload against the FP64 oracle solved to the gap")
     ap.add_argument("--threads", type=int, default=0, help="oracle host threads (0: the cores available, at most 16)")
     ap.add_argument("--fused", type=int, default=None, help="with --oracle: the GPU's launch variant (0-6)")
     ap.add_argument("--block", type=int, default=None, help="with --oracle: the GPU's workgroup / tile size")
     args = ap.parse_args()
     from tgvreconclaude_amd import Context, scene
+    if args.config3:
+        sc = scene.sphere_scene()
+        ctx = Context(0)
+        hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
+        res = config3(ctx, sc, hist, args.threads or min(16, len(os.sched_getaffinity(0))))
+        res.update({"workload": "sphere_depth7", "cubes": int(len(sc.cubes))})
+        with open(args.out, "w") as f:
+            json.dump(res, f, indent=1)
+        ctx.close()
+        return
     t0 = time.time()
     sc = scene.torus_box_scene()
     print(f"[accuracy] scene: {len(sc.cubes)} cubes ({time.time() - t0:.0f} s)", flush=True)

@@ -34,3 +34,22 @@ def test_bf16_against_fp64(ctx, csg8):
     assert r["iters_per_level"] == r32["iters_per_level"]
     assert r["linf_u"] <= 0.05
     assert r["hausdorff_leaf"] <= 4.0
+
+
+def test_precisions_on_the_sphere_workload(ctx, sphere):
+    # the sphere workload's precision report (FP32, BF16 p, q with FP32 u, v,
+    # BF16 everything) against the FP64 oracle solved to the same gap
+    from accuracy import config3
+    sc, hist = sphere
+    res = config3(ctx, sc, hist, threads=8, cap=3000)
+    r = {x["precision"]: x for x in res["results"]}
+    print({k: (v["reached"], max(v["final_rel_gap"]), v["linf_u_same_iters"], v["hausdorff_leaf"])
+           for k, v in r.items()})
+    assert r["fp32"]["reached"]
+    assert r["fp32"]["linf_u_same_iters"] <= 1e-4
+    assert r["fp32"]["hausdorff_leaf"] <= 0.5
+    # 8 significand bits stall the gap: p, q in BF16 well above 1e-4, all
+    # fields in BF16 higher still; the storage error grows in the same order
+    assert not r["bf16-dual"]["reached"] and not r["bf16"]["reached"]
+    assert max(r["bf16-dual"]["final_rel_gap"]) < max(r["bf16"]["final_rel_gap"])
+    assert r["fp32"]["linf_u_same_iters"] < r["bf16-dual"]["linf_u_same_iters"] < r["bf16"]["linf_u_same_iters"]

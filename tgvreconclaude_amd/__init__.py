@@ -17,11 +17,11 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from ._native import BF16, CUBE_DT, FP32, FP64, LEAF_DT, OP_ADJ, OP_REF, STATE_DT, Params, Report
+from ._native import BF16, BF16_DUAL, CUBE_DT, FP32, FP64, LEAF_DT, OP_ADJ, OP_REF, STATE_DT, Params, Report
 
 __all__ = [
     "Context", "TGVParams", "compute_histograms", "tgv_minimize", "tgv_minimize_oc", "median_indicator",
-    "CUBE_DT", "STATE_DT", "LEAF_DT", "FP64", "FP32", "BF16", "OP_REF", "OP_ADJ", "make_view", "step_schedule",
+    "CUBE_DT", "STATE_DT", "LEAF_DT", "FP64", "FP32", "BF16", "BF16_DUAL", "OP_REF", "OP_ADJ", "make_view", "step_schedule",
 ]
 
 

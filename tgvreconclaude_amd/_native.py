@@ -15,7 +15,7 @@ STATE_DT = np.dtype([("u", "<f4"), ("v", "<f4", 3), ("p", "<f4", 3), ("q", "<f4"
 LEAF_DT = np.dtype([("hi", "<u4"), ("lo", "<u4"), ("depth", "<i4"), ("first_idx", "<u4"), ("last_idx", "<u4")])
 assert CUBE_DT.itemsize == 16 and STATE_DT.itemsize == 52 and LEAF_DT.itemsize == 20
 
-FP64, FP32, BF16 = 0, 1, 2
+FP64, FP32, BF16, BF16_DUAL = 0, 1, 2, 3
 OP_REF, OP_ADJ = 0, 1
 MAX_LEVELS = 22
 

@@ -1205,9 +1205,9 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         for (int k = 0; k < 6; ++k) q[k] = pq[(3 + k) * sa + s];
         dual_step(p, q, bar, a.sigma, a.a0, a.a1);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pq[c * sa + s] = IO::rt(p[c]);
+        for (int c = 0; c < 3; ++c) pq[c * sa + s] = IO::rtd(p[c]);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) pq[(3 + k) * sa + s] = IO::rt(q[k]);
+        for (int k = 0; k < 6; ++k) pq[(3 + k) * sa + s] = IO::rtd(q[k]);
     };
     auto load_uv = [&](long g, T o[4]) {
 #pragma unroll
@@ -1403,9 +1403,9 @@ __global__ void __launch_bounds__(kLevel1) k_level1(KArgs<S> a, int iters) {
             }
             dual_step(p, q, bar, a.sigma, a.a0, a.a1);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) p[c] = IO::rt(p[c]);
+            for (int c = 0; c < 3; ++c) p[c] = IO::rtd(p[c]);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) q[k] = IO::rt(q[k]);
+            for (int k = 0; k < 6; ++k) q[k] = IO::rtd(q[k]);
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) lpq[c][t] = p[c];
@@ -2020,13 +2020,23 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Address of state field f of storage S: with mixed storage the dual fields
+// (p, q) hold BF16 and their address carries the mark Io<mixs> tests.
+template <class S>
+S* field_ptr(DevBuf& b, int f) {
+    S* ptr = b.as<S>();
+    if (std::is_same<S, mixs>::value && f >= F_PX && f <= F_Q5)
+        ptr = reinterpret_cast<S*>(reinterpret_cast<uintptr_t>(ptr) | 1u);
+    return ptr;
+}
+
 template <class S>
 KArgs<S> make_args(tgvx_ctx* c, const tgvx_params* prm) {
     using T = typename Io<S>::T;
     Problem& P = c->prob;
     KArgs<S> a;
     std::memset(&a, 0, sizeof(a));
-    for (int i = 0; i < 17; ++i) a.s[i] = P.d_state[i].as<S>();
+    for (int i = 0; i < 17; ++i) a.s[i] = field_ptr<S>(P.d_state[i], i);
     const bool adj = prm && prm->op_mode == TGVX_OP_ADJ;
     for (int ax = 0; ax < 3; ++ax) {
         a.nba[ax] = P.d_nba[ax].as<int32_t>();
@@ -2143,7 +2153,7 @@ Set13<S> alt_set(tgvx_ctx* c) {
     Set13<S> m;
     for (int f = 0; f < 13; ++f) {
         P.d_alt[f].ensure(P.elem * std::max<uint64_t>(P.n, 1));
-        m.f[f] = P.d_alt[f].as<S>();
+        m.f[f] = field_ptr<S>(P.d_alt[f], f);
     }
     return m;
 }
@@ -3208,7 +3218,7 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
                   int op_mode, const uint64_t* blk_start, uint32_t nblocks) {
     if (nb <= 0 || nb > 32) throw Error(TGVX_E_ARG, "nb must be in 1..32");
     if (n >= (1ull << 30)) throw Error(TGVX_E_ARG, "at most 2^30 - 1 cubes");
-    if (precision < TGVX_FP64 || precision > TGVX_BF16) throw Error(TGVX_E_ARG, "bad precision");
+    if (precision < TGVX_FP64 || precision > TGVX_BF16_DUAL) throw Error(TGVX_E_ARG, "bad precision");
     if (n > 0 && (!cubes || !bins)) throw Error(TGVX_E_ARG, "null pointer");
     const double t0 = now_ms();
     g_graphs[c].clear();
@@ -3220,7 +3230,9 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     P.nb = nb;
     P.precision = precision;
     P.op_mode = op_mode;
-    P.elem = precision == TGVX_FP64 ? 8 : (precision == TGVX_FP32 ? 4 : 2);
+    // bytes per element of the state arrays (mixed storage: FP32-sized
+    // arrays, the dual fields using the first half as BF16)
+    P.elem = precision == TGVX_FP64 ? 8 : (precision == TGVX_BF16 ? 2 : 4);
     // blocks: contiguous input ranges [bs[b], bs[b+1]) (one block by default)
     std::vector<uint64_t> bs;
     if (blk_start && nblocks > 0) {
@@ -3431,6 +3443,7 @@ void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx
     switch (P.precision) {
         case TGVX_FP64: solve_t<double>(c, p, init_uv, rep, leaves, nleaves, init_stride); break;
         case TGVX_FP32: solve_t<float>(c, p, init_uv, rep, leaves, nleaves, init_stride); break;
+        case TGVX_BF16_DUAL: solve_t<mixs>(c, p, init_uv, rep, leaves, nleaves, init_stride); break;
         default: solve_t<bf16s>(c, p, init_uv, rep, leaves, nleaves, init_stride); break;
     }
 }
@@ -3464,6 +3477,7 @@ void problem_solve_streaming(tgvx_ctx* c, const tgvx_params* p, const tgvx_treet
     switch (P.precision) {
         case TGVX_FP64: solve_stream_t<double>(c, p, leaves, nleaves, rep); break;
         case TGVX_FP32: solve_stream_t<float>(c, p, leaves, nleaves, rep); break;
+        case TGVX_BF16_DUAL: solve_stream_t<mixs>(c, p, leaves, nleaves, rep); break;
         default: solve_stream_t<bf16s>(c, p, leaves, nleaves, rep); break;
     }
 }
@@ -3474,8 +3488,9 @@ void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out) {
     const size_t n = P.n, e = P.elem;
     std::vector<uint8_t> col(e * n);
     const size_t out_e = P.precision == TGVX_FP64 ? 8 : 4;
+    bool half = P.precision == TGVX_BF16;  // the current field holds BF16
     auto widen = [&](const uint8_t* src, uint64_t g, uint8_t* dst) {
-        if (P.precision == TGVX_BF16) {
+        if (half) {
             uint16_t h;
             std::memcpy(&h, src + 2 * g, 2);
             const uint32_t u = (uint32_t)h << 16;
@@ -3489,7 +3504,8 @@ void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out) {
         uint8_t* out = (uint8_t*)(is_bar ? ubar_out : state_out);
         if (!out) continue;
         const int rec = is_bar ? 4 : 13, slot = is_bar ? f - F_UB : f;
-        if (n) TGVX_HIP(hipMemcpy(col.data(), P.d_state[f].p, e * n, hipMemcpyDeviceToHost));
+        half = P.precision == TGVX_BF16 || (P.precision == TGVX_BF16_DUAL && f >= F_PX && f <= F_Q5);
+        if (n) TGVX_HIP(hipMemcpy(col.data(), P.d_state[f].p, (half ? 2 : e) * n, hipMemcpyDeviceToHost));
         for (uint64_t g = 0; g < n; ++g) widen(col.data(), g, out + ((size_t)P.perm[g] * rec + slot) * out_e);
     }
 }
@@ -3511,6 +3527,7 @@ void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* Pv, doubl
     switch (P.precision) {
         case TGVX_FP64: go(double{}); break;
         case TGVX_FP32: go(float{}); break;
+        case TGVX_BF16_DUAL: go(mixs{}); break;
         default: go(bf16s{}); break;
     }
 }
@@ -3523,6 +3540,7 @@ void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* re
     switch (P.precision) {
         case TGVX_FP64: autotune_t<double>(c, p, iters, rep); break;
         case TGVX_FP32: autotune_t<float>(c, p, iters, rep); break;
+        case TGVX_BF16_DUAL: autotune_t<mixs>(c, p, iters, rep); break;
         default: autotune_t<bf16s>(c, p, iters, rep); break;
     }
 }
@@ -3534,6 +3552,7 @@ void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double
     switch (P.precision) {
         case TGVX_FP64: bench_t<double>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
         case TGVX_FP32: bench_t<float>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
+        case TGVX_BF16_DUAL: bench_t<mixs>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
         default: bench_t<bf16s>(c, depth, p, iters, total_ms, dual_ms, primal_ms, fused_ms); break;
     }
 }
@@ -3566,6 +3585,7 @@ void tile_sets(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_s
     switch (P.precision) {
         case TGVX_FP64: tile_sets_t<double>(c, depth, tile, usable, max_sets, lds_bytes); break;
         case TGVX_FP32: tile_sets_t<float>(c, depth, tile, usable, max_sets, lds_bytes); break;
+        case TGVX_BF16_DUAL: tile_sets_t<mixs>(c, depth, tile, usable, max_sets, lds_bytes); break;
         default: tile_sets_t<bf16s>(c, depth, tile, usable, max_sets, lds_bytes); break;
     }
 }

@@ -261,6 +261,7 @@ void surface_extract(tgvx_ctx* c, const float* u_in, const uint8_t* has_in, cons
                 k_srf_u_state<double><<<blocks_for(n), 256, 0, st>>>(P.d_state[0].as<double>(), n, uf.as<float>());
                 break;
             case TGVX_FP32:
+            case TGVX_BF16_DUAL:  // u is FP32 in both
                 k_srf_u_state<float><<<blocks_for(n), 256, 0, st>>>(P.d_state[0].as<float>(), n, uf.as<float>());
                 break;
             default:
