@@ -65,6 +65,13 @@ struct KArgs {
     // [fz_off, fz_off + fz_n)), for the fused kernel's frozen reads; or null
     const S* fzaos;
     uint32_t fz_off, fz_n;
+    // reference rules, two-iteration kernel: per level cube t (position
+    // lvl_off + t) and axis ax, record ax * lvl_n + t holds, where the
+    // reference table's +a neighbour lies outside the level, its u_bar, v_bar
+    // (rfz_bar) and, where the -a neighbour does, the p, q the primal step
+    // reads of it (rfz_dual: kF[ax]); both constant while the level iterates
+    const S* rfz_bar;
+    const S* rfz_dual;
 };
 
 template <class S>
@@ -1064,9 +1071,13 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         g = s < TILE ? lo + s : (long)hl[s - TILE];
         return !masked(a, g);
     };
-    auto frozen = [&](int v, T o[4]) {
+    // frozen u_bar, v_bar of the +a neighbour (axis ax) of the cube at g,
+    // encoded v <= -2
+    auto frozen = [&](int v, T o[4], int ax, long g) {
         const long j = -2L - v;
-        if (a.fzaos && (uint32_t)j - a.fz_off < a.fz_n) {
+        if (REF && a.rfz_bar) {
+            IO::ld4(a.rfz_bar, (long)ax * a.lvl_n + (g - (long)a.lvl_off), o);
+        } else if (a.fzaos && (uint32_t)j - a.fz_off < a.fz_n) {
             IO::ld4(a.fzaos, (long)((uint32_t)j - a.fz_off), o);
         } else {
 #pragma unroll
@@ -1101,11 +1112,21 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         if (REF && !((uint32_t)j - a.lvl_off < a.lvl_n)) return IO::ld(a.s[f], j);
         return IO::ld(src.f[f], j);
     };
+    // the incoming values (kF[ax]) of the cube at position j, the -a
+    // neighbour of the cube at g, outside the tile's sets
+    auto gather_in = [&](int ax, long j, long g, T o[4]) {
+        if (REF && a.rfz_dual && !((uint32_t)j - a.lvl_off < a.lvl_n)) {
+            IO::ld4(a.rfz_dual, (long)ax * a.lvl_n + (g - (long)a.lvl_off), o);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o[e] = ld_out(kF[ax][e], j);
+        }
+    };
     // passes 3-5 of slot s (the expressions of primal_adj_at): own p, q and
     // the incoming ones from LDS (cv >= 0) or from src (cv <= -2, p, q of the
     // first iteration only; pin holds them if have_in); u_bar, v_bar to LDS,
     // the new u, v in nw
-    auto primal = [&](int s, uint32_t w, const T od[4], const int cv[3], T nw[4], bool have_in,
+    auto primal = [&](int s, long g, uint32_t w, const T od[4], const int cv[3], T nw[4], bool have_in,
                       const T (&pin)[3][4]) {
         T p[3], qs[6];
 #pragma unroll
@@ -1134,10 +1155,11 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
             } else if (have_in) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) in[ax][e] = v <= -2 ? pin[ax][e] : T(0);
+            } else if (v <= -2) {
+                gather_in(ax, -2L - v, g, in[ax]);
             } else {
-                const long j = v <= -2 ? -2L - v : 0L;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) in[ax][e] = v <= -2 ? ld_out(kF[ax][e], j) : T(0);
+                for (int e = 0; e < 4; ++e) in[ax][e] = T(0);
             }
         }
         if constexpr (REF) {
@@ -1179,7 +1201,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         for (int c = 0; c < 4; ++c) ub[c * sa + s] = IO::rt(T(2) * nw[c] - od[c]);
     };
     // dual n+2 of slot s; fz: the frozen neighbours already read (have_fz)
-    auto dual2 = [&](int s, const int bv[3], bool have_fz, const T (&fz)[3][4]) {
+    auto dual2 = [&](int s, long g, const int bv[3], bool have_fz, const T (&fz)[3][4]) {
         T bar[4][4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) bar[0][c] = ub[c * sa + s];
@@ -1191,7 +1213,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
 #pragma unroll
                     for (int c = 0; c < 4; ++c) bar[1 + ax][c] = fz[ax][c];
                 } else {
-                    frozen(v, bar[1 + ax]);
+                    frozen(v, bar[1 + ax], ax, g);
                 }
             } else {
 #pragma unroll
@@ -1240,7 +1262,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         stage_w(tid, g0, tid & ~63);
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
-            if (b0[ax] <= -2) frozen(b0[ax], fz0[ax]);
+            if (b0[ax] <= -2) frozen(b0[ax], fz0[ax], ax, g0);
     }
     if (l1) {
 #pragma unroll
@@ -1255,12 +1277,8 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         if constexpr (PF) {
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
-                if (b1[ax] <= -2) frozen(b1[ax], fz1[ax]);
-                if (c1[ax] <= -2) {
-                    const long j = -2L - c1[ax];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) in1[ax][e] = ld_out(kF[ax][e], j);
-                }
+                if (b1[ax] <= -2) frozen(b1[ax], fz1[ax], ax, g1);
+                if (c1[ax] <= -2) gather_in(ax, -2L - c1[ax], g1, in1[ax]);
             }
         }
     }
@@ -1271,15 +1289,15 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     __syncthreads();
     // ---- primal n+1 over PA
     T nw0[4] = {T(0), T(0), T(0), T(0)}, nw1[4] = {T(0), T(0), T(0), T(0)};
-    if (l0) primal(tid, w0, od0, c0, nw0, false, in1);
-    if (l1) primal(s1, w1, od1, c1, nw1, PF, in1);
+    if (l0) primal(tid, g0, w0, od0, c0, nw0, false, in1);
+    if (l1) primal(s1, g1, w1, od1, c1, nw1, PF, in1);
     for (int s = tid + 2 * TILE; s < nA; s += TILE) {
         long g;
         if (!live(s, g)) continue;
         const int cv[3] = {cs[s], cs[CA + s], cs[2 * CA + s]};
         T od[4], nw[4];
         load_uv(g, od);
-        primal(s, a.info[g], od, cv, nw, false, in1);
+        primal(s, g, a.info[g], od, cv, nw, false, in1);
         if (s < nP) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) uvx[c * sx + s - 2 * TILE] = IO::rt(nw[c]);
@@ -1292,21 +1310,21 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     }
     __syncthreads();
     // ---- dual n+2 over DB
-    if (l0) dual2(tid, b0, true, fz0);
-    if (l1 && s1 < nD) dual2(s1, b1, PF, fz1);
+    if (l0) dual2(tid, g0, b0, true, fz0);
+    if (l1 && s1 < nD) dual2(s1, g1, b1, PF, fz1);
     for (int s = tid + 2 * TILE; s < nD; s += TILE) {
         long g;
         if (!live(s, g)) continue;
         const int bv[3] = {bs[s], bs[CA + s], bs[2 * CA + s]};
-        dual2(s, bv, false, fz0);
+        dual2(s, g, bv, false, fz0);
     }
     __syncthreads();
     // ---- primal n+2 over PC
     T own[4] = {T(0), T(0), T(0), T(0)};
-    if (l0) primal(tid, w0, nw0, c0, own, false, in1);
+    if (l0) primal(tid, g0, w0, nw0, c0, own, false, in1);
     if (l1 && s1 < nP) {
         T x[4];
-        primal(s1, w1, nw1, c1, x, false, in1);
+        primal(s1, g1, w1, nw1, c1, x, false, in1);
     }
     for (int s = tid + 2 * TILE; s < nP; s += TILE) {
         long g;
@@ -1315,7 +1333,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         T od[4], x[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) od[c] = uvx[c * sx + s - 2 * TILE];
-        primal(s, a.info[g], od, cv, x, false, in1);
+        primal(s, g, a.info[g], od, cv, x, false, in1);
     }
     __syncthreads();
     // ---- dual n+3 over the tile
@@ -1606,6 +1624,33 @@ __global__ void k_fz_aos(KArgs<S> a, S* __restrict__ rec) {
     const long g = (long)a.fz_off + t;
 #pragma unroll
     for (int c = 0; c < 4; ++c) IO::st(rec, 4l * t + c, IO::ld(a.s[F_UB + c], g));
+}
+
+// The reference rules' out-of-level neighbour values of a level (rfz_bar,
+// rfz_dual of KArgs), gathered once at level start: the two-iteration kernel
+// then reads one record per such neighbour, next to its own cube's, instead
+// of four scattered fields per iteration.
+template <class S>
+__global__ void k_ref_frozen(KArgs<S> a, S* __restrict__ bar, S* __restrict__ dual) {
+    using IO = Io<S>;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.lvl_n) return;
+    const long g = (long)a.lvl_off + t;
+    auto out = [&](int j) { return j >= 0 && !((uint32_t)j - a.lvl_off < a.lvl_n); };
+    constexpr int kF[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const long r = (long)ax * a.lvl_n + t;
+        const int jp = a.nbp[ax][g], jm = a.nbm[ax][g];
+        if (out(jp)) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) IO::st(bar, 4 * r + c, IO::ld(a.s[F_UB + c], jp));
+        }
+        if (out(jm)) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) IO::st(dual, 4 * r + e, IO::ld(a.s[kF[ax][e]], jm));
+        }
+    }
 }
 
 // ---- initialisation
@@ -2081,6 +2126,26 @@ void prepare_level(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, size_t li) {
     Problem& P = c->prob;
     a.fzaos = nullptr;
     a.fz_off = a.fz_n = 0;
+    a.rfz_bar = a.rfz_dual = nullptr;
+    if (p->op_mode == TGVX_OP_REF && !(p->fused == 0 && !p->tune)) {
+        // the reference rules' out-of-level neighbours (two-iteration kernel)
+        const Level& L = P.levels[li];
+        // sized for the largest level at the first use, so that the buffer
+        // never moves under a captured graph of an earlier level
+        uint32_t max_n = 1;
+        for (const Level& l : P.levels) max_n = std::max(max_n, l.n);
+        P.d_rfz.ensure(2 * 4 * 3 * (size_t)max_n * P.elem);
+        const size_t rec = 4 * 3 * (size_t)max_n;
+        S* bar = P.d_rfz.as<S>();
+        S* dual = reinterpret_cast<S*>(static_cast<char*>(P.d_rfz.p) + rec * P.elem);
+        KArgs<S> g = a;
+        g.lvl_off = L.off;
+        g.lvl_n = L.n;
+        if (L.n > 0) k_ref_frozen<S><<<(L.n + 255) / 256, 256, 0, c->stream>>>(g, bar, dual);
+        a.rfz_bar = bar;
+        a.rfz_dual = dual;
+        return;
+    }
     if (p->op_mode != TGVX_OP_ADJ || li == 0) return;
     const Level& prev = P.levels[li - 1];
     a.fz_off = prev.off;

@@ -118,6 +118,7 @@ struct Problem {
     DevBuf d_save[4];                // scratch copy of u, v of one level
     DevBuf d_alt[13];                // second (u, v, p, q) set of the fused iteration
     DevBuf d_snb[6], d_sflag, d_spar;
+    DevBuf d_rfz;                    // reference rules: out-of-level neighbour records of a level
     DevBuf d_fzaos;                  // the previous level's u_bar, v_bar as 4-value records  // streaming mode: per-leaf faces, flags, bisection parent
     DevBuf d_init;                   // float x 4 per position (warm start field), optional
     DevBuf d_orig;                   // uint32 position -> input index
