@@ -79,8 +79,12 @@ def main():
     ap.add_argument("--no-ref", action="store_true", help="skip the sweep with the reference's update rules")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
                     help="CPU baseline budget over its three timings (0 = skip)")
-    ap.add_argument("--workload", choices=["depth10", "batched"], default="depth10",
-                    help="batched: 256 independent blocks of log-uniform size in one problem vs one problem per block")
+    ap.add_argument("--workload", choices=["depth10", "batched", "streaming"], default="depth10",
+                    help="batched: 256 independent blocks of log-uniform size in one problem vs one problem per "
+                         "block; streaming: the depth-10 scene's streaming solve resident and out of core")
+    ap.add_argument("--leaf-cubes", type=int, default=1 << 20, help="streaming: cubes per treetop leaf at most")
+    ap.add_argument("--budgets-mb", default="1024,256",
+                    help="streaming: device budgets of the out-of-core solve, MiB, comma-separated")
     ap.add_argument("--blocks", type=int, default=256)
     ap.add_argument("--tune", type=int, default=1, help="1: autotune launch configurations before timing")
     ap.add_argument("--batched-gap", type=float, default=0.0,
@@ -122,6 +126,8 @@ def main():
     prec = {"fp32": T.FP32, "bf16": T.BF16, "bf16-dual": T.BF16_DUAL, "fp64": T.FP64}[args.precision]
     if args.workload == "batched":
         return batched(args, T, scene, prec)
+    if args.workload == "streaming":
+        return streaming(args, T, scene)
     t0 = time.time()
     log(f"[bench] generating scene: {args.views} views {args.width}x{args.height}, depth <= {args.max_depth}")
     sc = scene.torus_box_scene(views=args.views, W=args.width, H=args.height, max_depth=args.max_depth)
@@ -409,6 +415,57 @@ def main():
         "value_with_setup": with_setup,
         "octree_build": octree,
     }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def streaming(args, T, scene):
+    """tgv_minimize_streaming (the reference's stage 6, tgv.cpp:802-1070) on the
+    depth-10 scene split into treetop leaves: the resident solve (every cube's
+    state in HBM) against the out-of-core solve with the state in pinned host
+    memory and at most a given budget of per-cube data on the device. A step
+    is one coarse-to-fine sweep with `iters` iterations per level, uploads,
+    host work and downloads included (the reference reads and writes its
+    files in the same place)."""
+    import numpy as np
+
+    t0 = time.time()
+    sc = scene.torus_box_scene(views=args.views, W=args.width, H=args.height, max_depth=args.max_depth)
+    cubes = sc.cubes
+    log(f"[bench] scene: {len(cubes)} cubes in {time.time() - t0:.1f} s")
+    ctx = T.Context(0)
+    arch, cus = ctx.device_info()
+    hist = ctx.histograms(cubes, sc.views(), sc.origin, sc.r_root, 10)
+    leaves = scene.treetop(cubes, args.leaf_cubes)
+    p = T.TGVParams(alpha0=2.0, alpha1=1.0, iters=args.iters)
+    ctx.solve_streaming(cubes, hist, leaves, p)  # warm-up
+    t1 = time.perf_counter()
+    ref, rrep = ctx.solve_streaming(cubes, hist, leaves, p)
+    t_res = time.perf_counter() - t1
+    li = rrep.leaf_iters
+    log(f"[bench] resident streaming solve: {t_res:.3f} s ({li / t_res:.4g} leaf-iter/s incl. load)")
+    runs = []
+    for mb in [int(x) for x in args.budgets_mb.split(",") if x]:
+        st, rep, orep = ctx.solve_streaming_ooc(cubes, hist, leaves, p, device_bytes=mb << 20)
+        t1 = time.perf_counter()
+        st, rep, orep = ctx.solve_streaming_ooc(cubes, hist, leaves, p, device_bytes=mb << 20)
+        dt = time.perf_counter() - t1
+        same = bool(np.array_equal(np.ascontiguousarray(st).view(np.uint8), np.ascontiguousarray(ref).view(np.uint8)))
+        runs.append({"budget_mb": mb, "s": round(dt, 3), "leaf_iter_per_s": li / dt, "identical_to_resident": same,
+                     "iterate_and_transfer_ms": round(rep.iterate_ms, 1), **orep,
+                     "pcie_gbs": orep["moved_bytes"] / max(rep.iterate_ms, 1e-9) / 1e6})
+        log(f"[bench] out of core, {mb} MiB: {dt:.3f} s, {orep['batches']} batches, "
+            f"{orep['moved_bytes'] / 1e9:.2f} GB moved, identical {same}")
+    resident_bytes = len(cubes) * (17 * 4 + 4 + 4 * 4 + 6 * 4 + 1 + 4 + 12)
+    out = {"metric": "streaming TGV leaf-iter/s, resident and out of core, 1 GPU",
+           "value": runs[-1]["leaf_iter_per_s"] if runs else li / t_res, "unit": "leaf-iter/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "torus_box_depth10_streaming", "cubes": int(len(cubes)), "leaves": int(len(leaves)),
+                      "leaf_cubes_max": args.leaf_cubes, "iters_per_level": args.iters, "op_mode": "reference",
+                      "device": arch, "cus": cus},
+           "resident": {"s": round(t_res, 3), "leaf_iter_per_s": li / t_res,
+                        "device_state_bytes_estimate": resident_bytes},
+           "out_of_core": runs}
     print(json.dumps(out), flush=True)
     ctx.close()
 

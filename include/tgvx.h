@@ -205,6 +205,29 @@ int tgvx_solve_streaming(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, cons
                          const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* params,
                          void* state_out, tgvx_report* report);
 
+/* tgvx_solve_streaming out of core (no reference counterpart for the
+ * budget; the semantics are tgv_minimize_streaming's, tgv.cpp:802-1070): the
+ * state of every cube stays in pinned host memory, as the reference keeps it
+ * in its state file, and at most device_bytes of per-cube data are on the
+ * device at a time, in two slots that alternate so that one batch of treetop
+ * leaves moves over PCIe while the other iterates. Per level the parent copy
+ * and the level-start snapshot run on the host; the leaves holding cubes of
+ * the level run in batches of whole leaves. The sorted codes, depths and leaf
+ * of every cube (16 B per cube, reported as index_bytes) stay on the device
+ * for the neighbour lookups. FP32; results equal tgvx_solve_streaming's bit for
+ * bit. A leaf whose cubes of one level need more than a slot is an error. */
+typedef struct {
+    int32_t batches;        /* leaf batches run over all levels */
+    uint64_t slot_cubes;    /* capacity of a slot in cubes */
+    uint64_t device_bytes;  /* both slots */
+    uint64_t index_bytes;   /* codes, depths and leaf of every cube */
+    uint64_t moved_bytes;   /* host <-> device */
+    double tables_ms;       /* neighbour tables (device lookups) */
+} tgvx_ooc_report;
+int tgvx_solve_streaming_ooc(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+                             const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* params,
+                             uint64_t device_bytes, void* state_out, tgvx_report* report, tgvx_ooc_report* ooc);
+
 /* Resident problem: upload cubes + histograms once, build tables on the device. */
 int tgvx_problem_load(tgvx_ctx* ctx, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins,
                       int32_t nb, int32_t precision, int32_t op_mode);

@@ -1,0 +1,54 @@
+"""The streaming solve out of core: the state in pinned host memory, at most a
+given number of bytes of per-cube data on the device, treetop leaves run in
+batches on two alternating slots. Bit for bit the resident streaming solve
+(which the oracle pins) for every budget, from many small batches per level
+to one."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tgvreconclaude_amd import TGVParams, scene
+
+pytestmark = pytest.mark.gpu
+
+PER_CUBE = 4 * 28 + 1  # bytes of slot data per local cube
+
+
+def _bits(st):
+    st = np.concatenate([st["u"][:, None], st["v"], st["p"], st["q"]], axis=1) if st.dtype.names else st
+    return st.view(np.uint32)
+
+
+@pytest.mark.parametrize("slot_cubes", [3000, 20000, 10 ** 6])
+def test_sphere_out_of_core_equals_resident(ctx, sphere, slot_cubes):
+    sc, hist = sphere
+    leaves = scene.treetop(sc.cubes, 3000)
+    p = TGVParams(alpha0=2.0, alpha1=1.0, iters=40)
+    ref, _ = ctx.solve_streaming(sc.cubes, hist, leaves, p)
+    budget = 2 * PER_CUBE * slot_cubes
+    st, rep, orep = ctx.solve_streaming_ooc(sc.cubes, hist, leaves, p, device_bytes=budget)
+    print(orep)
+    assert np.array_equal(_bits(st), _bits(ref))
+    assert orep["device_bytes"] <= budget
+    nlev = len(np.unique(sc.cubes["depth"]))
+    if slot_cubes <= 3000:
+        assert orep["batches"] > 2 * nlev  # several batches per level, alternating slots
+    assert len(rep.levels()) == nlev
+
+
+def test_csg_out_of_core_equals_oracle(ctx):
+    sc = scene.csg_scene(41, views=12, W=320, H=240, max_depth=7)
+    hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 8)
+    leaves = scene.treetop(sc.cubes, 2000)
+    p = TGVParams(alpha0=0.5, alpha1=0.3, iters=25)
+    st, _, orep = ctx.solve_streaming_ooc(sc.cubes, hist, leaves, p, device_bytes=2 * PER_CUBE * 4000)
+    ref = O.tgv_minimize_streaming(sc.cubes, hist, leaves, O.params(alpha0=0.5, alpha1=0.3, iters=25))
+    assert np.array_equal(_bits(st), ref.view(np.uint32))
+    assert orep["batches"] > len(np.unique(sc.cubes["depth"]))
+
+
+def test_budget_below_a_leaf_is_refused(ctx, sphere):
+    sc, hist = sphere
+    leaves = scene.treetop(sc.cubes, 40000)
+    with pytest.raises(RuntimeError, match="too small|budget"):
+        ctx.solve_streaming_ooc(sc.cubes, hist, leaves, TGVParams(iters=2), device_bytes=2 * PER_CUBE * 100)

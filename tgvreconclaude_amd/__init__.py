@@ -171,6 +171,20 @@ class Context:
                                                  len(lv), C.byref(params), N.ptr(st), C.byref(rep)))
         return st, rep
 
+    def solve_streaming_ooc(self, cubes, hists, leaves, params=None, device_bytes=256 << 20):
+        """tgv_minimize_streaming with at most device_bytes of per-cube data on the
+        device (FP32); returns (state, Report, out-of-core report dict)."""
+        params = params or TGVParams()
+        cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)
+        h = np.ascontiguousarray(hists, dtype=np.uint32)
+        lv = np.ascontiguousarray(leaves, dtype=LEAF_DT)
+        st = self._state_array(len(cubes), FP32)
+        rep, orep = Report(), N.OocReport()
+        self._check(self._L.tgvx_solve_streaming_ooc(self.h, N.ptr(cubes), len(cubes), N.ptr(h), h.shape[1],
+                                                     N.ptr(lv), len(lv), C.byref(params), int(device_bytes),
+                                                     N.ptr(st), C.byref(rep), C.byref(orep)))
+        return st, rep, orep.as_dict()
+
     def solve_oc(self, cubes, hists, leaves, params=None):
         params = params or TGVParams()
         cubes = np.ascontiguousarray(cubes, dtype=CUBE_DT)

@@ -69,6 +69,14 @@ class TuneReport(C.Structure):
                  "default_us": self.default_us[i]} for i in range(self.n_levels)]
 
 
+class OocReport(C.Structure):
+    _fields_ = [("batches", C.c_int32), ("slot_cubes", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("index_bytes", C.c_uint64), ("moved_bytes", C.c_uint64), ("tables_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Block(C.Structure):
     _fields_ = [("cubes", C.c_void_p), ("n", C.c_uint64), ("bins", C.c_void_p)]
 
@@ -107,7 +115,7 @@ TGVX_SYMBOLS = [
     "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune", "tgvx_problem_solve_levels",
     "tgvx_surface_extract", "tgvx_surface_copy", "tgvx_solve_streaming", "tgvx_octree_build",
     "tgvx_octree_balance", "tgvx_octree_copy", "tgvx_octree_merge", "tgvx_tile_sets", "tgvx_histograms_noalloc",
-    "tgvx_problem_solve_from",
+    "tgvx_problem_solve_from", "tgvx_solve_streaming_ooc",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
@@ -167,6 +175,8 @@ def lib():
         L.tgvx_solve_streaming.argtypes = [vp, vp, C.c_uint64, vp, i32, vp, C.c_uint64, C.POINTER(Params), vp,
                                            C.POINTER(Report)]
         L.tgvx_problem_block_report.argtypes = [vp, vp, vp]
+        L.tgvx_solve_streaming_ooc.argtypes = [vp, vp, C.c_uint64, vp, i32, vp, C.c_uint64, C.POINTER(Params),
+                                               C.c_uint64, vp, C.POINTER(Report), C.POINTER(OocReport)]
         L.tgvx_solve_batched.argtypes = [vp, vp, C.c_uint32, i32, C.POINTER(Params), vp, C.POINTER(Report)]
         L.tgvx_octree_build.argtypes = [vp, C.POINTER(DepthView), i32, vp, C.c_float, i32, i32, C.POINTER(u64),
                                         C.POINTER(OctreeReport)]

@@ -294,6 +294,16 @@ int tgvx_solve_streaming(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const 
     });
 }
 
+int tgvx_solve_streaming_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int32_t nb,
+                             const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* p,
+                             uint64_t device_bytes, void* state_out, tgvx_report* rep, tgvx_ooc_report* ooc) {
+    return guarded(c, [&] {
+        if (p && p->precision != TGVX_FP32) throw tgvx::Error(TGVX_E_ARG, "the out-of-core solve stores FP32");
+        tgvx::solve_stream_ooc(c, cubes, n, bins, nb, leaves, nleaves, p, device_bytes, static_cast<float*>(state_out),
+                               rep, ooc);
+    });
+}
+
 int tgvx_autotune(tgvx_ctx* c, const tgvx_params* p, int32_t iters, tgvx_tune_report* rep) {
     return guarded(c, [&] { tgvx::autotune(c, p, iters, rep); });
 }

@@ -1577,6 +1577,51 @@ __global__ void k_stream_inputs(const uint64_t* __restrict__ pcode, const int32_
     depth[i] = pdepth[inv[i]];
 }
 
+// The streaming tables of input cubes [i0, i0 + cn) in input order (global
+// input indices), for the out-of-core solve: the same lookups as
+// k_stream_tables.
+__global__ void k_stream_tables_ooc(const uint64_t* __restrict__ code, const int32_t* __restrict__ depth, uint32_t n,
+                                    const int32_t* __restrict__ leaf_of, const uint32_t* __restrict__ lfirst,
+                                    const uint32_t* __restrict__ llast, uint32_t i0, uint32_t cn,
+                                    int32_t* __restrict__ nb, uint8_t* __restrict__ flag, int32_t* __restrict__ par) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cn) return;
+    const uint32_t i = i0 + t;
+    const int l = leaf_of[i];
+    if (l < 0) {
+        for (int f = 0; f < 6; ++f) nb[6 * (size_t)t + f] = -1;
+        flag[t] = 0;
+        par[t] = -1;
+        return;
+    }
+    const uint32_t f0 = lfirst[l], f1 = llast[l];
+    const uint64_t c = code[i];
+    const int d = depth[i];
+    uint32_t fl = 64u;
+    for (int f = 0; f < 6; ++f) {
+        const uint64_t k = face_nb(c, f, d);
+        int64_t j = -1;
+        if (k != kNone) {
+            j = leaf_code(code, f0, f1, k);
+            if (j < 0) j = bisect_code(code, n, k);
+            if (j < 0 && d > 0) {
+                const uint64_t pk = ancestor(k, d - 1);
+                j = leaf_code(code, f0, f1, pk);
+                if (j < 0) j = bisect_code(code, n, pk);
+            }
+        }
+        nb[6 * (size_t)t + f] = (int32_t)j;
+        if (j >= 0) {
+            const bool in = (uint64_t)j >= f0 && (uint64_t)j <= f1;
+            if ((f & 1) == 0 && in && depth[j] == d) fl |= 1u << f;
+            if ((f & 1) == 1 && !in) fl |= 1u << f;
+        }
+    }
+    flag[t] = (uint8_t)fl;
+    const int64_t pj = d > 0 ? bisect_code(code, n, ancestor(c, d - 1)) : -1;
+    par[t] = (int32_t)pj;
+}
+
 // Parent copy of one leaf's part of a level (tgv.cpp:895-911): read every
 // parent first, then write.
 template <class S>
@@ -3450,13 +3495,13 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     up(d_dd, ddepth.data(), 4 * std::max<size_t>(ddepth.size(), 1));
     up(d_dp, dpos.data(), 4 * std::max<size_t>(dpos.size(), 1));
     up(P.d_info, info.data(), 4 * std::max<uint64_t>(n, 1));
-    {
-        // histograms in position order (read by the warm-start features)
-        std::vector<uint32_t> pb((size_t)std::max<uint64_t>(n, 1) * nb, 0u);
-        for (uint64_t g = 0; g < n; ++g)
-            std::memcpy(&pb[(size_t)g * nb], bins + (size_t)P.perm[g] * nb, sizeof(uint32_t) * nb);
-        up(P.d_bins, pb.data(), 4 * pb.size());
-    }
+    // histograms in position order (read by the warm-start features); the
+    // host copy lives until the stream is drained at the end of the load,
+    // since the copy is only ordered on the stream
+    std::vector<uint32_t> pb((size_t)std::max<uint64_t>(n, 1) * nb, 0u);
+    for (uint64_t g = 0; g < n; ++g)
+        std::memcpy(&pb[(size_t)g * nb], bins + (size_t)P.perm[g] * nb, sizeof(uint32_t) * nb);
+    up(P.d_bins, pb.data(), 4 * pb.size());
     for (int ax = 0; ax < 3; ++ax) {
         P.d_nba[ax].ensure(4 * std::max<uint64_t>(n, 1));
         P.d_nbam[ax].ensure(4 * std::max<uint64_t>(n, 1));
@@ -3544,6 +3589,364 @@ void problem_solve_streaming(tgvx_ctx* c, const tgvx_params* p, const tgvx_treet
         case TGVX_FP32: solve_stream_t<float>(c, p, leaves, nleaves, rep); break;
         case TGVX_BF16_DUAL: solve_stream_t<mixs>(c, p, leaves, nleaves, rep); break;
         default: solve_stream_t<bf16s>(c, p, leaves, nleaves, rep); break;
+    }
+}
+
+namespace {
+// Pinned host memory owned by one solve.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    void alloc(size_t b) {
+        TGVX_HIP(hipHostMalloc(&p, std::max<size_t>(b, 16), hipHostMallocDefault));
+        bytes = b;
+    }
+    template <class X>
+    X* as() const { return static_cast<X*>(p); }
+};
+
+// One of the two alternating device slots of the out-of-core solve: SoA
+// arrays of capacity C cubes in one allocation (state[13], bars[4],
+// snapshot[4] as float; info, tables[6] as 32-bit words; flags), the pinned
+// staging of the same layout and the pinned copy of the state block back.
+struct OocSlot {
+    size_t cap = 0;
+    DevBuf dev;
+    PinnedBuf up, down;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<uint32_t> act;  // global indices of the batch's iterated cubes
+    bool busy = false;
+    size_t w = 0;  // the current batch's array stride: its local cubes
+    static size_t words(size_t cap) { return (13 + 4 + 4 + 1 + 6) * cap + (cap + 3) / 4; }
+    float* st(float* b) const { return b; }
+    float* bar(float* b) const { return b + 13 * w; }
+    float* fz(float* b) const { return b + 17 * w; }
+    uint32_t* info(float* b) const { return reinterpret_cast<uint32_t*>(b + 21 * w); }
+    int32_t* nb(float* b) const { return reinterpret_cast<int32_t*>(b + 22 * w); }
+    uint8_t* flag(float* b) const { return reinterpret_cast<uint8_t*>(b + 28 * w); }
+    ~OocSlot() {
+        if (done) (void)hipEventDestroy(done);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+}  // namespace
+
+// tgv_minimize_streaming (tgv.cpp:802-1070) out of core: the state of every
+// cube lives in pinned host memory (the reference keeps it in its state file)
+// and at most `budget` bytes of per-cube data are on the device at a time.
+// Per level, on the host: the parent copy leaf after leaf (bisection parent,
+// tgv.cpp:872-934) and the level-start snapshot of (u, v) (build_bars_frozen).
+// Then the treetop leaves holding cubes of the level run in batches: a
+// batch's iterated cubes, the cubes they read (their leaf's other cubes, other
+// leaves' and border cubes: snapshot values) and its tables go to a device
+// slot, the level's iterations run there with the resident solve's kernels
+// (streaming mode), and the iterated cubes' state comes back. The two slots
+// alternate on two streams, so one batch's transfers overlap the other's
+// iterations; within a level the leaves are independent (borders read the
+// snapshot, border p, q are zero), so the result is the resident solve's, bit
+// for bit. The sorted codes, depths and leaf of every cube (16 B per cube)
+// stay on the device for the neighbour lookups (CubeFileReader::lookup's
+// bisection, tgv.cpp:673-683), like the reference's open cube file.
+void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const uint32_t* bins, int nbins,
+                      const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* p, uint64_t budget,
+                      float* state_out, tgvx_report* rep, tgvx_ooc_report* orep) {
+    if (!p || !state_out || (n64 && (!cubes || !bins)) || (nleaves && !leaves)) throw Error(TGVX_E_ARG, "null pointer");
+    if (nbins <= 0 || nbins > 32) throw Error(TGVX_E_ARG, "nb must be in 1..32");
+    if (n64 >= (1ull << 30)) throw Error(TGVX_E_ARG, "at most 2^30 - 1 cubes");
+    if (p->op_mode != TGVX_OP_REF) throw Error(TGVX_E_ARG, "the streaming solve runs the reference rules (op_mode 0)");
+    if (p->iters < 0) throw Error(TGVX_E_ARG, "negative iteration count");
+    const uint32_t n = (uint32_t)n64;
+    tgvx_report r;
+    std::memset(&r, 0, sizeof(r));
+    const double t0 = now_ms();
+    // ---- inputs: codes (sorted), depths, per-cube constants, leaves
+    std::vector<uint64_t> code(n);
+    std::vector<int32_t> depth(n);
+    std::vector<uint32_t> info(n);
+    int min_d = 21, max_d = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        code[i] = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
+        depth[i] = cubes[i].depth;
+        if (depth[i] < 0 || depth[i] > 21) throw Error(TGVX_E_ARG, "cube depth outside 0..21");
+        if (i && code[i] < code[i - 1]) throw Error(TGVX_E_ARG, "cubes must be sorted by Morton code");
+        min_d = std::min(min_d, depth[i]);
+        max_d = std::max(max_d, depth[i]);
+        const uint32_t* b = bins + (size_t)i * nbins;
+        uint64_t tot = 0;
+        for (int k = 0; k < nbins; ++k) tot += b[k];
+        if (tot > InfoWord::kWMax) throw Error(TGVX_E_ARG, "vote total of a cube exceeds 2^23 - 1");
+        uint32_t kmed = 0, cum = 0;
+        for (int k = 0; k < nbins; ++k) {
+            cum += b[k];
+            if (cum * 2u >= (uint32_t)tot) {
+                kmed = (uint32_t)k;
+                break;
+            }
+        }
+        info[i] = (uint32_t)tot | (kmed << 24);
+    }
+    std::vector<int32_t> leaf_of(n, -1);
+    std::vector<uint32_t> lf(std::max<uint64_t>(nleaves, 1), 1), ll(std::max<uint64_t>(nleaves, 1), 0);
+    for (uint64_t l = 0; l < nleaves; ++l) {
+        if (leaves[l].first_idx > leaves[l].last_idx || leaves[l].first_idx >= n) continue;
+        lf[l] = leaves[l].first_idx;
+        ll[l] = std::min<uint32_t>(leaves[l].last_idx, n - 1);
+        for (uint32_t i = lf[l]; i <= ll[l]; ++i)
+            if (leaf_of[i] < 0) leaf_of[i] = (int32_t)l;
+    }
+    // state: u = median (tgv.cpp:821-845), everything else 0
+    PinnedBuf st, snap;
+    st.alloc(sizeof(float) * 13 * (size_t)n);
+    snap.alloc(sizeof(float) * 4 * (size_t)n);
+    float* S = st.as<float>();
+    float* Z = snap.as<float>();
+    std::memset(S, 0, sizeof(float) * 13 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) S[13 * (size_t)i] = median_of<float>(info[i], nbins);
+    if (n == 0) {
+        if (rep) *rep = r;
+        return;
+    }
+    // ---- per-cube tables on the device, in chunks: neighbours as each
+    // cube's leaf sees them (global indices), flags, bisection parent
+    std::vector<int32_t> nbg(6 * (size_t)n), par(n);
+    std::vector<uint8_t> sflag(n);
+    {
+        DevBuf d_code, d_depth, d_leaf, d_lf, d_ll, d_nb, d_fl, d_par;
+        d_code.ensure(8ull * n);
+        d_depth.ensure(4ull * n);
+        d_leaf.ensure(4ull * n);
+        d_lf.ensure(4 * lf.size());
+        d_ll.ensure(4 * ll.size());
+        TGVX_HIP(hipMemcpyAsync(d_code.p, code.data(), 8ull * n, hipMemcpyHostToDevice, c->stream));
+        TGVX_HIP(hipMemcpyAsync(d_depth.p, depth.data(), 4ull * n, hipMemcpyHostToDevice, c->stream));
+        TGVX_HIP(hipMemcpyAsync(d_leaf.p, leaf_of.data(), 4ull * n, hipMemcpyHostToDevice, c->stream));
+        TGVX_HIP(hipMemcpyAsync(d_lf.p, lf.data(), 4 * lf.size(), hipMemcpyHostToDevice, c->stream));
+        TGVX_HIP(hipMemcpyAsync(d_ll.p, ll.data(), 4 * ll.size(), hipMemcpyHostToDevice, c->stream));
+        const uint32_t chunk = 1u << 20;
+        d_nb.ensure(24ull * chunk);
+        d_fl.ensure(chunk);
+        d_par.ensure(4ull * chunk);
+        for (uint32_t i0 = 0; i0 < n; i0 += chunk) {
+            const uint32_t cn = std::min(chunk, n - i0);
+            k_stream_tables_ooc<<<(cn + 255) / 256, 256, 0, c->stream>>>(
+                d_code.as<uint64_t>(), d_depth.as<int32_t>(), n, d_leaf.as<int32_t>(), d_lf.as<uint32_t>(),
+                d_ll.as<uint32_t>(), i0, cn, d_nb.as<int32_t>(), d_fl.as<uint8_t>(), d_par.as<int32_t>());
+            TGVX_HIP(hipGetLastError());
+            TGVX_HIP(hipMemcpyAsync(nbg.data() + 6 * (size_t)i0, d_nb.p, 24ull * cn, hipMemcpyDeviceToHost, c->stream));
+            TGVX_HIP(hipMemcpyAsync(sflag.data() + i0, d_fl.p, cn, hipMemcpyDeviceToHost, c->stream));
+            TGVX_HIP(hipMemcpyAsync(par.data() + i0, d_par.p, 4ull * cn, hipMemcpyDeviceToHost, c->stream));
+            TGVX_HIP(hipStreamSynchronize(c->stream));
+        }
+    }
+    r.setup_ms = now_ms() - t0;
+    // ---- device slots: two, each a capacity of C local cubes
+    const size_t per_cube = 4 * (13 + 4 + 4 + 1 + 6) + 1;
+    const size_t cap = budget / (2 * per_cube);
+    if (cap < 64) throw Error(TGVX_E_ARG, "device budget below 2 x 64 cubes of slot data");
+    OocSlot slot[2];
+    for (OocSlot& sl : slot) {
+        sl.cap = cap;
+        sl.dev.ensure(4 * OocSlot::words(cap));
+        sl.up.alloc(4 * OocSlot::words(cap));
+        sl.down.alloc(sizeof(float) * 13 * cap);
+        TGVX_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        TGVX_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    }
+    const float sg = p->sigma > 0.f ? p->sigma : 0.25f, ta = p->tau > 0.f ? p->tau : 0.25f;
+    auto finish = [&](OocSlot& sl) {
+        if (!sl.busy) return;
+        TGVX_HIP(hipEventSynchronize(sl.done));
+        const float* D = sl.down.as<float>();
+        for (size_t a = 0; a < sl.act.size(); ++a)
+            for (int f = 0; f < 13; ++f) S[13 * (size_t)sl.act[a] + f] = D[f * sl.w + a];
+        sl.busy = false;
+    };
+    std::vector<int32_t> loc(n, -1);
+    std::vector<uint32_t> mark(n, 0u);
+    uint32_t stamp = 0;
+    double it_ms = 0.0;
+    int li = 0, nb_batches = 0;
+    uint64_t moved = 0;
+    for (int L = min_d; L <= max_d; ++L) {
+        // the leaves' cubes of depth L, leaf after leaf
+        std::vector<std::vector<uint32_t>> by_leaf(std::max<uint64_t>(nleaves, 1));
+        uint64_t nL = 0;
+        for (uint64_t l = 0; l < nleaves; ++l) {
+            if (lf[l] > ll[l]) continue;
+            for (uint32_t i = lf[l]; i <= ll[l]; ++i)
+                if (depth[i] == L && leaf_of[i] == (int32_t)l) by_leaf[l].push_back(i);
+            nL += by_leaf[l].size();
+        }
+        bool any = false;
+        for (uint32_t i = 0; i < n && !any; ++i) any = depth[i] == L;
+        if (!any) continue;
+        // parent copy, leaf after leaf: each leaf reads its parents first
+        if (L > min_d) {
+            std::vector<float> tmp;
+            for (uint64_t l = 0; l < nleaves; ++l) {
+                const std::vector<uint32_t>& v = by_leaf[l];
+                tmp.assign(4 * v.size(), 0.f);
+                for (size_t k = 0; k < v.size(); ++k)
+                    if (par[v[k]] >= 0) std::memcpy(&tmp[4 * k], S + 13 * (size_t)par[v[k]], 16);
+                for (size_t k = 0; k < v.size(); ++k) {
+                    float* s = S + 13 * (size_t)v[k];
+                    if (par[v[k]] >= 0) std::memcpy(s, &tmp[4 * k], 16);
+                    std::memset(s + 4, 0, 9 * sizeof(float));
+                }
+            }
+        }
+        // level-start snapshot of every cube's (u, v)
+        for (uint32_t i = 0; i < n; ++i) std::memcpy(Z + 4 * (size_t)i, S + 13 * (size_t)i, 16);
+        const double tl0 = now_ms();
+        // batches of whole leaves; a leaf's cubes read at most 7 local slots each
+        uint64_t l = 0, used_slot = 0;
+        while (l < nleaves) {
+            if (by_leaf[l].empty()) {
+                ++l;
+                continue;
+            }
+            OocSlot& sl = slot[used_slot & 1];
+            finish(sl);
+            float* U = sl.up.as<float>();
+            const size_t C = sl.cap;
+            sl.act.clear();
+            std::vector<uint32_t> others;
+            // the batch: whole leaves while the distinct cubes they iterate
+            // and read fit the slot (counted with a stamp per batch)
+            ++stamp;
+            uint64_t l1 = l;
+            size_t count = 0;
+            std::vector<uint32_t> fresh;
+            while (l1 < nleaves) {
+                fresh.clear();
+                auto see = [&](int32_t j) {
+                    if (j >= 0 && mark[j] != stamp) {
+                        mark[j] = stamp;
+                        fresh.push_back((uint32_t)j);
+                    }
+                };
+                for (uint32_t i : by_leaf[l1]) {
+                    see((int32_t)i);
+                    for (int f = 0; f < 6; ++f) see(nbg[6 * (size_t)i + f]);
+                }
+                if (count + fresh.size() > C) {
+                    for (uint32_t j : fresh) mark[j] = 0;
+                    break;
+                }
+                count += fresh.size();
+                ++l1;
+            }
+            if (l1 == l) throw Error(TGVX_E_ARG, "device budget too small for treetop leaf " + std::to_string(l));
+            for (uint64_t k = l; k < l1; ++k)
+                for (uint32_t i : by_leaf[k]) {
+                    loc[i] = (int32_t)sl.act.size();
+                    sl.act.push_back(i);
+                }
+            const size_t na = sl.act.size();
+            auto local = [&](int32_t j) -> int32_t {
+                if (j < 0) return -1;
+                if (loc[j] < 0) {
+                    loc[j] = (int32_t)(na + others.size());
+                    others.push_back((uint32_t)j);
+                }
+                return loc[j];
+            };
+            // local indices of every neighbour first: they fix the batch's size
+            std::vector<int32_t> nbl(6 * na);
+            for (size_t a = 0; a < na; ++a)
+                for (int f = 0; f < 6; ++f) nbl[6 * a + f] = local(nbg[6 * (size_t)sl.act[a] + f]);
+            const size_t W = na + others.size();
+            sl.w = W;
+            int32_t* NB = sl.nb(U);
+            for (size_t a = 0; a < na; ++a) {
+                const uint32_t i = sl.act[a];
+                for (int f = 0; f < 6; ++f) NB[f * W + a] = nbl[6 * a + f];
+                sl.info(U)[a] = info[i];
+                sl.flag(U)[a] = sflag[i];
+            }
+            auto put = [&](size_t a, uint32_t i) {
+                for (int f = 0; f < 13; ++f) sl.st(U)[f * W + a] = S[13 * (size_t)i + f];
+                for (int k = 0; k < 4; ++k) sl.fz(U)[k * W + a] = Z[4 * (size_t)i + k];
+            };
+            for (size_t a = 0; a < na; ++a) {
+                put(a, sl.act[a]);
+                for (int k = 0; k < 4; ++k) sl.bar(U)[k * W + a] = Z[4 * (size_t)sl.act[a] + k];
+            }
+            for (size_t k = 0; k < others.size(); ++k) put(na + k, others[k]);
+            for (uint32_t i : sl.act) loc[i] = -1;
+            for (uint32_t i : others) loc[i] = -1;
+            // up, iterate, down
+            if (W > C) throw Error(TGVX_E_STATE, "out-of-core batch exceeds its slot");
+            TGVX_HIP(hipMemcpyAsync(sl.dev.p, U, 4 * OocSlot::words(W), hipMemcpyHostToDevice, sl.stream));
+            float* dv = sl.dev.as<float>();
+            KArgs<float> a;
+            std::memset(&a, 0, sizeof(a));
+            for (int f = 0; f < 13; ++f) a.s[f] = dv + f * W;
+            for (int k = 0; k < 4; ++k) {
+                a.s[F_UB + k] = dv + (13 + k) * W;
+                a.fz[k] = dv + (17 + k) * W;
+            }
+            a.info = reinterpret_cast<const uint32_t*>(dv + 21 * W);
+            for (int ax = 0; ax < 3; ++ax) {
+                a.nbp[ax] = reinterpret_cast<const int32_t*>(dv + (22 + 2 * ax) * W);
+                a.nbm[ax] = reinterpret_cast<const int32_t*>(dv + (23 + 2 * ax) * W);
+            }
+            a.sflag = reinterpret_cast<const uint8_t*>(dv + 28 * W);
+            a.nbins = nbins;
+            a.off = 0;
+            a.n = (uint32_t)na;
+            a.lvl_off = 0;
+            a.lvl_n = (uint32_t)na;
+            a.sigma = sg;
+            a.tau = ta;
+            a.a0 = p->alpha0;
+            a.a1 = p->alpha1;
+            a.tl = ta * p->lambda;
+            a.init_stride = 4;
+            const unsigned grid = (unsigned)((na + 255) / 256);
+            for (int it = 0; it < p->iters; ++it) {
+                k_dual<float, 2, true><<<grid, 256, 0, sl.stream>>>(a);
+                k_primal_ref<float, true><<<grid, 256, 0, sl.stream>>>(a);
+            }
+            TGVX_HIP(hipGetLastError());
+            TGVX_HIP(hipMemcpyAsync(sl.down.p, dv, sizeof(float) * 13 * W, hipMemcpyDeviceToHost, sl.stream));
+            TGVX_HIP(hipEventRecord(sl.done, sl.stream));
+            sl.busy = true;
+            moved += 4 * OocSlot::words(W) + sizeof(float) * 13 * W;
+            ++used_slot;
+            ++nb_batches;
+            l = l1;
+        }
+        finish(slot[0]);
+        finish(slot[1]);
+        it_ms += now_ms() - tl0;
+        if (li < TGVX_MAX_LEVELS) {
+            r.level_depth[li] = L;
+            r.level_leaves[li] = (uint32_t)nL;
+            r.level_iters[li] = p->iters;
+        }
+        r.leaf_iters += (double)nL * p->iters;
+        ++li;
+    }
+    std::memcpy(state_out, S, sizeof(float) * 13 * (size_t)n);
+    r.n_levels = std::min(li, TGVX_MAX_LEVELS);
+    r.solve_ms = now_ms() - t0;
+    r.iterate_ms = it_ms;
+    if (rep) *rep = r;
+    if (orep) {
+        orep->batches = nb_batches;
+        orep->slot_cubes = cap;
+        orep->device_bytes = 2 * 4 * OocSlot::words(cap);
+        orep->index_bytes = 16ull * n;
+        orep->moved_bytes = moved;
+        orep->tables_ms = r.setup_ms;
     }
 }
 

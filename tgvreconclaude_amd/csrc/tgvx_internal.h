@@ -171,6 +171,9 @@ void problem_solve(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx
 void problem_solve_levels(tgvx_ctx* c, const tgvx_params* p, const int32_t* level_iters, int32_t n_levels,
                           tgvx_report* rep);
 void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out);
+void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb,
+                      const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* p, uint64_t budget,
+                      float* state_out, tgvx_report* rep, tgvx_ooc_report* orep);
 void problem_solve_streaming(tgvx_ctx* c, const tgvx_params* p, const tgvx_treetop_leaf* leaves, uint64_t nleaves,
                              tgvx_report* rep);
 void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* P, double* D);
