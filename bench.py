@@ -41,7 +41,8 @@ ALG_BYTES = {k: 34 * e + 8 + 24 for k, e in ELEM.items()}
 # p, q in BF16, u, v and the extrapolations in FP32: 16 values of 4 B and 18 of 2 B
 ALG_BYTES["bf16-dual"] = 16 * 4 + 18 * 2 + 8 + 24
 KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj",
-                3: "k_fused_tile (p, q staged in LDS)", 4: "k_fused_halo", 5: "k_fused2", 6: "k_level1"}
+                3: "k_fused_tile (p, q staged in LDS)", 4: "k_fused_halo", 5: "k_fused2", 6: "k_level1",
+                7: "k_persist"}
 
 
 def log(msg):
@@ -63,11 +64,12 @@ def main():
                     help="workgroup / tile size (0: 1024 for the two-iteration kernel, 512 for the other tiled "
                          "fused variants, else 256); with the variant, the autotuner's base configuration")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5, 6], default=5,
+    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5, 6, 7], default=5,
                     help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "
                          "kernel; 3: LDS-tiled with the tile's p, q staged in LDS; 4: LDS-tiled with a "
                          "tabulated halo; 5: two iterations per launch over tabulated tile sets; 6: levels of at "
-                         "most 1024 cubes in one workgroup for all their iterations, larger ones as 5")
+                         "most 1024 cubes in one workgroup for all their iterations, larger ones as 5; 7: a level "
+                         "in one cooperative launch with grid barriers (levels too large for it as 6)")
     ap.add_argument("--no-fused", action="store_true", help="same as --fused 0")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start time-to-gap runs")
     ap.add_argument("--no-octree", action="store_true",
@@ -79,9 +81,11 @@ def main():
     ap.add_argument("--no-ref", action="store_true", help="skip the sweep with the reference's update rules")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
                     help="CPU baseline budget over its three timings (0 = skip)")
-    ap.add_argument("--workload", choices=["depth10", "batched", "streaming"], default="depth10",
+    ap.add_argument("--workload", choices=["depth10", "batched", "streaming", "variants"], default="depth10",
                     help="batched: 256 independent blocks of log-uniform size in one problem vs one problem per "
-                         "block; streaming: the depth-10 scene's streaming solve resident and out of core")
+                         "block; streaming: the depth-10 scene's streaming solve resident and out of core; "
+                         "variants: per-level iteration time of each launch variant of the depth-10 scene, "
+                         "grid-barrier cost, whole sweeps with one variant")
     ap.add_argument("--leaf-cubes", type=int, default=1 << 20, help="streaming: cubes per treetop leaf at most")
     ap.add_argument("--budgets-mb", default="1024,256",
                     help="streaming: device budgets of the out-of-core solve, MiB, comma-separated")
@@ -128,6 +132,8 @@ def main():
         return batched(args, T, scene, prec)
     if args.workload == "streaming":
         return streaming(args, T, scene)
+    if args.workload == "variants":
+        return variants(args, T, scene)
     t0 = time.time()
     log(f"[bench] generating scene: {args.views} views {args.width}x{args.height}, depth <= {args.max_depth}")
     sc = scene.torus_box_scene(views=args.views, W=args.width, H=args.height, max_depth=args.max_depth)
@@ -216,15 +222,21 @@ def main():
     if tuning:
         variant = tuning[-1]["variant"]
     fused = variant != 0
+    tile = tuning[-1]["block"] if tuning else args.block
+    xcd = tuning[-1]["xcd"] if tuning else 1
+    if variant in (6, 7):
+        # the variants for small levels time the finest level's fused pass as
+        # the two-iteration kernel they fall back to there
+        variant, tile, xcd = 5, (1024 if n_f >= 1 << 18 else 512), 1
     # fused_ms is per iteration; the two-iteration kernel's launch is two
-    two = variant == 5 and ctx.tile_sets(finest, tuning[-1]["block"] if tuning else args.block)[0]
+    two = variant == 5 and ctx.tile_sets(finest, tile)[0]
     if variant == 5 and not two:
         variant = 1  # the level's tile sets do not fit: one-iteration tiled passes
     it_ms = fused_ms if fused else dual_ms + primal_ms
     kernel = f"{KERNEL_NAMES[variant]}, depth {finest}, {n_f} leaves" + (
         " (one launch = two iterations)" if two else (" (one launch = one iteration)" if fused else ""))
-    if tuning:
-        kernel += f", workgroup {tuning[-1]['block']}, {'XCD-contiguous' if tuning[-1]['xcd'] else 'plain'} tile order"
+    if fused:
+        kernel += f", workgroup {tile}, {'XCD-contiguous' if xcd else 'plain'} tile order"
     ab = ALG_BYTES[args.precision]
     achieved = ab * n_f / (it_ms * 1e-3) / 1e9
     log(f"[bench] depth {finest} ({n_f} leaves): fused {fused_ms * 1e3:.1f} us; two-kernel dual {dual_ms * 1e3:.1f} us "
@@ -372,7 +384,7 @@ def main():
     if args.precision == "fp32" and os.path.exists(pj):
         name = KERNEL_NAMES[variant]
         if variant in (1, 3, 4, 5):
-            name += f", tile {tuning[-1]['block'] if tuning else args.block}"
+            name += f", tile {tile}"
         rec = json.load(open(pj)).get(name)
         if rec and rec["leaves"] == n_f:
             traffic_it = rec["traffic_bytes_per_iteration"]
@@ -415,6 +427,64 @@ def main():
         "value_with_setup": with_setup,
         "octree_build": octree,
     }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def variants(args, T, scene):
+    """Where launches cost: on the depth-10 scene (exact-adjoint rules, FP32),
+    each level of at most 2^18 cubes timed for 50 iterations with each launch
+    variant (the better of three runs, graphs for the variants that launch
+    per iteration), the cost of one grid barrier of the persistent level
+    kernel by grid size, and whole sweeps (50 iterations per level) with one
+    variant for every level."""
+    t0 = time.time()
+    sc = scene.torus_box_scene(views=args.views, W=args.width, H=args.height, max_depth=args.max_depth)
+    log(f"[bench] scene: {len(sc.cubes)} cubes in {time.time() - t0:.1f} s")
+    ctx = T.Context(0)
+    arch, cus = ctx.device_info()
+    hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
+    ctx.load(sc.cubes, hist, precision=T.FP32, op_mode=T.OP_ADJ)
+    cfgs = [(0, 256), (4, 128), (5, 512), (5, 1024), (6, 1024), (7, 1024), (7, 256)]
+    per_level = []
+    for d, n in ctx.levels():
+        if n > 1 << 18:
+            continue
+        row = {"depth": d, "cubes": n}
+        for v, b in cfgs:
+            p = T.TGVParams(alpha0=2.0, alpha1=1.0, iters=50, precision=T.FP32, op_mode=T.OP_ADJ, fused=v, block=b)
+            ctx.level_bench(d, p, 50)  # graphs and tables built
+            row[f"{KERNEL_NAMES[v]}/{b}"] = round(min(ctx.level_bench(d, p, 50)[0] for _ in range(3)) * 1e3 / 50, 2)
+        per_level.append(row)
+        log(f"[bench] depth {d} ({n} cubes), us per iteration: "
+            f"{ {k: v for k, v in row.items() if k not in ('depth', 'cubes')} }")
+    barrier = []
+    for wgs, b in [(1, 1024), (4, 1024), (16, 1024), (64, 1024), (256, 1024), (64, 256), (256, 256), (1024, 256)]:
+        us = min(ctx.grid_barrier_us(wgs, b) for _ in range(2))
+        barrier.append({"workgroups": wgs, "block": b, "us_per_barrier": round(us, 3)})
+        log(f"[bench] grid barrier, {wgs} workgroups of {b}: {us:.2f} us")
+    hip = __import__("ctypes").CDLL("libamdhip64.so")
+    sweeps = []
+    for v, b in [(6, 1024), (7, 1024), (7, 256)]:
+        p = T.TGVParams(alpha0=2.0, alpha1=1.0, iters=50, precision=T.FP32, op_mode=T.OP_ADJ, fused=v, block=b)
+        ctx.run(p)
+        hip.hipDeviceSynchronize()
+        t1 = time.perf_counter()
+        li = 0.0
+        for _ in range(args.steps):
+            rep = ctx.run(p)
+            li += rep.leaf_iters
+        hip.hipDeviceSynchronize()
+        dt = time.perf_counter() - t1
+        sweeps.append({"variant": KERNEL_NAMES[v], "block": b, "ms_per_sweep": round(dt / args.steps * 1e3, 3),
+                       "leaf_iter_per_s": li / dt,
+                       "level_ms": [(l["depth"], round(l["ms"], 3)) for l in rep.levels()]})
+        log(f"[bench] sweep, {KERNEL_NAMES[v]}/{b} where it fits: {dt / args.steps * 1e3:.2f} ms")
+    out = {"metric": "per-level TGV iteration time by launch variant, 1 GPU", "unit": "us per iteration",
+           "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "torus_box_depth10", "leaves": int(len(sc.cubes)), "op_mode": "adjoint",
+                      "iters": 50, "device": arch, "cus": cus},
+           "per_level": per_level, "grid_barrier": barrier, "sweeps": sweeps}
     print(json.dumps(out), flush=True)
     ctx.close()
 

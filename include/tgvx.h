@@ -111,8 +111,12 @@ typedef struct {
                               per launch over tile sets tabulated once per level and tile (a
                               level whose sets do not fit runs as 1); 6 a level of at most 1024
                               cubes in one workgroup for all its iterations (larger levels run
-                              as 5, tiles of 1024 from 2^18 cubes on, else 512). For 1, 3, 4
-                              and 5, block is the tile (128, 256 = default, 512, 1024) */
+                              as 5, tiles of 1024 from 2^18 cubes on, else 512); 7 a level in
+                              one cooperative launch for all its iterations, two grid barriers
+                              per iteration, workgroups of block = 256 or 1024 (default; a level
+                              whose workgroups cannot all be resident at once runs as 6 does).
+                              For 1, 3, 4 and 5, block is the tile (128, 256 = default, 512,
+                              1024) */
     int32_t tune;          /* 1: take each level's launch configuration from tgvx_autotune's table
                               (levels without an entry use fused / block as given) */
     float sigma, tau;      /* primal-dual steps; 0 = the reference's 0.25 (tgv.cpp:149-150).
@@ -262,6 +266,15 @@ int tgvx_problem_levels(tgvx_ctx* ctx, int32_t* depths, uint32_t* counts, int32_
 int tgvx_level_bench(tgvx_ctx* ctx, int32_t depth, const tgvx_params* params, int32_t iters,
                      double* total_ms, double* dual_ms, double* primal_ms, double* fused_ms);
 
+/* Grid-barrier cost (no reference counterpart; measurement for the choice
+ * of params.fused = 7): microseconds per barrier of one cooperative launch of
+ * `workgroups` workgroups of `block` (256 or 1024) threads that does nothing
+ * but `barriers` (1..100000) of the persistent level kernel's grid barriers
+ * (difference of two launches, HIP events on the library's stream). Fails
+ * when the workgroups cannot all be resident at once. */
+int tgvx_grid_barrier_bench(tgvx_ctx* ctx, int32_t workgroups, int32_t block, int32_t barriers,
+                            double* us_per_barrier);
+
 /* Tile sets of the two-iteration kernel (params.fused = 5; no reference
  * counterpart) for the level at `depth` of the resident problem (exact-adjoint
  * mode) and a tile of 128, 256 or 512 cubes, built if not yet built: usable = 1
@@ -328,8 +341,8 @@ int tgvx_octree_copy(tgvx_ctx* ctx, tgvx_cube* out);
  * For every level of the loaded problem, times `iters` iterations of each
  * launch configuration (the better of two timings, after a warm-up; pass the
  * solve's chunk, e.g. 50) -- kernel variant (dual + primal kernels, the fused
- * variants 1-6 of tgvx_params.fused; exact-adjoint mode only for the fused
- * ones), workgroup / tile size (64 to 1024) and tile order (XCD-contiguous or
+ * variants 1-7 of tgvx_params.fused; exact-adjoint mode only for the fused
+ * ones but 5), workgroup / tile size (64 to 1024) and tile order (XCD-contiguous or
  * plain) -- and stores the fastest in the context, per (precision, operator
  * mode, floor(log2 leaves)), counting 1 us per kernel launch and iteration on
  * top of the measured time (what a launch costs more inside a coarse-to-fine
@@ -343,7 +356,8 @@ typedef struct {
     uint32_t level_leaves[TGVX_MAX_LEVELS];
     int32_t variant[TGVX_MAX_LEVELS]; /* 0 two kernels, 1 tiled fused, 2 untiled fused, 3 tiled + LDS p, q,
                                          4 tiled + tabulated halo, 5 two iterations per launch,
-                                         6 the whole level in one workgroup */
+                                         6 the whole level in one workgroup,
+                                         7 the whole level in one cooperative launch */
     int32_t block[TGVX_MAX_LEVELS];
     int32_t xcd[TGVX_MAX_LEVELS];
     double best_us[TGVX_MAX_LEVELS];    /* per iteration, chosen configuration */

@@ -397,6 +397,13 @@ class Context:
                                              C.byref(p), C.byref(f)))
         return t.value, d.value, p.value, f.value
 
+    def grid_barrier_us(self, workgroups, block=1024, barriers=2000):
+        """Microseconds per grid barrier of the persistent level kernel in a
+        cooperative launch of `workgroups` workgroups of `block` threads."""
+        us = C.c_double()
+        self._check(self._L.tgvx_grid_barrier_bench(self.h, workgroups, block, barriers, C.byref(us)))
+        return us.value
+
     def tile_sets(self, depth, tile):
         """The two-iteration kernel's tile sets at `depth` (exact-adjoint problem):
         (usable, (largest PA, DB, PC), LDS bytes per workgroup)."""

@@ -168,6 +168,14 @@ int tgvx_level_bench(tgvx_ctx* c, int32_t depth, const tgvx_params* p, int32_t i
     });
 }
 
+int tgvx_grid_barrier_bench(tgvx_ctx* c, int32_t workgroups, int32_t block, int32_t barriers,
+                            double* us_per_barrier) {
+    return guarded(c, [&] {
+        if (!us_per_barrier) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        tgvx::grid_barrier_bench(c, workgroups, block, barriers, us_per_barrier);
+    });
+}
+
 int tgvx_tile_sets(tgvx_ctx* c, int32_t depth, int32_t tile, int32_t* usable, int32_t* max_sets,
                    int64_t* lds_bytes) {
     return guarded(c, [&] {

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <set>
@@ -1495,6 +1496,186 @@ __global__ void __launch_bounds__(kLevel1) k_level1(KArgs<S> a, int iters) {
     for (int c = 0; c < 4; ++c) IO::st(a.s[F_UB + c], g, ub[c]);
 }
 
+// ---- a whole level in one cooperative launch (no reference counterpart):
+// one cube per thread over as many workgroups as the level needs, all
+// co-resident, `iters` iterations with two grid-wide barriers each. Thread
+// state (u, v, p, q, u_bar, v_bar, table entries) stays in registers; the
+// exchange goes through the main set: u_bar, v_bar are published, barrier,
+// the dual step reads the +a neighbours' (k_dual's expressions), p, q are
+// published, barrier, the primal step reads the incoming ones
+// (primal_adj_at's expressions). A value is never overwritten before the
+// barrier that follows its last read, so the state after the launch is that
+// of `iters` two-kernel iterations, bit for bit.
+//
+// Barrier: one arrival counter per launch (cleared by the host before it);
+// barrier k is complete once the counter reaches (k + 1) * gridDim.x. The
+// arrival releases and the wait acquires at agent scope: the XCDs' L2 caches
+// are not coherent with each other, so the release writes the arriving L2
+// back and the acquire invalidates the waiting one. The wait is bounded:
+// after spin_max polls, or once another workgroup has given up, it sets
+// *err and every workgroup returns, so the launch always ends; the host
+// reads *err and fails the solve.
+struct GridBar {
+    uint32_t* cnt;
+    int32_t* err;
+    uint32_t spin_max;
+};
+
+__device__ inline bool grid_barrier(const GridBar& b, uint32_t k) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t target = (k + 1u) * gridDim.x;
+        __hip_atomic_fetch_add(b.cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        uint32_t polls = 0;
+        while (__hip_atomic_load(b.cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            ++polls;
+            if (polls > b.spin_max ||
+                ((polls & 63u) == 0u && __hip_atomic_load(b.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                __hip_atomic_store(b.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+template <class S, int B>
+__global__ void __launch_bounds__(B) k_persist(KArgs<S> a, int iters, GridBar gb) {
+    using IO = Io<S>;
+    using T = typename IO::T;
+    const uint32_t t = blockIdx.x * (uint32_t)B + threadIdx.x;
+    const bool inr = t < a.n;
+    const long g = (long)a.off + (inr ? t : 0u);
+    const bool run = inr && !masked(a, g);
+    T u = T(0), v[3] = {T(0), T(0), T(0)}, p[3] = {T(0), T(0), T(0)}, ub[4] = {T(0), T(0), T(0), T(0)};
+    T q[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    uint32_t w = 0u;
+    int nb[3] = {-1, -1, -1}, inc[3] = {-1, -1, -1};
+    if (run) {
+        u = IO::ld(a.s[F_U], g);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = IO::ld(a.s[F_VX + c], g);
+            p[c] = IO::ld(a.s[F_PX + c], g);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q[k] = IO::ld(a.s[F_Q0 + k], g);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ub[c] = IO::ld(a.s[F_UB + c], g);
+        w = a.info[g];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            nb[ax] = a.nbp[ax][g];
+            inc[ax] = a.nbt[ax][g];
+        }
+    }
+    uint32_t k = 0;  // barriers passed
+    for (int it = 0; it < iters; ++it) {
+        // passes 1-2 (k_dual): a missing +a neighbour gives the cube's own
+        // value; one outside the level gives its u_bar, v_bar, which no
+        // launch of this level writes
+        if (run) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) IO::st(a.s[F_UB + c], g, ub[c]);
+        }
+        if (!grid_barrier(gb, k++)) return;
+        if (run) {
+            T bar[4][4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bar[0][c] = ub[c];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) bar[1 + ax][c] = nb[ax] < 0 ? ub[c] : IO::ld(a.s[F_UB + c], nb[ax]);
+            }
+            dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                p[c] = IO::rtd(p[c]);
+                IO::st(a.s[F_PX + c], g, p[c]);
+            }
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                q[m] = IO::rtd(q[m]);
+                IO::st(a.s[F_Q0 + m], g, q[m]);
+            }
+        }
+        if (!grid_barrier(gb, k++)) return;
+        // passes 3-5 (primal_adj_at, then the extrapolation)
+        if (run) {
+            T dp = T(0);
+            T dq[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const bool hs = InfoWord::has(w, ax);
+                dp = hs ? dp + p[ax] : dp;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T w2 = (c == ax) ? T(1) : T(0.5);
+                    dq[c] = hs ? dq[c] + w2 * q[qidx(c, ax) - F_Q0] : dq[c];
+                }
+            }
+            constexpr int kF[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
+            T in[3][4];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const long s = inc[ax] >= 0 ? (long)inc[ax] : g;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) in[ax][e] = IO::ld(a.s[kF[ax][e]], s);
+            }
+            if (inc[0] >= 0) {
+                dp -= in[0][0];
+                dq[0] -= T(1) * in[0][1];
+                dq[1] -= T(0.5) * in[0][2];
+                dq[2] -= T(0.5) * in[0][3];
+            }
+            if (inc[1] >= 0) {
+                dp -= in[1][0];
+                dq[0] -= T(0.5) * in[1][1];
+                dq[1] -= T(1) * in[1][2];
+                dq[2] -= T(0.5) * in[1][3];
+            }
+            if (inc[2] >= 0) {
+                dp -= in[2][0];
+                dq[0] -= T(0.5) * in[2][1];
+                dq[1] -= T(0.5) * in[2][2];
+                dq[2] -= T(1) * in[2][3];
+            }
+            T nw[4];
+            nw[0] = prox(u + a.tau * dp, w, a.nbins, a.tl);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nw[1 + c] = v[c] + a.tau * (p[c] + dq[c]);
+            ub[0] = IO::rt(T(2) * nw[0] - u);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ub[1 + c] = IO::rt(T(2) * nw[1 + c] - v[c]);
+            u = IO::rt(nw[0]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = IO::rt(nw[1 + c]);
+        }
+    }
+    if (!run) return;
+    IO::st(a.s[F_U], g, u);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) IO::st(a.s[F_VX + c], g, v[c]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) IO::st(a.s[F_UB + c], g, ub[c]);
+}
+
+// `nbar` grid barriers and nothing else: the cost of one barrier of k_persist
+// at a given grid (no reference counterpart).
+template <int B>
+__global__ void __launch_bounds__(B) k_barrier_bench(GridBar gb, int nbar) {
+    for (int k = 0; k < nbar; ++k)
+        if (!grid_barrier(gb, (uint32_t)k)) return;
+}
+
 // ---- streaming mode (tgv_minimize_streaming, tgv.cpp:515-1070)
 // The cube file's bisection (CubeFileReader::lookup, tgv.cpp:674-683): the
 // first probe whose code matches, not necessarily the last cube with it.
@@ -2204,7 +2385,7 @@ int block_of(const tgvx_params* p) {
     if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
         throw Error(TGVX_E_ARG, "block must be 64, 128, 256, 512 or 1024");
     const bool tiled = p && ((p->op_mode == TGVX_OP_ADJ && (p->fused == 1 || p->fused == 3 || p->fused == 4)) ||
-                             p->fused == 5 || p->fused == 6);
+                             p->fused == 5 || p->fused == 6 || p->fused == 7);
     if (b > 256 && !tiled) throw Error(TGVX_E_ARG, "blocks above 256 exist only for the tiled fused kernels");
     return b;
 }
@@ -2214,7 +2395,9 @@ int block_of(const tgvx_params* p) {
 // with the tile's p, q staged in LDS, 4 = tiled with a precomputed halo,
 // 5 = two iterations per launch over tabulated tile sets, 6 = a level of at
 // most kLevel1 cubes in one workgroup for all its iterations (larger levels
-// run variant 5 with tiles of 512);
+// run variant 5 with tiles of 512), 7 = a level in one cooperative launch
+// with grid barriers (workgroups of 256 or 1024; a level whose workgroups
+// cannot all be co-resident runs as variant 6 does);
 // block = the workgroup size (the tile size for variants 1, 3, 4); xcd =
 // XCD-aware tile order.
 struct Launch {
@@ -2232,15 +2415,16 @@ int size_bucket(uint32_t n) {
 
 Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool frozen) {
     const bool adj = p->op_mode == TGVX_OP_ADJ;
-    if (p->fused < 0 || p->fused > 6) throw Error(TGVX_E_ARG, "fused must be 0 to 6");
+    if (p->fused < 0 || p->fused > 7) throw Error(TGVX_E_ARG, "fused must be 0 to 7");
     // the reference's rules have the two-kernel iteration and the two-iteration
-    // kernel (5; 6 runs every level as 5, its small-level kernel being the
-    // exact-adjoint one's); the frozen-border solve has the two kernels only
-    const int var = frozen ? 0 : adj ? p->fused : (p->fused == 5 || p->fused == 6 ? p->fused : 0);
+    // kernel (5; 6 and 7 run every level as 5, their small-level kernels being
+    // the exact-adjoint one's); the frozen-border solve has the two kernels only
+    const int var = frozen ? 0 : adj ? p->fused : (p->fused >= 5 ? p->fused : 0);
     Launch L{var, block_of(p), 1};
     const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4 || L.variant == 5;
     if (tiled && p->block <= 0) L.block = kFuseTile;
     if (L.variant == 6) L = Launch{6, kLevel1, 0};
+    if (L.variant == 7) L = Launch{7, L.block == 256 ? 256 : 1024, 0};
     if (tiled && L.block == 64)
         throw Error(TGVX_E_ARG, "the tiled fused kernel takes block 128 or more");
     if (p->tune && !frozen) {
@@ -2609,6 +2793,96 @@ struct GraphCache {
 };
 std::map<const tgvx_ctx*, GraphCache> g_graphs;
 
+// Polls of a grid-barrier wait before it gives up (each poll sleeps about
+// 64 cycles and reads the counter from memory: about a second in all, far
+// above any legitimate wait of a few microseconds). The environment variable
+// TGVX_GRID_SPIN replaces it (0 makes every wait that has to poll give up at
+// once, which is how the tests drive the failure path).
+constexpr uint32_t kGridSpin = 1u << 20;
+inline uint32_t grid_spin() {
+    const char* e = std::getenv("TGVX_GRID_SPIN");
+    return e && *e ? (uint32_t)std::strtoul(e, nullptr, 10) : kGridSpin;
+}
+
+// Whether a level of n cubes fits one cooperative launch of k_persist with
+// workgroups of `block` threads (all workgroups co-resident).
+template <class S>
+bool persist_fits(tgvx_ctx* c, uint32_t n, int block) {
+    const void* f = block == 1024 ? (const void*)k_persist<S, 1024> : (const void*)k_persist<S, 256>;
+    int per_cu = 0, cus = 0;
+    TGVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, block, 0));
+    TGVX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const uint64_t grid = ((uint64_t)n + block - 1) / block;
+    return n > 0 && per_cu > 0 && grid <= (uint64_t)per_cu * (uint64_t)cus;
+}
+
+// Variant 7: `iters` iterations of a's range in one cooperative launch; false
+// (nothing launched) when its workgroups cannot all be co-resident.
+template <class S>
+bool launch_persist(tgvx_ctx* c, KArgs<S>& a, int iters, int block) {
+    if (!persist_fits<S>(c, a.n, block)) return false;
+    const bool fresh = c->grid_bar.p == nullptr;
+    c->grid_bar.ensure(512);
+    int32_t* const w = c->grid_bar.as<int32_t>();
+    // the counter restarts at 0 for every launch; the error flag is cleared
+    // once, and again by the host after it reports a failure
+    k_fill_i32<<<1, 128, 0, c->stream>>>(w, fresh ? 128u : 1u, 0);
+    GridBar gb{reinterpret_cast<uint32_t*>(w), w + 64, grid_spin()};
+    const uint32_t grid = (a.n + block - 1) / block;
+    void* args[] = {&a, &iters, &gb};
+    const void* f = block == 1024 ? (const void*)k_persist<S, 1024> : (const void*)k_persist<S, 256>;
+    TGVX_HIP(hipLaunchCooperativeKernel(f, dim3(grid), dim3(block), args, 0, c->stream));
+    c->grid_used = true;
+    return true;
+}
+
+// Microseconds per grid barrier of a cooperative launch of `wgs` workgroups
+// of `block` threads: two launches of 10 and 10 + nbar barriers, on events.
+void grid_barrier_bench_run(tgvx_ctx* c, int wgs, int block, int nbar, double* us) {
+    if (block != 256 && block != 1024) throw Error(TGVX_E_ARG, "block must be 256 or 1024");
+    if (nbar < 1 || nbar > 100000) throw Error(TGVX_E_ARG, "barriers must be in 1..100000");
+    const void* f = block == 1024 ? (const void*)k_barrier_bench<1024> : (const void*)k_barrier_bench<256>;
+    int per_cu = 0, cus = 0;
+    TGVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, block, 0));
+    TGVX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    if (wgs < 1 || (int64_t)wgs > (int64_t)per_cu * cus)
+        throw Error(TGVX_E_ARG, "workgroups must be in 1.." + std::to_string((int64_t)per_cu * cus));
+    const bool fresh = c->grid_bar.p == nullptr;
+    c->grid_bar.ensure(512);
+    int32_t* const w = c->grid_bar.as<int32_t>();
+    GridBar gb{reinterpret_cast<uint32_t*>(w), w + 64, grid_spin()};
+    float ms[2] = {0.f, 0.f};
+    const int counts[2] = {10, 10 + nbar};
+    for (int r = 0; r < 2; ++r) {
+        k_fill_i32<<<1, 128, 0, c->stream>>>(w, fresh && r == 0 ? 128u : 1u, 0);
+        int k = counts[r];
+        void* args[] = {&gb, &k};
+        TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+        TGVX_HIP(hipLaunchCooperativeKernel(f, dim3(wgs), dim3(block), args, 0, c->stream));
+        TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+        c->grid_used = true;
+        TGVX_HIP(hipEventSynchronize(c->ev[1]));
+        TGVX_HIP(hipEventElapsedTime(&ms[r], c->ev[0], c->ev[1]));
+    }
+    *us = 1e3 * (double)(ms[1] - ms[0]) / nbar;
+}
+
+// After the stream has drained past persistent launches: fail if one of
+// their grid barriers gave up (the state is then not the iteration's).
+void check_grid(tgvx_ctx* c) {
+    if (!c->grid_used) return;
+    c->grid_used = false;
+    int32_t e = 0;
+    int32_t* const w = c->grid_bar.as<int32_t>();
+    TGVX_HIP(hipMemcpyAsync(&e, w + 64, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    TGVX_HIP(hipStreamSynchronize(c->stream));
+    if (e != 0) {
+        k_fill_i32<<<1, 128, 0, c->stream>>>(w, 128u, 0);
+        TGVX_HIP(hipStreamSynchronize(c->stream));
+        throw Error(TGVX_E_STATE, "persistent level kernel: a grid barrier gave up waiting");
+    }
+}
+
 // Variant 6 on a level larger than kLevel1: the two-iteration kernel, tiles of
 // 1024 once the level has one per CU (2^18 cubes; measured 2.6-6 % faster than
 // 512 at 0.38-4 M cubes, 12 % slower at 97 k), else 512.
@@ -2622,6 +2896,13 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         if (!frozen && p->op_mode == TGVX_OP_ADJ && a.n <= (uint32_t)kLevel1 && a.off == a.lvl_off &&
             a.n == a.lvl_n) {
             k_level1<S><<<1, kLevel1, 0, c->stream>>>(a, iters);
+            TGVX_HIP(hipGetLastError());
+            return;
+        }
+        L = level1_fallback(a.n);
+    }
+    if (L.variant == 7) {
+        if (!frozen && p->op_mode == TGVX_OP_ADJ && !a.sflag && launch_persist<S>(c, a, iters, L.block)) {
             TGVX_HIP(hipGetLastError());
             return;
         }
@@ -2871,6 +3152,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             run_iters<S>(c, a, p, fixed, false);
             TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
             TGVX_HIP(hipEventSynchronize(c->ev[1]));
+            check_grid(c);
             float ms = 0.f;
             TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
             it_ms += ms;
@@ -2926,6 +3208,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                 run_iters<S>(c, a, p, k, false);
                 TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
                 TGVX_HIP(hipEventSynchronize(c->ev[1]));
+                check_grid(c);
                 done += k;
                 g0 = now_ms();
                 eval_gap_blocks<S>(c, a, p, li, rg);
@@ -2954,6 +3237,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                 run_iters<S>(c, a, p, k, false);
                 TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
                 TGVX_HIP(hipEventSynchronize(c->ev[1]));
+                check_grid(c);
                 done += k;
                 g0 = now_ms();
                 eval_gap<S>(c, a, p, &Pv, &Dv);
@@ -2974,6 +3258,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         r.leaf_iters += work >= 0.0 ? work : (double)L.n * done;
     }
     TGVX_HIP(hipStreamSynchronize(c->stream));
+    check_grid(c);
     if (std::find(deferred.begin(), deferred.end(), 1) != deferred.end()) {
         std::vector<double> pd(2 * deferred.size());
         TGVX_HIP(hipMemcpy(pd.data(), P.d_lvlpd.p, sizeof(double) * pd.size(), hipMemcpyDeviceToHost));
@@ -3023,6 +3308,7 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     run_launch<S>(c, a, p, iters, false, cfg);
     TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
     TGVX_HIP(hipEventSynchronize(c->ev[1]));
+    check_grid(c);
     float ms = 0.f;
     TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     *total_ms = ms;
@@ -3063,12 +3349,12 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
     const bool ref = p->op_mode != TGVX_OP_ADJ;
     // the reference's rules: the two-iteration kernel when configured and the
     // level's tile sets fit (they have no one-iteration fused kernel)
-    const bool ref2 = ref && (cfg.variant == 5 || cfg.variant == 6) &&
-                      halo2_tab<S, true>(c, a, cfg.variant == 6 ? level1_fallback(a.n).block : cfg.block).ok;
+    const bool ref2 = ref && cfg.variant >= 5 &&
+                      halo2_tab<S, true>(c, a, cfg.variant >= 6 ? level1_fallback(a.n).block : cfg.block).ok;
     if (!ref || ref2) {
         Launch fz = cfg;
         if (fz.variant == 0) fz = Launch{1, kFuseTile, cfg.xcd};
-        if (fz.variant == 6) fz = level1_fallback(a.n);
+        if (fz.variant >= 6) fz = level1_fallback(a.n);
         // the two-iteration kernel: per launch / 2 (one iteration per
         // launch where the level's tile sets do not fit)
         int per = 1;
@@ -3091,6 +3377,7 @@ void bench_t(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* to
             }
             TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
             TGVX_HIP(hipEventSynchronize(c->ev[1]));
+            check_grid(c);
             float m1 = 0.f;
             TGVX_HIP(hipEventElapsedTime(&m1, c->ev[0], c->ev[1]));
             sf += m1;
@@ -3117,7 +3404,10 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
         }
         for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{5, b, x});
     }
-    if (p->op_mode == TGVX_OP_ADJ) cand.push_back(Launch{6, kLevel1, 0});
+    if (p->op_mode == TGVX_OP_ADJ) {
+        cand.push_back(Launch{6, kLevel1, 0});
+        for (int b : {1024, 256}) cand.push_back(Launch{7, b, 0});
+    }
     // Kernel launches per iteration of a candidate over `iters` iterations of
     // a level of n cubes. Inside a sweep every launch costs about 1 us more
     // than when one level's graph is timed alone (measured at depths 4-6 of
@@ -3131,6 +3421,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
             case 0: return 2.0;
             case 5: return (2.0 + (k - 1.0) / 2.0) / k;
             case 6: return n <= (uint32_t)kLevel1 ? 1.0 / k : (2.0 + (k - 1.0) / 2.0) / k;
+            case 7: return 2.0 / k;  // the counter's clear and the launch
             default: return (k + 1.0) / k;
         }
     };
@@ -3146,6 +3437,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
         Launch win = base;
         for (const Launch& cf : cand) {
             if (cf.variant == 6 && L.n > (uint32_t)kLevel1) continue;  // would run as 5
+            if (cf.variant == 7 && !persist_fits<S>(c, L.n, cf.block)) continue;
             run_launch<S>(c, a, &q, iters, false, cf);  // warm-up (and graph build)
             double us = 1e300;
             for (int rep = 0; rep < 2; ++rep) {  // the better of two timings
@@ -3153,6 +3445,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
                 run_launch<S>(c, a, &q, iters, false, cf);
                 TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
                 TGVX_HIP(hipEventSynchronize(c->ev[1]));
+                check_grid(c);
                 float ms = 0.f;
                 TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
                 us = std::min(us, 1e3 * ms / iters);
@@ -4011,6 +4304,11 @@ void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* re
         case TGVX_BF16_DUAL: autotune_t<mixs>(c, p, iters, rep); break;
         default: autotune_t<bf16s>(c, p, iters, rep); break;
     }
+}
+
+void grid_barrier_bench(tgvx_ctx* c, int wgs, int block, int nbar, double* us) {
+    grid_barrier_bench_run(c, wgs, block, nbar, us);
+    check_grid(c);
 }
 
 void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,

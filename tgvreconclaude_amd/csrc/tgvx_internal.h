@@ -117,9 +117,9 @@ struct Problem {
     DevBuf d_state[17];              // u vx vy vz px py pz q0..q5 ub vbx vby vbz
     DevBuf d_save[4];                // scratch copy of u, v of one level
     DevBuf d_alt[13];                // second (u, v, p, q) set of the fused iteration
-    DevBuf d_snb[6], d_sflag, d_spar;
+    DevBuf d_snb[6], d_sflag, d_spar;  // streaming mode: per-leaf faces, flags, bisection parent
     DevBuf d_rfz;                    // reference rules: out-of-level neighbour records of a level
-    DevBuf d_fzaos;                  // the previous level's u_bar, v_bar as 4-value records  // streaming mode: per-leaf faces, flags, bisection parent
+    DevBuf d_fzaos;                  // the previous level's u_bar, v_bar as 4-value records
     DevBuf d_init;                   // float x 4 per position (warm start field), optional
     DevBuf d_orig;                   // uint32 position -> input index
     DevBuf d_part;                   // gap partial sums
@@ -160,6 +160,10 @@ struct tgvx_ctx {
     uint64_t oct_n = 0;
     // launch configurations chosen by tgvx_autotune
     std::map<std::tuple<int, int, int>, tgvx::TuneEntry> tuned;
+    // the persistent level kernel's grid barrier: [0] arrival count, [64]
+    // error flag; grid_used: a launch since the flag was last read
+    tgvx::DevBuf grid_bar;
+    bool grid_used = false;
 };
 
 namespace tgvx {
@@ -180,6 +184,7 @@ void problem_gap(tgvx_ctx* c, int depth, const tgvx_params* p, double* P, double
 void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double* total_ms, double* dual_ms,
                  double* primal_ms, double* fused_ms);
 void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* rep);
+void grid_barrier_bench(tgvx_ctx* c, int wgs, int block, int nbar, double* us);
 void tile_sets(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_sets, int64_t* lds_bytes);
 // octree.hip
 void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, const float origin[3], float r_root,
