@@ -72,6 +72,8 @@ def main():
                          "in one cooperative launch with grid barriers (levels too large for it as 6)")
     ap.add_argument("--no-fused", action="store_true", help="same as --fused 0")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start time-to-gap runs")
+    ap.add_argument("--no-oc", action="store_true",
+                    help="skip timing the drop-in tgv_minimize_oc (frozen-border treetop leaves)")
     ap.add_argument("--no-octree", action="store_true",
                     help="skip timing the octree build (depth maps -> balanced cubes) on the device and the host")
     ap.add_argument("--step-ratio", type=float, default=1024.0,
@@ -334,6 +336,25 @@ def main():
                      "autotune": [(l["depth"], l["variant"], l["block"], l["xcd"]) for l in rtune] if rtune else None}
         log(f"[bench] reference rules: {ref_sweep['value']:.4g} leaf-iter/s; depth {finest} "
             f"{r_it * 1e3:.1f} us per iteration (two kernels {(rd_ms + rp_ms) * 1e3:.1f} us)")
+
+    # the drop-in tgv_minimize_oc (tgv.cpp:482-507): treetop leaves of at
+    # most 2^20 cubes solved one after another per level, out-of-leaf
+    # u_bar, v_bar frozen at the level start; one call = problem load +
+    # sweep (the iteration time is the report's, events around each level's
+    # leaf loop). Replaces the resident problem.
+    if not args.no_ref and not args.no_oc:
+        leaves = scene.treetop(cubes, 1 << 20)
+        po = T.TGVParams(alpha0=2.0, alpha1=1.0, iters=args.iters, precision=prec, op_mode=T.OP_REF, fused=6)
+        ctx.solve_oc(cubes, hist, leaves, po)
+        t1 = time.perf_counter()
+        _, orep = ctx.solve_oc(cubes, hist, leaves, po)
+        dt_oc = time.perf_counter() - t1
+        ref_sweep["oc"] = {"leaves": int(len(leaves)), "wall_s": round(dt_oc, 3), "iterate_ms": round(orep.iterate_ms, 2),
+                           "leaf_iter_per_s": orep.leaf_iters / (orep.iterate_ms * 1e-3),
+                           "definition": "tgv_minimize_oc, treetop leaves of at most 2^20 cubes, one call (load "
+                                         "included in wall_s); leaf_iter_per_s over the iterations only"}
+        log(f"[bench] reference rules, frozen-border leaves ({len(leaves)}): iterations {orep.iterate_ms:.1f} ms "
+            f"({ref_sweep['oc']['leaf_iter_per_s']:.4g} leaf-iter/s), call {dt_oc:.3f} s")
 
     cpu = None
     if args.cpu_seconds > 0:

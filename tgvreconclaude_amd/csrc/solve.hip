@@ -1855,22 +1855,44 @@ __global__ void k_fz_aos(KArgs<S> a, S* __restrict__ rec) {
 // The reference rules' out-of-level neighbour values of a level (rfz_bar,
 // rfz_dual of KArgs), gathered once at level start: the two-iteration kernel
 // then reads one record per such neighbour, next to its own cube's, instead
-// of four scattered fields per iteration.
-template <class S>
-__global__ void k_ref_frozen(KArgs<S> a, S* __restrict__ bar, S* __restrict__ dual) {
+// of four scattered fields per iteration. count (when not null): the level's
+// out-of-level references are counted into *count (one atomic per
+// workgroup) and nothing is gathered. SNAP (frozen-border leaves, where the
+// range [lvl_off, lvl_off + lvl_n) is one treetop leaf's part of a level):
+// u_bar, v_bar come from the level-start snapshot fz (run_level_frozen,
+// tgv.cpp:274-309), p, q from the main set (live, constant while the leaf
+// iterates).
+template <class S, bool SNAP = false>
+__global__ void __launch_bounds__(256) k_ref_frozen(KArgs<S> a, S* __restrict__ bar, S* __restrict__ dual,
+                                                    uint32_t* __restrict__ count) {
     using IO = Io<S>;
+    __shared__ uint32_t s_cnt;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.lvl_n) return;
-    const long g = (long)a.lvl_off + t;
+    const bool in = t < a.lvl_n;
+    const long g = (long)a.lvl_off + (in ? t : 0u);
     auto out = [&](int j) { return j >= 0 && !((uint32_t)j - a.lvl_off < a.lvl_n); };
     constexpr int kF[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
+    if (count) {
+        if (threadIdx.x == 0) s_cnt = 0u;
+        __syncthreads();
+        uint32_t k = 0;
+        if (in) {
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) k += (out(a.nbp[ax][g]) ? 1u : 0u) + (out(a.nbm[ax][g]) ? 1u : 0u);
+        }
+        if (k) atomicAdd(&s_cnt, k);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_cnt) atomicAdd(count, s_cnt);
+        return;
+    }
+    if (!in) return;
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         const long r = (long)ax * a.lvl_n + t;
         const int jp = a.nbp[ax][g], jm = a.nbm[ax][g];
         if (out(jp)) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) IO::st(bar, 4 * r + c, IO::ld(a.s[F_UB + c], jp));
+            for (int c = 0; c < 4; ++c) IO::st(bar, 4 * r + c, IO::ld(SNAP ? a.fz[c] : a.s[F_UB + c], jp));
         }
         if (out(jm)) {
 #pragma unroll
@@ -2347,6 +2369,21 @@ void set_steps(KArgs<S>& a, const tgvx_params* p, int depth) {
 
 // Before level li iterates in the exact-adjoint mode: the previous level's
 // bars (the only frozen values it reads) as 4-value records.
+// The records of the reference rules' out-of-level neighbours (u_bar, v_bar,
+// then p, q; 4 values per level cube and axis each), sized for the largest
+// level at the first use, so that the buffer never moves under a captured
+// graph of an earlier level.
+template <class S>
+void rfz_bufs(tgvx_ctx* c, S** bar, S** dual) {
+    Problem& P = c->prob;
+    uint32_t max_n = 1;
+    for (const Level& l : P.levels) max_n = std::max(max_n, l.n);
+    P.d_rfz.ensure(2 * 4 * 3 * (size_t)max_n * P.elem);
+    const size_t rec = 4 * 3 * (size_t)max_n;
+    *bar = P.d_rfz.as<S>();
+    *dual = reinterpret_cast<S*>(static_cast<char*>(P.d_rfz.p) + rec * P.elem);
+}
+
 template <class S>
 void prepare_level(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, size_t li) {
     Problem& P = c->prob;
@@ -2356,18 +2393,31 @@ void prepare_level(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, size_t li) {
     if (p->op_mode == TGVX_OP_REF && !(p->fused == 0 && !p->tune)) {
         // the reference rules' out-of-level neighbours (two-iteration kernel)
         const Level& L = P.levels[li];
-        // sized for the largest level at the first use, so that the buffer
-        // never moves under a captured graph of an earlier level
-        uint32_t max_n = 1;
-        for (const Level& l : P.levels) max_n = std::max(max_n, l.n);
-        P.d_rfz.ensure(2 * 4 * 3 * (size_t)max_n * P.elem);
-        const size_t rec = 4 * 3 * (size_t)max_n;
-        S* bar = P.d_rfz.as<S>();
-        S* dual = reinterpret_cast<S*>(static_cast<char*>(P.d_rfz.p) + rec * P.elem);
+        S *bar, *dual;
+        rfz_bufs<S>(c, &bar, &dual);
         KArgs<S> g = a;
         g.lvl_off = L.off;
         g.lvl_n = L.n;
-        if (L.n > 0) k_ref_frozen<S><<<(L.n + 255) / 256, 256, 0, c->stream>>>(g, bar, dual);
+        if (L.n == 0) return;
+        // Records pay where many references leave the level: measured on the
+        // depth-10 torus + box scene, levels with 40-60 % of their six
+        // references outside (depths 4-9) run the two-iteration kernel 18-47 %
+        // faster with them, the finest level (18 %) 8 % slower, its sparse
+        // record reads fetching whole lines. Decided once per level and
+        // problem from a count (one synchronisation).
+        if (P.rfz_use.size() != P.levels.size()) P.rfz_use.assign(P.levels.size(), -1);
+        if (P.rfz_use[li] < 0) {
+            P.d_rfz_cnt.ensure(sizeof(uint32_t));
+            uint32_t* cnt = P.d_rfz_cnt.as<uint32_t>();
+            k_fill_i32<<<1, 64, 0, c->stream>>>(reinterpret_cast<int32_t*>(cnt), 1u, 0);
+            k_ref_frozen<S><<<(L.n + 255) / 256, 256, 0, c->stream>>>(g, bar, dual, cnt);
+            uint32_t h = 0;
+            TGVX_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+            TGVX_HIP(hipStreamSynchronize(c->stream));
+            P.rfz_use[li] = (uint64_t)h * 10u >= (uint64_t)L.n * 6u * 3u ? 1 : 0;  // at least 30 %
+        }
+        if (!P.rfz_use[li]) return;  // out-of-level reads from the main set
+        k_ref_frozen<S><<<(L.n + 255) / 256, 256, 0, c->stream>>>(g, bar, dual, nullptr);
         a.rfz_bar = bar;
         a.rfz_dual = dual;
         return;
@@ -2418,8 +2468,9 @@ Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool froz
     if (p->fused < 0 || p->fused > 7) throw Error(TGVX_E_ARG, "fused must be 0 to 7");
     // the reference's rules have the two-kernel iteration and the two-iteration
     // kernel (5; 6 and 7 run every level as 5, their small-level kernels being
-    // the exact-adjoint one's); the frozen-border solve has the two kernels only
-    const int var = frozen ? 0 : adj ? p->fused : (p->fused >= 5 ? p->fused : 0);
+    // the exact-adjoint one's), also for frozen-border leaves; the exact-adjoint
+    // frozen-border solve has the two kernels only
+    const int var = !adj ? (p->fused >= 5 ? p->fused : 0) : frozen ? 0 : p->fused;
     Launch L{var, block_of(p), 1};
     const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4 || L.variant == 5;
     if (tiled && p->block <= 0) L.block = kFuseTile;
@@ -2728,12 +2779,18 @@ void launch_fused_pass(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set1
 // fused (primal n, dual n+1) passes alternating between the main and the
 // alternate set, primal of the last back into the main set.
 template <class S, bool X>
-void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode) {
+void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode, bool frozen) {
     const int wg = L.block > 256 ? 256 : L.block;  // the dual / primal kernels take at most 256
     const unsigned grid = (a.n + wg - 1) / wg;
     Set13<S> cur = main_set(a), oth = alt_set<S>(c);
     const Set13<S> mainset = cur;
-    k_dual<S, 0, X><<<grid, wg, 0, c->stream>>>(a);
+    // frozen-border leaves (reference rules only): out-of-leaf u_bar, v_bar
+    // from the snapshot in the dual kernel, from the records in k_fused2
+    auto dual = [&] {
+        if (frozen) k_dual<S, 1, X><<<grid, wg, 0, c->stream>>>(a);
+        else k_dual<S, 0, X><<<grid, wg, 0, c->stream>>>(a);
+    };
+    dual();
     int i = 1;
     if (op_mode == TGVX_OP_REF) {
         // the reference's rules: two iterations per launch while two remain,
@@ -2745,7 +2802,7 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op
         for (; i < iters; ++i) {
             k_primal_ref_set<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, cur.f[0] != mainset.f[0]);
             cur = mainset;
-            k_dual<S, 0, X><<<grid, wg, 0, c->stream>>>(a);
+            dual();
         }
         k_primal_ref_set<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, cur.f[0] != mainset.f[0]);
         return;
@@ -2764,23 +2821,28 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op
     k_primal_adj<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, copy);
 }
 template <class S>
-void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode) {
-    if (L.xcd) launch_fused_x<S, true>(c, a, iters, L, op_mode);
-    else launch_fused_x<S, false>(c, a, iters, L, op_mode);
+void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode, bool frozen = false) {
+    if (L.xcd) launch_fused_x<S, true>(c, a, iters, L, op_mode, frozen);
+    else launch_fused_x<S, false>(c, a, iters, L, op_mode, frozen);
 }
 
 // Cached graph of `iters` iterations of one level.
 // Everything a captured graph bakes in: the launch shape and the kernel
 // arguments' scalars (steps, weights), so a solve with other parameters on
 // the same problem never replays a stale graph.
+// A frozen-border leaf's graph also bakes in the leaf's input index range
+// (its in-leaf test); the snapshot it reads lives as long as the problem.
 struct GraphKey {
     uint32_t off, n, lvl_off, lvl_n;
     int iters, op, block, prec, variant, xcd, mask;
     double sigma, tau, a0, a1, tl;
+    int frozen;
+    uint32_t leaf_first, leaf_last;
     bool operator<(const GraphKey& o) const {
-        return std::tie(off, n, lvl_off, lvl_n, iters, op, block, prec, variant, xcd, mask, sigma, tau, a0, a1, tl) <
+        return std::tie(off, n, lvl_off, lvl_n, iters, op, block, prec, variant, xcd, mask, sigma, tau, a0, a1, tl,
+                        frozen, leaf_first, leaf_last) <
                std::tie(o.off, o.n, o.lvl_off, o.lvl_n, o.iters, o.op, o.block, o.prec, o.variant, o.xcd, o.mask,
-                        o.sigma, o.tau, o.a0, o.a1, o.tl);
+                        o.sigma, o.tau, o.a0, o.a1, o.tl, o.frozen, o.leaf_first, o.leaf_last);
     }
 };
 struct GraphCache {
@@ -2908,6 +2970,9 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         }
         L = level1_fallback(a.n);
     }
+    // a frozen-border leaf's fused iterations read its out-of-leaf neighbours
+    // from the records; without them, the two kernels
+    if (frozen && !a.rfz_bar) L.variant = 0;
     const bool fused = L.variant != 0 && iters >= 2;
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (fused && L.variant == 4) halo_tab<S>(c, a, L.block);
@@ -2915,7 +2980,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         if (p->op_mode == TGVX_OP_REF) halo2_tab<S, true>(c, a, L.block);
         else halo2_tab<S>(c, a, L.block);
     }
-    if (p->use_graph && !frozen && iters > 1) {
+    if (p->use_graph && iters > 1) {
         GraphKey key{a.off,
                      a.n,
                      a.lvl_off,
@@ -2931,15 +2996,18 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
                      (double)a.tau,
                      (double)a.a0,
                      (double)a.a1,
-                     (double)a.tl};
+                     (double)a.tl,
+                     frozen ? 1 : 0,
+                     frozen ? a.leaf_first : 0u,
+                     frozen ? a.leaf_last : 0u};
         GraphCache& gc = g_graphs[c];
         auto it = gc.m.find(key);
         if (it == gc.m.end()) {
             hipGraph_t graph;
             TGVX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            if (fused) launch_fused<S>(c, a, iters, L, p->op_mode);
+            if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen);
             else
-                for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, false);
+                for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, frozen);
             TGVX_HIP(hipStreamEndCapture(c->stream, &graph));
             hipGraphExec_t exec;
             TGVX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -2949,7 +3017,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         TGVX_HIP(hipGraphLaunch(it->second, c->stream));
         return;
     }
-    if (fused) launch_fused<S>(c, a, iters, L, p->op_mode);
+    if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen);
     else
         for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, frozen);
     TGVX_HIP(hipGetLastError());
@@ -3124,6 +3192,14 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             }
             a.orig = P.d_orig.as<uint32_t>();
             const uint32_t* pb = P.perm.data() + L.off;
+            // reference rules with a fused variant: each leaf's part of the
+            // level is the two-iteration kernel's level, its out-of-leaf
+            // neighbours gathered into records at the leaf's start (u_bar,
+            // v_bar from the snapshot, p, q live)
+            const bool ref_fused = p->op_mode == TGVX_OP_REF && launch_for(c, p, L.n, true).variant != 0;
+            S *rbar = nullptr, *rdual = nullptr;
+            if (ref_fused) rfz_bufs<S>(c, &rbar, &rdual);
+            TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
             for (uint64_t l = 0; l < nleaves; ++l) {
                 const uint32_t* lo = std::lower_bound(pb, pb + L.n, leaves[l].first_idx);
                 const uint32_t* hi = std::upper_bound(pb, pb + L.n, leaves[l].last_idx);
@@ -3132,8 +3208,24 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                 a.n = (uint32_t)(hi - lo);
                 a.leaf_first = leaves[l].first_idx;
                 a.leaf_last = leaves[l].last_idx;
+                if (ref_fused) {
+                    a.lvl_off = a.off;
+                    a.lvl_n = a.n;
+                    k_ref_frozen<S, true><<<(a.n + 255) / 256, 256, 0, c->stream>>>(a, rbar, rdual, nullptr);
+                    a.rfz_bar = rbar;
+                    a.rfz_dual = rdual;
+                }
                 run_iters<S>(c, a, p, p->iters, true);
             }
+            a.lvl_off = L.off;
+            a.lvl_n = L.n;
+            a.rfz_bar = a.rfz_dual = nullptr;
+            TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+            TGVX_HIP(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            it_ms += ms;
+            if (li < (size_t)TGVX_MAX_LEVELS) r.level_ms[li] = ms;
             a.off = L.off;
             a.n = L.n;
             done = p->iters;
@@ -3629,6 +3721,7 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     P.loaded = false;
     P.halo.clear();
     P.halo2.clear();
+    P.rfz_use.clear();
     P.n = n;
     P.nb = nb;
     P.precision = precision;

@@ -119,6 +119,8 @@ struct Problem {
     DevBuf d_alt[13];                // second (u, v, p, q) set of the fused iteration
     DevBuf d_snb[6], d_sflag, d_spar;  // streaming mode: per-leaf faces, flags, bisection parent
     DevBuf d_rfz;                    // reference rules: out-of-level neighbour records of a level
+    DevBuf d_rfz_cnt;                // uint32: out-of-level references of a level (count)
+    std::vector<int8_t> rfz_use;     // per level: records used (1), not (0), not yet decided (-1)
     DevBuf d_fzaos;                  // the previous level's u_bar, v_bar as 4-value records
     DevBuf d_init;                   // float x 4 per position (warm start field), optional
     DevBuf d_orig;                   // uint32 position -> input index
