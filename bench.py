@@ -185,8 +185,10 @@ def main():
     log(f"[bench] problem resident in {t_load:.2f} s; levels {levels}")
 
     variant = 0 if args.no_fused else args.fused
+    # the step is the reference's tgv_minimize: each level's iterations, no
+    # energy evaluation (the per-level P, D report is timed separately below)
     prm = T.TGVParams(alpha0=2.0, alpha1=1.0, precision=prec, op_mode=T.OP_ADJ, iters=args.iters,
-                      use_graph=0 if args.no_graph else 1, block=args.block, fused=variant)
+                      use_graph=0 if args.no_graph else 1, block=args.block, fused=variant, no_energy=1)
     tuning = None
     if args.tune:
         tuning = ctx.autotune(prm, iters=args.iters)  # the step's own chunk of iterations per level
@@ -206,11 +208,32 @@ def main():
         leaf_iters += rep.leaf_iters
     hip.hipDeviceSynchronize()
     dt = time.perf_counter() - ts
-    # where a step's time goes: each level's iterations (events around its
-    # launches inside the last timed step)
-    level_ms = [(l["depth"], round(l["ms"], 4)) for l in rep.levels()]
     value = leaf_iters / dt
     log(f"[bench] {args.steps} steps in {dt:.3f} s: {value:.4g} leaf-iter/s")
+    # the same steps with each level's FP64 energy and dual value evaluated
+    # after its iterations (this library's report; the reference has none)
+    prm.no_energy = 0
+    ctx.run(prm)
+    hip.hipDeviceSynchronize()
+    t1 = time.perf_counter()
+    li_e = 0.0
+    for i in range(args.steps):
+        li_e += ctx.run(prm).leaf_iters
+    hip.hipDeviceSynchronize()
+    dt_e = time.perf_counter() - t1
+    with_energy = {"value": li_e / dt_e, "unit": "leaf-iter/s", "ms_per_step": round(dt_e / args.steps * 1e3, 4),
+                   "definition": "the same steps with each level's FP64 energy P and dual value D evaluated "
+                                 "after its iterations"}
+    log(f"[bench] with the per-level energy report: {li_e / dt_e:.4g} leaf-iter/s")
+    prm.no_energy = 1
+    # where a step's time goes: each level's iterations (events around its
+    # launches, per-level graphs so that the events sit between them)
+    ug = prm.use_graph
+    if ug:
+        prm.use_graph = 2
+    rep = ctx.run(prm)
+    prm.use_graph = ug
+    level_ms = [(l["depth"], round(l["ms"], 4)) for l in rep.levels()]
     # the same leaf-iterations with the one-time setup counted in: histogram
     # voting on the GPU (with the depth maps' upload) and the problem load
     setup = t_hist + t_load
@@ -441,6 +464,7 @@ def main():
         "time_to_gap": gap,
         "warm_start_speedup": warm,
         "reference_rules_sweep": ref_sweep,
+        "with_energy_report": with_energy,
         "level_ms_per_step": level_ms,
         "cpu_baseline": cpu,
         "autotune": tuning,

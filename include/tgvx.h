@@ -126,6 +126,10 @@ typedef struct {
        (tau). No reference counterpart. */
     float sigma_depth[TGVX_MAX_LEVELS];
     float tau_depth[TGVX_MAX_LEVELS];
+    int32_t no_energy;     /* fixed-iteration solves (gap_tol <= 0): 1 skips the evaluation of each
+                              level's P and D after its iterations (level_energy / level_dual are
+                              then 0); 0 (default) evaluates them. The reference computes neither.
+                              No reference counterpart. */
 } tgvx_params;
 
 typedef struct {

@@ -1,0 +1,84 @@
+"""A level's head launch and the whole-sweep graph.
+
+A cold level run with the two-iteration kernel opens with its head launch
+(dual 1, primal 1, dual 2 from the initial state), and a fixed-iteration
+solve run a second time with the same parameters replays as one graph. Both
+only move data differently, so every state must equal the two-kernel
+iteration bit for bit, for odd and even iteration counts, in every storage
+precision and both operator modes, and the per-level energies of the graph
+replay must equal those of the level-by-level run.
+"""
+import numpy as np
+import pytest
+
+from tgvreconclaude_amd import BF16, BF16_DUAL, FP32, FP64, OP_ADJ, OP_REF, TGVParams
+
+pytestmark = pytest.mark.gpu
+
+
+def _raw(st):
+    return np.ascontiguousarray(st).view(np.uint8)
+
+
+@pytest.mark.parametrize("prec", [FP32, FP64, BF16, BF16_DUAL])
+@pytest.mark.parametrize("mode", [OP_ADJ, OP_REF])
+@pytest.mark.parametrize("iters", [2, 3, 4, 7, 50])
+def test_head_launch_identical(ctx, sphere, prec, mode, iters):
+    sc, hist = sphere
+    base = dict(alpha0=2.0, alpha1=1.0, iters=iters, precision=prec, op_mode=mode)
+    ref, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=0, block=256, **base))
+    for block in (128, 512):
+        got, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=5, block=block, **base))
+        assert np.array_equal(_raw(got), _raw(ref)), (prec, mode, iters, block)
+
+
+@pytest.mark.parametrize("prec,mode", [(FP32, OP_ADJ), (FP64, OP_ADJ), (FP32, OP_REF), (BF16, OP_ADJ)])
+@pytest.mark.parametrize("no_energy", [0, 1])
+def test_sweep_graph_identical(ctx, sphere, prec, mode, no_energy):
+    sc, hist = sphere
+    ctx.load(sc.cubes, hist, precision=prec, op_mode=mode)
+    base = dict(alpha0=2.0, alpha1=1.0, iters=30, precision=prec, op_mode=mode, fused=6, no_energy=no_energy)
+    # level by level, per-level graphs (use_graph = 2) and plain launches
+    r_lv = ctx.run(TGVParams(use_graph=2, **base))
+    st_lv = ctx.state()
+    ctx.run(TGVParams(use_graph=0, **base))
+    assert np.array_equal(_raw(ctx.state()), _raw(st_lv))
+    # first run: level by level; second: captured; third: replayed
+    p = TGVParams(use_graph=1, **base)
+    for k in range(3):
+        r = ctx.run(p)
+        assert np.array_equal(_raw(ctx.state()), _raw(st_lv)), k
+        assert r.n_levels == r_lv.n_levels
+        assert r.leaf_iters == r_lv.leaf_iters
+        for li in range(r.n_levels):
+            assert r.level_iters[li] == r_lv.level_iters[li]
+            assert r.level_energy[li] == r_lv.level_energy[li]
+            assert r.level_dual[li] == r_lv.level_dual[li]
+            if no_energy:
+                assert r.level_energy[li] == 0.0 and r.level_dual[li] == 0.0
+        if not no_energy:
+            assert any(r.level_energy[li] != 0.0 for li in range(r.n_levels))
+
+
+def test_sweep_graph_follows_parameters(ctx, sphere):
+    """A cached sweep is replayed only for the parameters it was captured with."""
+    sc, hist = sphere
+    ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+    want = {}
+    for sg in (0.5, 0.25):
+        base = dict(alpha0=2.0, alpha1=1.0, iters=20, precision=FP32, op_mode=OP_ADJ, fused=6, sigma=sg, tau=sg)
+        ctx.run(TGVParams(use_graph=0, **base))
+        want[sg] = ctx.state()
+    for sg in (0.5, 0.5, 0.5, 0.25, 0.25, 0.25, 0.5, 0.5):
+        base = dict(alpha0=2.0, alpha1=1.0, iters=20, precision=FP32, op_mode=OP_ADJ, fused=6, sigma=sg, tau=sg)
+        ctx.run(TGVParams(use_graph=1, **base))
+        assert np.array_equal(_raw(ctx.state()), _raw(want[sg])), sg
+    # per-level iteration counts are part of what a sweep bakes in
+    n_lv = len(ctx.levels())
+    base = dict(alpha0=2.0, alpha1=1.0, precision=FP32, op_mode=OP_ADJ, fused=6, use_graph=1)
+    for counts in ([4] * n_lv, [6] * n_lv, [4] * n_lv):
+        for _ in range(3):
+            ctx.run_levels(TGVParams(**base), counts)
+            got = ctx.state()
+        ctx.run_levels(TGVParams(**dict(base, use_graph=0)), counts)
+        assert np.array_equal(_raw(got), _raw(ctx.state())), counts

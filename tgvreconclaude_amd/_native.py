@@ -34,7 +34,7 @@ class Params(C.Structure):
                 ("dual_eps", C.c_float), ("warm_start", C.c_int32), ("warm_dual", C.c_int32),
                 ("fused", C.c_int32), ("tune", C.c_int32),
                 ("sigma", C.c_float), ("tau", C.c_float),
-                ("sigma_depth", C.c_float * 22), ("tau_depth", C.c_float * 22)]
+                ("sigma_depth", C.c_float * 22), ("tau_depth", C.c_float * 22), ("no_energy", C.c_int32)]
 
 
 class Report(C.Structure):

@@ -1050,7 +1050,12 @@ extern __shared__ uint4 k_fused2_lds[];
 // the primal step reads p, q of the -a neighbours of the reference table and
 // always its own; a -a neighbour of another depth gives its p, q from the
 // main set, which the level never writes.
-template <class S, int TILE, bool XCD = true, bool REF = false>
+// HEAD: the first launch of a level that starts from its initial state
+// (u_bar = u, v_bar = v, p = q = 0, as k_reset and k_parent_init leave it):
+// dual 1 over DB, primal 1 over PC, dual 2 over T, so src holds (u^0, v^0)
+// and dst receives (u^1, v^1, p^2, q^2); src's p, q are not read, the
+// launch's first dual step starts from zero.
+template <class S, int TILE, bool XCD = true, bool REF = false, bool HEAD = false>
 __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13<S> dst, Halo2Tab h,
                                                  const int32_t* __restrict__ tiles, int sa, int sx) {
     using IO = Io<S>;
@@ -1236,6 +1241,13 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
 #pragma unroll
         for (int c = 0; c < 4; ++c) o[c] = IO::ld(src.f[F_U + c], g);
     };
+    // HEAD: slot s starts from u_bar = (u, v) = o and p = q = 0
+    auto seed = [&](int s, const T o[4]) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ub[c * sa + s] = o[c];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) pq[k * sa + s] = T(0);
+    };
     // ---- stage p, q of PA. Slots tid (0) and tid + TILE (1) keep their
     // table entries, info words, u, v and (slot 0) frozen neighbours in
     // registers. With tiles of 1024 a CU holds one workgroup whatever the
@@ -1260,7 +1272,8 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         }
         w0 = a.info[g0];
         load_uv(g0, od0);
-        stage_w(tid, g0, tid & ~63);
+        if constexpr (HEAD) seed(tid, od0);
+        else stage_w(tid, g0, tid & ~63);
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
             if (b0[ax] <= -2) frozen(b0[ax], fz0[ax], ax, g0);
@@ -1274,43 +1287,53 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         }
         w1 = a.info[g1];
         load_uv(g1, od1);
-        stage_w(s1, g1, TILE + (tid & ~63));
+        if constexpr (HEAD) seed(s1, od1);
+        else stage_w(s1, g1, TILE + (tid & ~63));
         if constexpr (PF) {
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
                 if (b1[ax] <= -2) frozen(b1[ax], fz1[ax], ax, g1);
-                if (c1[ax] <= -2) gather_in(ax, -2L - c1[ax], g1, in1[ax]);
+                if (!HEAD && c1[ax] <= -2) gather_in(ax, -2L - c1[ax], g1, in1[ax]);
             }
         }
     }
     for (int s = tid + 2 * TILE; s < nA; s += TILE) {
         long g;
-        if (live(s, g)) stage(s, g);
-    }
-    __syncthreads();
-    // ---- primal n+1 over PA
-    T nw0[4] = {T(0), T(0), T(0), T(0)}, nw1[4] = {T(0), T(0), T(0), T(0)};
-    if (l0) primal(tid, g0, w0, od0, c0, nw0, false, in1);
-    if (l1) primal(s1, g1, w1, od1, c1, nw1, PF, in1);
-    for (int s = tid + 2 * TILE; s < nA; s += TILE) {
-        long g;
         if (!live(s, g)) continue;
-        const int cv[3] = {cs[s], cs[CA + s], cs[2 * CA + s]};
-        T od[4], nw[4];
-        load_uv(g, od);
-        primal(s, g, a.info[g], od, cv, nw, false, in1);
-        if (s < nP) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) uvx[c * sx + s - 2 * TILE] = IO::rt(nw[c]);
+        if constexpr (HEAD) {
+            T od[4];
+            load_uv(g, od);
+            seed(s, od);
+        } else {
+            stage(s, g);
         }
     }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        nw0[c] = IO::rt(nw0[c]);  // the stored u, v of the first iteration
-        nw1[c] = IO::rt(nw1[c]);
-    }
     __syncthreads();
-    // ---- dual n+2 over DB
+    // ---- primal n+1 over PA (HEAD: none, the launch starts with the dual)
+    T nw0[4] = {T(0), T(0), T(0), T(0)}, nw1[4] = {T(0), T(0), T(0), T(0)};
+    if constexpr (!HEAD) {
+        if (l0) primal(tid, g0, w0, od0, c0, nw0, false, in1);
+        if (l1) primal(s1, g1, w1, od1, c1, nw1, PF, in1);
+        for (int s = tid + 2 * TILE; s < nA; s += TILE) {
+            long g;
+            if (!live(s, g)) continue;
+            const int cv[3] = {cs[s], cs[CA + s], cs[2 * CA + s]};
+            T od[4], nw[4];
+            load_uv(g, od);
+            primal(s, g, a.info[g], od, cv, nw, false, in1);
+            if (s < nP) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) uvx[c * sx + s - 2 * TILE] = IO::rt(nw[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            nw0[c] = IO::rt(nw0[c]);  // the stored u, v of the first iteration
+            nw1[c] = IO::rt(nw1[c]);
+        }
+        __syncthreads();
+    }
+    // ---- dual n+2 over DB (HEAD: dual 1)
     if (l0) dual2(tid, g0, b0, true, fz0);
     if (l1 && s1 < nD) dual2(s1, g1, b1, PF, fz1);
     for (int s = tid + 2 * TILE; s < nD; s += TILE) {
@@ -1320,12 +1343,13 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         dual2(s, g, bv, false, fz0);
     }
     __syncthreads();
-    // ---- primal n+2 over PC
+    // ---- primal n+2 over PC (HEAD: primal 1, from the staged (u, v); a
+    // slot's own u_bar is read before its primal step replaces it)
     T own[4] = {T(0), T(0), T(0), T(0)};
-    if (l0) primal(tid, g0, w0, nw0, c0, own, false, in1);
+    if (l0) primal(tid, g0, w0, HEAD ? od0 : nw0, c0, own, false, in1);
     if (l1 && s1 < nP) {
         T x[4];
-        primal(s1, g1, w1, nw1, c1, x, false, in1);
+        primal(s1, g1, w1, HEAD ? od1 : nw1, c1, x, false, in1);
     }
     for (int s = tid + 2 * TILE; s < nP; s += TILE) {
         long g;
@@ -1333,11 +1357,11 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         const int cv[3] = {cs[s], cs[CA + s], cs[2 * CA + s]};
         T od[4], x[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) od[c] = uvx[c * sx + s - 2 * TILE];
+        for (int c = 0; c < 4; ++c) od[c] = HEAD ? ub[c * sa + s] : uvx[c * sx + s - 2 * TILE];
         primal(s, g, a.info[g], od, cv, x, false, in1);
     }
     __syncthreads();
-    // ---- dual n+3 over the tile
+    // ---- dual n+3 over the tile (HEAD: dual 2)
     if (!l0) return;
     T bar[4][4];
 #pragma unroll
@@ -1369,14 +1393,16 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
 // p, q through LDS (two barriers). The dual and primal steps are k_dual's and
 // primal_adj_at's expressions on the main set, so the state after the launch
 // is that of `iters` two-kernel iterations, bit for bit.
+// NT threads, the smallest of 64, 256, 1024 that covers the level, so a small
+// level's barriers wait for as few waves as it needs.
 constexpr int kLevel1 = 1024;
 
-template <class S>
-__global__ void __launch_bounds__(kLevel1) k_level1(KArgs<S> a, int iters) {
+template <class S, int NT = kLevel1>
+__global__ void __launch_bounds__(NT) k_level1(KArgs<S> a, int iters) {
     using IO = Io<S>;
     using T = typename IO::T;
-    __shared__ T lub[4][kLevel1];
-    __shared__ T lpq[9][kLevel1];
+    __shared__ T lub[4][NT];
+    __shared__ T lpq[9][NT];
     const int t = (int)threadIdx.x;
     const long g = (long)a.off + (t < (int)a.n ? t : 0);
     const bool run = t < (int)a.n && !masked(a, g);
@@ -2700,9 +2726,10 @@ const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
     return *P.halo2.emplace(key, std::move(hb)).first->second;
 }
 
-// two iterations (primal n+1 ... dual n+3) from src into dst; false when the
-// level's tile sets do not fit (nothing launched)
-template <class S, bool X, bool REF = false>
+// two iterations (primal n+1 ... dual n+3) from src into dst, or with HEAD
+// the level's first one and a half (dual 1, primal 1, dual 2); false when
+// the level's tile sets do not fit (nothing launched)
+template <class S, bool X, bool REF = false, bool HEAD = false>
 bool launch_fused2_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, int tile) {
     const Halo2Bufs& hb = halo2_tab<S, REF>(c, a, tile);
     if (!hb.ok) return false;
@@ -2716,16 +2743,16 @@ bool launch_fused2_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<
         const int32_t* tl = hb.tiles.as<int32_t>() + cl.first;
         switch (tile) {
             case 128:
-                k_fused2<S, 128, X, REF><<<cl.count, 128, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 128, X, REF, HEAD><<<cl.count, 128, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
                 break;
             case 256:
-                k_fused2<S, 256, X, REF><<<cl.count, 256, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 256, X, REF, HEAD><<<cl.count, 256, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
                 break;
             case 1024:
-                k_fused2<S, 1024, X, REF><<<cl.count, 1024, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 1024, X, REF, HEAD><<<cl.count, 1024, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
                 break;
             default:
-                k_fused2<S, 512, X, REF><<<cl.count, 512, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 512, X, REF, HEAD><<<cl.count, 512, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
                 break;
         }
     }
@@ -2775,24 +2802,52 @@ void launch_fused_pass(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set1
     else launch_fused_pass_x<S, false>(c, a, src, dst, L);
 }
 
-// `iters` >= 2 exact-adjoint iterations as: dual of the first, iters - 1
-// fused (primal n, dual n+1) passes alternating between the main and the
-// alternate set, primal of the last back into the main set.
+// u, v from the set uv, p, q from the set pq
+template <class S>
+Set13<S> mix_sets(const Set13<S>& uv, const Set13<S>& pq) {
+    Set13<S> m = pq;
+    for (int f = F_U; f <= F_VZ; ++f) m.f[f] = uv.f[f];
+    return m;
+}
+
+// `iters` >= 2 iterations as: dual of the first, iters - 1 fused (primal n,
+// dual n+1) passes alternating between the main and the alternate set,
+// primal of the last back into the main set. A level that starts from its
+// initial state (cold: u_bar = u, v_bar = v, p = q = 0) with the
+// two-iteration kernel opens with its head launch instead (dual 1, primal 1,
+// dual 2), which writes u, v to the alternate set and p, q to the main set;
+// the sets then alternate field group by field group, so after an even
+// number of two-iteration launches p, q are back in the main set and the
+// last primal step leaves them in place.
 template <class S, bool X>
-void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode, bool frozen) {
+void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode, bool frozen, bool cold) {
     const int wg = L.block > 256 ? 256 : L.block;  // the dual / primal kernels take at most 256
     const unsigned grid = (a.n + wg - 1) / wg;
     Set13<S> cur = main_set(a), oth = alt_set<S>(c);
-    const Set13<S> mainset = cur;
+    const Set13<S> mainset = cur, altset = oth;
     // frozen-border leaves (reference rules only): out-of-leaf u_bar, v_bar
     // from the snapshot in the dual kernel, from the records in k_fused2
     auto dual = [&] {
         if (frozen) k_dual<S, 1, X><<<grid, wg, 0, c->stream>>>(a);
         else k_dual<S, 0, X><<<grid, wg, 0, c->stream>>>(a);
     };
-    dual();
+    const bool ref = op_mode == TGVX_OP_REF;
     int i = 1;
-    if (op_mode == TGVX_OP_REF) {
+    bool head = false;
+    if (cold && L.variant == 5) {
+        const Set13<S> hd = mix_sets(altset, mainset);
+        head = ref ? launch_fused2_x<S, X, true, true>(c, a, mainset, hd, L.block)
+                   : launch_fused2_x<S, X, false, true>(c, a, mainset, hd, L.block);
+        if (head) {
+            cur = hd;
+            oth = mix_sets(mainset, altset);
+            i = 2;
+        }
+    }
+    if (!head) dual();
+    // the last primal step copies p, q into the main set unless they are there
+    auto copy = [&] { return cur.f[F_PX] != mainset.f[F_PX] ? 1 : 0; };
+    if (ref) {
         // the reference's rules: two iterations per launch while two remain,
         // then primal and dual kernels on the main set
         for (; i + 1 < iters; i += 2) {
@@ -2800,11 +2855,12 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op
             std::swap(cur, oth);
         }
         for (; i < iters; ++i) {
-            k_primal_ref_set<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, cur.f[0] != mainset.f[0]);
+            k_primal_ref_set<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, copy());
             cur = mainset;
+            oth = altset;
             dual();
         }
-        k_primal_ref_set<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, cur.f[0] != mainset.f[0]);
+        k_primal_ref_set<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, copy());
         return;
     }
     if (L.variant == 5) {
@@ -2817,13 +2873,13 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op
         launch_fused_pass_x<S, X>(c, a, cur, oth, L);
         std::swap(cur, oth);
     }
-    const int copy = cur.f[0] != mainset.f[0];
-    k_primal_adj<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, copy);
+    k_primal_adj<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, copy());
 }
 template <class S>
-void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode, bool frozen = false) {
-    if (L.xcd) launch_fused_x<S, true>(c, a, iters, L, op_mode, frozen);
-    else launch_fused_x<S, false>(c, a, iters, L, op_mode, frozen);
+void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode, bool frozen = false,
+                  bool cold = false) {
+    if (L.xcd) launch_fused_x<S, true>(c, a, iters, L, op_mode, frozen, cold);
+    else launch_fused_x<S, false>(c, a, iters, L, op_mode, frozen, cold);
 }
 
 // Cached graph of `iters` iterations of one level.
@@ -2838,19 +2894,30 @@ struct GraphKey {
     double sigma, tau, a0, a1, tl;
     int frozen;
     uint32_t leaf_first, leaf_last;
+    int cold;
     bool operator<(const GraphKey& o) const {
         return std::tie(off, n, lvl_off, lvl_n, iters, op, block, prec, variant, xcd, mask, sigma, tau, a0, a1, tl,
-                        frozen, leaf_first, leaf_last) <
+                        frozen, leaf_first, leaf_last, cold) <
                std::tie(o.off, o.n, o.lvl_off, o.lvl_n, o.iters, o.op, o.block, o.prec, o.variant, o.xcd, o.mask,
-                        o.sigma, o.tau, o.a0, o.a1, o.tl, o.frozen, o.leaf_first, o.leaf_last);
+                        o.sigma, o.tau, o.a0, o.a1, o.tl, o.frozen, o.leaf_first, o.leaf_last, o.cold);
     }
 };
 struct GraphCache {
     std::map<GraphKey, hipGraphExec_t> m;
+    // the whole fixed-iteration sweep as one graph, for the parameters (and
+    // per-level iteration counts) in sweep_key; seen_key: the parameters of
+    // the last sweep run without one (a sweep is captured on its second run,
+    // when every table and buffer its launches read exists)
+    hipGraphExec_t sweep = nullptr;
+    std::vector<char> sweep_key, seen_key;
     ~GraphCache() { clear(); }
     void clear() {
         for (auto& kv : m) (void)hipGraphExecDestroy(kv.second);
         m.clear();
+        if (sweep) (void)hipGraphExecDestroy(sweep);
+        sweep = nullptr;
+        sweep_key.clear();
+        seen_key.clear();
     }
 };
 std::map<const tgvx_ctx*, GraphCache> g_graphs;
@@ -2951,13 +3018,16 @@ void check_grid(tgvx_ctx* c) {
 inline Launch level1_fallback(uint32_t n) { return Launch{5, n >= (1u << 18) ? 1024 : 512, 1}; }
 
 template <class S>
-void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, const Launch& L0) {
+void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, const Launch& L0,
+                bool cold = false) {
     if (iters <= 0 || a.n == 0) return;
     Launch L = L0;
     if (L.variant == 6) {
         if (!frozen && p->op_mode == TGVX_OP_ADJ && a.n <= (uint32_t)kLevel1 && a.off == a.lvl_off &&
             a.n == a.lvl_n) {
-            k_level1<S><<<1, kLevel1, 0, c->stream>>>(a, iters);
+            if (a.n <= 64) k_level1<S, 64><<<1, 64, 0, c->stream>>>(a, iters);
+            else if (a.n <= 256) k_level1<S, 256><<<1, 256, 0, c->stream>>>(a, iters);
+            else k_level1<S, kLevel1><<<1, kLevel1, 0, c->stream>>>(a, iters);
             TGVX_HIP(hipGetLastError());
             return;
         }
@@ -2980,7 +3050,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         if (p->op_mode == TGVX_OP_REF) halo2_tab<S, true>(c, a, L.block);
         else halo2_tab<S>(c, a, L.block);
     }
-    if (p->use_graph && iters > 1) {
+    if (p->use_graph && iters > 1 && !c->in_capture) {
         GraphKey key{a.off,
                      a.n,
                      a.lvl_off,
@@ -2999,13 +3069,14 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
                      (double)a.tl,
                      frozen ? 1 : 0,
                      frozen ? a.leaf_first : 0u,
-                     frozen ? a.leaf_last : 0u};
+                     frozen ? a.leaf_last : 0u,
+                     fused && cold ? 1 : 0};
         GraphCache& gc = g_graphs[c];
         auto it = gc.m.find(key);
         if (it == gc.m.end()) {
             hipGraph_t graph;
             TGVX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen);
+            if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen, cold);
             else
                 for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, frozen);
             TGVX_HIP(hipStreamEndCapture(c->stream, &graph));
@@ -3017,15 +3088,17 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         TGVX_HIP(hipGraphLaunch(it->second, c->stream));
         return;
     }
-    if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen);
+    if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen, cold);
     else
         for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, frozen);
     TGVX_HIP(hipGetLastError());
 }
 
+// cold: the level starts from its initial state (u_bar = u, v_bar = v,
+// p = q = 0), which lets the two-iteration kernel open with its head launch
 template <class S>
-void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen) {
-    run_launch<S>(c, a, p, iters, frozen, launch_for(c, p, a.n, frozen));
+void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, bool cold = false) {
+    run_launch<S>(c, a, p, iters, frozen, launch_for(c, p, a.n, frozen), cold);
 }
 
 template <class S>
@@ -3086,6 +3159,115 @@ void eval_gap_blocks(tgvx_ctx* c, KArgs<S> a, const tgvx_params* p, size_t li, s
     for (uint32_t b = 0; b < nb; ++b) rg[b] = rel_gap(pd[2 * b], pd[2 * b + 1]);
 }
 
+// A fixed-iteration solve of one block from the initial state as one graph
+// (no reference counterpart): k_reset, then per level the previous level's
+// u_bar, v_bar records, the parent initialisation, the iterations and,
+// unless no_energy, the FP64 energy, all nodes of one graph, so nothing
+// waits on the host between levels. The launches are the ones solve_t makes
+// for such a solve, in the same order. A sweep is captured on its second run
+// with the same parameters, when every table and buffer it reads exists
+// (they are never reallocated while the problem stays loaded); the per-level
+// times are not taken (use_graph = 2 keeps the per-level graphs and takes
+// them). Returns false when the solve is not of this kind or has not been
+// seen before: the caller runs it level by level.
+template <class S>
+void enqueue_sweep(tgvx_ctx* c, const tgvx_params* p, KArgs<S>& a) {
+    Problem& P = c->prob;
+    k_reset<S><<<(unsigned)((P.n + 255) / 256), 256, 0, c->stream>>>(a, (uint32_t)P.n);
+    for (size_t li = 0; li < P.levels.size(); ++li) {
+        const Level& L = P.levels[li];
+        a.off = a.lvl_off = L.off;
+        a.n = a.lvl_n = L.n;
+        set_steps<S>(a, p, L.depth);
+        prepare_level<S>(c, a, p, li);
+        if (li > 0 && L.n > 0) k_parent_init<S><<<(L.n + 255) / 256, 256, 0, c->stream>>>(a);
+        run_iters<S>(c, a, p, P.level_iters.empty() ? p->iters : P.level_iters[li], false, true);
+        if (!p->no_energy) eval_gap_to<S>(c, a, p, P.d_lvlpd.as<double>() + 2 * li);
+    }
+    TGVX_HIP(hipGetLastError());
+}
+
+template <class S>
+bool sweep_graph(tgvx_ctx* c, const tgvx_params* p, tgvx_report* rep) {
+    Problem& P = c->prob;
+    if (p->use_graph != 1 || p->gap_tol > 0.0 || P.nblocks != 1 || p->warm_start || p->dual_eps > 0.0f ||
+        P.levels.size() > (size_t)TGVX_MAX_LEVELS || P.n == 0)
+        return false;
+    for (const Level& L : P.levels)
+        if (launch_for(c, p, L.n, false).variant == 7) return false;  // cooperative launches stay eager
+    std::vector<char> key(sizeof(tgvx_params) + sizeof(int32_t) * P.level_iters.size());
+    std::memcpy(key.data(), p, sizeof(tgvx_params));
+    if (!P.level_iters.empty())
+        std::memcpy(key.data() + sizeof(tgvx_params), P.level_iters.data(), sizeof(int32_t) * P.level_iters.size());
+    GraphCache& gc = g_graphs[c];
+    if (!(gc.sweep && gc.sweep_key == key)) {
+        if (gc.seen_key != key) {
+            gc.seen_key = key;
+            return false;
+        }
+        KArgs<S> a = make_args<S>(c, p);
+        a.init_stride = 4;
+        TGVX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->in_capture = true;
+        try {
+            enqueue_sweep<S>(c, p, a);
+        } catch (...) {
+            // not capturable after all (a table missing): run level by level
+            c->in_capture = false;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            gc.seen_key.clear();
+            return false;
+        }
+        c->in_capture = false;
+        hipGraph_t graph = nullptr;
+        TGVX_HIP(hipStreamEndCapture(c->stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        TGVX_HIP(e);
+        if (gc.sweep) (void)hipGraphExecDestroy(gc.sweep);
+        gc.sweep = exec;
+        gc.sweep_key = key;
+    }
+    tgvx_report r;
+    std::memset(&r, 0, sizeof(r));
+    P.block_iters.assign(P.levels.size(), -1);
+    P.block_gap.assign(P.levels.size(), 0.0);
+    const double t0 = now_ms();
+    TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
+    TGVX_HIP(hipGraphLaunch(gc.sweep, c->stream));
+    TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
+    TGVX_HIP(hipEventSynchronize(c->ev[1]));
+    check_grid(c);
+    float ms = 0.f;
+    TGVX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    std::vector<double> pd(2 * P.levels.size(), 0.0);
+    if (!p->no_energy && !pd.empty())
+        TGVX_HIP(hipMemcpy(pd.data(), P.d_lvlpd.p, sizeof(double) * pd.size(), hipMemcpyDeviceToHost));
+    for (size_t li = 0; li < P.levels.size(); ++li) {
+        const Level& L = P.levels[li];
+        const int it = P.level_iters.empty() ? p->iters : P.level_iters[li];
+        r.level_depth[li] = L.depth;
+        r.level_leaves[li] = L.n;
+        r.level_iters[li] = it;
+        r.level_energy[li] = pd[2 * li];
+        r.level_dual[li] = pd[2 * li + 1];
+        r.leaf_iters += (double)L.n * it;
+    }
+    r.n_levels = (int)P.levels.size();
+    r.solve_ms = now_ms() - t0;
+    r.iterate_ms = ms;
+    if (rep) {
+        const double setup = rep->setup_ms;
+        *rep = r;
+        rep->setup_ms = setup;
+    }
+    return true;
+}
+
 template <class S>
 void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_report* rep,
              const tgvx_treetop_leaf* leaves, uint64_t nleaves, int init_stride) {
@@ -3101,6 +3283,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         throw Error(TGVX_E_ARG, "warm_start excludes an explicit initial field and the frozen-border solve");
     if (P.nblocks > 1 && (p->warm_start || init_uv || leaves))
         throw Error(TGVX_E_ARG, "a problem of several blocks takes neither warm starts nor treetop leaves");
+    if (!init_uv && !leaves && sweep_graph<S>(c, p, rep)) return;
     P.block_iters.assign(P.levels.size() * P.nblocks, -1);
     P.block_gap.assign(P.levels.size() * P.nblocks, 0.0);
     a.init_stride = 4;
@@ -3179,6 +3362,8 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             recover();
         }
         TGVX_HIP(hipGetLastError());
+        // the level's first iterations start from its initial state
+        const bool cold = !field && !rec;
         int done = 0;
         double work = -1.0;  // cube-iterations actually run, when not n * done
         double Pv = 0.0, Dv = 0.0;
@@ -3215,7 +3400,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                     a.rfz_bar = rbar;
                     a.rfz_dual = rdual;
                 }
-                run_iters<S>(c, a, p, p->iters, true);
+                run_iters<S>(c, a, p, p->iters, true, cold);
             }
             a.lvl_off = L.off;
             a.lvl_n = L.n;
@@ -3234,14 +3419,14 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             // P, D are read after the last level
             done = P.level_iters.empty() ? p->iters : P.level_iters[li];
             TGVX_HIP(hipEventRecord(c->ev[4 + 2 * li], c->stream));
-            run_iters<S>(c, a, p, done, false);
+            run_iters<S>(c, a, p, done, false, cold);
             TGVX_HIP(hipEventRecord(c->ev[5 + 2 * li], c->stream));
-            eval_gap_to<S>(c, a, p, P.d_lvlpd.as<double>() + 2 * li);
-            deferred[li] = 1;
+            if (!p->no_energy) eval_gap_to<S>(c, a, p, P.d_lvlpd.as<double>() + 2 * li);
+            deferred[li] = p->no_energy ? 2 : 1;
         } else if (!to_gap) {
             const int fixed = P.level_iters.empty() ? p->iters : P.level_iters[li];
             TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-            run_iters<S>(c, a, p, fixed, false);
+            run_iters<S>(c, a, p, fixed, false, cold);
             TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
             TGVX_HIP(hipEventSynchronize(c->ev[1]));
             check_grid(c);
@@ -3250,10 +3435,10 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             it_ms += ms;
             done = fixed;
             const double g0 = now_ms();
-            eval_gap<S>(c, a, p, &Pv, &Dv);
+            if (!p->no_energy) eval_gap<S>(c, a, p, &Pv, &Dv);
             if (P.nblocks > 1) {
-                std::vector<double> rg;
-                eval_gap_blocks<S>(c, a, p, li, rg);
+                std::vector<double> rg(P.nblocks, 0.0);
+                if (!p->no_energy) eval_gap_blocks<S>(c, a, p, li, rg);
                 for (uint32_t b = 0; b < P.nblocks; ++b) {
                     const bool has = P.seg[li][b + 1] > P.seg[li][b];
                     P.block_iters[li * P.nblocks + b] = has ? done : -1;
@@ -3297,7 +3482,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                 const int k = std::min(chunk, max_iters - done);
                 upload();  // blocks that reached the gap keep their state
                 TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-                run_iters<S>(c, a, p, k, false);
+                run_iters<S>(c, a, p, k, false, cold && done == 0);
                 TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
                 TGVX_HIP(hipEventSynchronize(c->ev[1]));
                 check_grid(c);
@@ -3326,7 +3511,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                     throw Error(TGVX_E_NUMERIC, "non-finite energy or gap at depth " + std::to_string(L.depth));
                 const int k = std::min(chunk, max_iters - done);
                 TGVX_HIP(hipEventRecord(c->ev[0], c->stream));
-                run_iters<S>(c, a, p, k, false);
+                run_iters<S>(c, a, p, k, false, cold && done == 0);
                 TGVX_HIP(hipEventRecord(c->ev[1], c->stream));
                 TGVX_HIP(hipEventSynchronize(c->ev[1]));
                 check_grid(c);
@@ -3351,13 +3536,15 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
     }
     TGVX_HIP(hipStreamSynchronize(c->stream));
     check_grid(c);
-    if (std::find(deferred.begin(), deferred.end(), 1) != deferred.end()) {
-        std::vector<double> pd(2 * deferred.size());
-        TGVX_HIP(hipMemcpy(pd.data(), P.d_lvlpd.p, sizeof(double) * pd.size(), hipMemcpyDeviceToHost));
+    if (std::find_if(deferred.begin(), deferred.end(), [](int d) { return d != 0; }) != deferred.end()) {
+        // deferred 1: P, D evaluated on the device; 2: iterations only
+        std::vector<double> pd(2 * deferred.size(), 0.0);
+        if (std::find(deferred.begin(), deferred.end(), 1) != deferred.end())
+            TGVX_HIP(hipMemcpy(pd.data(), P.d_lvlpd.p, sizeof(double) * pd.size(), hipMemcpyDeviceToHost));
         for (size_t li = 0; li < deferred.size(); ++li) {
             if (!deferred[li]) continue;
-            r.level_energy[li] = pd[2 * li];
-            r.level_dual[li] = pd[2 * li + 1];
+            r.level_energy[li] = deferred[li] == 1 ? pd[2 * li] : 0.0;
+            r.level_dual[li] = deferred[li] == 1 ? pd[2 * li + 1] : 0.0;
             float ms = 0.f;
             TGVX_HIP(hipEventElapsedTime(&ms, c->ev[4 + 2 * li], c->ev[5 + 2 * li]));
             it_ms += ms;

@@ -166,6 +166,9 @@ struct tgvx_ctx {
     // error flag; grid_used: a launch since the flag was last read
     tgvx::DevBuf grid_bar;
     bool grid_used = false;
+    // a whole fixed-iteration sweep is being captured into one graph: the
+    // level launches go onto the stream as kernels, not as graphs of their own
+    bool in_capture = false;
 };
 
 namespace tgvx {

@@ -74,9 +74,11 @@ tgvx_params solve_params(const TGVParams& p) {
     q.iters = p.iters;
     q.precision = TGVX_FP32;
     q.op_mode = TGVX_OP_REF;
-    // two iterations per launch over tabulated tile sets (the frozen-border
-    // solve keeps the dual and primal kernels); results are the same bits
+    // two iterations per launch over tabulated tile sets, the frozen-border
+    // solve's leaves included; results are the same bits
     q.fused = 6;
+    // the reference evaluates no energy
+    q.no_energy = 1;
     return q;
 }
 
