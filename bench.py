@@ -42,7 +42,7 @@ ALG_BYTES = {k: 34 * e + 8 + 24 for k, e in ELEM.items()}
 ALG_BYTES["bf16-dual"] = 16 * 4 + 18 * 2 + 8 + 24
 KERNEL_NAMES = {0: "k_dual + k_primal_adj", 1: "k_fused_tile", 2: "k_fused_adj",
                 3: "k_fused_tile (p, q staged in LDS)", 4: "k_fused_halo", 5: "k_fused2", 6: "k_level1",
-                7: "k_persist"}
+                7: "k_persist", 8: "k_fused4"}
 
 
 def log(msg):
@@ -64,12 +64,13 @@ def main():
                     help="workgroup / tile size (0: 1024 for the two-iteration kernel, 512 for the other tiled "
                          "fused variants, else 256); with the variant, the autotuner's base configuration")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5, 6, 7], default=5,
+    ap.add_argument("--fused", type=int, choices=[0, 1, 2, 3, 4, 5, 6, 7, 8], default=5,
                     help="0: dual + primal kernels per iteration; 1: LDS-tiled fused kernel; 2: untiled fused "
                          "kernel; 3: LDS-tiled with the tile's p, q staged in LDS; 4: LDS-tiled with a "
                          "tabulated halo; 5: two iterations per launch over tabulated tile sets; 6: levels of at "
                          "most 1024 cubes in one workgroup for all their iterations, larger ones as 5; 7: a level "
-                         "in one cooperative launch with grid barriers (levels too large for it as 6)")
+                         "in one cooperative launch with grid barriers (levels too large for it as 6); 8: four "
+                         "iterations per launch over tabulated tile sets where they fit, else as 5")
     ap.add_argument("--no-fused", action="store_true", help="same as --fused 0")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start time-to-gap runs")
     ap.add_argument("--no-oc", action="store_true",
@@ -249,7 +250,7 @@ def main():
     fused = variant != 0
     tile = tuning[-1]["block"] if tuning else args.block
     xcd = tuning[-1]["xcd"] if tuning else 1
-    if variant in (6, 7):
+    if variant in (6, 7, 8):
         # the variants for small levels time the finest level's fused pass as
         # the two-iteration kernel they fall back to there
         variant, tile, xcd = 5, (1024 if n_f >= 1 << 18 else 512), 1

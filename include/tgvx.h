@@ -114,8 +114,12 @@ typedef struct {
                               as 5, tiles of 1024 from 2^18 cubes on, else 512); 7 a level in
                               one cooperative launch for all its iterations, two grid barriers
                               per iteration, workgroups of block = 256 or 1024 (default; a level
-                              whose workgroups cannot all be resident at once runs as 6 does).
-                              For 1, 3, 4 and 5, block is the tile (128, 256 = default, 512,
+                              whose workgroups cannot all be resident at once runs as 6 does);
+                              8 four iterations per launch over tile sets tabulated once per level
+                              and tile, where four remain and the sets fit (tiles of 128, 256 =
+                              default), otherwise as 5 (both operator modes; for small levels,
+                              whose launches are bound by latency).
+                              For 1, 3, 4, 5 and 8, block is the tile (128, 256 = default, 512,
                               1024) */
     int32_t tune;          /* 1: take each level's launch configuration from tgvx_autotune's table
                               (levels without an entry use fused / block as given) */
@@ -289,6 +293,11 @@ int tgvx_grid_barrier_bench(tgvx_ctx* ctx, int32_t workgroups, int32_t block, in
  * fused_ms of tgvx_level_bench is per iteration (half a launch). */
 int tgvx_tile_sets(tgvx_ctx* ctx, int32_t depth, int32_t tile, int32_t* usable, int32_t* max_sets,
                    int64_t* lds_bytes);
+/* The same for the four-iteration kernel (params.fused = 8; tiles of 128, 256): usable,
+ * max_slots = the largest tile's slots (its nested sets and the constant slots of other-depth
+ * neighbours), lds_bytes = that tile's LDS per workgroup. No reference counterpart. */
+int tgvx_tile_sets4(tgvx_ctx* ctx, int32_t depth, int32_t tile, int32_t* usable, int32_t* max_slots,
+                    int64_t* lds_bytes);
 
 /* ---- iso-surface extraction (marching_cubes.cpp:63-181 extract_surface), on the device
  * Over the resident problem's cubes with u = the given field (n floats in input

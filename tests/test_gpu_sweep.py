@@ -82,3 +82,59 @@ def test_sweep_graph_follows_parameters(ctx, sphere):
             got = ctx.state()
         ctx.run_levels(TGVParams(**dict(base, use_graph=0)), counts)
         assert np.array_equal(_raw(got), _raw(ctx.state())), counts
+
+
+@pytest.mark.parametrize("prec", [FP32, FP64, BF16, BF16_DUAL])
+@pytest.mark.parametrize("mode", [OP_ADJ, OP_REF])
+@pytest.mark.parametrize("iters", [4, 5, 6, 9, 30])
+def test_four_iteration_launch_identical(ctx, sphere, prec, mode, iters):
+    """Four iterations per launch (fused = 8), cold and after a head launch,
+    with the remainder as two-iteration launches and single passes."""
+    sc, hist = sphere
+    base = dict(alpha0=2.0, alpha1=1.0, iters=iters, precision=prec, op_mode=mode)
+    ref, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=0, block=256, **base))
+    for block in (128, 256, 512):  # 512: two iterations per launch throughout
+        got, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=8, block=block, **base))
+        assert np.array_equal(_raw(got), _raw(ref)), (prec, mode, iters, block)
+
+
+def test_four_iteration_sets_usable(ctx, sphere):
+    """The sphere workload's levels of 512 cubes or more have usable
+    four-iteration sets with tiles of 256 in single precision, so the tests
+    above run the four-iteration kernel there (with tiles of 128 the sets of
+    some levels exceed the cap and those levels run two iterations per launch)."""
+    sc, hist = sphere
+    ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_ADJ)
+    for depth, n in ctx.levels():
+        if n < 512:
+            continue
+        ok, slots, lds = ctx.tile_sets4(depth, 256)
+        assert ok, (depth, slots, lds)
+        assert 256 < slots <= 8 * 256 and lds <= 160 * 1024
+        ok, slots, lds = ctx.tile_sets4(depth, 128)
+        assert slots <= 8 * 128 and (not ok or lds <= 160 * 1024)
+
+
+def test_four_iteration_to_gap(ctx, sphere):
+    """Chunks after the first start from a running state (no head launch)."""
+    sc, hist = sphere
+    base = dict(alpha0=2.0, alpha1=1.0, precision=FP32, op_mode=OP_ADJ, gap_tol=1e-3, chunk=50,
+                max_iters=2000, sigma=8.0, tau=1.0 / 128)
+    ref, r0 = ctx.solve(sc.cubes, hist, TGVParams(fused=0, block=256, **base))
+    got, r1 = ctx.solve(sc.cubes, hist, TGVParams(fused=8, block=256, **base))
+    assert [l["iters"] for l in r1.levels()] == [l["iters"] for l in r0.levels()]
+    assert np.array_equal(_raw(got), _raw(ref))
+
+
+def test_four_iteration_autotune_identical(ctx, sphere):
+    sc, hist = sphere
+    for mode in (OP_ADJ, OP_REF):
+        p = TGVParams(alpha0=2.0, alpha1=1.0, iters=50, precision=FP32, op_mode=mode, fused=8, block=256)
+        ctx.load(sc.cubes, hist, precision=FP32, op_mode=mode)
+        ctx.run(TGVParams(alpha0=2.0, alpha1=1.0, iters=50, precision=FP32, op_mode=mode, fused=0, block=256))
+        want = ctx.state()
+        ctx.autotune(p, iters=50)
+        p.tune = 1
+        for _ in range(3):  # level by level, captured, replayed
+            ctx.run(p)
+            assert np.array_equal(_raw(ctx.state()), _raw(want)), mode

@@ -404,6 +404,13 @@ class Context:
         self._check(self._L.tgvx_grid_barrier_bench(self.h, workgroups, block, barriers, C.byref(us)))
         return us.value
 
+    def tile_sets4(self, depth, tile):
+        """The four-iteration kernel's tile sets at `depth` (exact-adjoint problem):
+        (usable, largest slot count, LDS bytes per workgroup)."""
+        ok, ns, lds = C.c_int32(), C.c_int32(), C.c_int64()
+        self._check(self._L.tgvx_tile_sets4(self.h, depth, tile, C.byref(ok), C.byref(ns), C.byref(lds)))
+        return bool(ok.value), ns.value, lds.value
+
     def tile_sets(self, depth, tile):
         """The two-iteration kernel's tile sets at `depth` (exact-adjoint problem):
         (usable, (largest PA, DB, PC), LDS bytes per workgroup)."""

@@ -115,7 +115,7 @@ TGVX_SYMBOLS = [
     "tgvx_problem_block_report", "tgvx_solve_batched", "tgvx_autotune", "tgvx_problem_solve_levels",
     "tgvx_surface_extract", "tgvx_surface_copy", "tgvx_solve_streaming", "tgvx_octree_build",
     "tgvx_octree_balance", "tgvx_octree_copy", "tgvx_octree_merge", "tgvx_tile_sets", "tgvx_histograms_noalloc",
-    "tgvx_problem_solve_from", "tgvx_solve_streaming_ooc", "tgvx_grid_barrier_bench",
+    "tgvx_problem_solve_from", "tgvx_solve_streaming_ooc", "tgvx_grid_barrier_bench", "tgvx_tile_sets4",
 ]
 MLP_NPARAMS = 6404
 SCENE_SYMBOLS = ["tgvs_ring_cameras", "tgvs_random_cameras", "tgvs_render", "tgvs_build_octree",
@@ -161,6 +161,7 @@ def lib():
         L.tgvx_level_bench.argtypes = [vp, i32, C.POINTER(Params), i32, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tgvx_tile_sets.argtypes = [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_int64)]
+        L.tgvx_tile_sets4.argtypes = [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_int64)]
         L.tgvx_grid_barrier_bench.argtypes = [vp, i32, i32, i32, C.POINTER(C.c_double)]
         L.tgvx_mlp_set.argtypes = [vp, vp]
         L.tgvx_mlp_get.argtypes = [vp, vp]

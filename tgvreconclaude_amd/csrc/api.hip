@@ -184,6 +184,14 @@ int tgvx_tile_sets(tgvx_ctx* c, int32_t depth, int32_t tile, int32_t* usable, in
     });
 }
 
+int tgvx_tile_sets4(tgvx_ctx* c, int32_t depth, int32_t tile, int32_t* usable, int32_t* max_slots,
+                    int64_t* lds_bytes) {
+    return guarded(c, [&] {
+        if (!usable || !max_slots || !lds_bytes) throw tgvx::Error(TGVX_E_ARG, "null pointer");
+        tgvx::tile_sets4(c, depth, tile, usable, max_slots, lds_bytes);
+    });
+}
+
 int tgvx_mlp_set(tgvx_ctx* c, const float* w) {
     return guarded(c, [&] {
         if (!w) throw tgvx::Error(TGVX_E_ARG, "null pointer");

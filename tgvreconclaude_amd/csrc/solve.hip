@@ -108,6 +108,17 @@ __device__ inline void clip6(T* q, T r) {
 // prox_data (tgv.cpp:75-94): shrink towards the weighted median by
 // tau*lambda*W; the median depends only on the histogram, so it comes from
 // the packed word instead of a scan of the bins every iteration.
+// The same with the weighted median given (median_of's value for w).
+template <class T>
+__device__ inline T prox_m(T z, uint32_t w, T med, T tl) {
+    const uint32_t W = InfoWord::W(w);
+    if (W == 0u) return z;
+    const T diff = z - med;
+    const T shrink = tl * (T)W;
+    if (diff > shrink) return z - shrink;
+    if (diff < -shrink) return z + shrink;
+    return med;
+}
 template <class T>
 __device__ inline T prox(T z, uint32_t w, int nb, T tl) {
     const uint32_t W = InfoWord::W(w);
@@ -1064,8 +1075,13 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     T* const ub = reinterpret_cast<T*>(k_fused2_lds);  // [4][sa]
     T* const pq = ub + 4 * sa;                         // [9][sa]: p[3], q[6]
     T* const uvx = pq + 9 * sa;                        // [4][sx]
+    // the weighted median of every bin (median_of's expression, once per
+    // workgroup instead of a division per prox step); read after the
+    // staging barrier
+    __shared__ T s_med[32];
     const uint32_t tile = (uint32_t)tiles[tile_id<XCD>()];
     const int tid = (int)threadIdx.x;
+    if (tid < 32) s_med[tid] = median_of<T>(((uint32_t)tid << 24) | 1u, a.nbins);
     const long lo = (long)a.off + (long)tile * TILE;
     const int cntT = (int)min((uint32_t)TILE, a.n - tile * TILE);
     const int nP = h.cnt[4 * tile], nD = h.cnt[4 * tile + 1], nA = h.cnt[4 * tile + 2];
@@ -1171,7 +1187,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         if constexpr (REF) {
             // tgv.cpp:192-217 (k_primal_ref): a missing -a neighbour gives 0
             const T div_p = (p[0] - in[0][0]) + (p[1] - in[1][0]) + (p[2] - in[2][0]);
-            nw[0] = prox(od[0] + a.tau * div_p, w, a.nbins, a.tl);
+            nw[0] = prox_m(od[0] + a.tau * div_p, w, s_med[InfoWord::bin(w)], a.tl);
             const T dqx = (qs[0] - in[0][1]) + (qs[3] - in[1][1]) + (qs[4] - in[2][1]);
             const T dqy = (qs[3] - in[0][2]) + (qs[1] - in[1][2]) + (qs[5] - in[2][2]);
             const T dqz = (qs[4] - in[0][3]) + (qs[5] - in[1][3]) + (qs[2] - in[2][3]);
@@ -1200,7 +1216,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
             dq[1] -= T(0.5) * in[2][2];
             dq[2] -= T(1) * in[2][3];
         }
-        nw[0] = prox(od[0] + a.tau * dp, w, a.nbins, a.tl);
+        nw[0] = prox_m(od[0] + a.tau * dp, w, s_med[InfoWord::bin(w)], a.tl);
 #pragma unroll
         for (int c = 0; c < 3; ++c) nw[1 + c] = od[1 + c] + a.tau * (p[c] + dq[c]);
 #pragma unroll
@@ -1386,6 +1402,422 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     for (int k = 0; k < 6; ++k) IO::stn(dst.f[F_Q0 + k], g0, q[k]);
 }
 
+// ---- four iterations per launch (no reference counterpart). Where a level
+// is small, its two-iteration launches are bound by latency, not bandwidth
+// (one workgroup runs its tile through staging and four dependent phases
+// while most CUs idle), so these levels take four iterations per launch over
+// deeper tile sets, at the price of more redundant work at the sets' edges.
+// For a tile T the nested sets are G_0 = T and G_k = G_{k-1} plus its +a
+// neighbours (k odd) or its -a sources (k even) within the level, k = 1..7;
+// the launch runs primal n+1 over G_7, dual n+2 over G_6, primal n+2 over
+// G_5, ..., primal n+4 over G_1 and dual n+5 over G_0, every value in LDS.
+// Neighbours of other depths (constant while the level iterates) become
+// constant slots after G_7, holding their u_bar, v_bar and, with the
+// reference's rules, the p, q the primal step reads of them. Each value is
+// computed by the expressions of k_fused2, so the results are bit-identical
+// to every other launch variant.
+constexpr int kHalo4Cap = 8;   // slots per tile: at most kHalo4Cap * TILE
+constexpr int kHalo4Sets = 8;  // G_0 .. G_7
+constexpr int kHalo4Cnt = 10;  // per tile: ends of G_0 .. G_7, end of the constant slots, failed
+
+struct Halo4Tab {
+    const int32_t* cnt;    // per tile, kHalo4Cnt
+    const int32_t* list;   // per tile, (kHalo4Cap - 1) * TILE: position of slot TILE + k
+    const int32_t* bslot;  // per tile, 3 x kHalo4Cap * TILE: the +a source of slot s < end(G_6):
+                           // -1 the cube itself, >= 0 a slot (a constant slot for another depth)
+    const int32_t* cslot;  // per tile, 3 x kHalo4Cap * TILE: for slot s < end(G_7) the cube whose
+                           // +a neighbour it is (reference rules: its -a neighbour): -1 none,
+                           // >= 0 a slot, <= -2 the position -2 - v in the level outside the
+                           // sets (slots beyond G_5 only, read from the launch's input)
+};
+
+// One workgroup per tile; [off, off + n) is the whole level. As
+// k_halo2_table: members outside the tile go into an LDS hash set stage by
+// stage, then each class (G_k \ G_{k-1}, the constant slots) is numbered in
+// position order.
+template <int TILE, bool REF = false>
+__global__ void __launch_bounds__(TILE) k_halo4_table(const int32_t* __restrict__ nba0,
+                                                      const int32_t* __restrict__ nba1,
+                                                      const int32_t* __restrict__ nba2,
+                                                      const int32_t* __restrict__ nbt0,
+                                                      const int32_t* __restrict__ nbt1,
+                                                      const int32_t* __restrict__ nbt2, uint32_t off, uint32_t n,
+                                                      int32_t* __restrict__ cnt, int32_t* __restrict__ list,
+                                                      int32_t* __restrict__ bslot, int32_t* __restrict__ cslot) {
+    constexpr int CA = kHalo4Cap * TILE, NH = CA - TILE;
+    constexpr int LH = log2_ceil(2 * NH), H = 1 << LH;
+    __shared__ int32_t hkey[H], hval[H], lst[NH], rk[NH];
+    __shared__ int32_t nh, failed, ends[kHalo4Sets + 1];
+    const int32_t* const nba[3] = {nba0, nba1, nba2};
+    const int32_t* const nbt[3] = {nbt0, nbt1, nbt2};
+    const uint32_t tile = blockIdx.x;
+    const int tid = (int)threadIdx.x;
+    const long lo = (long)off + (long)tile * TILE;
+    const int cntT = (int)min((uint32_t)TILE, n - tile * TILE);
+    const long hi = lo + cntT, lvl_hi = (long)off + n;
+    for (int i = tid; i < H; i += TILE) hkey[i] = -1;
+    if (tid == 0) {
+        nh = 0;
+        failed = 0;
+        ends[0] = TILE;
+    }
+    __syncthreads();
+    auto in_tile = [&](long j) { return j >= lo && j < hi; };
+    auto in_level = [&](long j) { return j >= (long)off && j < lvl_hi; };
+    auto hash = [](int32_t j) { return ((uint32_t)j * 0x9E3779B1u) >> (32 - LH); };
+    auto insert = [&](int32_t j) {
+        uint32_t hp = hash(j);
+        for (int it = 0; it < H; ++it) {
+            const int32_t prev = atomicCAS(&hkey[hp], -1, j);
+            if (prev == -1) {
+                const int k = atomicAdd(&nh, 1);
+                if (k < NH) {
+                    lst[k] = j;
+                    hval[hp] = TILE + k;
+                } else {
+                    hval[hp] = -1;
+                    atomicOr(&failed, 1);
+                }
+                return;
+            }
+            if (prev == j) return;
+            hp = (hp + 1u) & (uint32_t)(H - 1);
+        }
+        atomicOr(&failed, 1);
+    };
+    auto find = [&](int32_t j) -> int32_t {
+        uint32_t hp = hash(j);
+        for (int it = 0; it < H; ++it) {
+            const int32_t k = hkey[hp];
+            if (k == j) return hval[hp];
+            if (k == -1) break;
+            hp = (hp + 1u) & (uint32_t)(H - 1);
+        }
+        return -1;
+    };
+    auto lookup = [&](int32_t j) -> int32_t {
+        const int32_t v = find(j);
+        if (v < 0) atomicOr(&failed, 1);
+        return v;
+    };
+    auto pos = [&](int s) -> long { return s < TILE ? lo + s : (long)lst[s - TILE]; };
+    auto valid = [&](int s) { return !(s >= cntT && s < TILE); };
+    // G_1 .. G_7: +a neighbours for odd k, -a sources for even k, within the
+    // level (the exact-adjoint sources always are)
+    for (int k = 1; k < kHalo4Sets; ++k) {
+        const int prev = ends[k - 1];
+        const bool plus = (k & 1) != 0;
+        for (int s = tid; s < prev; s += TILE) {
+            if (!valid(s)) continue;
+            const long g = pos(s);
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const int32_t j = plus ? nba[ax][g] : nbt[ax][g];
+                if (j < 0) continue;
+                if (!in_level(j)) {
+                    if (!plus && !REF) atomicOr(&failed, 1);
+                    continue;
+                }
+                if (!in_tile(j)) insert(j);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) ends[k] = TILE + min((int)nh, NH);
+        __syncthreads();
+    }
+    // constant slots: the other-depth +a neighbours of the dual slots and,
+    // with the reference's rules, the other-depth -a neighbours of the
+    // primal slots
+    const int e6 = ends[6], e7 = ends[7];
+    for (int s = tid; s < e7; s += TILE) {
+        if (!valid(s)) continue;
+        const long g = pos(s);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (s < e6) {
+                const int32_t j = nba[ax][g];
+                if (j >= 0 && !in_level(j)) insert(j);
+            }
+            if (REF) {
+                const int32_t k = nbt[ax][g];
+                if (k >= 0 && !in_level(k)) insert(k);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) ends[kHalo4Sets] = TILE + min((int)nh, NH);
+    __syncthreads();
+    const int eF = ends[kHalo4Sets];
+    // position order within each class
+    const int nX = eF - TILE;
+    for (int k = tid; k < nX; k += TILE) {
+        const int sl = TILE + k;
+        int c = 1;
+        while (c < kHalo4Sets && sl >= ends[c]) ++c;
+        const int k0 = ends[c - 1] - TILE, k1 = ends[c] - TILE;
+        const int32_t v = lst[k];
+        int r = k0;
+        for (int m = k0; m < k1; ++m) r += lst[m] < v ? 1 : 0;
+        rk[k] = r;
+    }
+    int32_t mv[kHalo4Cap - 1];
+#pragma unroll
+    for (int i = 0; i < kHalo4Cap - 1; ++i) {
+        const int k = tid + i * TILE;
+        mv[i] = k < nX ? lst[k] : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kHalo4Cap - 1; ++i) {
+        const int k = tid + i * TILE;
+        if (k < nX) lst[rk[k]] = mv[i];
+    }
+    for (int i = tid; i < H; i += TILE) {
+        const int32_t sv = hval[i];
+        if (hkey[i] != -1 && sv >= TILE) hval[i] = TILE + rk[sv - TILE];
+    }
+    __syncthreads();
+    int32_t* const L = list + (long)tile * NH;
+    for (int k = tid; k < nX; k += TILE) L[k] = lst[k];
+    int32_t* const B = bslot + (long)tile * 3 * CA;
+    int32_t* const C = cslot + (long)tile * 3 * CA;
+    const int e5 = ends[5];
+    for (int s = tid; s < e7; s += TILE) {
+        if (!valid(s)) continue;
+        const long g = pos(s);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (s < e6) {
+                const int32_t j = nba[ax][g];
+                B[ax * CA + s] = j < 0 ? -1 : (in_tile(j) ? (int32_t)(j - lo) : lookup(j));
+            }
+            const int32_t k = nbt[ax][g];
+            int32_t v;
+            if (k < 0) v = -1;
+            else if (in_tile(k)) v = (int32_t)(k - lo);
+            else if (!in_level(k)) v = lookup(k);  // a constant slot (reference rules)
+            else if (s < e5) v = lookup(k);        // in G_6
+            else {
+                v = find(k);
+                if (v < 0) v = -2 - k;
+            }
+            C[ax * CA + s] = v;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 0; k <= kHalo4Sets; ++k) cnt[kHalo4Cnt * tile + k] = ends[k];
+        cnt[kHalo4Cnt * tile + kHalo4Sets + 1] = (failed || nh > NH) ? 1 : 0;
+    }
+}
+
+// The dynamic LDS of k_fused4: u_bar, v_bar (4), p, q (9) and the stored
+// u, v (4) of every slot, each as an array of sa values.
+extern __shared__ uint4 k_fused4_lds[];
+
+// Threads per workgroup: more than the tile, so that each phase has at most
+// a few slots per thread and sixteen waves per CU hide the LDS and divide
+// latencies (with one thread per tile cube the launch is latency-bound).
+constexpr int kFused4Threads = 1024;
+
+// One workgroup per tile, in XCD-contiguous order. src holds (u^n, v^n,
+// p^n+1, q^n+1), dst receives (u^n+4, v^n+4, p^n+5, q^n+5) of the tiles'
+// cubes; sa = the slots of the largest tile's sets and constant slots.
+template <class S, int TILE, bool REF = false>
+__global__ void __launch_bounds__(kFused4Threads) k_fused4(KArgs<S> a, Set13<S> src, Set13<S> dst, Halo4Tab h,
+                                                           int sa) {
+    constexpr int NT = kFused4Threads;
+    using IO = Io<S>;
+    using T = typename IO::T;
+    constexpr int CA = kHalo4Cap * TILE, NH = CA - TILE;
+    // iterated slots per thread: slot tid + r * NT is thread tid's r-th; its
+    // position, info word and slot tables stay in registers for the launch
+    constexpr int R = (CA + NT - 1) / NT;
+    static_assert(R <= 2, "tiles of at most 256");
+    T* const ub = reinterpret_cast<T*>(k_fused4_lds);  // [4][sa]
+    T* const pq = ub + 4 * sa;                         // [9][sa]: p[3], q[6]
+    T* const uv = pq + 9 * sa;                         // [4][sa]: u, v as stored
+    __shared__ T s_med[32];
+    const uint32_t tile = (uint32_t)tile_id<true>();
+    const int tid = (int)threadIdx.x;
+    if (tid < 32) s_med[tid] = median_of<T>(((uint32_t)tid << 24) | 1u, a.nbins);
+    const long lo = (long)a.off + (long)tile * TILE;
+    const int cntT = (int)min((uint32_t)TILE, a.n - tile * TILE);
+    const int32_t* const e = h.cnt + (long)tile * kHalo4Cnt;
+    int ends[kHalo4Sets + 1];
+#pragma unroll
+    for (int k = 0; k <= kHalo4Sets; ++k) ends[k] = e[k];
+    const int e6 = ends[6], e7 = ends[7], eF = ends[kHalo4Sets];
+    const int32_t* const hl = h.list + (long)tile * NH;
+    const int32_t* const bs = h.bslot + (long)tile * 3 * CA;
+    const int32_t* const cs = h.cslot + (long)tile * 3 * CA;
+    auto live = [&](int s, long& g) {
+        if (s >= cntT && s < TILE) return false;
+        g = s < TILE ? lo + s : (long)hl[s - TILE];
+        return !masked(a, g);
+    };
+    constexpr int kF[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
+    // ---- staging: the thread's iterated slots (records, u, v, p, q from
+    // src); u_bar, v_bar (and with the reference's rules p, q) of the
+    // constant slots from the main set, which the level never writes
+    bool lv[R];
+    long gr[R];
+    uint32_t wr[R];
+    int br[R][3], cr[R][3];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int s = tid + r * NT;
+        lv[r] = s < e7 && live(s, gr[r]);
+        wr[r] = 0u;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) br[r][ax] = cr[r][ax] = -1;
+        if (!lv[r]) continue;
+        const long g = gr[r];
+        wr[r] = a.info[g];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            cr[r][ax] = cs[ax * CA + s];
+            if (s < e6) br[r][ax] = bs[ax * CA + s];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) uv[c * sa + s] = IO::ld(src.f[F_U + c], g);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) pq[k * sa + s] = IO::ld(src.f[F_PX + k], g);
+    }
+    for (int s = e7 + tid; s < eF; s += NT) {
+        const long g = (long)hl[s - TILE];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ub[c * sa + s] = IO::ld(a.s[F_UB + c], g);
+        if (REF) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) pq[k * sa + s] = IO::ld(a.s[F_PX + k], g);
+        }
+    }
+    __syncthreads();
+    // passes 3-5 of the thread's slot r (k_fused2's primal expressions): own
+    // p, q and the incoming ones from LDS, or (first phase only) from src
+    auto primal = [&](int r) {
+        const int s = tid + r * NT;
+        const uint32_t w = wr[r];
+        T od[4], nw[4], p[3], qs[6];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) od[c] = uv[c * sa + s];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = pq[c * sa + s];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qs[k] = pq[(3 + k) * sa + s];
+        T in[3][4];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int v = cr[r][ax];
+            if (v >= 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) in[ax][q] = pq[(kF[ax][q] - F_PX) * sa + v];
+            } else if (v <= -2) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) in[ax][q] = IO::ld(src.f[kF[ax][q]], -2L - v);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) in[ax][q] = T(0);
+            }
+        }
+        const T med = s_med[InfoWord::bin(w)];
+        if constexpr (REF) {
+            const T div_p = (p[0] - in[0][0]) + (p[1] - in[1][0]) + (p[2] - in[2][0]);
+            nw[0] = prox_m(od[0] + a.tau * div_p, w, med, a.tl);
+            const T dqx = (qs[0] - in[0][1]) + (qs[3] - in[1][1]) + (qs[4] - in[2][1]);
+            const T dqy = (qs[3] - in[0][2]) + (qs[1] - in[1][2]) + (qs[5] - in[2][2]);
+            const T dqz = (qs[4] - in[0][3]) + (qs[5] - in[1][3]) + (qs[2] - in[2][3]);
+            nw[1] = od[1] + a.tau * (p[0] + dqx);
+            nw[2] = od[2] + a.tau * (p[1] + dqy);
+            nw[3] = od[3] + a.tau * (p[2] + dqz);
+        } else {
+            T dp = T(0);
+            T dq[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const bool hs = InfoWord::has(w, ax);
+                dp = hs ? dp + p[ax] : dp;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T w2 = (c == ax) ? T(1) : T(0.5);
+                    dq[c] = hs ? dq[c] + w2 * qs[qidx(c, ax) - F_Q0] : dq[c];
+                }
+            }
+            if (cr[r][0] != -1) {
+                dp -= in[0][0];
+                dq[0] -= T(1) * in[0][1];
+                dq[1] -= T(0.5) * in[0][2];
+                dq[2] -= T(0.5) * in[0][3];
+            }
+            if (cr[r][1] != -1) {
+                dp -= in[1][0];
+                dq[0] -= T(0.5) * in[1][1];
+                dq[1] -= T(1) * in[1][2];
+                dq[2] -= T(0.5) * in[1][3];
+            }
+            if (cr[r][2] != -1) {
+                dp -= in[2][0];
+                dq[0] -= T(0.5) * in[2][1];
+                dq[1] -= T(0.5) * in[2][2];
+                dq[2] -= T(1) * in[2][3];
+            }
+            nw[0] = prox_m(od[0] + a.tau * dp, w, med, a.tl);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nw[1 + c] = od[1 + c] + a.tau * (p[c] + dq[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            ub[c * sa + s] = IO::rt(T(2) * nw[c] - od[c]);
+            uv[c * sa + s] = IO::rt(nw[c]);
+        }
+    };
+    // passes 1-2 of the thread's slot r (k_dual's expressions)
+    auto dual = [&](int r) {
+        const int s = tid + r * NT;
+        T bar[4][4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bar[0][c] = ub[c * sa + s];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int v = br[r][ax];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bar[1 + ax][c] = v < 0 ? bar[0][c] : ub[c * sa + v];
+        }
+        T p[3], q[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = pq[c * sa + s];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q[k] = pq[(3 + k) * sa + s];
+        dual_step(p, q, bar, a.sigma, a.a0, a.a1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pq[c * sa + s] = IO::rtd(p[c]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pq[(3 + k) * sa + s] = IO::rtd(q[k]);
+    };
+    // primal over G_7, dual over G_6, ..., primal over G_1, dual over G_0
+#pragma unroll 1
+    for (int k = kHalo4Sets - 1; k >= 1; k -= 2) {
+        const int np = ends[k], nd = ends[k - 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (lv[r] && tid + r * NT < np) primal(r);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (lv[r] && tid + r * NT < nd) dual(r);
+        __syncthreads();
+    }
+    // ---- the tile's cubes (slots 0 .. TILE - 1: the thread's slot 0)
+    if (tid < TILE && lv[0]) {
+        const long g = gr[0];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) IO::stn(dst.f[F_U + c], g, uv[c * sa + tid]);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) IO::stn(dst.f[F_PX + k], g, pq[k * sa + tid]);
+    }
+}
+
 // ---- a whole level in one workgroup (no reference counterpart), for levels
 // of at most kLevel1 cubes: `iters` iterations in one launch. Thread t owns
 // the level's cube t and keeps its u, v, p, q, u_bar, v_bar, table entries and
@@ -1418,6 +1850,7 @@ __global__ void __launch_bounds__(NT) k_level1(KArgs<S> a, int iters) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) ub[c] = IO::ld(a.s[F_UB + c], g);
     const uint32_t w = a.info[g];
+    const T med = median_of<T>(w, a.nbins);  // constant over the iterations
     // +a sources: >= 0 an LDS slot, -1 this cube, -2 frozen (values in fz);
     // incoming edges: the LDS slot or -1
     int src[3], inc[3];
@@ -1498,7 +1931,7 @@ __global__ void __launch_bounds__(NT) k_level1(KArgs<S> a, int iters) {
                 dq[2] -= T(1) * in[2][3];
             }
             T nw[4];
-            nw[0] = prox(u + a.tau * dp, w, a.nbins, a.tl);
+            nw[0] = prox_m(u + a.tau * dp, w, med, a.tl);
 #pragma unroll
             for (int c = 0; c < 3; ++c) nw[1 + c] = v[c] + a.tau * (p[c] + dq[c]);
             ub[0] = IO::rt(T(2) * nw[0] - u);
@@ -2461,7 +2894,7 @@ int block_of(const tgvx_params* p) {
     if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
         throw Error(TGVX_E_ARG, "block must be 64, 128, 256, 512 or 1024");
     const bool tiled = p && ((p->op_mode == TGVX_OP_ADJ && (p->fused == 1 || p->fused == 3 || p->fused == 4)) ||
-                             p->fused == 5 || p->fused == 6 || p->fused == 7);
+                             p->fused == 5 || p->fused == 6 || p->fused == 7 || p->fused == 8);
     if (b > 256 && !tiled) throw Error(TGVX_E_ARG, "blocks above 256 exist only for the tiled fused kernels");
     return b;
 }
@@ -2473,8 +2906,10 @@ int block_of(const tgvx_params* p) {
 // most kLevel1 cubes in one workgroup for all its iterations (larger levels
 // run variant 5 with tiles of 512), 7 = a level in one cooperative launch
 // with grid barriers (workgroups of 256 or 1024; a level whose workgroups
-// cannot all be co-resident runs as variant 6 does);
-// block = the workgroup size (the tile size for variants 1, 3, 4); xcd =
+// cannot all be co-resident runs as variant 6 does), 8 = four iterations per
+// launch over tabulated tile sets where four remain and the sets fit (tiles
+// of 128 or 256), otherwise as variant 5;
+// block = the workgroup size (the tile size for variants 1, 3, 4, 5, 8); xcd =
 // XCD-aware tile order.
 struct Launch {
     int variant, block, xcd;
@@ -2491,14 +2926,14 @@ int size_bucket(uint32_t n) {
 
 Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool frozen) {
     const bool adj = p->op_mode == TGVX_OP_ADJ;
-    if (p->fused < 0 || p->fused > 7) throw Error(TGVX_E_ARG, "fused must be 0 to 7");
+    if (p->fused < 0 || p->fused > 8) throw Error(TGVX_E_ARG, "fused must be 0 to 8");
     // the reference's rules have the two-kernel iteration and the two-iteration
     // kernel (5; 6 and 7 run every level as 5, their small-level kernels being
     // the exact-adjoint one's), also for frozen-border leaves; the exact-adjoint
     // frozen-border solve has the two kernels only
     const int var = !adj ? (p->fused >= 5 ? p->fused : 0) : frozen ? 0 : p->fused;
     Launch L{var, block_of(p), 1};
-    const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4 || L.variant == 5;
+    const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4 || L.variant == 5 || L.variant == 8;
     if (tiled && p->block <= 0) L.block = kFuseTile;
     if (L.variant == 6) L = Launch{6, kLevel1, 0};
     if (L.variant == 7) L = Launch{7, L.block == 256 ? 256 : 1024, 0};
@@ -2674,8 +3109,11 @@ const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
         TGVX_HIP(hipStreamSynchronize(c->stream));
         bool failed = false;
         const size_t el = sizeof(typename Io<S>::T);
+        // the kernel's static LDS (its median table) comes on top of the
+        // dynamic part a class is launched with
+        const size_t st = 32 * el;
         auto need = [&](uint32_t t) {
-            return el * (13 * (size_t)cnt[4 * t + 2] + 4 * (size_t)std::max(0, cnt[4 * t] - 2 * tile));
+            return st + el * (13 * (size_t)cnt[4 * t + 2] + 4 * (size_t)std::max(0, cnt[4 * t] - 2 * tile));
         };
         std::vector<size_t> needs(ntiles);
         for (uint32_t t = 0; t < ntiles; ++t) {
@@ -2708,7 +3146,7 @@ const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
             }
             cl.count = (uint32_t)order.size() - cl.first;
             cl.lds = el * (13 * (size_t)cl.sa + 4 * (size_t)cl.sx);
-            hb->lds = std::max(hb->lds, cl.lds);
+            hb->lds = std::max(hb->lds, st + cl.lds);
         }
         hb->ok = !failed && hb->lds <= lds_budget(c);
         if (hb->ok) {
@@ -2759,11 +3197,91 @@ bool launch_fused2_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<
     return true;
 }
 
+// The four-iteration kernel's tile sets for a's level (the whole level) and a
+// tile of 128 or 256, built on first use outside any graph capture.
+// ok = no tile's sets exceeded the cap and the largest tile's LDS fits.
+template <class S, bool REF = false>
+const Halo4Bufs& halo4_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
+    Problem& P = c->prob;
+    const auto key = std::make_tuple(a.off, a.n, tile, (int)sizeof(S) + (REF ? 1000 : 0));
+    auto it = P.halo4.find(key);
+    if (it != P.halo4.end()) return *it->second;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    TGVX_HIP(hipStreamIsCapturing(c->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) throw Error(TGVX_E_STATE, "tile sets built inside a graph capture");
+    auto hb = std::make_unique<Halo4Bufs>();
+    const uint32_t ntiles = (a.n + tile - 1) / tile;
+    const size_t ca = (size_t)kHalo4Cap * tile;
+    const bool whole = a.off == a.lvl_off && a.n == a.lvl_n;
+    if (whole && ntiles > 0 && (tile == 128 || tile == 256)) {
+        hb->cnt.ensure(sizeof(int32_t) * kHalo4Cnt * ntiles);
+        hb->list.ensure(sizeof(int32_t) * (ca - tile) * ntiles);
+        hb->bslot.ensure(sizeof(int32_t) * 3 * ca * ntiles);
+        hb->cslot.ensure(sizeof(int32_t) * 3 * ca * ntiles);
+        const int32_t* const* nb_plus = REF ? a.nbp : a.nba;
+        const int32_t* const* nb_in = REF ? a.nbm : a.nbt;
+        int32_t* const o[4] = {hb->cnt.as<int32_t>(), hb->list.as<int32_t>(), hb->bslot.as<int32_t>(),
+                               hb->cslot.as<int32_t>()};
+        switch (tile) {
+            case 128:
+                k_halo4_table<128, REF><<<ntiles, 128, 0, c->stream>>>(nb_plus[0], nb_plus[1], nb_plus[2], nb_in[0],
+                                                                        nb_in[1], nb_in[2], a.off, a.n, o[0], o[1],
+                                                                        o[2], o[3]);
+                break;
+            default:
+                k_halo4_table<256, REF><<<ntiles, 256, 0, c->stream>>>(nb_plus[0], nb_plus[1], nb_plus[2], nb_in[0],
+                                                                        nb_in[1], nb_in[2], a.off, a.n, o[0], o[1],
+                                                                        o[2], o[3]);
+                break;
+        }
+        TGVX_HIP(hipGetLastError());
+        std::vector<int32_t> cnt((size_t)kHalo4Cnt * ntiles);
+        TGVX_HIP(hipMemcpyAsync(cnt.data(), hb->cnt.p, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost,
+                                c->stream));
+        TGVX_HIP(hipStreamSynchronize(c->stream));
+        bool failed = false;
+        for (uint32_t t = 0; t < ntiles; ++t) {
+            hb->sa = std::max(hb->sa, cnt[(size_t)kHalo4Cnt * t + kHalo4Sets]);
+            failed = failed || cnt[(size_t)kHalo4Cnt * t + kHalo4Sets + 1] != 0;
+        }
+        const size_t el = sizeof(typename Io<S>::T);
+        hb->lds = el * (32 + 17 * (size_t)hb->sa);  // the median table and 17 arrays of sa
+        hb->ok = !failed && hb->lds <= lds_budget(c);
+        if (!hb->ok) {
+            hb->cnt.release();
+            hb->list.release();
+            hb->bslot.release();
+            hb->cslot.release();
+        }
+    }
+    return *P.halo4.emplace(key, std::move(hb)).first->second;
+}
+
+// four iterations (primal n+1 ... dual n+5) from src into dst; false when
+// the level's four-iteration sets do not fit (nothing launched)
+template <class S, bool REF = false>
+bool launch_fused4(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, int tile) {
+    const Halo4Bufs& hb = halo4_tab<S, REF>(c, a, tile);
+    if (!hb.ok) return false;
+    Halo4Tab h;
+    h.cnt = hb.cnt.as<int32_t>();
+    h.list = hb.list.as<int32_t>();
+    h.bslot = hb.bslot.as<int32_t>();
+    h.cslot = hb.cslot.as<int32_t>();
+    const uint32_t ntiles = (a.n + tile - 1) / tile;
+    const size_t dyn = hb.lds - 32 * sizeof(typename Io<S>::T);
+    switch (tile) {
+        case 128: k_fused4<S, 128, REF><<<ntiles, kFused4Threads, dyn, c->stream>>>(a, src, dst, h, hb.sa); break;
+        default: k_fused4<S, 256, REF><<<ntiles, kFused4Threads, dyn, c->stream>>>(a, src, dst, h, hb.sa); break;
+    }
+    return true;
+}
+
 // one fused pass (primal n, dual n+1) from src into dst
 template <class S, bool X>
 void launch_fused_pass_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<S>& dst, const Launch& L0) {
-    // variant 5 advances one iteration with the tiled kernel of the same tile
-    const Launch L = L0.variant == 5 ? Launch{1, L0.block, L0.xcd} : L0;
+    // variants 5 and 8 advance one iteration with the tiled kernel of the same tile
+    const Launch L = L0.variant == 5 || L0.variant == 8 ? Launch{1, L0.block, L0.xcd} : L0;
     if (L.variant == 4) {
         const HaloTab h = halo_tab<S>(c, a, L.block);
         switch (L.block) {
@@ -2834,7 +3352,10 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op
     const bool ref = op_mode == TGVX_OP_REF;
     int i = 1;
     bool head = false;
-    if (cold && L.variant == 5) {
+    const bool two = L.variant == 5 || L.variant == 8;
+    // variant 8: four iterations per launch while four remain (whole levels)
+    const bool four = L.variant == 8 && !frozen;
+    if (cold && two) {
         const Set13<S> hd = mix_sets(altset, mainset);
         head = ref ? launch_fused2_x<S, X, true, true>(c, a, mainset, hd, L.block)
                    : launch_fused2_x<S, X, false, true>(c, a, mainset, hd, L.block);
@@ -2848,8 +3369,13 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op
     // the last primal step copies p, q into the main set unless they are there
     auto copy = [&] { return cur.f[F_PX] != mainset.f[F_PX] ? 1 : 0; };
     if (ref) {
-        // the reference's rules: two iterations per launch while two remain,
-        // then primal and dual kernels on the main set
+        // the reference's rules: four (variant 8) or two iterations per
+        // launch while as many remain, then primal and dual kernels on the
+        // main set
+        for (; four && i + 3 < iters; i += 4) {
+            if (!launch_fused4<S, true>(c, a, cur, oth, L.block)) break;
+            std::swap(cur, oth);
+        }
         for (; i + 1 < iters; i += 2) {
             if (!launch_fused2_x<S, X, true>(c, a, cur, oth, L.block)) break;
             std::swap(cur, oth);
@@ -2863,7 +3389,11 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op
         k_primal_ref_set<S, X><<<grid, wg, 0, c->stream>>>(a, cur, mainset, copy());
         return;
     }
-    if (L.variant == 5) {
+    for (; four && i + 3 < iters; i += 4) {
+        if (!launch_fused4<S>(c, a, cur, oth, L.block)) break;
+        std::swap(cur, oth);
+    }
+    if (two) {
         for (; i + 1 < iters; i += 2) {
             if (!launch_fused2_x<S, X>(c, a, cur, oth, L.block)) break;
             std::swap(cur, oth);
@@ -3046,9 +3576,13 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
     const bool fused = L.variant != 0 && iters >= 2;
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (fused && L.variant == 4) halo_tab<S>(c, a, L.block);
-    if (fused && L.variant == 5) {
+    if (fused && (L.variant == 5 || L.variant == 8)) {
         if (p->op_mode == TGVX_OP_REF) halo2_tab<S, true>(c, a, L.block);
         else halo2_tab<S>(c, a, L.block);
+    }
+    if (fused && L.variant == 8 && !frozen && iters >= 4) {
+        if (p->op_mode == TGVX_OP_REF) halo4_tab<S, true>(c, a, L.block);
+        else halo4_tab<S>(c, a, L.block);
     }
     if (p->use_graph && iters > 1 && !c->in_capture) {
         GraphKey key{a.off,
@@ -3683,6 +4217,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
         }
         for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{5, b, x});
     }
+    for (int b : {256, 128}) cand.push_back(Launch{8, b, 1});
     if (p->op_mode == TGVX_OP_ADJ) {
         cand.push_back(Launch{6, kLevel1, 0});
         for (int b : {1024, 256}) cand.push_back(Launch{7, b, 0});
@@ -3701,6 +4236,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
             case 5: return (2.0 + (k - 1.0) / 2.0) / k;
             case 6: return n <= (uint32_t)kLevel1 ? 1.0 / k : (2.0 + (k - 1.0) / 2.0) / k;
             case 7: return 2.0 / k;  // the counter's clear and the launch
+            case 8: return (2.0 + (k - 1.0) / 4.0) / k;
             default: return (k + 1.0) / k;
         }
     };
@@ -3716,6 +4252,7 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
         Launch win = base;
         for (const Launch& cf : cand) {
             if (cf.variant == 6 && L.n > (uint32_t)kLevel1) continue;  // would run as 5
+            if (cf.variant == 8 && L.n > (1u << 20)) continue;         // bandwidth-bound: two per launch
             if (cf.variant == 7 && !persist_fits<S>(c, L.n, cf.block)) continue;
             run_launch<S>(c, a, &q, iters, false, cf);  // warm-up (and graph build)
             double us = 1e300;
@@ -3763,13 +4300,14 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
     release_graphs(c);  // the candidates' graphs
     // keep only the tables the chosen configurations use (the candidates'
     // tables of other tile sizes hold about 100 B per cube each)
-    std::set<std::tuple<uint32_t, uint32_t, int>> keep4, keep5;
+    std::set<std::tuple<uint32_t, uint32_t, int>> keep4, keep5, keep8;
     for (const Level& L : P.levels) {
         auto it = c->tuned.find(std::make_tuple(P.precision, p->op_mode, size_bucket(L.n)));
         if (it == c->tuned.end()) continue;
         const TuneEntry& e = it->second;
         if (e.variant == 4) keep4.insert(std::make_tuple(L.off, L.n, e.block));
-        if (e.variant == 5) keep5.insert(std::make_tuple(L.off, L.n, e.block));
+        if (e.variant == 5 || e.variant == 8) keep5.insert(std::make_tuple(L.off, L.n, e.block));
+        if (e.variant == 8) keep8.insert(std::make_tuple(L.off, L.n, e.block));
         if (e.variant == 6 && L.n > (uint32_t)kLevel1)
             keep5.insert(std::make_tuple(L.off, L.n, level1_fallback(L.n).block));
     }
@@ -3778,6 +4316,10 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
     for (auto it = P.halo2.begin(); it != P.halo2.end();) {
         const auto k = std::make_tuple(std::get<0>(it->first), std::get<1>(it->first), std::get<2>(it->first));
         it = keep5.count(k) ? std::next(it) : P.halo2.erase(it);
+    }
+    for (auto it = P.halo4.begin(); it != P.halo4.end();) {
+        const auto k = std::make_tuple(std::get<0>(it->first), std::get<1>(it->first), std::get<2>(it->first));
+        it = keep8.count(k) ? std::next(it) : P.halo4.erase(it);
     }
 }
 
@@ -3908,6 +4450,7 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     P.loaded = false;
     P.halo.clear();
     P.halo2.clear();
+    P.halo4.clear();
     P.rfz_use.clear();
     P.n = n;
     P.nb = nb;
@@ -4633,6 +5176,36 @@ void tile_sets(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_s
         case TGVX_FP32: tile_sets_t<float>(c, depth, tile, usable, max_sets, lds_bytes); break;
         case TGVX_BF16_DUAL: tile_sets_t<mixs>(c, depth, tile, usable, max_sets, lds_bytes); break;
         default: tile_sets_t<bf16s>(c, depth, tile, usable, max_sets, lds_bytes); break;
+    }
+}
+
+template <class S>
+void tile_sets4_t(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_slots, int64_t* lds_bytes) {
+    Problem& P = c->prob;
+    const Level* L = nullptr;
+    for (const Level& l : P.levels)
+        if (l.depth == depth) L = &l;
+    if (!L) throw Error(TGVX_E_ARG, "no cubes at depth " + std::to_string(depth));
+    if (tile != 128 && tile != 256) throw Error(TGVX_E_ARG, "the four-iteration kernel takes tiles of 128 or 256");
+    tgvx_params q;
+    tgvx_default_params(&q);
+    q.op_mode = TGVX_OP_ADJ;
+    const KArgs<S> a = level_args<S>(c, &q, *L);
+    const Halo4Bufs& hb = halo4_tab<S>(c, a, tile);
+    *usable = hb.ok ? 1 : 0;
+    *max_slots = hb.sa;
+    *lds_bytes = (int64_t)hb.lds;
+}
+
+void tile_sets4(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_slots, int64_t* lds_bytes) {
+    Problem& P = c->prob;
+    if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
+    if (P.op_mode != TGVX_OP_ADJ) throw Error(TGVX_E_STATE, "the four-iteration kernel needs the exact-adjoint tables");
+    switch (P.precision) {
+        case TGVX_FP64: tile_sets4_t<double>(c, depth, tile, usable, max_slots, lds_bytes); break;
+        case TGVX_FP32: tile_sets4_t<float>(c, depth, tile, usable, max_slots, lds_bytes); break;
+        case TGVX_BF16_DUAL: tile_sets4_t<mixs>(c, depth, tile, usable, max_slots, lds_bytes); break;
+        default: tile_sets4_t<bf16s>(c, depth, tile, usable, max_slots, lds_bytes); break;
     }
 }
 

@@ -78,6 +78,16 @@ struct Halo2Bufs {
     } cls[2];
 };
 
+// The four-iteration kernel's tile sets of one level and tile size: per
+// tile the ends of its nested sets and constant slots, the halo positions and
+// the slot tables; lds = the largest tile's LDS in bytes.
+struct Halo4Bufs {
+    DevBuf cnt, list, bslot, cslot;
+    bool ok = false;
+    int sa = 0;
+    size_t lds = 0;
+};
+
 struct Level {
     int depth;
     uint32_t off, n;  // contiguous range of permuted positions
@@ -131,6 +141,7 @@ struct Problem {
     std::map<std::tuple<uint32_t, uint32_t, int>, std::unique_ptr<HaloBufs>> halo;
     // tile sets of the two-iteration kernel, keyed likewise (and by the state's element size)
     std::map<std::tuple<uint32_t, uint32_t, int, int>, std::unique_ptr<Halo2Bufs>> halo2;
+    std::map<std::tuple<uint32_t, uint32_t, int, int>, std::unique_ptr<Halo4Bufs>> halo4;
 };
 
 }  // namespace tgvx
@@ -191,6 +202,7 @@ void level_bench(tgvx_ctx* c, int depth, const tgvx_params* p, int iters, double
 void autotune(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* rep);
 void grid_barrier_bench(tgvx_ctx* c, int wgs, int block, int nbar, double* us);
 void tile_sets(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_sets, int64_t* lds_bytes);
+void tile_sets4(tgvx_ctx* c, int depth, int tile, int32_t* usable, int32_t* max_slots, int64_t* lds_bytes);
 // octree.hip
 void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, const float origin[3], float r_root,
                   int32_t max_depth, int32_t last_stage, uint64_t* n_out, tgvx_octree_report* rep);
