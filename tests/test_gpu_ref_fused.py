@@ -60,7 +60,7 @@ def test_ref_csg_scene_tuned(ctx):
     p = TGVParams(alpha0=2.0, alpha1=1.0, iters=50, precision=FP32, op_mode=OP_REF, fused=6)
     ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_REF)
     tuned = ctx.autotune(p, iters=50)
-    assert all(l["variant"] in (0, 5) for l in tuned)
+    assert all(l["variant"] in (0, 5, 8) for l in tuned)  # the reference rules' variants
     p.tune = 1
     ctx.run(p)
     got = _flat(ctx.state())

@@ -38,7 +38,7 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
     levels = ctx.autotune(p, iters=iters)
     assert [(l["depth"], l["leaves"]) for l in levels] == [tuple(x) for x in ctx.levels()]
     for l in levels:
-        assert l["variant"] in ((0, 1, 2, 3, 4, 5, 6, 7) if mode == OP_ADJ else (0, 5))
+        assert l["variant"] in ((0, 1, 2, 3, 4, 5, 6, 7, 8) if mode == OP_ADJ else (0, 5, 8))
         assert l["block"] in (64, 128, 256, 512, 1024) and l["xcd"] in (0, 1)
         if l["variant"] in (1, 3, 4):
             assert l["block"] in (128, 256, 512, 1024)
@@ -48,6 +48,8 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
             assert l["block"] == 1024 and l["leaves"] <= 1024
         elif l["variant"] == 7:
             assert l["block"] in (256, 1024)
+        elif l["variant"] == 8:
+            assert l["block"] in (128, 256)
         else:
             assert l["block"] <= 256
         # candidates are ranked by time + 1 us per kernel launch and iteration
@@ -58,6 +60,8 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
                 return (2.0 + (iters - 1) / 2.0) / iters
             if v == 7:
                 return 2.0 / iters
+            if v == 8:
+                return (2.0 + (iters - 1) / 4.0) / iters
             return 1.0 / iters if v == 6 else (iters + 1.0) / iters
         own = 0 if mode == OP_REF else p.fused
         assert 0 < l["best_us"] and 0 < l["default_us"]

@@ -1864,6 +1864,16 @@ __global__ void __launch_bounds__(NT) k_level1(KArgs<S> a, int iters) {
         const int k = a.nbt[ax][g];
         inc[ax] = k < 0 ? -1 : k - (int)a.off;
     }
+    // the +a neighbour's slot (this cube's own where there is none: its
+    // u_bar, v_bar are published too), read unconditionally, the frozen
+    // values selected after the read (no branch per component)
+    int nsl[3];
+    bool frz[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        nsl[ax] = src[ax] >= 0 ? src[ax] : t;
+        frz[ax] = src[ax] == -2;
+    }
     for (int it = 0; it < iters; ++it) {
         // passes 1-2 (k_dual)
 #pragma unroll
@@ -1876,8 +1886,10 @@ __global__ void __launch_bounds__(NT) k_level1(KArgs<S> a, int iters) {
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    bar[1 + ax][c] = src[ax] == -1 ? ub[c] : (src[ax] == -2 ? fz[ax][c] : lub[c][src[ax]]);
+                for (int c = 0; c < 4; ++c) {
+                    const T x = lub[c][nsl[ax]];
+                    bar[1 + ax][c] = frz[ax] ? fz[ax][c] : x;
+                }
             }
             dual_step(p, q, bar, a.sigma, a.a0, a.a1);
 #pragma unroll
@@ -2363,7 +2375,10 @@ __global__ void __launch_bounds__(256) k_ref_frozen(KArgs<S> a, S* __restrict__ 
 // ---- initialisation
 // tgv.cpp:372-385: u = median, everything else 0, u_bar = u.
 template <class S>
-__global__ void k_reset(KArgs<S> a, uint32_t n) {
+// p, q are cleared at the first npq positions only: a caller passes fewer
+// than n when every later level sets its own p, q before anything reads them
+// (exact-adjoint solves, whose other-depth reads are u_bar, v_bar only).
+__global__ void k_reset(KArgs<S> a, uint32_t n, uint32_t npq) {
     using IO = Io<S>;
     using T = typename IO::T;
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2372,7 +2387,11 @@ __global__ void k_reset(KArgs<S> a, uint32_t n) {
     IO::st(a.s[F_U], g, m);
     IO::st(a.s[F_UB], g, m);
 #pragma unroll
-    for (int f = F_VX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
+    for (int f = F_VX; f <= F_VZ; ++f) IO::st(a.s[f], g, T(0));
+    if (g < (long)npq) {
+#pragma unroll
+        for (int f = F_PX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
+    }
 #pragma unroll
     for (int f = F_VBX; f <= F_VBZ; ++f) IO::st(a.s[f], g, T(0));
 }
@@ -2381,11 +2400,19 @@ __global__ void k_reset(KArgs<S> a, uint32_t n) {
 // = 0, u_bar = u, v_bar = v. A parent found at the same depth holds the code
 // of its own parent and copies itself, so reading it while it is written
 // returns the same value: the gather order does not matter.
-template <class S>
-__global__ void k_parent_init(KArgs<S> a) {
+// FULL = false (the level opens with the head launch of the two-iteration
+// kernel, which reads u, v only): u, v alone. rec (when not null): the
+// previous level's u_bar, v_bar records of k_fz_aos, in the same launch.
+template <class S, bool FULL = true>
+__global__ void k_parent_init(KArgs<S> a, S* __restrict__ rec) {
     using IO = Io<S>;
     using T = typename IO::T;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rec && t < a.fz_n) {
+        const long f = (long)a.fz_off + t;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) IO::st(rec, 4l * t + c, IO::ld(a.s[F_UB + c], f));
+    }
     if (t >= a.n) return;
     const long g = (long)a.off + t;
     const int pi = a.parent[g];
@@ -2396,6 +2423,7 @@ __global__ void k_parent_init(KArgs<S> a) {
     IO::st(a.s[F_VX], g, vx);
     IO::st(a.s[F_VY], g, vy);
     IO::st(a.s[F_VZ], g, vz);
+    if (!FULL) return;
 #pragma unroll
     for (int f = F_PX; f <= F_Q5; ++f) IO::st(a.s[f], g, T(0));
     IO::st(a.s[F_UB], g, u);
@@ -2844,7 +2872,9 @@ void rfz_bufs(tgvx_ctx* c, S** bar, S** dual) {
 }
 
 template <class S>
-void prepare_level(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, size_t li) {
+// launch_fz = false: the exact-adjoint records of the previous level are
+// left to the caller's parent initialisation (same launch).
+void prepare_level(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, size_t li, bool launch_fz = true) {
     Problem& P = c->prob;
     a.fzaos = nullptr;
     a.fz_off = a.fz_n = 0;
@@ -2885,7 +2915,7 @@ void prepare_level(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, size_t li) {
     const Level& prev = P.levels[li - 1];
     a.fz_off = prev.off;
     a.fz_n = prev.n;
-    k_fz_aos<S><<<(prev.n + 255) / 256, 256, 0, c->stream>>>(a, P.d_fzaos.as<S>());
+    if (launch_fz) k_fz_aos<S><<<(prev.n + 255) / 256, 256, 0, c->stream>>>(a, P.d_fzaos.as<S>());
     a.fzaos = P.d_fzaos.as<S>();
 }
 
@@ -3338,7 +3368,8 @@ Set13<S> mix_sets(const Set13<S>& uv, const Set13<S>& pq) {
 // number of two-iteration launches p, q are back in the main set and the
 // last primal step leaves them in place.
 template <class S, bool X>
-void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode, bool frozen, bool cold) {
+void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode, bool frozen, bool cold,
+                    bool need_head) {
     const int wg = L.block > 256 ? 256 : L.block;  // the dual / primal kernels take at most 256
     const unsigned grid = (a.n + wg - 1) / wg;
     Set13<S> cur = main_set(a), oth = alt_set<S>(c);
@@ -3365,6 +3396,7 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op
             i = 2;
         }
     }
+    if (need_head && !head) throw Error(TGVX_E_STATE, "a level initialised for the head launch runs without it");
     if (!head) dual();
     // the last primal step copies p, q into the main set unless they are there
     auto copy = [&] { return cur.f[F_PX] != mainset.f[F_PX] ? 1 : 0; };
@@ -3407,9 +3439,9 @@ void launch_fused_x(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op
 }
 template <class S>
 void launch_fused(tgvx_ctx* c, KArgs<S>& a, int iters, const Launch& L, int op_mode, bool frozen = false,
-                  bool cold = false) {
-    if (L.xcd) launch_fused_x<S, true>(c, a, iters, L, op_mode, frozen, cold);
-    else launch_fused_x<S, false>(c, a, iters, L, op_mode, frozen, cold);
+                  bool cold = false, bool need_head = false) {
+    if (L.xcd) launch_fused_x<S, true>(c, a, iters, L, op_mode, frozen, cold, need_head);
+    else launch_fused_x<S, false>(c, a, iters, L, op_mode, frozen, cold, need_head);
 }
 
 // Cached graph of `iters` iterations of one level.
@@ -3549,7 +3581,7 @@ inline Launch level1_fallback(uint32_t n) { return Launch{5, n >= (1u << 18) ? 1
 
 template <class S>
 void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, const Launch& L0,
-                bool cold = false) {
+                bool cold = false, bool need_head = false) {
     if (iters <= 0 || a.n == 0) return;
     Launch L = L0;
     if (L.variant == 6) {
@@ -3574,6 +3606,9 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
     // from the records; without them, the two kernels
     if (frozen && !a.rfz_bar) L.variant = 0;
     const bool fused = L.variant != 0 && iters >= 2;
+    // a level initialised for the head launch (u, v only) must take it
+    if (need_head && !(fused && (L.variant == 5 || L.variant == 8)))
+        throw Error(TGVX_E_STATE, "a level initialised for the head launch runs without it");
     if (fused) alt_set<S>(c);  // allocate outside any capture
     if (fused && L.variant == 4) halo_tab<S>(c, a, L.block);
     if (fused && (L.variant == 5 || L.variant == 8)) {
@@ -3610,7 +3645,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         if (it == gc.m.end()) {
             hipGraph_t graph;
             TGVX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen, cold);
+            if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen, cold, need_head);
             else
                 for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, frozen);
             TGVX_HIP(hipStreamEndCapture(c->stream, &graph));
@@ -3622,7 +3657,7 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
         TGVX_HIP(hipGraphLaunch(it->second, c->stream));
         return;
     }
-    if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen, cold);
+    if (fused) launch_fused<S>(c, a, iters, L, p->op_mode, frozen, cold, need_head);
     else
         for (int i = 0; i < iters; ++i) launch_iter<S>(c, a, p->op_mode, L, frozen);
     TGVX_HIP(hipGetLastError());
@@ -3631,8 +3666,37 @@ void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool 
 // cold: the level starts from its initial state (u_bar = u, v_bar = v,
 // p = q = 0), which lets the two-iteration kernel open with its head launch
 template <class S>
-void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, bool cold = false) {
-    run_launch<S>(c, a, p, iters, frozen, launch_for(c, p, a.n, frozen), cold);
+void run_iters(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, bool cold = false,
+               bool need_head = false) {
+    run_launch<S>(c, a, p, iters, frozen, launch_for(c, p, a.n, frozen), cold, need_head);
+}
+
+// Whether run_iters(cold) opens this whole level with the head launch: the
+// choice run_launch and launch_fused_x make (variant 5 or 8, directly or as
+// variant 6's fallback above kLevel1 cubes, and the level's tile sets fit).
+template <class S>
+bool opens_with_head(tgvx_ctx* c, const KArgs<S>& a, const tgvx_params* p, int iters) {
+    if (iters < 2 || a.n == 0 || a.off != a.lvl_off || a.n != a.lvl_n) return false;
+    Launch L = launch_for(c, p, a.n, false);
+    if (L.variant == 6) {
+        if (p->op_mode == TGVX_OP_ADJ && a.n <= (uint32_t)kLevel1) return false;
+        L = level1_fallback(a.n);
+    }
+    if (L.variant != 5 && L.variant != 8) return false;
+    return p->op_mode == TGVX_OP_REF ? halo2_tab<S, true>(c, a, L.block).ok : halo2_tab<S>(c, a, L.block).ok;
+}
+
+// The cold initialisation of level li > 0 (tgv.cpp:391-409), with the
+// previous level's u_bar, v_bar records (exact-adjoint mode, left to it by
+// prepare_level(..., false)) in the same launch. uv_only: the level opens
+// with the head launch, which reads u, v alone.
+template <class S>
+void parent_init_cold(tgvx_ctx* c, const KArgs<S>& a, bool uv_only) {
+    S* rec = a.fzaos ? c->prob.d_fzaos.as<S>() : nullptr;
+    const uint32_t m = std::max(a.n, rec ? a.fz_n : 0u);
+    if (m == 0) return;
+    if (uv_only) k_parent_init<S, false><<<(m + 255) / 256, 256, 0, c->stream>>>(a, rec);
+    else k_parent_init<S, true><<<(m + 255) / 256, 256, 0, c->stream>>>(a, rec);
 }
 
 template <class S>
@@ -3707,15 +3771,19 @@ void eval_gap_blocks(tgvx_ctx* c, KArgs<S> a, const tgvx_params* p, size_t li, s
 template <class S>
 void enqueue_sweep(tgvx_ctx* c, const tgvx_params* p, KArgs<S>& a) {
     Problem& P = c->prob;
-    k_reset<S><<<(unsigned)((P.n + 255) / 256), 256, 0, c->stream>>>(a, (uint32_t)P.n);
+    // exact-adjoint levels set their own p, q before anything reads them
+    const uint32_t npq = p->op_mode == TGVX_OP_ADJ ? P.levels[0].n : (uint32_t)P.n;
+    k_reset<S><<<(unsigned)((P.n + 255) / 256), 256, 0, c->stream>>>(a, (uint32_t)P.n, npq);
     for (size_t li = 0; li < P.levels.size(); ++li) {
         const Level& L = P.levels[li];
         a.off = a.lvl_off = L.off;
         a.n = a.lvl_n = L.n;
         set_steps<S>(a, p, L.depth);
-        prepare_level<S>(c, a, p, li);
-        if (li > 0 && L.n > 0) k_parent_init<S><<<(L.n + 255) / 256, 256, 0, c->stream>>>(a);
-        run_iters<S>(c, a, p, P.level_iters.empty() ? p->iters : P.level_iters[li], false, true);
+        prepare_level<S>(c, a, p, li, false);
+        const int it = P.level_iters.empty() ? p->iters : P.level_iters[li];
+        const bool head = li > 0 && opens_with_head<S>(c, a, p, it);
+        if (li > 0) parent_init_cold<S>(c, a, head);
+        run_iters<S>(c, a, p, it, false, true, head);
         if (!p->no_energy) eval_gap_to<S>(c, a, p, P.d_lvlpd.as<double>() + 2 * li);
     }
     TGVX_HIP(hipGetLastError());
@@ -3835,8 +3903,12 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         a.init_stride = init_stride;
     }
     const double t_solve0 = now_ms();
+    // an exact-adjoint solve from the cold initialisation sets every later
+    // level's p, q before anything reads them
+    const bool lean = p->op_mode == TGVX_OP_ADJ && !init_uv && !p->warm_start && !leaves && P.nblocks == 1;
     if (P.n > 0) {
-        k_reset<S><<<(unsigned)((P.n + 255) / 256), 256, 0, c->stream>>>(a, (uint32_t)P.n);
+        const uint32_t npq = lean && !P.levels.empty() ? P.levels[0].n : (uint32_t)P.n;
+        k_reset<S><<<(unsigned)((P.n + 255) / 256), 256, 0, c->stream>>>(a, (uint32_t)P.n, npq);
         TGVX_HIP(hipGetLastError());
     }
     double it_ms = 0.0, gap_ms = 0.0;
@@ -3852,13 +3924,21 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         set_steps<S>(a, p, L.depth);
         const unsigned grid = (L.n + 255) / 256;
         int warm_kept = 0;
-        if (!leaves) prepare_level<S>(c, a, p, li);
+        const bool field = init_uv || p->warm_start;
+        // a cold level's parent initialisation takes the previous level's
+        // records along (prepare_level leaves them to it)
+        if (!leaves) prepare_level<S>(c, a, p, li, field);
         // the level's initialisation; with dual_eps the dual follows from (u, v)
         const bool rec = p->dual_eps > 0.0f && !leaves;
         auto recover = [&] {
             if (rec) k_dual_recover<S><<<grid, 256, 0, c->stream>>>(a, (typename Io<S>::T)p->dual_eps);
         };
-        const bool field = init_uv || p->warm_start;
+        // fixed iterations from the cold initialisation, level by level with
+        // nothing read between: a level that opens with the head launch
+        // needs u, v only
+        const bool fixed_deferred = !to_gap && P.nblocks == 1 && li < (size_t)TGVX_MAX_LEVELS && !leaves;
+        const int fixed_iters = P.level_iters.empty() ? p->iters : P.level_iters[li];
+        bool need_head = false;
         if (p->warm_start) {
             // the network needs the parent level's final state: features now
             warm_init(c, L.off, L.n);
@@ -3869,7 +3949,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
             const int keep = rec && p->warm_dual ? 1 : 0;
             if (p->warm_fallback) {
                 // keep the warm start only where it does not start from a larger gap
-                if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
+                if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a, nullptr);
                 recover();
                 double Pc, Dc, Pw, Dw;
                 eval_gap<S>(c, a, p, &Pc, &Dc);
@@ -3884,7 +3964,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                 }
             } else {
                 if (keep) {
-                    if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
+                    if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a, nullptr);
                     recover();
                 }
                 k_field_init<S><<<grid, 256, 0, c->stream>>>(a, 0, keep);
@@ -3892,7 +3972,11 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
                 warm_kept = 1;
             }
         } else {
-            if (li > 0) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a);
+            if (li > 0) {
+                need_head = fixed_deferred && !rec && opens_with_head<S>(c, a, p, fixed_iters);
+                if (leaves) k_parent_init<S><<<grid, 256, 0, c->stream>>>(a, nullptr);
+                else parent_init_cold<S>(c, a, need_head);
+            }
             recover();
         }
         TGVX_HIP(hipGetLastError());
@@ -3951,9 +4035,9 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         } else if (!to_gap && P.nblocks == 1 && li < (size_t)TGVX_MAX_LEVELS) {
             // fixed iterations: nothing waits for the level; its time and
             // P, D are read after the last level
-            done = P.level_iters.empty() ? p->iters : P.level_iters[li];
+            done = fixed_iters;
             TGVX_HIP(hipEventRecord(c->ev[4 + 2 * li], c->stream));
-            run_iters<S>(c, a, p, done, false, cold);
+            run_iters<S>(c, a, p, done, false, cold, need_head);
             TGVX_HIP(hipEventRecord(c->ev[5 + 2 * li], c->stream));
             if (!p->no_energy) eval_gap_to<S>(c, a, p, P.d_lvlpd.as<double>() + 2 * li);
             deferred[li] = p->no_energy ? 2 : 1;
@@ -4380,7 +4464,7 @@ void solve_stream_t(tgvx_ctx* c, const tgvx_params* p, const tgvx_treetop_leaf* 
     a.sflag = P.d_sflag.as<uint8_t>();
     for (int i = 0; i < 4; ++i) a.fz[i] = P.d_save[i].as<S>();
     // tgv.cpp:821-845: u = median, everything else 0
-    if (n) k_reset<S><<<grid_all, 256, 0, c->stream>>>(a, n);
+    if (n) k_reset<S><<<grid_all, 256, 0, c->stream>>>(a, n, n);
     uint32_t max_level = 1;
     for (const Level& L : P.levels) max_level = std::max(max_level, L.n);
     DevBuf tmp;
