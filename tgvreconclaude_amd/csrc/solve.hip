@@ -1280,6 +1280,12 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     uint32_t w0 = 0u, w1 = 0u;
     T od0[4] = {T(0), T(0), T(0), T(0)}, od1[4] = {T(0), T(0), T(0), T(0)};
     T fz0[3][4], fz1[3][4], in1[3][4];
+    // reference rules (tiles of 1024, single precision): slot 0's -a
+    // neighbours of other depths, whose p, q both primal steps read, are
+    // gathered while staging too (the exact-adjoint sources are all in the
+    // tile's sets)
+    constexpr bool PF0 = PF && REF;
+    T in0[3][4];
     if (l0) {
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
@@ -1293,6 +1299,11 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
             if (b0[ax] <= -2) frozen(b0[ax], fz0[ax], ax, g0);
+        if constexpr (PF0) {
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax)
+                if (c0[ax] <= -2) gather_in(ax, -2L - c0[ax], g0, in0[ax]);
+        }
     }
     if (l1) {
 #pragma unroll
@@ -1328,7 +1339,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     // ---- primal n+1 over PA (HEAD: none, the launch starts with the dual)
     T nw0[4] = {T(0), T(0), T(0), T(0)}, nw1[4] = {T(0), T(0), T(0), T(0)};
     if constexpr (!HEAD) {
-        if (l0) primal(tid, g0, w0, od0, c0, nw0, false, in1);
+        if (l0) primal(tid, g0, w0, od0, c0, nw0, PF0, in0);
         if (l1) primal(s1, g1, w1, od1, c1, nw1, PF, in1);
         for (int s = tid + 2 * TILE; s < nA; s += TILE) {
             long g;
@@ -1362,7 +1373,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     // ---- primal n+2 over PC (HEAD: primal 1, from the staged (u, v); a
     // slot's own u_bar is read before its primal step replaces it)
     T own[4] = {T(0), T(0), T(0), T(0)};
-    if (l0) primal(tid, g0, w0, HEAD ? od0 : nw0, c0, own, false, in1);
+    if (l0) primal(tid, g0, w0, HEAD ? od0 : nw0, c0, own, PF0, in0);
     if (l1 && s1 < nP) {
         T x[4];
         primal(s1, g1, w1, HEAD ? od1 : nw1, c1, x, false, in1);
