@@ -2784,9 +2784,66 @@ __global__ void k_tables_geo(const uint64_t* __restrict__ codes, const int32_t* 
     parent[g] = d > 0 ? pair_lookup(dc, dd, dp, U, ancestor(c, d - 1), d - 1) : -1;
 }
 
+// The state of position g as the records of input cube perm[g]: (u, v, p, q)
+// in st (13 values), (u_bar, v_bar) in bar (4), widened to the arithmetic type.
+template <class S>
+__global__ void k_state_out(KArgs<S> a, const uint32_t* __restrict__ perm, uint32_t n,
+                            typename Io<S>::T* __restrict__ st, typename Io<S>::T* __restrict__ bar) {
+    using IO = Io<S>;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const size_t i = perm[g];
+    if (st) {
+#pragma unroll
+        for (int f = 0; f < 13; ++f) st[13 * i + f] = IO::ld(a.s[f], g);
+    }
+    if (bar) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) bar[4 * i + k] = IO::ld(a.s[F_UB + k], g);
+    }
+}
+
 __global__ void k_fill_i32(int32_t* p, uint32_t n, int32_t v) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) p[g] = v;
+}
+
+// The per-position inputs of a load, from the cubes and histograms in input
+// order and the position -> input permutation: code, depth, the histogram
+// (nb bins) and the packed word of vote total and median bin
+// (histogram.cpp:21-31: the first k with 2 * cumulative >= total). A total
+// above the word's 24 bits sets *bad.
+__global__ void k_load_positions(const tgvx_cube* __restrict__ cubes, const uint32_t* __restrict__ bins,
+                                 const uint32_t* __restrict__ perm, uint32_t n, int nb, uint64_t* __restrict__ code,
+                                 int32_t* __restrict__ depth, uint32_t* __restrict__ pbins,
+                                 uint32_t* __restrict__ info, int32_t* __restrict__ bad) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t i = perm[g];
+    const tgvx_cube cb = cubes[i];
+    code[g] = ((uint64_t)cb.hi << 32) | cb.lo;
+    depth[g] = cb.depth;
+    const uint32_t* b = bins + (size_t)i * nb;
+    uint32_t* o = pbins + (size_t)g * nb;
+    uint64_t tot = 0;
+    for (int k = 0; k < nb; ++k) {
+        const uint32_t x = b[k];
+        o[k] = x;
+        tot += x;
+    }
+    if (tot > InfoWord::kWMax) {
+        atomicOr(bad, 1);
+        tot = 0;
+    }
+    uint32_t kmed = 0, cum = 0;
+    for (int k = 0; k < nb; ++k) {
+        cum += b[k];
+        if (cum * 2u >= (uint32_t)tot) {
+            kmed = (uint32_t)k;
+            break;
+        }
+    }
+    info[g] = (uint32_t)tot | (kmed << 24);
 }
 
 // The depth-aware table is one-to-one within a level, so each target has at
@@ -4627,38 +4684,38 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
         }
         uoff[b + 1] = (uint32_t)ucode.size();
     }
-    // per-position code, depth, vote total and median bin
-    std::vector<uint64_t> pcode(n);
-    std::vector<int32_t> pdepth(n);
-    std::vector<uint32_t> info(n);
-    for (uint64_t g = 0; g < n; ++g) {
-        const uint32_t i = P.perm[g];
-        pcode[g] = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
-        pdepth[g] = cubes[i].depth;
-        const uint32_t* b = bins + (size_t)i * nb;
-        uint64_t tot = 0;
-        for (int k = 0; k < nb; ++k) tot += b[k];
-        if (tot > InfoWord::kWMax) throw Error(TGVX_E_ARG, "vote total of a cube exceeds 2^23 - 1");
-        // median bin (histogram.cpp:21-31): first k with 2 * cumulative >= total
-        uint32_t kmed = 0, cum = 0;
-        for (int k = 0; k < nb; ++k) {
-            cum += b[k];
-            if (cum * 2u >= (uint32_t)tot) {
-                kmed = (uint32_t)k;
-                break;
-            }
-        }
-        info[g] = (uint32_t)tot | (kmed << 24);
-    }
     auto up = [&](DevBuf& b, const void* src, size_t bytes) {
         b.ensure(bytes);
         if (bytes) TGVX_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
     };
+    // per-position code, depth, histogram, vote total and median bin, on the
+    // device from the inputs in input order and the permutation (the position
+    // -> input map stays resident: the frozen-border mode's in-leaf test
+    // reads it). The host buffers are read by the copies, which complete
+    // before the synchronisation below.
     DevBuf& d_code = P.d_code;
     DevBuf& d_depth = P.d_depth;
-    DevBuf d_uc, d_ui, d_uoff, d_doff;
-    up(d_code, pcode.data(), 8 * n);
-    up(d_depth, pdepth.data(), 4 * n);
+    DevBuf d_uc, d_ui, d_uoff, d_doff, d_cubes, d_rbins, d_bad;
+    up(P.d_orig, P.perm.data(), 4 * std::max<uint64_t>(n, 1));
+    d_code.ensure(8 * std::max<uint64_t>(n, 1));
+    d_depth.ensure(4 * std::max<uint64_t>(n, 1));
+    P.d_info.ensure(4 * std::max<uint64_t>(n, 1));
+    P.d_bins.ensure(4 * (size_t)std::max<uint64_t>(n, 1) * nb);
+    if (n > 0) {
+        up(d_cubes, cubes, sizeof(tgvx_cube) * n);
+        up(d_rbins, bins, 4 * (size_t)n * nb);
+        d_bad.ensure(sizeof(int32_t));
+        k_fill_i32<<<1, 64, 0, c->stream>>>(d_bad.as<int32_t>(), 1u, 0);
+        k_load_positions<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+            d_cubes.as<tgvx_cube>(), d_rbins.as<uint32_t>(), P.d_orig.as<uint32_t>(), (uint32_t)n, nb,
+            d_code.as<uint64_t>(), d_depth.as<int32_t>(), P.d_bins.as<uint32_t>(), P.d_info.as<uint32_t>(),
+            d_bad.as<int32_t>());
+        TGVX_HIP(hipGetLastError());
+        int32_t bad = 0;
+        TGVX_HIP(hipMemcpyAsync(&bad, d_bad.p, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+        TGVX_HIP(hipStreamSynchronize(c->stream));
+        if (bad) throw Error(TGVX_E_ARG, "vote total of a cube exceeds 2^23 - 1");
+    }
     up(d_uc, ucode.data(), 8 * std::max<size_t>(ucode.size(), 1));
     up(d_ui, uidx.data(), 4 * std::max<size_t>(uidx.size(), 1));
     const bool multi = nblocks > 1;
@@ -4685,11 +4742,15 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     for (uint32_t b = 0; b < nblocks; ++b) {
         std::vector<std::pair<uint64_t, uint32_t>> pk;
         pk.reserve(bs[b + 1] - bs[b]);
-        for (uint64_t i = bs[b]; i < bs[b + 1]; ++i)
-            pk.push_back({((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i});
-        std::stable_sort(pk.begin(), pk.end(), [&](const auto& x, const auto& y) {
+        auto before = [&](const std::pair<uint64_t, uint32_t>& x, const std::pair<uint64_t, uint32_t>& y) {
             return x.first != y.first ? x.first < y.first : cubes[x.second].depth < cubes[y.second].depth;
-        });
+        };
+        bool sorted = true;  // the octree builder's output already is
+        for (uint64_t i = bs[b]; i < bs[b + 1]; ++i) {
+            pk.push_back({((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i});
+            if (pk.size() > 1 && before(pk.back(), pk[pk.size() - 2])) sorted = false;
+        }
+        if (!sorted) std::stable_sort(pk.begin(), pk.end(), before);
         for (size_t i = 0; i < pk.size(); ++i) {
             if (i + 1 < pk.size() && pk[i + 1].first == pk[i].first &&
                 cubes[pk[i + 1].second].depth == cubes[pk[i].second].depth)
@@ -4705,14 +4766,6 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     up(d_dc, dcode.data(), 8 * std::max<size_t>(dcode.size(), 1));
     up(d_dd, ddepth.data(), 4 * std::max<size_t>(ddepth.size(), 1));
     up(d_dp, dpos.data(), 4 * std::max<size_t>(dpos.size(), 1));
-    up(P.d_info, info.data(), 4 * std::max<uint64_t>(n, 1));
-    // histograms in position order (read by the warm-start features); the
-    // host copy lives until the stream is drained at the end of the load,
-    // since the copy is only ordered on the stream
-    std::vector<uint32_t> pb((size_t)std::max<uint64_t>(n, 1) * nb, 0u);
-    for (uint64_t g = 0; g < n; ++g)
-        std::memcpy(&pb[(size_t)g * nb], bins + (size_t)P.perm[g] * nb, sizeof(uint32_t) * nb);
-    up(P.d_bins, pb.data(), 4 * pb.size());
     for (int ax = 0; ax < 3; ++ax) {
         P.d_nba[ax].ensure(4 * std::max<uint64_t>(n, 1));
         P.d_nbam[ax].ensure(4 * std::max<uint64_t>(n, 1));
@@ -4743,8 +4796,6 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     for (const Level& L : P.levels) max_level = std::max(max_level, L.n);
     P.d_fzaos.ensure(4 * P.elem * (size_t)max_level);
     for (int i = 0; i < 4; ++i) P.d_save[i].ensure(P.elem * std::max<uint64_t>(n, 1));
-    // position -> input index (the frozen-border mode's in-leaf test reads it)
-    up(P.d_orig, P.perm.data(), 4 * std::max<uint64_t>(n, 1));
     TGVX_HIP(hipStreamSynchronize(c->stream));
     P.loaded = true;
     (void)t0;
@@ -5164,28 +5215,32 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
 void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out) {
     Problem& P = c->prob;
     if (!P.loaded) throw Error(TGVX_E_STATE, "no problem loaded");
-    const size_t n = P.n, e = P.elem;
-    std::vector<uint8_t> col(e * n);
-    const size_t out_e = P.precision == TGVX_FP64 ? 8 : 4;
-    bool half = P.precision == TGVX_BF16;  // the current field holds BF16
-    auto widen = [&](const uint8_t* src, uint64_t g, uint8_t* dst) {
-        if (half) {
-            uint16_t h;
-            std::memcpy(&h, src + 2 * g, 2);
-            const uint32_t u = (uint32_t)h << 16;
-            std::memcpy(dst, &u, 4);
-        } else {
-            std::memcpy(dst, src + e * g, e);
-        }
+    const size_t n = P.n;
+    if (n == 0 || (!state_out && !ubar_out)) return;
+    // records in input order on the device (one scattered write per cube),
+    // then one copy each
+    auto go = [&](auto tag) {
+        using S = decltype(tag);
+        using T = typename Io<S>::T;
+        tgvx_params q;
+        tgvx_default_params(&q);
+        const KArgs<S> a = make_args<S>(c, &q);
+        DevBuf st, bar;
+        if (state_out) st.ensure(13 * sizeof(T) * n);
+        if (ubar_out) bar.ensure(4 * sizeof(T) * n);
+        k_state_out<S><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+            a, P.d_orig.as<uint32_t>(), (uint32_t)n, state_out ? st.as<T>() : nullptr, ubar_out ? bar.as<T>() : nullptr);
+        TGVX_HIP(hipGetLastError());
+        if (state_out)
+            TGVX_HIP(hipMemcpyAsync(state_out, st.p, 13 * sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+        if (ubar_out) TGVX_HIP(hipMemcpyAsync(ubar_out, bar.p, 4 * sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+        TGVX_HIP(hipStreamSynchronize(c->stream));
     };
-    for (int f = 0; f < 17; ++f) {
-        const bool is_bar = f >= F_UB;
-        uint8_t* out = (uint8_t*)(is_bar ? ubar_out : state_out);
-        if (!out) continue;
-        const int rec = is_bar ? 4 : 13, slot = is_bar ? f - F_UB : f;
-        half = P.precision == TGVX_BF16 || (P.precision == TGVX_BF16_DUAL && f >= F_PX && f <= F_Q5);
-        if (n) TGVX_HIP(hipMemcpy(col.data(), P.d_state[f].p, (half ? 2 : e) * n, hipMemcpyDeviceToHost));
-        for (uint64_t g = 0; g < n; ++g) widen(col.data(), g, out + ((size_t)P.perm[g] * rec + slot) * out_e);
+    switch (P.precision) {
+        case TGVX_FP64: go(double{}); break;
+        case TGVX_FP32: go(float{}); break;
+        case TGVX_BF16_DUAL: go(mixs{}); break;
+        default: go(bf16s{}); break;
     }
 }
 
