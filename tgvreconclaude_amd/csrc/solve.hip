@@ -2691,13 +2691,15 @@ __global__ void __launch_bounds__(kGapBlock) k_sum_pairs(const double* part, uin
 // Lookup in the code map: ucode ascending, uidx = position of the last input
 // cube with that code (the reference's unordered_map "last writer wins").
 __device__ inline int code_lookup(const uint64_t* uc, const int32_t* ui, uint32_t U, uint64_t key) {
+    // the last entry with the key (the arrays may repeat a code: the last
+    // writer wins, as in the reference's map)
     uint32_t lo = 0, hi = U;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (uc[mid] < key) lo = mid + 1;
+        if (uc[mid] <= key) lo = mid + 1;
         else hi = mid;
     }
-    return (lo < U && uc[lo] == key) ? ui[lo] : -1;
+    return (lo > 0 && uc[lo - 1] == key) ? ui[lo - 1] : -1;
 }
 
 // Several independent octrees (blocks) may share one problem: each position's
@@ -2734,13 +2736,13 @@ __global__ void k_tables(const uint64_t* __restrict__ codes, const int32_t* __re
 // else -1; duplicates of a (code, depth) pair other than the last get none.
 __device__ inline int pair_lookup(const uint64_t* dc, const int32_t* dd, const int32_t* dp, uint32_t U, uint64_t key,
                                   int depth) {
-    uint32_t lo = 0, hi = U;
+    uint32_t lo = 0, hi = U;  // the last entry with the pair
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (dc[mid] < key || (dc[mid] == key && dd[mid] < depth)) lo = mid + 1;
+        if (dc[mid] < key || (dc[mid] == key && dd[mid] <= depth)) lo = mid + 1;
         else hi = mid;
     }
-    return (lo < U && dc[lo] == key && dd[lo] == depth) ? dp[lo] : -1;
+    return (lo > 0 && dc[lo - 1] == key && dd[lo - 1] == depth) ? dp[lo - 1] : -1;
 }
 
 __global__ void k_tables_geo(const uint64_t* __restrict__ codes, const int32_t* __restrict__ depth, uint32_t n,
@@ -2812,17 +2814,24 @@ __global__ void k_fill_i32(int32_t* p, uint32_t n, int32_t v) {
 // order and the position -> input permutation: code, depth, the histogram
 // (nb bins) and the packed word of vote total and median bin
 // (histogram.cpp:21-31: the first k with 2 * cumulative >= total). A total
-// above the word's 24 bits sets *bad.
+// above the word's 24 bits sets *bad. icode, idepth (when not null): code and
+// depth in input order.
 __global__ void k_load_positions(const tgvx_cube* __restrict__ cubes, const uint32_t* __restrict__ bins,
                                  const uint32_t* __restrict__ perm, uint32_t n, int nb, uint64_t* __restrict__ code,
                                  int32_t* __restrict__ depth, uint32_t* __restrict__ pbins,
-                                 uint32_t* __restrict__ info, int32_t* __restrict__ bad) {
+                                 uint32_t* __restrict__ info, int32_t* __restrict__ bad,
+                                 uint64_t* __restrict__ icode, int32_t* __restrict__ idepth) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const uint32_t i = perm[g];
     const tgvx_cube cb = cubes[i];
-    code[g] = ((uint64_t)cb.hi << 32) | cb.lo;
+    const uint64_t cc = ((uint64_t)cb.hi << 32) | cb.lo;
+    code[g] = cc;
     depth[g] = cb.depth;
+    if (icode) {  // the same in input order
+        icode[i] = cc;
+        idepth[i] = cb.depth;
+    }
     const uint32_t* b = bins + (size_t)i * nb;
     uint32_t* o = pbins + (size_t)g * nb;
     uint64_t tot = 0;
@@ -4661,28 +4670,53 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
         }
         for (uint32_t k = 0; k < nblocks; ++k) sg[k + 1] += sg[k];
     }
-    // code map of each block: sorted codes, last writer (largest input index)
-    std::vector<uint64_t> ucode;
-    std::vector<int32_t> uidx;
-    std::vector<uint32_t> uoff(nblocks + 1, 0);
-    ucode.reserve(n);
-    uidx.reserve(n);
-    for (uint32_t b = 0; b < nblocks; ++b) {
-        std::vector<std::pair<uint64_t, uint32_t>> kv;
-        kv.reserve(bs[b + 1] - bs[b]);
-        bool sorted = true;
-        for (uint64_t i = bs[b]; i < bs[b + 1]; ++i) {
-            kv.push_back({((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i});
-            if (kv.size() > 1 && kv.back().first < kv[kv.size() - 2].first) sorted = false;
+    // The code maps the lookups search: per block, codes ascending, the last
+    // input cube of each code (code-only map, the reference's last writer)
+    // and of each (code, depth) pair (depth-aware map). Input in (code,
+    // depth) order within each block -- the octree builder's output -- is
+    // searched as it is (the lookups take the last of equal entries); other
+    // input is sorted and deduplicated here.
+    bool in_order = true;
+    for (uint32_t b = 0; b < nblocks && in_order; ++b)
+        for (uint64_t i = bs[b] + 1; i < bs[b + 1]; ++i) {
+            const uint64_t x = ((uint64_t)cubes[i - 1].hi << 32) | cubes[i - 1].lo,
+                           y = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
+            if (y < x || (y == x && cubes[i].depth < cubes[i - 1].depth)) {
+                in_order = false;
+                break;
+            }
         }
-        if (!sorted)
+    std::vector<uint64_t> ucode, dcode;
+    std::vector<int32_t> uidx, ddepth, dpos;
+    std::vector<uint32_t> uoff(nblocks + 1, 0), doff(nblocks + 1, 0);
+    if (in_order) {
+        for (uint32_t b = 0; b <= nblocks; ++b) uoff[b] = doff[b] = (uint32_t)bs[b];
+    } else {
+        for (uint32_t b = 0; b < nblocks; ++b) {
+            std::vector<std::pair<uint64_t, uint32_t>> kv;
+            kv.reserve(bs[b + 1] - bs[b]);
+            for (uint64_t i = bs[b]; i < bs[b + 1]; ++i)
+                kv.push_back({((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i});
             std::stable_sort(kv.begin(), kv.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
-        for (size_t i = 0; i < kv.size(); ++i) {
-            if (i + 1 < kv.size() && kv[i + 1].first == kv[i].first) continue;
-            ucode.push_back(kv[i].first);
-            uidx.push_back((int32_t)P.inv[kv[i].second]);
+            for (size_t i = 0; i < kv.size(); ++i) {
+                if (i + 1 < kv.size() && kv[i + 1].first == kv[i].first) continue;
+                ucode.push_back(kv[i].first);
+                uidx.push_back((int32_t)P.inv[kv[i].second]);
+            }
+            uoff[b + 1] = (uint32_t)ucode.size();
+            std::stable_sort(kv.begin(), kv.end(), [&](const auto& x, const auto& y) {
+                return x.first != y.first ? x.first < y.first : cubes[x.second].depth < cubes[y.second].depth;
+            });
+            for (size_t i = 0; i < kv.size(); ++i) {
+                if (i + 1 < kv.size() && kv[i + 1].first == kv[i].first &&
+                    cubes[kv[i + 1].second].depth == cubes[kv[i].second].depth)
+                    continue;
+                dcode.push_back(kv[i].first);
+                ddepth.push_back(cubes[kv[i].second].depth);
+                dpos.push_back((int32_t)P.inv[kv[i].second]);
+            }
+            doff[b + 1] = (uint32_t)dcode.size();
         }
-        uoff[b + 1] = (uint32_t)ucode.size();
     }
     auto up = [&](DevBuf& b, const void* src, size_t bytes) {
         b.ensure(bytes);
@@ -4695,8 +4729,13 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     // before the synchronisation below.
     DevBuf& d_code = P.d_code;
     DevBuf& d_depth = P.d_depth;
-    DevBuf d_uc, d_ui, d_uoff, d_doff, d_cubes, d_rbins, d_bad;
+    DevBuf d_uc, d_ui, d_uoff, d_doff, d_cubes, d_rbins, d_bad, d_icode, d_idepth, d_inv, d_dc, d_dd, d_dp;
     up(P.d_orig, P.perm.data(), 4 * std::max<uint64_t>(n, 1));
+    if (in_order) {
+        d_icode.ensure(8 * std::max<uint64_t>(n, 1));
+        d_idepth.ensure(4 * std::max<uint64_t>(n, 1));
+        up(d_inv, P.inv.data(), 4 * std::max<uint64_t>(n, 1));
+    }
     d_code.ensure(8 * std::max<uint64_t>(n, 1));
     d_depth.ensure(4 * std::max<uint64_t>(n, 1));
     P.d_info.ensure(4 * std::max<uint64_t>(n, 1));
@@ -4709,19 +4748,33 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
         k_load_positions<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
             d_cubes.as<tgvx_cube>(), d_rbins.as<uint32_t>(), P.d_orig.as<uint32_t>(), (uint32_t)n, nb,
             d_code.as<uint64_t>(), d_depth.as<int32_t>(), P.d_bins.as<uint32_t>(), P.d_info.as<uint32_t>(),
-            d_bad.as<int32_t>());
+            d_bad.as<int32_t>(), in_order ? d_icode.as<uint64_t>() : nullptr,
+            in_order ? d_idepth.as<int32_t>() : nullptr);
         TGVX_HIP(hipGetLastError());
         int32_t bad = 0;
         TGVX_HIP(hipMemcpyAsync(&bad, d_bad.p, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
         TGVX_HIP(hipStreamSynchronize(c->stream));
         if (bad) throw Error(TGVX_E_ARG, "vote total of a cube exceeds 2^23 - 1");
     }
-    up(d_uc, ucode.data(), 8 * std::max<size_t>(ucode.size(), 1));
-    up(d_ui, uidx.data(), 4 * std::max<size_t>(uidx.size(), 1));
+    if (!in_order) {
+        up(d_uc, ucode.data(), 8 * std::max<size_t>(ucode.size(), 1));
+        up(d_ui, uidx.data(), 4 * std::max<size_t>(uidx.size(), 1));
+        up(d_dc, dcode.data(), 8 * std::max<size_t>(dcode.size(), 1));
+        up(d_dd, ddepth.data(), 4 * std::max<size_t>(ddepth.size(), 1));
+        up(d_dp, dpos.data(), 4 * std::max<size_t>(dpos.size(), 1));
+    }
+    const uint64_t* const map_code = in_order ? d_icode.as<uint64_t>() : d_uc.as<uint64_t>();
+    const int32_t* const map_pos = in_order ? d_inv.as<int32_t>() : d_ui.as<int32_t>();
+    const uint32_t map_n = in_order ? (uint32_t)n : (uint32_t)ucode.size();
+    const uint64_t* const pair_code = in_order ? d_icode.as<uint64_t>() : d_dc.as<uint64_t>();
+    const int32_t* const pair_depth = in_order ? d_idepth.as<int32_t>() : d_dd.as<int32_t>();
+    const int32_t* const pair_pos = in_order ? d_inv.as<int32_t>() : d_dp.as<int32_t>();
+    const uint32_t pair_n = in_order ? (uint32_t)n : (uint32_t)dcode.size();
     const bool multi = nblocks > 1;
     if (multi) {
         up(P.d_blk, pblk.data(), 4 * n);
         up(d_uoff, uoff.data(), 4 * uoff.size());
+        up(d_doff, doff.data(), 4 * doff.size());
         P.d_bact.ensure(nblocks);
     }
     const int32_t* kblk = multi ? P.d_blk.as<int32_t>() : nullptr;
@@ -4729,43 +4782,11 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     P.d_parent.ensure(4 * std::max<uint64_t>(n, 1));
     if (n > 0) {
         k_tables<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
-            d_code.as<uint64_t>(), d_depth.as<int32_t>(), (uint32_t)n, d_uc.as<uint64_t>(), d_ui.as<int32_t>(),
-            (uint32_t)ucode.size(), kblk, d_uoff.as<uint32_t>(), P.d_nb[0].as<int32_t>(), P.d_nb[1].as<int32_t>(), P.d_nb[2].as<int32_t>(),
+            d_code.as<uint64_t>(), d_depth.as<int32_t>(), (uint32_t)n, map_code, map_pos, map_n, kblk,
+            d_uoff.as<uint32_t>(), P.d_nb[0].as<int32_t>(), P.d_nb[1].as<int32_t>(), P.d_nb[2].as<int32_t>(),
             P.d_nb[3].as<int32_t>(), P.d_nb[4].as<int32_t>(), P.d_nb[5].as<int32_t>(), P.d_parent.as<int32_t>());
         TGVX_HIP(hipGetLastError());
     }
-    // depth-aware table (exact-adjoint mode and the gap): per block, (code,
-    // depth) pairs ascending with the position of the last input cube of each
-    std::vector<uint64_t> dcode;
-    std::vector<int32_t> ddepth, dpos;
-    std::vector<uint32_t> doff(nblocks + 1, 0);
-    for (uint32_t b = 0; b < nblocks; ++b) {
-        std::vector<std::pair<uint64_t, uint32_t>> pk;
-        pk.reserve(bs[b + 1] - bs[b]);
-        auto before = [&](const std::pair<uint64_t, uint32_t>& x, const std::pair<uint64_t, uint32_t>& y) {
-            return x.first != y.first ? x.first < y.first : cubes[x.second].depth < cubes[y.second].depth;
-        };
-        bool sorted = true;  // the octree builder's output already is
-        for (uint64_t i = bs[b]; i < bs[b + 1]; ++i) {
-            pk.push_back({((uint64_t)cubes[i].hi << 32) | cubes[i].lo, (uint32_t)i});
-            if (pk.size() > 1 && before(pk.back(), pk[pk.size() - 2])) sorted = false;
-        }
-        if (!sorted) std::stable_sort(pk.begin(), pk.end(), before);
-        for (size_t i = 0; i < pk.size(); ++i) {
-            if (i + 1 < pk.size() && pk[i + 1].first == pk[i].first &&
-                cubes[pk[i + 1].second].depth == cubes[pk[i].second].depth)
-                continue;
-            dcode.push_back(pk[i].first);
-            ddepth.push_back(cubes[pk[i].second].depth);
-            dpos.push_back((int32_t)P.inv[pk[i].second]);
-        }
-        doff[b + 1] = (uint32_t)dcode.size();
-    }
-    if (multi) up(d_doff, doff.data(), 4 * doff.size());
-    DevBuf d_dc, d_dd, d_dp;
-    up(d_dc, dcode.data(), 8 * std::max<size_t>(dcode.size(), 1));
-    up(d_dd, ddepth.data(), 4 * std::max<size_t>(ddepth.size(), 1));
-    up(d_dp, dpos.data(), 4 * std::max<size_t>(dpos.size(), 1));
     for (int ax = 0; ax < 3; ++ax) {
         P.d_nba[ax].ensure(4 * std::max<uint64_t>(n, 1));
         P.d_nbam[ax].ensure(4 * std::max<uint64_t>(n, 1));
@@ -4775,8 +4796,7 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     if (n > 0) {
         const unsigned grid = (unsigned)((n + 255) / 256);
         k_tables_geo<<<grid, 256, 0, c->stream>>>(d_code.as<uint64_t>(), d_depth.as<int32_t>(), (uint32_t)n,
-                                                  d_dc.as<uint64_t>(), d_dd.as<int32_t>(), d_dp.as<int32_t>(),
-                                                  (uint32_t)dcode.size(), kblk, d_doff.as<uint32_t>(),
+                                                  pair_code, pair_depth, pair_pos, pair_n, kblk, d_doff.as<uint32_t>(),
                                                   P.d_nba[0].as<int32_t>(),
                                                   P.d_nba[1].as<int32_t>(), P.d_nba[2].as<int32_t>(),
                                                   P.d_nbam[0].as<int32_t>(), P.d_nbam[1].as<int32_t>(),
