@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <numeric>
 #include <set>
@@ -3874,10 +3875,13 @@ bool sweep_graph(tgvx_ctx* c, const tgvx_params* p, tgvx_report* rep) {
         return false;
     for (const Level& L : P.levels)
         if (launch_for(c, p, L.n, false).variant == 7) return false;  // cooperative launches stay eager
-    std::vector<char> key(sizeof(tgvx_params) + sizeof(int32_t) * P.level_iters.size());
-    std::memcpy(key.data(), p, sizeof(tgvx_params));
+    // the parameters' bytes up to the last field (the struct has no inner
+    // padding; its tail padding may hold anything) and the per-level counts
+    constexpr size_t kp = offsetof(tgvx_params, no_energy) + sizeof(int32_t);
+    std::vector<char> key(kp + sizeof(int32_t) * P.level_iters.size());
+    std::memcpy(key.data(), p, kp);
     if (!P.level_iters.empty())
-        std::memcpy(key.data() + sizeof(tgvx_params), P.level_iters.data(), sizeof(int32_t) * P.level_iters.size());
+        std::memcpy(key.data() + kp, P.level_iters.data(), sizeof(int32_t) * P.level_iters.size());
     GraphCache& gc = g_graphs[c];
     if (!(gc.sweep && gc.sweep_key == key)) {
         if (gc.seen_key != key) {
