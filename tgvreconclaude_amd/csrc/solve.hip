@@ -844,6 +844,9 @@ __host__ __device__ constexpr int log2_ceil(int x) {
     return l;
 }
 
+// Built by k_halo2_table in tile order; halo2_tab then reorders the rows into
+// launch order (LDS class by class) and replaces each count record's last
+// entry by its tile id.
 struct Halo2Tab {
     const int32_t* cnt;    // per tile: nPC, nDB, nPA, failed (the tile's sets exceed the cap)
     const int32_t* list;   // per tile, (kHalo2Cap - 1) * TILE: position of slot TILE + k
@@ -1057,7 +1060,10 @@ __device__ inline void lds_dma4(const float* g, float* l_base) {
 // info words and frozen neighbours).
 extern __shared__ uint4 k_fused2_lds[];
 
-// tiles: the tile ids this launch runs (the tiles of one LDS class).
+// h: the tables of the tiles this launch runs (one LDS class), in launch
+// order: workgroup k (XCD-contiguous or plain) runs the tile of record k,
+// whose count record holds nPC, nDB, nPA and the tile id, so that every table
+// read of the staging depends on the workgroup id alone.
 // REF: the reference's update rules (k_dual and k_primal_ref's expressions):
 // the primal step reads p, q of the -a neighbours of the reference table and
 // always its own; a -a neighbour of another depth gives its p, q from the
@@ -1068,8 +1074,8 @@ extern __shared__ uint4 k_fused2_lds[];
 // and dst receives (u^1, v^1, p^2, q^2); src's p, q are not read, the
 // launch's first dual step starts from zero.
 template <class S, int TILE, bool XCD = true, bool REF = false, bool HEAD = false>
-__global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13<S> dst, Halo2Tab h,
-                                                 const int32_t* __restrict__ tiles, int sa, int sx) {
+__global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13<S> dst, Halo2Tab h, int sa,
+                                                 int sx) {
     using IO = Io<S>;
     using T = typename IO::T;
     constexpr int CA = kHalo2Cap * TILE, NH = CA - TILE;
@@ -1080,15 +1086,17 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     // workgroup instead of a division per prox step); read after the
     // staging barrier
     __shared__ T s_med[32];
-    const uint32_t tile = (uint32_t)tiles[tile_id<XCD>()];
+    const long kp = tile_id<XCD>();
+    const int32_t* const hd = h.cnt + 4 * kp;
+    const int nP = hd[0], nD = hd[1], nA = hd[2];
+    const uint32_t tile = (uint32_t)hd[3];
     const int tid = (int)threadIdx.x;
     if (tid < 32) s_med[tid] = median_of<T>(((uint32_t)tid << 24) | 1u, a.nbins);
     const long lo = (long)a.off + (long)tile * TILE;
     const int cntT = (int)min((uint32_t)TILE, a.n - tile * TILE);
-    const int nP = h.cnt[4 * tile], nD = h.cnt[4 * tile + 1], nA = h.cnt[4 * tile + 2];
-    const int32_t* const hl = h.list + (long)tile * NH;
-    const int32_t* const bs = h.bslot + (long)tile * 3 * CA;
-    const int32_t* const cs = h.cslot + (long)tile * 3 * CA;
+    const int32_t* const hl = h.list + kp * NH;
+    const int32_t* const bs = h.bslot + kp * 3 * CA;
+    const int32_t* const cs = h.cslot + kp * 3 * CA;
     auto live = [&](int s, long& g) {
         if (s >= cntT && s < TILE) return false;
         g = s < TILE ? lo + s : (long)hl[s - TILE];
@@ -1274,10 +1282,40 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     // LDS-only (in double precision these registers would spill).
     constexpr bool PF = TILE >= 1024 && sizeof(T) == 4;
     const int s1 = tid + TILE;
+    // the table entries of slot 0, read at once (within the tile's rows,
+    // whatever its size), and those of slot 1 with its position where the
+    // tile has that slot: the values they address are then one dependent
+    // read away from the count record
+    int32_t h1 = 0;
+    int t_b0[3], t_c0[3], t_b1[3] = {-1, -1, -1}, t_c1[3] = {-1, -1, -1};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        t_b0[ax] = bs[ax * CA + tid];
+        t_c0[ax] = cs[ax * CA + tid];
+    }
+    if (s1 < nA) {
+        h1 = hl[tid];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            t_b1[ax] = bs[ax * CA + s1];
+            t_c1[ax] = cs[ax * CA + s1];
+        }
+    }
     long g0 = 0, g1 = 0;
     const bool l0 = live(tid, g0);
-    const bool l1 = s1 < nA && live(s1, g1);
-    int b0[3] = {-1, -1, -1}, c0[3] = {-1, -1, -1}, b1[3] = {-1, -1, -1}, c1[3] = {-1, -1, -1};
+    bool l1 = s1 < nA;
+    if (l1) {
+        g1 = (long)h1;
+        l1 = !masked(a, g1);
+    }
+    int b0[3], c0[3], b1[3], c1[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        b0[ax] = l0 ? t_b0[ax] : -1;
+        c0[ax] = l0 ? t_c0[ax] : -1;
+        b1[ax] = l1 && s1 < nD ? t_b1[ax] : -1;
+        c1[ax] = l1 ? t_c1[ax] : -1;
+    }
     uint32_t w0 = 0u, w1 = 0u;
     T od0[4] = {T(0), T(0), T(0), T(0)}, od1[4] = {T(0), T(0), T(0), T(0)};
     T fz0[3][4], fz1[3][4], in1[3][4];
@@ -1288,11 +1326,6 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     constexpr bool PF0 = PF && REF;
     T in0[3][4];
     if (l0) {
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            b0[ax] = bs[ax * CA + tid];
-            c0[ax] = cs[ax * CA + tid];
-        }
         w0 = a.info[g0];
         load_uv(g0, od0);
         if constexpr (HEAD) seed(tid, od0);
@@ -1307,12 +1340,6 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
         }
     }
     if (l1) {
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) c1[ax] = cs[ax * CA + s1];
-        if (s1 < nD) {
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) b1[ax] = bs[ax * CA + s1];
-        }
         w1 = a.info[g1];
         load_uv(g1, od1);
         if constexpr (HEAD) seed(s1, od1);
@@ -3169,6 +3196,14 @@ size_t fused2_wg_per_cu(int tile) {
     return (size_t)std::max(1, 4 * waves / std::max(1, tile / 64));
 }
 
+// Row r of dst = row order[r] of src, rows of len int32 (one workgroup a row).
+__global__ void k_permute_rows(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
+                               const int32_t* __restrict__ order, uint32_t len) {
+    const int32_t* const s = src + (size_t)order[blockIdx.x] * len;
+    int32_t* const d = dst + (size_t)blockIdx.x * len;
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) d[i] = s[i];
+}
+
 // The two-iteration kernel's tile sets for a's level and a tile size, built
 // on first use (callers build them before any graph capture starts).
 template <int TILE, bool REF>
@@ -3258,10 +3293,35 @@ const Halo2Bufs& halo2_tab(tgvx_ctx* c, const KArgs<S>& a, int tile) {
         }
         hb->ok = !failed && hb->lds <= lds_budget(c);
         if (hb->ok) {
-            hb->tiles.ensure(sizeof(int32_t) * order.size());
-            TGVX_HIP(hipMemcpyAsync(hb->tiles.p, order.data(), sizeof(int32_t) * order.size(),
-                                    hipMemcpyHostToDevice, c->stream));
-            TGVX_HIP(hipStreamSynchronize(c->stream));
+            // the tables in launch order, each count record ending in its
+            // tile id (the kernel's table reads then depend on its workgroup
+            // id alone)
+            std::vector<int32_t> rec(4 * order.size());
+            for (size_t k = 0; k < order.size(); ++k) {
+                const int32_t t = order[k];
+                rec[4 * k] = cnt[4 * (size_t)t];
+                rec[4 * k + 1] = cnt[4 * (size_t)t + 1];
+                rec[4 * k + 2] = cnt[4 * (size_t)t + 2];
+                rec[4 * k + 3] = t;
+            }
+            DevBuf ord;
+            ord.ensure(sizeof(int32_t) * order.size());
+            TGVX_HIP(hipMemcpyAsync(ord.p, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+            TGVX_HIP(hipMemcpyAsync(hb->cnt.p, rec.data(), sizeof(int32_t) * rec.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+            auto permute = [&](DevBuf& b, size_t len) {
+                DevBuf o;
+                o.ensure(sizeof(int32_t) * len * ntiles);
+                k_permute_rows<<<ntiles, 256, 0, c->stream>>>(b.as<int32_t>(), o.as<int32_t>(), ord.as<int32_t>(),
+                                                              (uint32_t)len);
+                TGVX_HIP(hipGetLastError());
+                b.swap(o);
+                TGVX_HIP(hipStreamSynchronize(c->stream));  // before o (the old rows) is freed
+            };
+            permute(hb->list, ca - tile);
+            permute(hb->bslot, 3 * ca);
+            permute(hb->cslot, 3 * ca);
         } else {  // the level falls back to one iteration per launch
             hb->cnt.release();
             hb->list.release();
@@ -3280,25 +3340,26 @@ bool launch_fused2_x(tgvx_ctx* c, KArgs<S>& a, const Set13<S>& src, const Set13<
     const Halo2Bufs& hb = halo2_tab<S, REF>(c, a, tile);
     if (!hb.ok) return false;
     Halo2Tab h;
-    h.cnt = hb.cnt.as<int32_t>();
-    h.list = hb.list.as<int32_t>();
-    h.bslot = hb.bslot.as<int32_t>();
-    h.cslot = hb.cslot.as<int32_t>();
+    const size_t ca = (size_t)kHalo2Cap * tile;
     for (const Halo2Bufs::Class& cl : hb.cls) {
         if (cl.count == 0) continue;
-        const int32_t* tl = hb.tiles.as<int32_t>() + cl.first;
+        // the class's rows (the tables are in launch order)
+        h.cnt = hb.cnt.as<int32_t>() + 4 * (size_t)cl.first;
+        h.list = hb.list.as<int32_t>() + (ca - tile) * cl.first;
+        h.bslot = hb.bslot.as<int32_t>() + 3 * ca * cl.first;
+        h.cslot = hb.cslot.as<int32_t>() + 3 * ca * cl.first;
         switch (tile) {
             case 128:
-                k_fused2<S, 128, X, REF, HEAD><<<cl.count, 128, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 128, X, REF, HEAD><<<cl.count, 128, cl.lds, c->stream>>>(a, src, dst, h, cl.sa, cl.sx);
                 break;
             case 256:
-                k_fused2<S, 256, X, REF, HEAD><<<cl.count, 256, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 256, X, REF, HEAD><<<cl.count, 256, cl.lds, c->stream>>>(a, src, dst, h, cl.sa, cl.sx);
                 break;
             case 1024:
-                k_fused2<S, 1024, X, REF, HEAD><<<cl.count, 1024, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 1024, X, REF, HEAD><<<cl.count, 1024, cl.lds, c->stream>>>(a, src, dst, h, cl.sa, cl.sx);
                 break;
             default:
-                k_fused2<S, 512, X, REF, HEAD><<<cl.count, 512, cl.lds, c->stream>>>(a, src, dst, h, tl, cl.sa, cl.sx);
+                k_fused2<S, 512, X, REF, HEAD><<<cl.count, 512, cl.lds, c->stream>>>(a, src, dst, h, cl.sa, cl.sx);
                 break;
         }
     }

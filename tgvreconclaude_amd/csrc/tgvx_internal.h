@@ -9,6 +9,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #include "tgvx.h"
@@ -49,6 +50,10 @@ struct DevBuf {
     }
     template <class X>
     X* as() const { return static_cast<X*>(p); }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
 };
 
 // Per launch range and tile size: the fused tiled kernel's precomputed
@@ -65,9 +70,10 @@ struct HaloBufs {
 // PC over the tiles; lds: the largest dynamic LDS of a launch. The tiles run
 // in at most two launches (LDS classes): those whose LDS need keeps the
 // occupancy of the tile at the 90th percentile, then the rest, each launch
-// sized for its largest tile; tiles lists their ids class by class.
+// sized for its largest tile; the tables' rows are in that launch order,
+// class by class.
 struct Halo2Bufs {
-    DevBuf cnt, list, bslot, cslot, tiles;
+    DevBuf cnt, list, bslot, cslot;
     bool ok = false;
     int sa = 0, sd = 0, sp = 0;
     size_t lds = 0;
