@@ -7,7 +7,9 @@ outliers, octree clamped at depth 10 (about 6 M cubes), 10 bins.
     finest leaf (the workload's accuracy bar);
   * one 50-iteration chunk per level on the GPU against the CPU oracle at
     full size, bit for bit: FP64 in the exact-adjoint mode, and FP32 with the
-    reference's own update rules (tgv.cpp:137-232, the drop-in path).
+    reference's own update rules (tgv.cpp:137-232, the drop-in path);
+  * the two-iteration kernel at full size against the two-kernel iteration,
+    bit for bit, every storage precision, both rule sets, three tile sizes.
 """
 import os
 
@@ -16,7 +18,7 @@ import pytest
 
 import oracle as O
 from accuracy import compare
-from tgvreconclaude_amd import FP32, FP64, OP_ADJ, OP_REF, TGVParams, scene, step_schedule
+from tgvreconclaude_amd import BF16, BF16_DUAL, FP32, FP64, OP_ADJ, OP_REF, TGVParams, scene, step_schedule
 
 pytestmark = pytest.mark.gpu
 
@@ -69,3 +71,21 @@ def test_one_chunk_per_level_equals_oracle(ctx, torus_box, prec, mode):
     same = got.view(word) == np.ascontiguousarray(ref).view(word)
     assert same.all(), f"{(~same).any(axis=1).sum()} cubes differ"
     assert np.abs(ref[:, 4:]).max() > 0
+
+
+@pytest.mark.parametrize("prec", [FP32, FP64, BF16, BF16_DUAL])
+@pytest.mark.parametrize("mode", [OP_ADJ, OP_REF])
+def test_two_iteration_tiles_at_full_size(ctx, torus_box, prec, mode):
+    """The two-iteration kernel at the workload's own sizes, where its finest
+    levels split into two LDS classes with tables in launch order, equals the
+    two-kernel iteration bit for bit for tiles of 256, 512 and 1024 (head
+    launch, two-iteration launches and the single last steps: 7 iterations)."""
+    sc, hist = torus_box
+    ctx.load(sc.cubes, hist, precision=prec, op_mode=mode)
+    base = dict(alpha0=2.0, alpha1=1.0, iters=7, precision=prec, op_mode=mode, no_energy=1, use_graph=0)
+    ctx.run(TGVParams(fused=0, block=256, **base))
+    want = np.ascontiguousarray(ctx.state()).view(np.uint8)
+    for block in (1024, 512, 256):
+        ctx.run(TGVParams(fused=5, block=block, **base))
+        got = np.ascontiguousarray(ctx.state()).view(np.uint8)
+        assert np.array_equal(got, want), block
