@@ -102,19 +102,22 @@ typedef struct {
                               initialisation + network correction, with warm_fallback as above */
     int32_t warm_dual;     /* with an initial field and dual_eps > 0: 0 recovers p, q from the field,
                               1 keeps the p, q recovered from the cold initialisation */
-    int32_t fused;         /* exact-adjoint mode, results identical in all cases: 0 a dual and a
+    int32_t fused;         /* results identical in all cases: 0 a dual and a
                               primal kernel per iteration; 1 (default) one fused kernel per
                               iteration sharing a tile's primal values through LDS; 2 one fused
                               kernel recomputing every neighbour's primal values; 3 as 1 with the
                               tile's p, q staged in LDS first; 4 as 1 with the out-of-tile
-                              neighbours tabulated once per level and tile; 5 two iterations
+                              neighbours tabulated once per level and tile (1-4 exact-adjoint
+                              mode only; the reference's rules run them as 0); 5 two iterations
                               per launch over tile sets tabulated once per level and tile (a
-                              level whose sets do not fit runs as 1); 6 a level of at most 1024
-                              cubes in one workgroup for all its iterations (larger levels run
-                              as 5, tiles of 1024 from 2^18 cubes on, else 512); 7 a level in
+                              level whose sets do not fit runs as 1, with the reference's rules
+                              as 0); 6 a level of at most 1024 cubes in one workgroup for all
+                              its iterations, both operator modes (larger levels run as 5,
+                              tiles of 1024 from 2^18 cubes on, else 512); 7 a level in
                               one cooperative launch for all its iterations, two grid barriers
                               per iteration, workgroups of block = 256 or 1024 (default; a level
-                              whose workgroups cannot all be resident at once runs as 6 does);
+                              whose workgroups cannot all be resident at once runs as 6 does;
+                              exact-adjoint mode, the reference's rules run it as 5);
                               8 four iterations per launch over tile sets tabulated once per level
                               and tile, where four remain and the sets fit (tiles of 128, 256 =
                               default), otherwise as 5 (both operator modes; for small levels,

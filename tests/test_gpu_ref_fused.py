@@ -1,14 +1,15 @@
 """The reference's update rules (tgv.cpp:137-232, the drop-in tgv_minimize)
-with two iterations per launch over tabulated tile sets: bit for bit the
-two-kernel iteration and the CPU oracle, for every tile size, odd and even
-iteration counts, FP32 and FP64, inputs where octant-0 children share their
-parents' codes, and several blocks in one problem."""
+with two iterations per launch over tabulated tile sets, and small levels in
+one workgroup: bit for bit the two-kernel iteration and the CPU oracle, for
+every tile size, odd and even iteration counts, FP32 and FP64, inputs where
+octant-0 children share their parents' codes, and several blocks in one
+problem."""
 import numpy as np
 import pytest
 
 import oracle as O
 import test_ref_rules_pins as pins
-from tgvreconclaude_amd import FP32, FP64, OP_REF, TGVParams, scene
+from tgvreconclaude_amd import BF16, BF16_DUAL, FP32, FP64, OP_REF, TGVParams, scene
 
 pytestmark = pytest.mark.gpu
 
@@ -34,6 +35,18 @@ def test_ref_two_iterations_per_launch_equal_oracle(ctx, sphere, fused, block, i
     assert bad.size == 0, f"{bad.size} cubes differ"
 
 
+@pytest.mark.parametrize("prec", [FP64, BF16, BF16_DUAL])
+@pytest.mark.parametrize("iters", [1, 2, 7, 30])
+def test_ref_small_levels_in_one_workgroup(ctx, sphere, prec, iters):
+    """Levels of at most 1024 cubes in one workgroup (fused = 6) against the
+    two kernels, bit for bit, in every other storage precision."""
+    sc, hist = sphere
+    base = dict(alpha0=2.0, alpha1=1.0, iters=iters, precision=prec, op_mode=OP_REF)
+    want, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=0, block=256, **base))
+    got, _ = ctx.solve(sc.cubes, hist, TGVParams(fused=6, **base))
+    assert np.array_equal(np.ascontiguousarray(got).view(np.uint8), np.ascontiguousarray(want).view(np.uint8))
+
+
 def test_ref_fp64_equals_oracle(ctx, sphere):
     sc, hist = sphere
     p = TGVParams(alpha0=2.0, alpha1=1.0, iters=100, precision=FP64, op_mode=OP_REF, fused=6)
@@ -43,12 +56,13 @@ def test_ref_fp64_equals_oracle(ctx, sphere):
 
 
 @pytest.mark.parametrize("seed", pins.SEEDS)
-@pytest.mark.parametrize("block", [128, 512])
-def test_ref_code_collisions(ctx, seed, block):
+@pytest.mark.parametrize("fused,block", [(5, 128), (5, 512), (6, 0)])
+def test_ref_code_collisions(ctx, seed, fused, block):
     # octant-0 children sharing codes with coarser cubes, in both orders,
-    # coarser fallbacks and neighbours of other depths (read frozen or stale)
+    # coarser fallbacks and neighbours of other depths (read frozen or stale);
+    # fused = 6: every level of these small problems in one workgroup
     cs, h = pins.scene(seed)
-    p = TGVParams(alpha0=0.5, alpha1=0.3, iters=20, precision=FP32, op_mode=OP_REF, fused=5, block=block)
+    p = TGVParams(alpha0=0.5, alpha1=0.3, iters=20, precision=FP32, op_mode=OP_REF, fused=fused, block=block)
     st, _ = ctx.solve(cs, h, p)
     ref = O.tgv_minimize(cs, h, O.params(alpha0=0.5, alpha1=0.3, iters=20, op_mode=0))[0]
     assert np.array_equal(_bits(_flat(st)), _bits(ref))
@@ -60,7 +74,7 @@ def test_ref_csg_scene_tuned(ctx):
     p = TGVParams(alpha0=2.0, alpha1=1.0, iters=50, precision=FP32, op_mode=OP_REF, fused=6)
     ctx.load(sc.cubes, hist, precision=FP32, op_mode=OP_REF)
     tuned = ctx.autotune(p, iters=50)
-    assert all(l["variant"] in (0, 5, 8) for l in tuned)  # the reference rules' variants
+    assert all(l["variant"] in (0, 5, 6, 8) for l in tuned)  # the reference rules' variants
     p.tune = 1
     ctx.run(p)
     got = _flat(ctx.state())

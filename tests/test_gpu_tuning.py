@@ -38,7 +38,7 @@ def test_autotune_choice_and_identity(ctx, sphere, prec, mode):
     levels = ctx.autotune(p, iters=iters)
     assert [(l["depth"], l["leaves"]) for l in levels] == [tuple(x) for x in ctx.levels()]
     for l in levels:
-        assert l["variant"] in ((0, 1, 2, 3, 4, 5, 6, 7, 8) if mode == OP_ADJ else (0, 5, 8))
+        assert l["variant"] in ((0, 1, 2, 3, 4, 5, 6, 7, 8) if mode == OP_ADJ else (0, 5, 6, 8))
         assert l["block"] in (64, 128, 256, 512, 1024) and l["xcd"] in (0, 1)
         if l["variant"] in (1, 3, 4):
             assert l["block"] in (128, 256, 512, 1024)

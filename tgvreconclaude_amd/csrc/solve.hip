@@ -1862,13 +1862,16 @@ __global__ void __launch_bounds__(kFused4Threads) k_fused4(KArgs<S> a, Set13<S> 
 // the level's cube t and keeps its u, v, p, q, u_bar, v_bar, table entries and
 // frozen neighbours in registers; each iteration publishes u_bar, v_bar, then
 // p, q through LDS (two barriers). The dual and primal steps are k_dual's and
-// primal_adj_at's expressions on the main set, so the state after the launch
-// is that of `iters` two-kernel iterations, bit for bit.
+// primal_adj_at's expressions on the main set (REF: k_primal_ref's, the
+// reference's rules, tgv.cpp:192-217, with the -a neighbours of the
+// reference's table; one of another depth gives its p, q from the main set,
+// constant while the level iterates), so the state after the launch is that
+// of `iters` two-kernel iterations, bit for bit.
 // NT threads, the smallest of 64, 256, 1024 that covers the level, so a small
 // level's barriers wait for as few waves as it needs.
 constexpr int kLevel1 = 1024;
 
-template <class S, int NT = kLevel1>
+template <class S, int NT = kLevel1, bool REF = false>
 __global__ void __launch_bounds__(NT) k_level1(KArgs<S> a, int iters) {
     using IO = Io<S>;
     using T = typename IO::T;
@@ -1891,17 +1894,29 @@ __global__ void __launch_bounds__(NT) k_level1(KArgs<S> a, int iters) {
     const uint32_t w = a.info[g];
     const T med = median_of<T>(w, a.nbins);  // constant over the iterations
     // +a sources: >= 0 an LDS slot, -1 this cube, -2 frozen (values in fz);
-    // incoming edges: the LDS slot or -1
+    // incoming edges: the LDS slot or -1 (REF: -2 a -a neighbour of another
+    // depth, its p_ax and row ax of q in fin)
+    constexpr int kFf[3][4] = {{F_PX, F_Q0, F_Q3, F_Q4}, {F_PY, F_Q3, F_Q1, F_Q5}, {F_PZ, F_Q4, F_Q5, F_Q2}};
     int src[3], inc[3];
-    T fz[3][4];
+    T fz[3][4], fin[3][4];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         const int j = a.nbp[ax][g];
         src[ax] = j < 0 ? -1 : (in_level(j) ? j - (int)a.off : -2);
 #pragma unroll
         for (int c = 0; c < 4; ++c) fz[ax][c] = IO::ld(a.s[F_UB + c], j >= 0 ? (long)j : g);
-        const int k = a.nbt[ax][g];
-        inc[ax] = k < 0 ? -1 : k - (int)a.off;
+        if constexpr (REF) {
+            const int m = a.nbm[ax][g];
+            inc[ax] = m < 0 ? -1 : (in_level(m) ? m - (int)a.off : -2);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const T x = IO::ld(a.s[kFf[ax][e]], m >= 0 ? (long)m : g);
+                fin[ax][e] = inc[ax] == -2 ? x : T(0);
+            }
+        } else {
+            const int k = a.nbt[ax][g];
+            inc[ax] = k < 0 ? -1 : k - (int)a.off;
+        }
     }
     // the +a neighbour's slot (this cube's own where there is none: its
     // u_bar, v_bar are published too), read unconditionally, the frozen
@@ -1941,8 +1956,37 @@ __global__ void __launch_bounds__(NT) k_level1(KArgs<S> a, int iters) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) lpq[3 + k][t] = q[k];
         __syncthreads();
-        // passes 3-5 (primal_adj_at, then the extrapolation)
-        if (run) {
+        // passes 3-5 (primal_adj_at, then the extrapolation; REF:
+        // k_primal_ref's, a missing -a neighbour giving 0)
+        if (REF && run) {
+            constexpr int kF[3][4] = {{0, 3, 6, 7}, {1, 6, 4, 8}, {2, 7, 8, 5}};  // p_ax, q row ax in lpq
+            T in[3][4];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const int k = inc[ax] >= 0 ? inc[ax] : t;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const T x = lpq[kF[ax][e]][k];
+                    in[ax][e] = inc[ax] >= 0 ? x : fin[ax][e];
+                }
+            }
+            const T div_p = (p[0] - in[0][0]) + (p[1] - in[1][0]) + (p[2] - in[2][0]);
+            T nw[4];
+            nw[0] = prox_m(u + a.tau * div_p, w, med, a.tl);
+            const T dqx = (q[0] - in[0][1]) + (q[3] - in[1][1]) + (q[4] - in[2][1]);
+            const T dqy = (q[3] - in[0][2]) + (q[1] - in[1][2]) + (q[5] - in[2][2]);
+            const T dqz = (q[4] - in[0][3]) + (q[5] - in[1][3]) + (q[2] - in[2][3]);
+            nw[1] = v[0] + a.tau * (p[0] + dqx);
+            nw[2] = v[1] + a.tau * (p[1] + dqy);
+            nw[3] = v[2] + a.tau * (p[2] + dqz);
+            ub[0] = IO::rt(T(2) * nw[0] - u);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ub[1 + c] = IO::rt(T(2) * nw[1 + c] - v[c]);
+            u = IO::rt(nw[0]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = IO::rt(nw[1 + c]);
+        }
+        if (!REF && run) {
             T dp = T(0);
             T dq[3] = {T(0), T(0), T(0)};
 #pragma unroll
@@ -3062,10 +3106,11 @@ int size_bucket(uint32_t n) {
 Launch launch_for(const tgvx_ctx* c, const tgvx_params* p, uint32_t n, bool frozen) {
     const bool adj = p->op_mode == TGVX_OP_ADJ;
     if (p->fused < 0 || p->fused > 8) throw Error(TGVX_E_ARG, "fused must be 0 to 8");
-    // the reference's rules have the two-kernel iteration and the two-iteration
-    // kernel (5; 6 and 7 run every level as 5, their small-level kernels being
-    // the exact-adjoint one's), also for frozen-border leaves; the exact-adjoint
-    // frozen-border solve has the two kernels only
+    // the reference's rules have the two-kernel iteration, the two- and
+    // four-iteration kernels (5, 8), the whole-level kernel for small levels
+    // (6; larger levels as 5) and run 7 as 6 does beyond its small levels,
+    // also for frozen-border leaves (with 5); the exact-adjoint frozen-border
+    // solve has the two kernels only
     const int var = !adj ? (p->fused >= 5 ? p->fused : 0) : frozen ? 0 : p->fused;
     Launch L{var, block_of(p), 1};
     const bool tiled = L.variant == 1 || L.variant == 3 || L.variant == 4 || L.variant == 5 || L.variant == 8;
@@ -3718,17 +3763,32 @@ void check_grid(tgvx_ctx* c) {
 // 512 at 0.38-4 M cubes, 12 % slower at 97 k), else 512.
 inline Launch level1_fallback(uint32_t n) { return Launch{5, n >= (1u << 18) ? 1024 : 512, 1}; }
 
+// Whether variant 6 runs a's range as one k_level1 launch: a whole level of
+// at most kLevel1 cubes, outside the frozen-border and streaming solves (both
+// operator modes).
+template <class S>
+bool level1_runs(const KArgs<S>& a, bool frozen) {
+    return !frozen && !a.sflag && a.n <= (uint32_t)kLevel1 && a.off == a.lvl_off && a.n == a.lvl_n;
+}
+
 template <class S>
 void run_launch(tgvx_ctx* c, KArgs<S>& a, const tgvx_params* p, int iters, bool frozen, const Launch& L0,
                 bool cold = false, bool need_head = false) {
     if (iters <= 0 || a.n == 0) return;
     Launch L = L0;
     if (L.variant == 6) {
-        if (!frozen && p->op_mode == TGVX_OP_ADJ && a.n <= (uint32_t)kLevel1 && a.off == a.lvl_off &&
-            a.n == a.lvl_n) {
-            if (a.n <= 64) k_level1<S, 64><<<1, 64, 0, c->stream>>>(a, iters);
-            else if (a.n <= 256) k_level1<S, 256><<<1, 256, 0, c->stream>>>(a, iters);
-            else k_level1<S, kLevel1><<<1, kLevel1, 0, c->stream>>>(a, iters);
+        if (level1_runs(a, frozen)) {
+            const bool ref = p->op_mode == TGVX_OP_REF;
+            if (a.n <= 64) {
+                if (ref) k_level1<S, 64, true><<<1, 64, 0, c->stream>>>(a, iters);
+                else k_level1<S, 64><<<1, 64, 0, c->stream>>>(a, iters);
+            } else if (a.n <= 256) {
+                if (ref) k_level1<S, 256, true><<<1, 256, 0, c->stream>>>(a, iters);
+                else k_level1<S, 256><<<1, 256, 0, c->stream>>>(a, iters);
+            } else {
+                if (ref) k_level1<S, kLevel1, true><<<1, kLevel1, 0, c->stream>>>(a, iters);
+                else k_level1<S, kLevel1><<<1, kLevel1, 0, c->stream>>>(a, iters);
+            }
             TGVX_HIP(hipGetLastError());
             return;
         }
@@ -3818,7 +3878,7 @@ bool opens_with_head(tgvx_ctx* c, const KArgs<S>& a, const tgvx_params* p, int i
     if (iters < 2 || a.n == 0 || a.off != a.lvl_off || a.n != a.lvl_n) return false;
     Launch L = launch_for(c, p, a.n, false);
     if (L.variant == 6) {
-        if (p->op_mode == TGVX_OP_ADJ && a.n <= (uint32_t)kLevel1) return false;
+        if (level1_runs(a, false)) return false;
         L = level1_fallback(a.n);
     }
     if (L.variant != 5 && L.variant != 8) return false;
@@ -4444,10 +4504,9 @@ void autotune_t(tgvx_ctx* c, const tgvx_params* p, int iters, tgvx_tune_report* 
         for (int b : {256, 128, 512, 1024}) cand.push_back(Launch{5, b, x});
     }
     for (int b : {256, 128}) cand.push_back(Launch{8, b, 1});
-    if (p->op_mode == TGVX_OP_ADJ) {
-        cand.push_back(Launch{6, kLevel1, 0});
+    cand.push_back(Launch{6, kLevel1, 0});
+    if (p->op_mode == TGVX_OP_ADJ)
         for (int b : {1024, 256}) cand.push_back(Launch{7, b, 0});
-    }
     // Kernel launches per iteration of a candidate over `iters` iterations of
     // a level of n cubes. Inside a sweep every launch costs about 1 us more
     // than when one level's graph is timed alone (measured at depths 4-6 of
