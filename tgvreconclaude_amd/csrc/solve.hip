@@ -1348,7 +1348,10 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
                 if (b1[ax] <= -2) frozen(b1[ax], fz1[ax], ax, g1);
-                if (!HEAD && c1[ax] <= -2) gather_in(ax, -2L - c1[ax], g1, in1[ax]);
+                // reference rules: for a slot of PC, a -a neighbour outside
+                // the sets lies outside the level, so its p, q are constant
+                // and serve the second primal step too (the head launch's only)
+                if ((!HEAD || (REF && s1 < nP)) && c1[ax] <= -2) gather_in(ax, -2L - c1[ax], g1, in1[ax]);
             }
         }
     }
@@ -1404,7 +1407,7 @@ __global__ void __launch_bounds__(TILE) k_fused2(KArgs<S> a, Set13<S> src, Set13
     if (l0) primal(tid, g0, w0, HEAD ? od0 : nw0, c0, own, PF0, in0);
     if (l1 && s1 < nP) {
         T x[4];
-        primal(s1, g1, w1, HEAD ? od1 : nw1, c1, x, false, in1);
+        primal(s1, g1, w1, HEAD ? od1 : nw1, c1, x, PF && REF, in1);
     }
     for (int s = tid + 2 * TILE; s < nP; s += TILE) {
         long g;
