@@ -2613,25 +2613,46 @@ __device__ inline void gap_cube(const KArgs<S>& a, long g, double lam, double a0
     using T = typename IO::T;
     auto in_lvl = [&](int j) { return (uint32_t)j >= a.lvl_off && (uint32_t)j < a.lvl_off + a.lvl_n; };
     auto L = [&](int f, long j) -> double { return (double)IO::ld(a.s[f], j); };
-    const uint32_t Wc = InfoWord::W(a.info[g]);
-    const double W = (double)Wc, m = (double)median_of<T>(a.info[g], a.nbins), ui = L(F_U, g);
+    const uint32_t wd = a.info[g];
+    const uint32_t Wc = InfoWord::W(wd);
+    const double W = (double)Wc, m = (double)median_of<T>(wd, a.nbins), ui = L(F_U, g);
     const double v[3] = {L(F_VX, g), L(F_VY, g), L(F_VZ, g)};
     const double p[3] = {L(F_PX, g), L(F_PY, g), L(F_PZ, g)};
-    double gr[3], dv[3][3];
-    int jn[3];
+    double q[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = L(F_Q0 + k, g);
+    // every load unconditional (the cube's own address where a neighbour is
+    // missing, the value then unused), so that they are all in flight at
+    // once; the sums below skip the same terms in the same order
+    int jn[3], kn[3];
+    bool in[3];
+    double nu[3], nv[3][3], ip[3], iq[3][3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const int j = a.nba[ax][g];
-        jn[ax] = j;
-        if (j >= 0) {
-            const bool in = in_lvl(j);
-            gr[ax] = (in ? L(F_U, j) : L(F_UB, j)) - ui;
+        jn[ax] = a.nba[ax][g];
+        kn[ax] = a.nbt[ax][g];
+    }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) dv[c][ax] = (in ? L(F_VX + c, j) : L(F_VBX + c, j)) - v[c];
-        } else {
-            gr[ax] = 0.0;
-            for (int c = 0; c < 3; ++c) dv[c][ax] = 0.0;
-        }
+    for (int ax = 0; ax < 3; ++ax) {
+        const int j = jn[ax], k = kn[ax];
+        const long jj = j >= 0 ? j : g, kk = k >= 0 ? k : g;
+        // a neighbour in the level gives its u, v, one outside it (frozen)
+        // its u_bar, v_bar
+        in[ax] = j >= 0 && in_lvl(j);
+        nu[ax] = (double)IO::ld(in[ax] ? a.s[F_U] : a.s[F_UB], jj);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nv[ax][c] = (double)IO::ld(in[ax] ? a.s[F_VX + c] : a.s[F_VBX + c], jj);
+        ip[ax] = L(F_PX + ax, kk);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) iq[ax][c] = L(qidx(c, ax), kk);
+    }
+    double gr[3], dv[3][3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const bool h = jn[ax] >= 0;
+        gr[ax] = h ? nu[ax] - ui : 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dv[c][ax] = h ? nv[ax][c] - v[c] : 0.0;
     }
     const double k1[3] = {gr[0] - v[0], gr[1] - v[1], gr[2] - v[2]};
     const double e[6] = {dv[0][0], dv[1][1], dv[2][2], 0.5 * (dv[0][1] + dv[1][0]), 0.5 * (dv[0][2] + dv[2][0]),
@@ -2645,32 +2666,30 @@ __device__ inline void gap_cube(const KArgs<S>& a, long g, double lam, double a0
     double cst = 0.0, cu = 0.0, cv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const int j = jn[ax];
-        if (j < 0) continue;
+        if (jn[ax] < 0) continue;
         cu -= p[ax];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const double w = (c == ax) ? 1.0 : 0.5;
-            cv[c] -= w * L(qidx(c, ax), g);
+            cv[c] -= w * q[qidx(c, ax) - F_Q0];
         }
-        if (!in_lvl(j)) {
-            cst += p[ax] * L(F_UB, j);
+        if (!in[ax]) {
+            cst += p[ax] * nu[ax];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const double w = (c == ax) ? 1.0 : 0.5;
-                cst += w * L(qidx(c, ax), g) * L(F_VBX + c, j);
+                cst += w * q[qidx(c, ax) - F_Q0] * nv[ax][c];
             }
         }
     }
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const int k = a.nbt[ax][g];
-        if (k < 0) continue;
-        cu += L(F_PX + ax, k);
+        if (kn[ax] < 0) continue;
+        cu += ip[ax];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const double w = (c == ax) ? 1.0 : 0.5;
-            cv[c] += w * L(qidx(c, ax), k);
+            cv[c] += w * iq[ax][c];
         }
     }
     for (int c = 0; c < 3; ++c) cv[c] -= p[c];
@@ -2742,7 +2761,23 @@ __global__ void __launch_bounds__(kGapBlock) k_gap_seg(KArgs<S> a, const uint32_
 __global__ void __launch_bounds__(kGapBlock) k_sum_pairs(const double* part, uint32_t nparts, double* out) {
     __shared__ double sp[kGapBlock], sd[kGapBlock];
     double P = 0.0, D = 0.0;
-    for (uint32_t i = threadIdx.x; i < nparts; i += kGapBlock) {
+    // eight pairs' loads in flight at a time; the sums keep their order
+    constexpr int kBatch = 8;
+    uint32_t i = threadIdx.x;
+    for (; i + (kBatch - 1) * kGapBlock < nparts; i += kBatch * kGapBlock) {
+        double x[2 * kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            x[2 * k] = part[2 * (i + k * kGapBlock)];
+            x[2 * k + 1] = part[2 * (i + k * kGapBlock) + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            P += x[2 * k];
+            D += x[2 * k + 1];
+        }
+    }
+    for (; i < nparts; i += kGapBlock) {
         P += part[2 * i];
         D += part[2 * i + 1];
     }
