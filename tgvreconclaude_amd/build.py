@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-I" + INCLUDE, "-I" + CSRC,
 ]
-HIP_SOURCES = ["api.hip", "hist.hip", "solve.hip", "mlp.hip", "surface.hip", "octree.hip"]
+HIP_SOURCES = ["api.hip", "hist.hip", "solve.hip", "mlp.hip", "surface.hip", "octree.hip", "xfer.hip"]
 
 
 def _run(cmd):

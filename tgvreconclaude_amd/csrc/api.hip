@@ -77,6 +77,7 @@ void tgvx_destroy(tgvx_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    c->stage.reset();
     tgvx::release_graphs(c);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);

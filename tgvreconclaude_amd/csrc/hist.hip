@@ -228,8 +228,7 @@ void build_pyramids(tgvx_ctx* c, const tgvx_view* views, int nv, std::vector<Vie
     for (int v = 0; v < nv; ++v) {
         ViewMeta& m = meta[v];
         for (int l = 0; l < m.nlev; ++l) m.lv[l] = reinterpret_cast<const float*>(pyr + (uintptr_t)m.lv[l]);
-        TGVX_HIP(hipMemcpyAsync(const_cast<float*>(m.lv[0]), views[v].depths, sizeof(float) * (size_t)m.w[0] * m.h[0],
-                                hipMemcpyHostToDevice, c->stream));
+        h2d(c, const_cast<float*>(m.lv[0]), views[v].depths, sizeof(float) * (size_t)m.w[0] * m.h[0]);
         launch_mips(m, c->stream);
     }
 }
@@ -249,7 +248,7 @@ void histograms(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view
         TGVX_HIP(hipMemcpyAsync(c->h_meta.p, meta.data(), sizeof(ViewMeta) * nv, hipMemcpyHostToDevice, c->stream));
     c->h_cubes.ensure(sizeof(tgvx_cube) * n);
     c->h_bins.ensure(sizeof(uint32_t) * n * nb);
-    TGVX_HIP(hipMemcpyAsync(c->h_cubes.p, cubes, sizeof(tgvx_cube) * n, hipMemcpyHostToDevice, c->stream));
+    h2d(c, c->h_cubes.p, cubes, sizeof(tgvx_cube) * n);
     VoteArgs a;
     a.cubes = c->h_cubes.as<tgvx_cube>();
     a.n = n;
@@ -265,7 +264,7 @@ void histograms(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const tgvx_view
     if (blocks > 0x7fffffffull) throw Error(TGVX_E_ARG, "too many cubes");
     k_vote<<<(unsigned)blocks, T, sizeof(uint32_t) * nb * T, c->stream>>>(a);
     TGVX_HIP(hipGetLastError());
-    TGVX_HIP(hipMemcpyAsync(bins_out, c->h_bins.p, sizeof(uint32_t) * n * nb, hipMemcpyDeviceToHost, c->stream));
+    d2h(c, bins_out, c->h_bins.p, sizeof(uint32_t) * n * nb);
     TGVX_HIP(hipStreamSynchronize(c->stream));
 }
 
@@ -290,11 +289,11 @@ void median_init(tgvx_ctx* c, const uint32_t* bins, uint64_t n, int nb, float* o
     if (n == 0) return;
     c->h_bins.ensure(sizeof(uint32_t) * n * nb);
     c->h_pyr.ensure(sizeof(float) * n);
-    TGVX_HIP(hipMemcpyAsync(c->h_bins.p, bins, sizeof(uint32_t) * n * nb, hipMemcpyHostToDevice, c->stream));
+    h2d(c, c->h_bins.p, bins, sizeof(uint32_t) * n * nb);
     k_median<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->h_bins.as<uint32_t>(), n, nb,
                                                                    c->h_pyr.as<float>());
     TGVX_HIP(hipGetLastError());
-    TGVX_HIP(hipMemcpyAsync(out, c->h_pyr.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    d2h(c, out, c->h_pyr.p, sizeof(float) * n);
     TGVX_HIP(hipStreamSynchronize(c->stream));
 }
 

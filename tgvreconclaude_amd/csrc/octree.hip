@@ -731,7 +731,7 @@ void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, con
     uint64_t off = 0;
     for (int k = 0; k < nviews; ++k) {
         const uint64_t p = (uint64_t)views[k].width * views[k].height;
-        TGVX_HIP(hipMemcpyAsync(dmaps.as<float>() + off, views[k].depths, 4 * p, hipMemcpyHostToDevice, st));
+        h2d(c, dmaps.as<float>() + off, views[k].depths, 4 * p, st);
         off += p;
     }
     TGVX_HIP(hipStreamSynchronize(st));
@@ -861,7 +861,7 @@ void octree_merge(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, uint64_t* n_o
 void octree_copy(tgvx_ctx* c, tgvx_cube* out) {
     if (c->oct_n && !out) throw Error(TGVX_E_ARG, "null pointer");
     if (c->oct_n)
-        TGVX_HIP(hipMemcpy(out, c->oct_out.p, sizeof(tgvx_cube) * c->oct_n, hipMemcpyDeviceToHost));
+        d2h(c, out, c->oct_out.p, sizeof(tgvx_cube) * c->oct_n);
 }
 
 }  // namespace tgvx

@@ -4882,7 +4882,7 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     }
     auto up = [&](DevBuf& b, const void* src, size_t bytes) {
         b.ensure(bytes);
-        if (bytes) TGVX_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+        h2d(c, b.p, src, bytes);
     };
     // per-position code, depth, histogram, vote total and median bin, on the
     // device from the inputs in input order and the permutation (the position
@@ -5414,8 +5414,8 @@ void problem_state(tgvx_ctx* c, void* state_out, void* ubar_out) {
             a, P.d_orig.as<uint32_t>(), (uint32_t)n, state_out ? st.as<T>() : nullptr, ubar_out ? bar.as<T>() : nullptr);
         TGVX_HIP(hipGetLastError());
         if (state_out)
-            TGVX_HIP(hipMemcpyAsync(state_out, st.p, 13 * sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
-        if (ubar_out) TGVX_HIP(hipMemcpyAsync(ubar_out, bar.p, 4 * sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+            d2h(c, state_out, st.p, 13 * sizeof(T) * n);
+        if (ubar_out) d2h(c, ubar_out, bar.p, 4 * sizeof(T) * n);
         TGVX_HIP(hipStreamSynchronize(c->stream));
     };
     switch (P.precision) {

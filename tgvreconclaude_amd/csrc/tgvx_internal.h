@@ -56,6 +56,22 @@ struct DevBuf {
     }
 };
 
+// Two pinned host buffers that large host <-> device copies of caller buffers
+// alternate through, with the host-thread pool that fills and drains them
+// (xfer.hip). done[i]: recorded after the last DMA that used buf[i].
+class CopyPool;
+struct Staging {
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    int next = 0;
+    std::shared_ptr<CopyPool> pool;
+    Staging() = default;
+    Staging(const Staging&) = delete;
+    Staging& operator=(const Staging&) = delete;
+    ~Staging();
+};
+
 // Per launch range and tile size: the fused tiled kernel's precomputed
 // neighbour slots (three int32 per position) and per-tile lists of the
 // out-of-tile neighbours of the same level (3 * tile entries per tile).
@@ -183,12 +199,19 @@ struct tgvx_ctx {
     // error flag; grid_used: a launch since the flag was last read
     tgvx::DevBuf grid_bar;
     bool grid_used = false;
+    // pinned staging of large copies of caller buffers (made on first use)
+    std::unique_ptr<tgvx::Staging> stage;
     // a whole fixed-iteration sweep is being captured into one graph: the
     // level launches go onto the stream as kernels, not as graphs of their own
     bool in_capture = false;
 };
 
 namespace tgvx {
+// xfer.hip: copies of caller (pageable) host buffers on the context's stream
+// (or st): h2d returns once src may be reused, its DMAs ordered on the
+// stream; d2h returns with dst filled.
+void h2d(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st = nullptr);
+void d2h(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st = nullptr);
 // solve.hip
 void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb, int precision,
                   int op_mode, const uint64_t* blk_start = nullptr, uint32_t nblocks = 1);

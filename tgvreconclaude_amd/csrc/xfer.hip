@@ -1,0 +1,178 @@
+// Host <-> device copies of large caller buffers (no reference counterpart).
+//
+// The callers hand over pageable host memory (the reference's std::vectors,
+// NumPy arrays). A pageable copy is staged by the runtime through its own
+// buffers by one host thread; here the bytes go through two pinned buffers of
+// kChunk bytes that alternate, filled and drained by a small pool of host
+// threads, so that the host copy of one chunk overlaps the DMA of the other
+// and the host copy itself runs at several threads' memory bandwidth. Every
+// DMA is on the context's stream, so later work on the stream sees the data.
+#include <algorithm>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <thread>
+
+#include "tgvx_internal.h"
+
+namespace tgvx {
+
+namespace {
+constexpr size_t kChunk = 32ull << 20;  // bytes per pinned buffer
+constexpr size_t kSmall = 2ull << 20;   // below this the runtime's own path
+constexpr size_t kSlice = 1ull << 20;   // smallest share of one copy thread
+}  // namespace
+
+// A fixed pool of host threads; copy() splits one range among them (the
+// calling thread takes the first share) and returns when all are done.
+class CopyPool {
+  public:
+    explicit CopyPool(int n) : n_(std::max(1, n)) {
+        for (int i = 1; i < n_; ++i) th_.emplace_back([this, i] { run(i); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void copy(void* dst, const void* src, size_t bytes) {
+        const int parts = (int)std::min<size_t>((size_t)n_, std::max<size_t>(1, bytes / kSlice));
+        if (parts <= 1) {
+            std::memcpy(dst, src, bytes);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = static_cast<char*>(dst);
+            src_ = static_cast<const char*>(src);
+            bytes_ = bytes;
+            parts_ = parts;
+            left_ = parts - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        share(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return left_ == 0; });
+    }
+
+  private:
+    void share(int i) {
+        const size_t per = (bytes_ + parts_ - 1) / parts_;
+        const size_t lo = std::min(bytes_, per * (size_t)i), hi = std::min(bytes_, lo + per);
+        if (hi > lo) std::memcpy(dst_ + lo, src_ + lo, hi - lo);
+    }
+    void run(int i) {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                if (i >= parts_) continue;
+            }
+            share(i);
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (--left_ == 0) done_.notify_one();
+            }
+        }
+    }
+    const int n_;
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    bool stop_ = false;
+    uint64_t gen_ = 0;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t bytes_ = 0;
+    int parts_ = 0, left_ = 0;
+};
+
+Staging::~Staging() {
+    for (int i = 0; i < 2; ++i) {
+        if (done[i]) (void)hipEventSynchronize(done[i]);
+        if (done[i]) (void)hipEventDestroy(done[i]);
+        if (buf[i]) (void)hipHostFree(buf[i]);
+    }
+}
+
+namespace {
+Staging& staging(tgvx_ctx* c) {
+    if (!c->stage) {
+        auto s = std::make_unique<Staging>();
+        for (int i = 0; i < 2; ++i) {
+            TGVX_HIP(hipHostMalloc(&s->buf[i], kChunk, hipHostMallocDefault));
+            TGVX_HIP(hipEventCreateWithFlags(&s->done[i], hipEventDisableTiming));
+        }
+        const unsigned hw = std::thread::hardware_concurrency();
+        s->pool = std::make_shared<CopyPool>((int)std::min(8u, std::max(1u, hw / 2)));
+        c->stage = std::move(s);
+    }
+    return *c->stage;
+}
+}  // namespace
+
+void h2d(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return;
+    if (!st) st = c->stream;
+    if (bytes < kSmall) {
+        TGVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        return;
+    }
+    Staging& s = staging(c);
+    const char* in = static_cast<const char*>(src);
+    char* out = static_cast<char*>(dst);
+    for (size_t off = 0; off < bytes; off += kChunk) {
+        const size_t len = std::min(kChunk, bytes - off);
+        const int b = s.next;
+        s.next ^= 1;
+        if (s.used[b]) TGVX_HIP(hipEventSynchronize(s.done[b]));  // its previous DMA has read it
+        s.pool->copy(s.buf[b], in + off, len);
+        TGVX_HIP(hipMemcpyAsync(out + off, s.buf[b], len, hipMemcpyHostToDevice, st));
+        TGVX_HIP(hipEventRecord(s.done[b], st));
+        s.used[b] = true;
+    }
+}
+
+void d2h(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return;
+    if (!st) st = c->stream;
+    if (bytes < kSmall) {
+        TGVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        TGVX_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    Staging& s = staging(c);
+    const char* in = static_cast<const char*>(src);
+    char* out = static_cast<char*>(dst);
+    const size_t nch = (bytes + kChunk - 1) / kChunk;
+    auto issue = [&](size_t k) {
+        const size_t off = k * kChunk, len = std::min(kChunk, bytes - off);
+        const int b = (int)(k & 1);
+        // an earlier copy's DMA out of this buffer (possibly on another
+        // stream) has read it
+        if (s.used[b]) TGVX_HIP(hipEventSynchronize(s.done[b]));
+        TGVX_HIP(hipMemcpyAsync(s.buf[b], in + off, len, hipMemcpyDeviceToHost, st));
+        TGVX_HIP(hipEventRecord(s.done[b], st));
+        s.used[b] = true;
+    };
+    issue(0);
+    for (size_t k = 0; k < nch; ++k) {
+        // the next chunk's DMA into the other buffer, whose host copy of chunk
+        // k - 1 has finished, runs while this chunk is copied out
+        if (k + 1 < nch) issue(k + 1);
+        const int b = (int)(k & 1);
+        TGVX_HIP(hipEventSynchronize(s.done[b]));
+        const size_t off = k * kChunk, len = std::min(kChunk, bytes - off);
+        s.pool->copy(out + off, s.buf[b], len);
+    }
+    s.next = (int)(nch & 1);
+}
+
+}  // namespace tgvx
