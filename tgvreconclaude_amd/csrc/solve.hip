@@ -16,6 +16,7 @@
 // compiled with -ffp-contract=off, and HIP's division and square root are
 // correctly rounded, so FP32 / FP64 results equal the CPU definition bit for bit.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -4795,34 +4796,76 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     }
     P.nblocks = nblocks;
     P.block_start = bs;
-    // level-major permutation (counting sort by depth, stable)
-    std::vector<uint64_t> cnt(22, 0);
-    for (uint64_t i = 0; i < n; ++i) {
-        if (cubes[i].depth < 0 || cubes[i].depth > 21) throw Error(TGVX_E_ARG, "cube depth outside 0..21");
-        cnt[cubes[i].depth]++;
-    }
+    // level-major permutation: a stable counting sort by depth, in parallel
+    // over contiguous chunks of the input (per chunk and depth a count, then
+    // every chunk scatters from its own offsets, so each level keeps input
+    // order), with the depths checked and the input's (code, depth) order
+    // within each block tested in the same pass
+    const int nt = (int)std::min<uint64_t>((uint64_t)host_threads(c), std::max<uint64_t>(1, n >> 16));
+    const uint64_t per = (n + (uint64_t)nt - 1) / (uint64_t)nt;
+    std::vector<std::array<uint64_t, 22>> ccnt(nt);
+    std::vector<uint8_t> bad(nt, 0), unordered(nt, 0);
+    parallel_for(c, nt, [&](int k) {
+        std::array<uint64_t, 22>& cc = ccnt[k];
+        cc.fill(0);
+        const uint64_t lo = std::min(n, per * (uint64_t)k), hi = std::min(n, lo + per);
+        if (lo >= hi) return;
+        size_t b = (size_t)(std::upper_bound(bs.begin(), bs.end(), lo) - bs.begin()) - 1;  // lo's block
+        bool ord = true;
+        for (uint64_t i = lo; i < hi; ++i) {
+            const int d = cubes[i].depth;
+            if (d < 0 || d > 21) {
+                bad[k] = 1;
+                return;
+            }
+            cc[d]++;
+            while (i >= bs[b + 1]) ++b;
+            if (ord && i > bs[b]) {
+                const uint64_t x = ((uint64_t)cubes[i - 1].hi << 32) | cubes[i - 1].lo,
+                               y = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
+                ord = !(y < x || (y == x && d < cubes[i - 1].depth));
+            }
+        }
+        unordered[k] = ord ? 0 : 1;
+    });
+    for (int k = 0; k < nt; ++k)
+        if (bad[k]) throw Error(TGVX_E_ARG, "cube depth outside 0..21");
+    bool in_order = true;
+    for (int k = 0; k < nt; ++k) in_order = in_order && !unordered[k];
     P.levels.clear();
-    std::vector<uint64_t> start(22, 0);
+    std::vector<std::array<uint64_t, 22>> coff(nt);
     uint64_t acc = 0;
     for (int d = 0; d < 22; ++d) {
-        start[d] = acc;
-        if (cnt[d]) P.levels.push_back({d, (uint32_t)acc, (uint32_t)cnt[d]});
-        acc += cnt[d];
+        const uint64_t first = acc;
+        for (int k = 0; k < nt; ++k) {
+            coff[k][d] = acc;
+            acc += ccnt[k][d];
+        }
+        if (acc > first) P.levels.push_back({d, (uint32_t)first, (uint32_t)(acc - first)});
     }
-    P.perm.assign(n, 0);
-    P.inv.assign(n, 0);
-    for (uint64_t i = 0; i < n; ++i) {
-        const uint64_t g = start[cubes[i].depth]++;
-        P.perm[g] = (uint32_t)i;
-        P.inv[i] = (uint32_t)g;
-    }
+    P.perm.resize(n);
+    P.inv.resize(n);
+    parallel_for(c, nt, [&](int k) {
+        std::array<uint64_t, 22> o = coff[k];
+        const uint64_t lo = std::min(n, per * (uint64_t)k), hi = std::min(n, lo + per);
+        for (uint64_t i = lo; i < hi; ++i) {
+            const uint64_t g = o[cubes[i].depth]++;
+            P.perm[g] = (uint32_t)i;
+            P.inv[i] = (uint32_t)g;
+        }
+    });
     // per-level, per-block segments of the positions (a level's positions are
-    // in input order, hence grouped by block)
+    // in input order, hence grouped by block); the block of each position
+    // when there are several
     P.seg.assign(P.levels.size(), std::vector<uint32_t>(nblocks + 1, 0));
-    std::vector<int32_t> pblk(n);
+    std::vector<int32_t> pblk(nblocks > 1 ? n : 0);
     for (size_t li = 0; li < P.levels.size(); ++li) {
         const Level& L = P.levels[li];
         std::vector<uint32_t>& sg = P.seg[li];
+        if (nblocks == 1) {
+            sg[1] = L.n;
+            continue;
+        }
         uint32_t b = 0;
         for (uint32_t t = 0; t < L.n; ++t) {
             const uint64_t i = P.perm[L.off + t];
@@ -4838,16 +4881,6 @@ void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_
     // depth) order within each block -- the octree builder's output -- is
     // searched as it is (the lookups take the last of equal entries); other
     // input is sorted and deduplicated here.
-    bool in_order = true;
-    for (uint32_t b = 0; b < nblocks && in_order; ++b)
-        for (uint64_t i = bs[b] + 1; i < bs[b + 1]; ++i) {
-            const uint64_t x = ((uint64_t)cubes[i - 1].hi << 32) | cubes[i - 1].lo,
-                           y = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
-            if (y < x || (y == x && cubes[i].depth < cubes[i - 1].depth)) {
-                in_order = false;
-                break;
-            }
-        }
     std::vector<uint64_t> ucode, dcode;
     std::vector<int32_t> uidx, ddepth, dpos;
     std::vector<uint32_t> uoff(nblocks + 1, 0), doff(nblocks + 1, 0);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -212,6 +213,10 @@ namespace tgvx {
 // stream; d2h returns with dst filled.
 void h2d(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st = nullptr);
 void d2h(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st = nullptr);
+// the same pool's host threads: f(0) .. f(parts - 1) with parts at most
+// host_threads(c), part 0 on the calling thread
+int host_threads(tgvx_ctx* c);
+void parallel_for(tgvx_ctx* c, int parts, const std::function<void(int)>& f);
 // solve.hip
 void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb, int precision,
                   int op_mode, const uint64_t* blk_start = nullptr, uint32_t nblocks = 1);

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <thread>
 
@@ -23,12 +24,13 @@ constexpr size_t kSmall = 2ull << 20;   // below this the runtime's own path
 constexpr size_t kSlice = 1ull << 20;   // smallest share of one copy thread
 }  // namespace
 
-// A fixed pool of host threads; copy() splits one range among them (the
-// calling thread takes the first share) and returns when all are done.
+// A fixed pool of host threads. run(parts, f) calls f(0) .. f(parts - 1),
+// part 0 on the calling thread, the others on the pool's threads, and
+// returns when all have returned; copy() splits one byte range that way.
 class CopyPool {
   public:
     explicit CopyPool(int n) : n_(std::max(1, n)) {
-        for (int i = 1; i < n_; ++i) th_.emplace_back([this, i] { run(i); });
+        for (int i = 1; i < n_; ++i) th_.emplace_back([this, i] { loop(i); });
     }
     ~CopyPool() {
         {
@@ -38,44 +40,51 @@ class CopyPool {
         cv_.notify_all();
         for (auto& t : th_) t.join();
     }
-    void copy(void* dst, const void* src, size_t bytes) {
-        const int parts = (int)std::min<size_t>((size_t)n_, std::max<size_t>(1, bytes / kSlice));
-        if (parts <= 1) {
-            std::memcpy(dst, src, bytes);
+    int threads() const { return n_; }
+    void run(int parts, const std::function<void(int)>& f) {
+        parts = std::max(1, std::min(parts, n_));
+        if (parts == 1) {
+            f(0);
             return;
         }
         {
             std::lock_guard<std::mutex> lk(mu_);
-            dst_ = static_cast<char*>(dst);
-            src_ = static_cast<const char*>(src);
-            bytes_ = bytes;
+            fn_ = &f;
             parts_ = parts;
             left_ = parts - 1;
             ++gen_;
         }
         cv_.notify_all();
-        share(0);
+        f(0);
         std::unique_lock<std::mutex> lk(mu_);
         done_.wait(lk, [this] { return left_ == 0; });
+        fn_ = nullptr;
+    }
+    void copy(void* dst, const void* src, size_t bytes) {
+        const int parts = (int)std::min<size_t>((size_t)n_, std::max<size_t>(1, bytes / kSlice));
+        char* d = static_cast<char*>(dst);
+        const char* s = static_cast<const char*>(src);
+        const size_t per = (bytes + parts - 1) / parts;
+        run(parts, [&](int i) {
+            const size_t lo = std::min(bytes, per * (size_t)i), hi = std::min(bytes, lo + per);
+            if (hi > lo) std::memcpy(d + lo, s + lo, hi - lo);
+        });
     }
 
   private:
-    void share(int i) {
-        const size_t per = (bytes_ + parts_ - 1) / parts_;
-        const size_t lo = std::min(bytes_, per * (size_t)i), hi = std::min(bytes_, lo + per);
-        if (hi > lo) std::memcpy(dst_ + lo, src_ + lo, hi - lo);
-    }
-    void run(int i) {
+    void loop(int i) {
         uint64_t seen = 0;
         for (;;) {
+            const std::function<void(int)>* f;
             {
                 std::unique_lock<std::mutex> lk(mu_);
                 cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
                 if (stop_) return;
                 seen = gen_;
                 if (i >= parts_) continue;
+                f = fn_;
             }
-            share(i);
+            (*f)(i);
             {
                 std::lock_guard<std::mutex> lk(mu_);
                 if (--left_ == 0) done_.notify_one();
@@ -88,9 +97,7 @@ class CopyPool {
     std::condition_variable cv_, done_;
     bool stop_ = false;
     uint64_t gen_ = 0;
-    char* dst_ = nullptr;
-    const char* src_ = nullptr;
-    size_t bytes_ = 0;
+    const std::function<void(int)>* fn_ = nullptr;
     int parts_ = 0, left_ = 0;
 };
 
@@ -103,20 +110,28 @@ Staging::~Staging() {
 }
 
 namespace {
-Staging& staging(tgvx_ctx* c) {
-    if (!c->stage) {
-        auto s = std::make_unique<Staging>();
-        for (int i = 0; i < 2; ++i) {
-            TGVX_HIP(hipHostMalloc(&s->buf[i], kChunk, hipHostMallocDefault));
-            TGVX_HIP(hipEventCreateWithFlags(&s->done[i], hipEventDisableTiming));
-        }
+Staging& staging_of(tgvx_ctx* c) {
+    if (!c->stage) c->stage = std::make_unique<Staging>();
+    Staging& s = *c->stage;
+    if (!s.pool) {
         const unsigned hw = std::thread::hardware_concurrency();
-        s->pool = std::make_shared<CopyPool>((int)std::min(8u, std::max(1u, hw / 2)));
-        c->stage = std::move(s);
+        s.pool = std::make_shared<CopyPool>((int)std::min(8u, std::max(1u, hw / 2)));
     }
-    return *c->stage;
+    return s;
+}
+Staging& staging(tgvx_ctx* c) {
+    Staging& s = staging_of(c);
+    for (int i = 0; i < 2; ++i) {
+        if (!s.buf[i]) TGVX_HIP(hipHostMalloc(&s.buf[i], kChunk, hipHostMallocDefault));
+        if (!s.done[i]) TGVX_HIP(hipEventCreateWithFlags(&s.done[i], hipEventDisableTiming));
+    }
+    return s;
 }
 }  // namespace
+
+int host_threads(tgvx_ctx* c) { return staging_of(c).pool->threads(); }
+
+void parallel_for(tgvx_ctx* c, int parts, const std::function<void(int)>& f) { staging_of(c).pool->run(parts, f); }
 
 void h2d(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st) {
     if (bytes == 0) return;
