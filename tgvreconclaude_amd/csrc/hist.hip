@@ -222,15 +222,30 @@ void launch_mips(const ViewMeta& m, hipStream_t s) {
 void build_pyramids(tgvx_ctx* c, const tgvx_view* views, int nv, std::vector<ViewMeta>& meta) {
     meta.assign(nv, ViewMeta{});
     uint64_t total = 0;
-    for (int v = 0; v < nv; ++v) total = view_meta(views[v], meta[v], total);
-    c->h_pyr.ensure(total * sizeof(float));
-    char* pyr = c->h_pyr.as<char>();
+    for (int v = 0; v < nv; ++v) view_meta(views[v], meta[v], 0);
+    // every view's base level first, one after another (one gathered upload),
+    // then the views' coarser levels
+    std::vector<HostPiece> base(nv);
     for (int v = 0; v < nv; ++v) {
         ViewMeta& m = meta[v];
-        for (int l = 0; l < m.nlev; ++l) m.lv[l] = reinterpret_cast<const float*>(pyr + (uintptr_t)m.lv[l]);
-        h2d(c, const_cast<float*>(m.lv[0]), views[v].depths, sizeof(float) * (size_t)m.w[0] * m.h[0]);
-        launch_mips(m, c->stream);
+        base[v] = HostPiece{views[v].depths, sizeof(float) * (size_t)m.w[0] * m.h[0]};
+        m.lv[0] = reinterpret_cast<const float*>(total * sizeof(float));
+        total += (uint64_t)m.w[0] * m.h[0];
     }
+    for (int v = 0; v < nv; ++v) {
+        ViewMeta& m = meta[v];
+        for (int l = 1; l < m.nlev; ++l) {
+            m.lv[l] = reinterpret_cast<const float*>(total * sizeof(float));
+            total += (uint64_t)m.w[l] * m.h[l];
+        }
+    }
+    c->h_pyr.ensure(total * sizeof(float));
+    char* pyr = c->h_pyr.as<char>();
+    for (int v = 0; v < nv; ++v)
+        for (int l = 0; l < meta[v].nlev; ++l)
+            meta[v].lv[l] = reinterpret_cast<const float*>(pyr + (uintptr_t)meta[v].lv[l]);
+    h2dv(c, pyr, base.data(), nv);
+    for (int v = 0; v < nv; ++v) launch_mips(meta[v], c->stream);
 }
 }  // namespace
 

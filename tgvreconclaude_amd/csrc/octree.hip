@@ -728,12 +728,11 @@ void octree_build(tgvx_ctx* c, const tgvx_depth_view* views, int32_t nviews, con
     // depth maps in, one cube per pixel
     DevBuf dmaps;
     dmaps.ensure(4 * std::max<uint64_t>(total, 1));
+    std::vector<HostPiece> pieces(nviews);
+    for (int k = 0; k < nviews; ++k)
+        pieces[k] = HostPiece{views[k].depths, 4 * (size_t)views[k].width * views[k].height};
+    h2dv(c, dmaps.p, pieces.data(), nviews, st);
     uint64_t off = 0;
-    for (int k = 0; k < nviews; ++k) {
-        const uint64_t p = (uint64_t)views[k].width * views[k].height;
-        h2d(c, dmaps.as<float>() + off, views[k].depths, 4 * p, st);
-        off += p;
-    }
     TGVX_HIP(hipStreamSynchronize(st));
     const double t1 = ms_now();
     T[B_CODE].ensure(8 * std::max<uint64_t>(total, 1));

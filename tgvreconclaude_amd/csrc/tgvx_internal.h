@@ -212,6 +212,12 @@ namespace tgvx {
 // (or st): h2d returns once src may be reused, its DMAs ordered on the
 // stream; d2h returns with dst filled.
 void h2d(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st = nullptr);
+// the pieces' bytes one after another into dst
+struct HostPiece {
+    const void* src;
+    size_t bytes;
+};
+void h2dv(tgvx_ctx* c, void* dst, const HostPiece* pieces, int count, hipStream_t st = nullptr);
 void d2h(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st = nullptr);
 // the same pool's host threads: f(0) .. f(parts - 1) with parts at most
 // host_threads(c), part 0 on the calling thread

@@ -133,26 +133,53 @@ int host_threads(tgvx_ctx* c) { return staging_of(c).pool->threads(); }
 
 void parallel_for(tgvx_ctx* c, int parts, const std::function<void(int)>& f) { staging_of(c).pool->run(parts, f); }
 
-void h2d(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st) {
-    if (bytes == 0) return;
+void h2dv(tgvx_ctx* c, void* dst, const HostPiece* pieces, int count, hipStream_t st) {
     if (!st) st = c->stream;
+    // the pieces' start offsets in the concatenated stream
+    std::vector<size_t> at((size_t)std::max(count, 0) + 1, 0);
+    for (int i = 0; i < count; ++i) at[i + 1] = at[i] + pieces[i].bytes;
+    const size_t bytes = at[count > 0 ? count : 0];
+    if (bytes == 0) return;
+    char* out = static_cast<char*>(dst);
     if (bytes < kSmall) {
-        TGVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        for (int i = 0; i < count; ++i)
+            if (pieces[i].bytes)
+                TGVX_HIP(hipMemcpyAsync(out + at[i], pieces[i].src, pieces[i].bytes, hipMemcpyHostToDevice, st));
         return;
     }
     Staging& s = staging(c);
-    const char* in = static_cast<const char*>(src);
-    char* out = static_cast<char*>(dst);
+    // bytes [lo, hi) of the concatenated stream to d
+    auto gather = [&](char* d, size_t lo, size_t hi) {
+        size_t i = (size_t)(std::upper_bound(at.begin(), at.end(), lo) - at.begin()) - 1;
+        while (lo < hi) {
+            while (at[i + 1] <= lo) ++i;
+            const size_t len = std::min(hi, at[i + 1]) - lo;
+            std::memcpy(d, static_cast<const char*>(pieces[i].src) + (lo - at[i]), len);
+            d += len;
+            lo += len;
+        }
+    };
     for (size_t off = 0; off < bytes; off += kChunk) {
         const size_t len = std::min(kChunk, bytes - off);
         const int b = s.next;
         s.next ^= 1;
         if (s.used[b]) TGVX_HIP(hipEventSynchronize(s.done[b]));  // its previous DMA has read it
-        s.pool->copy(s.buf[b], in + off, len);
-        TGVX_HIP(hipMemcpyAsync(out + off, s.buf[b], len, hipMemcpyHostToDevice, st));
+        char* pin = static_cast<char*>(s.buf[b]);
+        const int parts = (int)std::min<size_t>((size_t)s.pool->threads(), std::max<size_t>(1, len / kSlice));
+        const size_t per = (len + parts - 1) / parts;
+        s.pool->run(parts, [&](int k) {
+            const size_t lo = std::min(len, per * (size_t)k), hi = std::min(len, lo + per);
+            if (hi > lo) gather(pin + lo, off + lo, off + hi);
+        });
+        TGVX_HIP(hipMemcpyAsync(out + off, pin, len, hipMemcpyHostToDevice, st));
         TGVX_HIP(hipEventRecord(s.done[b], st));
         s.used[b] = true;
     }
+}
+
+void h2d(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st) {
+    const HostPiece one{src, bytes};
+    h2dv(c, dst, &one, 1, st);
 }
 
 void d2h(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st) {
