@@ -4137,8 +4137,7 @@ void solve_t(tgvx_ctx* c, const tgvx_params* p, const float* init_uv, tgvx_repor
         std::vector<float> perm_init(w * P.n);
         for (uint64_t g = 0; g < P.n; ++g)
             std::memcpy(&perm_init[w * g], init_uv + w * (size_t)P.perm[g], sizeof(float) * w);
-        TGVX_HIP(hipMemcpyAsync(P.d_init.p, perm_init.data(), sizeof(float) * w * P.n, hipMemcpyHostToDevice,
-                                c->stream));
+        h2d(c, P.d_init.p, perm_init.data(), sizeof(float) * w * P.n);
         TGVX_HIP(hipStreamSynchronize(c->stream));
         a.init = P.d_init.as<float>();
         a.init_stride = init_stride;
@@ -4673,7 +4672,7 @@ void solve_stream_t(tgvx_ctx* c, const tgvx_params* p, const tgvx_treetop_leaf* 
     }
     auto up = [&](DevBuf& b, const void* src, size_t bytes) {
         b.ensure(bytes);
-        TGVX_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+        h2d(c, b.p, src, bytes);
     };
     DevBuf d_inv, d_leaf, d_lf, d_ll, d_code, d_depth;
     up(d_inv, P.inv.data(), 4ull * std::max<uint32_t>(n, 1));
@@ -5201,9 +5200,9 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
         d_leaf.ensure(4ull * n);
         d_lf.ensure(4 * lf.size());
         d_ll.ensure(4 * ll.size());
-        TGVX_HIP(hipMemcpyAsync(d_code.p, code.data(), 8ull * n, hipMemcpyHostToDevice, c->stream));
-        TGVX_HIP(hipMemcpyAsync(d_depth.p, depth.data(), 4ull * n, hipMemcpyHostToDevice, c->stream));
-        TGVX_HIP(hipMemcpyAsync(d_leaf.p, leaf_of.data(), 4ull * n, hipMemcpyHostToDevice, c->stream));
+        h2d(c, d_code.p, code.data(), 8ull * n);
+        h2d(c, d_depth.p, depth.data(), 4ull * n);
+        h2d(c, d_leaf.p, leaf_of.data(), 4ull * n);
         TGVX_HIP(hipMemcpyAsync(d_lf.p, lf.data(), 4 * lf.size(), hipMemcpyHostToDevice, c->stream));
         TGVX_HIP(hipMemcpyAsync(d_ll.p, ll.data(), 4 * ll.size(), hipMemcpyHostToDevice, c->stream));
         const uint32_t chunk = 1u << 20;

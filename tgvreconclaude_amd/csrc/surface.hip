@@ -254,7 +254,7 @@ void surface_extract(tgvx_ctx* c, const float* u_in, const uint8_t* has_in, cons
     DevBuf d_uin, d_hin;
     if (u_in) {
         d_uin.ensure(4ull * n);
-        TGVX_HIP(hipMemcpyAsync(d_uin.p, u_in, 4ull * n, hipMemcpyHostToDevice, st));
+        h2d(c, d_uin.p, u_in, 4ull * n, st);
     } else {
         switch (P.precision) {
             case TGVX_FP64:
@@ -272,7 +272,7 @@ void surface_extract(tgvx_ctx* c, const float* u_in, const uint8_t* has_in, cons
     if (has_in) {
         d_hin.ensure(n);
         hd.ensure(n);
-        TGVX_HIP(hipMemcpyAsync(d_hin.p, has_in, n, hipMemcpyHostToDevice, st));
+        h2d(c, d_hin.p, has_in, n, st);
     }
     if (u_in || has_in)
         k_srf_gather<<<blocks_for(n), 256, 0, st>>>(u_in ? d_uin.as<float>() : nullptr,
@@ -389,9 +389,8 @@ void surface_extract(tgvx_ctx* c, const float* u_in, const uint8_t* has_in, cons
 }
 
 void surface_copy(tgvx_ctx* c, float* verts, uint32_t* tris) {
-    if (c->srf_nv && verts)
-        TGVX_HIP(hipMemcpy(verts, c->srf_verts.p, 12 * c->srf_nv, hipMemcpyDeviceToHost));
-    if (c->srf_nt && tris) TGVX_HIP(hipMemcpy(tris, c->srf_tris.p, 12 * c->srf_nt, hipMemcpyDeviceToHost));
+    if (c->srf_nv && verts) d2h(c, verts, c->srf_verts.p, 12 * c->srf_nv);
+    if (c->srf_nt && tris) d2h(c, tris, c->srf_tris.p, 12 * c->srf_nt);
 }
 
 }  // namespace tgvx
