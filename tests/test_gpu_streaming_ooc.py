@@ -52,3 +52,57 @@ def test_budget_below_a_leaf_is_refused(ctx, sphere):
     leaves = scene.treetop(sc.cubes, 40000)
     with pytest.raises(RuntimeError, match="too small|budget"):
         ctx.solve_streaming_ooc(sc.cubes, hist, leaves, TGVParams(iters=2), device_bytes=2 * PER_CUBE * 100)
+
+
+@pytest.fixture(scope="module")
+def csg9(ctx):
+    sc = scene.csg_scene(21, views=24, W=640, H=480, max_depth=9)
+    hist = ctx.histograms(sc.cubes, sc.views(), sc.origin, sc.r_root, 10)
+    assert len(sc.cubes) >= 4 << 16  # the host passes split it over several threads
+    return sc, hist
+
+
+def test_large_out_of_core_equals_resident(ctx, csg9):
+    """Several host threads share each pass (input check, parent copy,
+    snapshot, batch gathering and scatter) at this size."""
+    sc, hist = csg9
+    leaves = scene.treetop(sc.cubes, 1 << 16)
+    p = TGVParams(alpha0=2.0, alpha1=1.0, iters=10)
+    ref, _ = ctx.solve_streaming(sc.cubes, hist, leaves, p)
+    st, rep, orep = ctx.solve_streaming_ooc(sc.cubes, hist, leaves, p, device_bytes=2 * PER_CUBE * 150000)
+    assert np.array_equal(_bits(st), _bits(ref))
+    assert orep["batches"] > len(np.unique(sc.cubes["depth"]))
+
+
+@pytest.mark.parametrize("parts", [2, 3, 4, 8])
+def test_unsorted_input_refused_at_a_range_boundary(ctx, csg9, parts):
+    sc, hist = csg9
+    cubes = sc.cubes.copy()
+    n = len(cubes)
+    k = (n + parts - 1) // parts
+    # cube k's code below every other (neighbouring codes can be equal)
+    cubes["hi"][k] = cubes["lo"][k] = 0
+    assert cubes["hi"][k - 1] or cubes["lo"][k - 1]
+    with pytest.raises(RuntimeError, match="sorted"):
+        ctx.solve_streaming_ooc(cubes, hist, scene.treetop(sc.cubes, 1 << 16), TGVParams(iters=1))
+
+
+def test_first_bad_cube_in_input_order_is_reported(ctx, csg9):
+    sc, hist = csg9
+    n = len(sc.cubes)
+    leaves = scene.treetop(sc.cubes, 1 << 16)
+    early, late = n // 8, n - 2
+    code = (sc.cubes["hi"].astype(np.uint64) << 32) | sc.cubes["lo"]
+    assert code[early] < code[early + 1] and code[late] < code[late + 1]
+    # an unsorted pair before a cube of impossible depth
+    cubes = sc.cubes.copy()
+    cubes[[early, early + 1]] = cubes[[early + 1, early]]
+    cubes["depth"][late] = 30
+    with pytest.raises(RuntimeError, match="sorted"):
+        ctx.solve_streaming_ooc(cubes, hist, leaves, TGVParams(iters=1))
+    # and the other way round
+    cubes = sc.cubes.copy()
+    cubes["depth"][early] = 30
+    cubes[[late, late + 1]] = cubes[[late + 1, late]]
+    with pytest.raises(RuntimeError, match="depth"):
+        ctx.solve_streaming_ooc(cubes, hist, leaves, TGVParams(iters=1))

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <set>
 #include <vector>
@@ -5146,36 +5147,75 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
     std::vector<uint64_t> code(n);
     std::vector<int32_t> depth(n);
     std::vector<uint32_t> info(n);
+    // host passes over the cubes run on the context's host threads in ranges
+    // of at least kGrain cubes; each writes only its own cubes' entries
+    constexpr size_t kGrain = 1u << 16;
     int min_d = 21, max_d = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        code[i] = ((uint64_t)cubes[i].hi << 32) | cubes[i].lo;
-        depth[i] = cubes[i].depth;
-        if (depth[i] < 0 || depth[i] > 21) throw Error(TGVX_E_ARG, "cube depth outside 0..21");
-        if (i && code[i] < code[i - 1]) throw Error(TGVX_E_ARG, "cubes must be sorted by Morton code");
-        min_d = std::min(min_d, depth[i]);
-        max_d = std::max(max_d, depth[i]);
-        const uint32_t* b = bins + (size_t)i * nbins;
-        uint64_t tot = 0;
-        for (int k = 0; k < nbins; ++k) tot += b[k];
-        if (tot > InfoWord::kWMax) throw Error(TGVX_E_ARG, "vote total of a cube exceeds 2^23 - 1");
-        uint32_t kmed = 0, cum = 0;
-        for (int k = 0; k < nbins; ++k) {
-            cum += b[k];
-            if (cum * 2u >= (uint32_t)tot) {
-                kmed = (uint32_t)k;
+    bool has_depth[22] = {};
+    // the first bad cube in input order and what is wrong with it
+    uint64_t bad_at = ~0ull;
+    int bad_why = 0;
+    std::mutex mu;
+    auto cube_code = [&](uint32_t i) { return ((uint64_t)cubes[i].hi << 32) | cubes[i].lo; };
+    parallel_range(c, n, kGrain, [&](size_t lo, size_t hi) {
+        int dmin = 21, dmax = 0, why = 0;
+        bool seen[22] = {};
+        size_t i = lo;
+        for (; i < hi; ++i) {
+            code[i] = cube_code((uint32_t)i);
+            depth[i] = cubes[i].depth;
+            if (depth[i] < 0 || depth[i] > 21) {
+                why = 1;
                 break;
             }
+            if (i && code[i] < cube_code((uint32_t)i - 1)) {
+                why = 2;
+                break;
+            }
+            dmin = std::min(dmin, depth[i]);
+            dmax = std::max(dmax, depth[i]);
+            seen[depth[i]] = true;
+            const uint32_t* b = bins + i * nbins;
+            uint64_t tot = 0;
+            for (int k = 0; k < nbins; ++k) tot += b[k];
+            if (tot > InfoWord::kWMax) {
+                why = 3;
+                break;
+            }
+            uint32_t kmed = 0, cum = 0;
+            for (int k = 0; k < nbins; ++k) {
+                cum += b[k];
+                if (cum * 2u >= (uint32_t)tot) {
+                    kmed = (uint32_t)k;
+                    break;
+                }
+            }
+            info[i] = (uint32_t)tot | (kmed << 24);
         }
-        info[i] = (uint32_t)tot | (kmed << 24);
-    }
+        std::lock_guard<std::mutex> lk(mu);
+        if (why && i < bad_at) {
+            bad_at = i;
+            bad_why = why;
+        }
+        min_d = std::min(min_d, dmin);
+        max_d = std::max(max_d, dmax);
+        for (int d = 0; d < 22; ++d) has_depth[d] = has_depth[d] || seen[d];
+    });
+    if (bad_why == 1) throw Error(TGVX_E_ARG, "cube depth outside 0..21");
+    if (bad_why == 2) throw Error(TGVX_E_ARG, "cubes must be sorted by Morton code");
+    if (bad_why == 3) throw Error(TGVX_E_ARG, "vote total of a cube exceeds 2^23 - 1");
     std::vector<int32_t> leaf_of(n, -1);
     std::vector<uint32_t> lf(std::max<uint64_t>(nleaves, 1), 1), ll(std::max<uint64_t>(nleaves, 1), 0);
     for (uint64_t l = 0; l < nleaves; ++l) {
         if (leaves[l].first_idx > leaves[l].last_idx || leaves[l].first_idx >= n) continue;
         lf[l] = leaves[l].first_idx;
         ll[l] = std::min<uint32_t>(leaves[l].last_idx, n - 1);
-        for (uint32_t i = lf[l]; i <= ll[l]; ++i)
-            if (leaf_of[i] < 0) leaf_of[i] = (int32_t)l;
+        // a cube in several leaves' ranges belongs to the first
+        const uint32_t f0 = lf[l];
+        parallel_range(c, (size_t)ll[l] - f0 + 1, kGrain, [&](size_t lo, size_t hi) {
+            for (size_t i = f0 + lo; i < f0 + hi; ++i)
+                if (leaf_of[i] < 0) leaf_of[i] = (int32_t)l;
+        });
     }
     // state: u = median (tgv.cpp:821-845), everything else 0
     PinnedBuf st, snap;
@@ -5183,8 +5223,10 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
     snap.alloc(sizeof(float) * 4 * (size_t)n);
     float* S = st.as<float>();
     float* Z = snap.as<float>();
-    std::memset(S, 0, sizeof(float) * 13 * (size_t)n);
-    for (uint32_t i = 0; i < n; ++i) S[13 * (size_t)i] = median_of<float>(info[i], nbins);
+    parallel_range(c, n, kGrain, [&](size_t lo, size_t hi) {
+        std::memset(S + 13 * lo, 0, sizeof(float) * 13 * (hi - lo));
+        for (size_t i = lo; i < hi; ++i) S[13 * i] = median_of<float>(info[i], nbins);
+    });
     if (n == 0) {
         if (rep) *rep = r;
         return;
@@ -5240,8 +5282,10 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
         if (!sl.busy) return;
         TGVX_HIP(hipEventSynchronize(sl.done));
         const float* D = sl.down.as<float>();
-        for (size_t a = 0; a < sl.act.size(); ++a)
-            for (int f = 0; f < 13; ++f) S[13 * (size_t)sl.act[a] + f] = D[f * sl.w + a];
+        parallel_range(c, sl.act.size(), kGrain, [&](size_t lo, size_t hi) {
+            for (size_t a = lo; a < hi; ++a)
+                for (int f = 0; f < 13; ++f) S[13 * (size_t)sl.act[a] + f] = D[f * sl.w + a];
+        });
         sl.busy = false;
     };
     std::vector<int32_t> loc(n, -1);
@@ -5252,34 +5296,42 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
     uint64_t moved = 0;
     for (int L = min_d; L <= max_d; ++L) {
         // the leaves' cubes of depth L, leaf after leaf
+        if (!has_depth[L]) continue;
         std::vector<std::vector<uint32_t>> by_leaf(std::max<uint64_t>(nleaves, 1));
+        const int nlt = (int)std::min<uint64_t>((uint64_t)host_threads(c), nleaves);
+        parallel_for(c, nlt, [&](int k) {
+            for (uint64_t l = (uint64_t)k; l < nleaves; l += (uint64_t)nlt) {
+                if (lf[l] > ll[l]) continue;
+                for (uint32_t i = lf[l]; i <= ll[l]; ++i)
+                    if (depth[i] == L && leaf_of[i] == (int32_t)l) by_leaf[l].push_back(i);
+            }
+        });
         uint64_t nL = 0;
-        for (uint64_t l = 0; l < nleaves; ++l) {
-            if (lf[l] > ll[l]) continue;
-            for (uint32_t i = lf[l]; i <= ll[l]; ++i)
-                if (depth[i] == L && leaf_of[i] == (int32_t)l) by_leaf[l].push_back(i);
-            nL += by_leaf[l].size();
-        }
-        bool any = false;
-        for (uint32_t i = 0; i < n && !any; ++i) any = depth[i] == L;
-        if (!any) continue;
-        // parent copy, leaf after leaf: each leaf reads its parents first
+        for (uint64_t l = 0; l < nleaves; ++l) nL += by_leaf[l].size();
+        // parent copy, leaf after leaf: each leaf reads all its parents, then
+        // writes its cubes
         if (L > min_d) {
             std::vector<float> tmp;
             for (uint64_t l = 0; l < nleaves; ++l) {
                 const std::vector<uint32_t>& v = by_leaf[l];
                 tmp.assign(4 * v.size(), 0.f);
-                for (size_t k = 0; k < v.size(); ++k)
-                    if (par[v[k]] >= 0) std::memcpy(&tmp[4 * k], S + 13 * (size_t)par[v[k]], 16);
-                for (size_t k = 0; k < v.size(); ++k) {
-                    float* s = S + 13 * (size_t)v[k];
-                    if (par[v[k]] >= 0) std::memcpy(s, &tmp[4 * k], 16);
-                    std::memset(s + 4, 0, 9 * sizeof(float));
-                }
+                parallel_range(c, v.size(), kGrain, [&](size_t lo, size_t hi) {
+                    for (size_t k = lo; k < hi; ++k)
+                        if (par[v[k]] >= 0) std::memcpy(&tmp[4 * k], S + 13 * (size_t)par[v[k]], 16);
+                });
+                parallel_range(c, v.size(), kGrain, [&](size_t lo, size_t hi) {
+                    for (size_t k = lo; k < hi; ++k) {
+                        float* s = S + 13 * (size_t)v[k];
+                        if (par[v[k]] >= 0) std::memcpy(s, &tmp[4 * k], 16);
+                        std::memset(s + 4, 0, 9 * sizeof(float));
+                    }
+                });
             }
         }
         // level-start snapshot of every cube's (u, v)
-        for (uint32_t i = 0; i < n; ++i) std::memcpy(Z + 4 * (size_t)i, S + 13 * (size_t)i, 16);
+        parallel_range(c, n, kGrain, [&](size_t lo, size_t hi) {
+            for (size_t i = lo; i < hi; ++i) std::memcpy(Z + 4 * i, S + 13 * i, 16);
+        });
         const double tl0 = now_ms();
         // batches of whole leaves; a leaf's cubes read at most 7 local slots each
         uint64_t l = 0, used_slot = 0;
@@ -5341,21 +5393,23 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
             const size_t W = na + others.size();
             sl.w = W;
             int32_t* NB = sl.nb(U);
-            for (size_t a = 0; a < na; ++a) {
-                const uint32_t i = sl.act[a];
-                for (int f = 0; f < 6; ++f) NB[f * W + a] = nbl[6 * a + f];
-                sl.info(U)[a] = info[i];
-                sl.flag(U)[a] = sflag[i];
-            }
             auto put = [&](size_t a, uint32_t i) {
                 for (int f = 0; f < 13; ++f) sl.st(U)[f * W + a] = S[13 * (size_t)i + f];
                 for (int k = 0; k < 4; ++k) sl.fz(U)[k * W + a] = Z[4 * (size_t)i + k];
             };
-            for (size_t a = 0; a < na; ++a) {
-                put(a, sl.act[a]);
-                for (int k = 0; k < 4; ++k) sl.bar(U)[k * W + a] = Z[4 * (size_t)sl.act[a] + k];
-            }
-            for (size_t k = 0; k < others.size(); ++k) put(na + k, others[k]);
+            parallel_range(c, na, kGrain, [&](size_t lo, size_t hi) {
+                for (size_t a = lo; a < hi; ++a) {
+                    const uint32_t i = sl.act[a];
+                    for (int f = 0; f < 6; ++f) NB[f * W + a] = nbl[6 * a + f];
+                    sl.info(U)[a] = info[i];
+                    sl.flag(U)[a] = sflag[i];
+                    put(a, i);
+                    for (int k = 0; k < 4; ++k) sl.bar(U)[k * W + a] = Z[4 * (size_t)i + k];
+                }
+            });
+            parallel_range(c, others.size(), kGrain, [&](size_t lo, size_t hi) {
+                for (size_t k = lo; k < hi; ++k) put(na + k, others[k]);
+            });
             for (uint32_t i : sl.act) loc[i] = -1;
             for (uint32_t i : others) loc[i] = -1;
             // up, iterate, down
@@ -5411,7 +5465,9 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
         r.leaf_iters += (double)nL * p->iters;
         ++li;
     }
-    std::memcpy(state_out, S, sizeof(float) * 13 * (size_t)n);
+    parallel_range(c, n, kGrain, [&](size_t lo, size_t hi) {
+        std::memcpy(state_out + 13 * lo, S + 13 * lo, sizeof(float) * 13 * (hi - lo));
+    });
     r.n_levels = std::min(li, TGVX_MAX_LEVELS);
     r.solve_ms = now_ms() - t0;
     r.iterate_ms = it_ms;

@@ -223,6 +223,9 @@ void d2h(tgvx_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st =
 // host_threads(c), part 0 on the calling thread
 int host_threads(tgvx_ctx* c);
 void parallel_for(tgvx_ctx* c, int parts, const std::function<void(int)>& f);
+// f(lo, hi) over consecutive ranges covering [0, count), one per part, with at
+// least `grain` items per part
+void parallel_range(tgvx_ctx* c, size_t count, size_t grain, const std::function<void(size_t, size_t)>& f);
 // solve.hip
 void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb, int precision,
                   int op_mode, const uint64_t* blk_start = nullptr, uint32_t nblocks = 1);

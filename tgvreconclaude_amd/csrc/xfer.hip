@@ -133,6 +133,17 @@ int host_threads(tgvx_ctx* c) { return staging_of(c).pool->threads(); }
 
 void parallel_for(tgvx_ctx* c, int parts, const std::function<void(int)>& f) { staging_of(c).pool->run(parts, f); }
 
+void parallel_range(tgvx_ctx* c, size_t count, size_t grain, const std::function<void(size_t, size_t)>& f) {
+    if (count == 0) return;
+    CopyPool& pool = *staging_of(c).pool;
+    const int parts = (int)std::min<size_t>((size_t)pool.threads(), std::max<size_t>(1, count / std::max<size_t>(grain, 1)));
+    const size_t per = (count + parts - 1) / parts;
+    pool.run(parts, [&](int k) {
+        const size_t lo = std::min(count, per * (size_t)k), hi = std::min(count, lo + per);
+        if (hi > lo) f(lo, hi);
+    });
+}
+
 void h2dv(tgvx_ctx* c, void* dst, const HostPiece* pieces, int count, hipStream_t st) {
     if (!st) st = c->stream;
     // the pieces' start offsets in the concatenated stream
