@@ -5257,10 +5257,9 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
                 d_code.as<uint64_t>(), d_depth.as<int32_t>(), n, d_leaf.as<int32_t>(), d_lf.as<uint32_t>(),
                 d_ll.as<uint32_t>(), i0, cn, d_nb.as<int32_t>(), d_fl.as<uint8_t>(), d_par.as<int32_t>());
             TGVX_HIP(hipGetLastError());
-            TGVX_HIP(hipMemcpyAsync(nbg.data() + 6 * (size_t)i0, d_nb.p, 24ull * cn, hipMemcpyDeviceToHost, c->stream));
-            TGVX_HIP(hipMemcpyAsync(sflag.data() + i0, d_fl.p, cn, hipMemcpyDeviceToHost, c->stream));
-            TGVX_HIP(hipMemcpyAsync(par.data() + i0, d_par.p, 4ull * cn, hipMemcpyDeviceToHost, c->stream));
-            TGVX_HIP(hipStreamSynchronize(c->stream));
+            d2h(c, nbg.data() + 6 * (size_t)i0, d_nb.p, 24ull * cn);
+            d2h(c, sflag.data() + i0, d_fl.p, cn);
+            d2h(c, par.data() + i0, d_par.p, 4ull * cn);
         }
     }
     r.setup_ms = now_ms() - t0;
