@@ -62,16 +62,19 @@ def csg9(ctx):
     return sc, hist
 
 
-def test_large_out_of_core_equals_resident(ctx, csg9):
-    """Several host threads share each pass (input check, parent copy,
-    snapshot, batch gathering and scatter) at this size."""
+@pytest.mark.parametrize("leaf_max,slot_cubes", [(1 << 16, 150000), (1 << 17, 300000), (1 << 19, 300000)])
+def test_large_out_of_core_equals_resident(ctx, csg9, leaf_max, slot_cubes):
+    """Several host threads share each pass at this size (input check, parent
+    copy, snapshot, gathering and scatter; with leaves or batches of more than
+    2^17 cubes of one depth also the batch cut and the local numbering)."""
     sc, hist = csg9
-    leaves = scene.treetop(sc.cubes, 1 << 16)
+    leaves = scene.treetop(sc.cubes, leaf_max)
     p = TGVParams(alpha0=2.0, alpha1=1.0, iters=10)
     ref, _ = ctx.solve_streaming(sc.cubes, hist, leaves, p)
-    st, rep, orep = ctx.solve_streaming_ooc(sc.cubes, hist, leaves, p, device_bytes=2 * PER_CUBE * 150000)
+    st, rep, orep = ctx.solve_streaming_ooc(sc.cubes, hist, leaves, p, device_bytes=2 * PER_CUBE * slot_cubes)
     assert np.array_equal(_bits(st), _bits(ref))
-    assert orep["batches"] > len(np.unique(sc.cubes["depth"]))
+    if leaf_max == 1 << 16:
+        assert orep["batches"] > len(np.unique(sc.cubes["depth"]))
 
 
 @pytest.mark.parametrize("parts", [2, 3, 4, 8])

@@ -5346,49 +5346,86 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
             sl.act.clear();
             std::vector<uint32_t> others;
             // the batch: whole leaves while the distinct cubes they iterate
-            // and read fit the slot (counted with a stamp per batch)
+            // and read fit the slot, counted with a stamp per batch. A
+            // leaf's cubes are stamped on the host threads: the thread whose
+            // exchange first sets a cube's stamp counts it, so the count is
+            // the leaf's new distinct cubes in any thread order
             ++stamp;
             uint64_t l1 = l;
             size_t count = 0;
-            std::vector<uint32_t> fresh;
+            std::vector<std::vector<uint32_t>> fresh(host_threads(c));
             while (l1 < nleaves) {
-                fresh.clear();
-                auto see = [&](int32_t j) {
-                    if (j >= 0 && mark[j] != stamp) {
-                        mark[j] = stamp;
-                        fresh.push_back((uint32_t)j);
+                const std::vector<uint32_t>& v = by_leaf[l1];
+                for (auto& fr : fresh) fr.clear();
+                const int parts = parallel_split(c, v.size(), kGrain, [&](int k, size_t lo, size_t hi) {
+                    std::vector<uint32_t>& fr = fresh[k];
+                    auto see = [&](int32_t j) {
+                        if (j >= 0 && __atomic_load_n(&mark[j], __ATOMIC_RELAXED) != stamp &&
+                            __atomic_exchange_n(&mark[j], stamp, __ATOMIC_RELAXED) != stamp)
+                            fr.push_back((uint32_t)j);
+                    };
+                    for (size_t x = lo; x < hi; ++x) {
+                        const uint32_t i = v[x];
+                        see((int32_t)i);
+                        for (int f = 0; f < 6; ++f) see(nbg[6 * (size_t)i + f]);
                     }
-                };
-                for (uint32_t i : by_leaf[l1]) {
-                    see((int32_t)i);
-                    for (int f = 0; f < 6; ++f) see(nbg[6 * (size_t)i + f]);
-                }
-                if (count + fresh.size() > C) {
-                    for (uint32_t j : fresh) mark[j] = 0;
+                });
+                size_t nf = 0;
+                for (int k = 0; k < parts; ++k) nf += fresh[k].size();
+                if (count + nf > C) {
+                    for (int k = 0; k < parts; ++k)
+                        for (uint32_t j : fresh[k]) mark[j] = 0;
                     break;
                 }
-                count += fresh.size();
+                count += nf;
                 ++l1;
             }
             if (l1 == l) throw Error(TGVX_E_ARG, "device budget too small for treetop leaf " + std::to_string(l));
-            for (uint64_t k = l; k < l1; ++k)
-                for (uint32_t i : by_leaf[k]) {
-                    loc[i] = (int32_t)sl.act.size();
-                    sl.act.push_back(i);
-                }
+            for (uint64_t k = l; k < l1; ++k) sl.act.insert(sl.act.end(), by_leaf[k].begin(), by_leaf[k].end());
             const size_t na = sl.act.size();
-            auto local = [&](int32_t j) -> int32_t {
-                if (j < 0) return -1;
-                if (loc[j] < 0) {
-                    loc[j] = (int32_t)(na + others.size());
-                    others.push_back((uint32_t)j);
+            parallel_range(c, na, kGrain, [&](size_t lo, size_t hi) {
+                for (size_t a = lo; a < hi; ++a) loc[sl.act[a]] = (int32_t)a;
+            });
+            // the other cubes the batch reads get local indices na, na + 1,
+            // ... in the order of their first reading (iterated cube a, then
+            // face f). On the host threads: a cube's claim -(2 + a) keeps
+            // the smallest reader a (an atomic max), then the thread holding
+            // that reader lists the cube at its first face and marks it
+            // listed (INT32_MIN); the threads' lists in range order are that
+            // first-reading order
+            std::vector<std::vector<uint32_t>>& mine = fresh;
+            for (auto& m : mine) m.clear();
+            parallel_split(c, na, kGrain, [&](int, size_t lo, size_t hi) {
+                for (size_t a = lo; a < hi; ++a) {
+                    const int32_t want = -2 - (int32_t)a;
+                    for (int f = 0; f < 6; ++f) {
+                        const int32_t j = nbg[6 * (size_t)sl.act[a] + f];
+                        if (j < 0) continue;
+                        int32_t cur = __atomic_load_n(&loc[j], __ATOMIC_RELAXED);
+                        while ((cur == -1 || cur < want) &&
+                               !__atomic_compare_exchange_n(&loc[j], &cur, want, false, __ATOMIC_RELAXED,
+                                                            __ATOMIC_RELAXED)) {
+                        }
+                    }
                 }
-                return loc[j];
-            };
-            // local indices of every neighbour first: they fix the batch's size
-            std::vector<int32_t> nbl(6 * na);
-            for (size_t a = 0; a < na; ++a)
-                for (int f = 0; f < 6; ++f) nbl[6 * a + f] = local(nbg[6 * (size_t)sl.act[a] + f]);
+            });
+            const int parts = parallel_split(c, na, kGrain, [&](int k, size_t lo, size_t hi) {
+                std::vector<uint32_t>& m = mine[k];
+                for (size_t a = lo; a < hi; ++a) {
+                    const int32_t want = -2 - (int32_t)a;
+                    for (int f = 0; f < 6; ++f) {
+                        const int32_t j = nbg[6 * (size_t)sl.act[a] + f];
+                        if (j >= 0 && __atomic_load_n(&loc[j], __ATOMIC_RELAXED) == want) {
+                            __atomic_store_n(&loc[j], INT32_MIN, __ATOMIC_RELAXED);
+                            m.push_back((uint32_t)j);
+                        }
+                    }
+                }
+            });
+            for (int k = 0; k < parts; ++k) others.insert(others.end(), mine[k].begin(), mine[k].end());
+            parallel_range(c, others.size(), kGrain, [&](size_t lo, size_t hi) {
+                for (size_t k = lo; k < hi; ++k) loc[others[k]] = (int32_t)(na + k);
+            });
             const size_t W = na + others.size();
             sl.w = W;
             int32_t* NB = sl.nb(U);
@@ -5399,7 +5436,10 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
             parallel_range(c, na, kGrain, [&](size_t lo, size_t hi) {
                 for (size_t a = lo; a < hi; ++a) {
                     const uint32_t i = sl.act[a];
-                    for (int f = 0; f < 6; ++f) NB[f * W + a] = nbl[6 * a + f];
+                    for (int f = 0; f < 6; ++f) {
+                        const int32_t j = nbg[6 * (size_t)i + f];
+                        NB[f * W + a] = j < 0 ? -1 : loc[j];
+                    }
                     sl.info(U)[a] = info[i];
                     sl.flag(U)[a] = sflag[i];
                     put(a, i);
@@ -5409,8 +5449,12 @@ void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const u
             parallel_range(c, others.size(), kGrain, [&](size_t lo, size_t hi) {
                 for (size_t k = lo; k < hi; ++k) put(na + k, others[k]);
             });
-            for (uint32_t i : sl.act) loc[i] = -1;
-            for (uint32_t i : others) loc[i] = -1;
+            parallel_range(c, na, kGrain, [&](size_t lo, size_t hi) {
+                for (size_t a = lo; a < hi; ++a) loc[sl.act[a]] = -1;
+            });
+            parallel_range(c, others.size(), kGrain, [&](size_t lo, size_t hi) {
+                for (size_t k = lo; k < hi; ++k) loc[others[k]] = -1;
+            });
             // up, iterate, down
             if (W > C) throw Error(TGVX_E_STATE, "out-of-core batch exceeds its slot");
             TGVX_HIP(hipMemcpyAsync(sl.dev.p, U, 4 * OocSlot::words(W), hipMemcpyHostToDevice, sl.stream));

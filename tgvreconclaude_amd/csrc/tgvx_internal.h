@@ -226,6 +226,9 @@ void parallel_for(tgvx_ctx* c, int parts, const std::function<void(int)>& f);
 // f(lo, hi) over consecutive ranges covering [0, count), one per part, with at
 // least `grain` items per part
 void parallel_range(tgvx_ctx* c, size_t count, size_t grain, const std::function<void(size_t, size_t)>& f);
+// the same with the part's number k (ranges ascend with k); returns the
+// number of parts, at most host_threads(c)
+int parallel_split(tgvx_ctx* c, size_t count, size_t grain, const std::function<void(int, size_t, size_t)>& f);
 // solve.hip
 void problem_load(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n, const uint32_t* bins, int nb, int precision,
                   int op_mode, const uint64_t* blk_start = nullptr, uint32_t nblocks = 1);

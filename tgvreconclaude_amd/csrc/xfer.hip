@@ -133,15 +133,20 @@ int host_threads(tgvx_ctx* c) { return staging_of(c).pool->threads(); }
 
 void parallel_for(tgvx_ctx* c, int parts, const std::function<void(int)>& f) { staging_of(c).pool->run(parts, f); }
 
-void parallel_range(tgvx_ctx* c, size_t count, size_t grain, const std::function<void(size_t, size_t)>& f) {
-    if (count == 0) return;
+int parallel_split(tgvx_ctx* c, size_t count, size_t grain, const std::function<void(int, size_t, size_t)>& f) {
+    if (count == 0) return 0;
     CopyPool& pool = *staging_of(c).pool;
     const int parts = (int)std::min<size_t>((size_t)pool.threads(), std::max<size_t>(1, count / std::max<size_t>(grain, 1)));
     const size_t per = (count + parts - 1) / parts;
     pool.run(parts, [&](int k) {
         const size_t lo = std::min(count, per * (size_t)k), hi = std::min(count, lo + per);
-        if (hi > lo) f(lo, hi);
+        if (hi > lo) f(k, lo, hi);
     });
+    return parts;
+}
+
+void parallel_range(tgvx_ctx* c, size_t count, size_t grain, const std::function<void(size_t, size_t)>& f) {
+    parallel_split(c, count, grain, [&](int, size_t lo, size_t hi) { f(lo, hi); });
 }
 
 void h2dv(tgvx_ctx* c, void* dst, const HostPiece* pieces, int count, hipStream_t st) {
