@@ -5130,7 +5130,9 @@ struct OocSlot {
 // snapshot, border p, q are zero), so the result is the resident solve's, bit
 // for bit. The sorted codes, depths and leaf of every cube (16 B per cube)
 // stay on the device for the neighbour lookups (CubeFileReader::lookup's
-// bisection, tgv.cpp:673-683), like the reference's open cube file.
+// bisection, tgv.cpp:673-683), like the reference's open cube file. The
+// host's passes over the cubes run on the context's host threads; each gives
+// the single-thread result (the same batches and the same slot layout).
 void solve_stream_ooc(tgvx_ctx* c, const tgvx_cube* cubes, uint64_t n64, const uint32_t* bins, int nbins,
                       const tgvx_treetop_leaf* leaves, uint64_t nleaves, const tgvx_params* p, uint64_t budget,
                       float* state_out, tgvx_report* rep, tgvx_ooc_report* orep) {
