@@ -15,6 +15,7 @@ namespace {
 template <class F>
 int guarded(tgvx_ctx* c, F&& f) {
     if (!c) return TGVX_E_ARG;
+    const tgvx::CacheScope scope(c->cache.get());
     try {
         TGVX_HIP(hipSetDevice(c->device));
         f();

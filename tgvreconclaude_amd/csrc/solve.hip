@@ -5109,6 +5109,9 @@ struct OocSlot {
     int32_t* nb(float* b) const { return reinterpret_cast<int32_t*>(b + 22 * w); }
     uint8_t* flag(float* b) const { return reinterpret_cast<uint8_t*>(b + 28 * w); }
     ~OocSlot() {
+        // the slot's work has ended before its device block goes back to the
+        // context's cache (a call that threw may leave some in flight)
+        if (stream) (void)hipStreamSynchronize(stream);
         if (done) (void)hipEventDestroy(done);
         if (stream) (void)hipStreamDestroy(stream);
     }

@@ -7,6 +7,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -29,23 +30,71 @@ struct Error : std::runtime_error {
             throw ::tgvx::Error(TGVX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
     } while (0)
 
-// Device allocation owned by the context; freed on reset / destruction.
+// Device blocks released by one context's buffers, kept for that context's
+// later allocations: all of a context's device work is in order on its one
+// stream, so a block released there can be handed out again at once, without
+// the device-wide wait of hipFree (repeated loads of problems of one size
+// reuse every block). At most kMaxHeld bytes are kept (the largest blocks
+// are freed first beyond that), and an allocation that finds the device full
+// frees every kept block and tries again. Freed with the last buffer that
+// refers to it.
+class BlockCache : public std::enable_shared_from_this<BlockCache> {
+  public:
+    static constexpr size_t kMaxHeld = 32ull << 30;
+    BlockCache() = default;
+    BlockCache(const BlockCache&) = delete;
+    BlockCache& operator=(const BlockCache&) = delete;
+    ~BlockCache();
+    // a kept block of b .. b + b / 8 + 2 MiB bytes (its size in *got), else a
+    // new one of b bytes
+    void* get(size_t b, size_t* got);
+    void put(void* p, size_t bytes);
+    void trim(size_t keep);  // free kept blocks, largest first, down to keep bytes
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+    size_t held_ = 0;
+};
+// the cache of the context whose call runs on this thread (set by every C-ABI
+// entry point for the call's duration), or null
+BlockCache* current_cache();
+struct CacheScope {
+    BlockCache* prev;
+    explicit CacheScope(BlockCache* c);
+    ~CacheScope();
+};
+
+// Device allocation owned by the context; released on reset / destruction,
+// into the cache of the context it was made for (else freed).
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    std::shared_ptr<BlockCache> owner;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (owner)
+                owner->put(p, bytes);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
+        owner.reset();
     }
     void ensure(size_t b) {
         if (b <= bytes && p) return;
         release();
         if (b == 0) b = 16;
+        if (BlockCache* bc = current_cache()) {
+            owner = bc->shared_from_this();
+            p = bc->get(b, &bytes);
+            return;
+        }
         TGVX_HIP(hipMalloc(&p, b));
         bytes = b;
     }
@@ -54,6 +103,7 @@ struct DevBuf {
     void swap(DevBuf& o) {
         std::swap(p, o.p);
         std::swap(bytes, o.bytes);
+        std::swap(owner, o.owner);
     }
 };
 
@@ -178,6 +228,7 @@ struct TuneEntry {
 }  // namespace tgvx
 
 struct tgvx_ctx {
+    std::shared_ptr<tgvx::BlockCache> cache = std::make_shared<tgvx::BlockCache>();
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4 + 2 * TGVX_MAX_LEVELS] = {};  // 4 general, then a pair per level

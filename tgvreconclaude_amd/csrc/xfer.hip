@@ -101,6 +101,60 @@ class CopyPool {
     int parts_ = 0, left_ = 0;
 };
 
+namespace {
+thread_local BlockCache* t_cache = nullptr;
+}  // namespace
+
+BlockCache* current_cache() { return t_cache; }
+CacheScope::CacheScope(BlockCache* c) : prev(t_cache) { t_cache = c; }
+CacheScope::~CacheScope() { t_cache = prev; }
+
+BlockCache::~BlockCache() { trim(0); }
+
+void* BlockCache::get(size_t b, size_t* got) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.lower_bound(b);
+        if (it != free_.end() && it->first <= b + b / 8 + (2ull << 20)) {
+            void* p = it->second;
+            *got = it->first;
+            held_ -= it->first;
+            free_.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, b) != hipSuccess) {
+        (void)hipGetLastError();
+        trim(0);
+        TGVX_HIP(hipMalloc(&p, b));
+    }
+    *got = b;
+    return p;
+}
+
+void BlockCache::put(void* p, size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    free_.emplace(bytes, p);
+    held_ += bytes;
+    while (held_ > kMaxHeld && !free_.empty()) {
+        auto last = std::prev(free_.end());
+        (void)hipFree(last->second);
+        held_ -= last->first;
+        free_.erase(last);
+    }
+}
+
+void BlockCache::trim(size_t keep) {
+    std::lock_guard<std::mutex> lk(mu_);
+    while (held_ > keep && !free_.empty()) {
+        auto last = std::prev(free_.end());
+        (void)hipFree(last->second);
+        held_ -= last->first;
+        free_.erase(last);
+    }
+}
+
 Staging::~Staging() {
     for (int i = 0; i < 2; ++i) {
         if (done[i]) (void)hipEventSynchronize(done[i]);
